@@ -1,0 +1,267 @@
+/*
+ * ysim_hip.h — C ABI of the MI355X (gfx950) cloth-collision and spring-dynamics
+ * library.  This is the whole drop-in boundary of the HIP backend: the C++
+ * `HIP` backend tag (ysim_amd/host/ysim_hip_backend.hpp), the Python test
+ * harness (ctypes) and the benchmark call only the functions declared here.
+ *
+ * Conventions
+ *   - every entry point is `extern "C"`, takes plain pointers and sizes and
+ *     returns an int status (YSIM_OK == 0); on failure the message is in
+ *     ysim_hip_last_error() (thread-local);
+ *   - pointers named d_* are device pointers obtained from ysim_hip_alloc;
+ *     pointers named h_* are host pointers;
+ *   - vectors are packed float3 (3 floats per vertex) and facets packed uint3
+ *     with mesh-local vertex ids, as in the reference's packed scene buffers
+ *     (include/sim/scene.hpp:390-422, 800-840);
+ *   - a context owns one HIP stream; calls on one context are ordered, calls on
+ *     distinct contexts are independent.  A context is not thread-safe.
+ *
+ * Reference interfaces each group replaces are cited per function
+ * (paths relative to the yeduk3/ysim repository).
+ */
+#ifndef YSIM_HIP_H
+#define YSIM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+enum {
+  YSIM_OK = 0,
+  YSIM_ERR_INVALID = 1,   /* bad argument / shape */
+  YSIM_ERR_HIP = 2,       /* HIP runtime error */
+  YSIM_ERR_CAPACITY = 3,  /* an output buffer was too small; required size reported */
+  YSIM_ERR_UNSUPPORTED = 4,
+  YSIM_ERR_STATE = 5      /* call out of order (e.g. refit before build) */
+};
+
+/* ------------------------------------------------------------------ PODs   */
+/* Byte-identical to the reference kernel structs (src/metal/bvh.metal:7-24,
+ * src/metal/common_types.metalh:4-17, include/sim/collision_pod.hpp:256-280,
+ * 389-396).  static_asserts in the library pin the sizes. */
+typedef struct ysim_aabb4 {        /* AABB4, include/sim/bvh.hpp:41-102 */
+  float min[3]; int32_t i0;        /* i0 = numPrimitives when used as scene box */
+  float max[3]; int32_t i1;
+} ysim_aabb4;
+
+typedef struct ysim_bvh_node {     /* BVHNode, src/metal/bvh.metal:14-19 */
+  float min[3]; int32_t child_a;   /* child_a < 0 marks a leaf */
+  float max[3]; int32_t child_b;   /* leaf: facet id (mesh-local) */
+} ysim_bvh_node;
+
+typedef struct ysim_broad_collision {  /* BroadCollision, 32 B */
+  uint32_t point, triangle;            /* indexPair (mesh-local ids) */
+  uint32_t query_obj, target_obj;      /* objPair (mesh array indices) */
+  uint32_t query_behavior, target_behavior;
+  uint32_t query_shape, target_shape;
+} ysim_broad_collision;
+
+typedef struct ysim_narrow_collision { /* NarrowCollision, 48 B */
+  uint32_t point, triangle;
+  uint32_t query_obj, target_obj;
+  float normal_distance[4];            /* xyz: unit normal, w: signed distance */
+  uint32_t query_behavior, target_behavior;
+  uint32_t query_shape, target_shape;
+} ysim_narrow_collision;
+
+typedef struct ysim_fastgrid_params {  /* FastGridClothBehaviorParams<float>, 44 B */
+  uint32_t particle_num_1d;
+  float stretch_rest_x, stretch_rest_y;
+  float shear_rest_a, shear_rest_b;
+  float bend_rest_x, bend_rest_y;
+  float kstretch, kshear, kbend;
+  float thickness;                     /* contact-response gate (physics.metal:284-291) */
+} ysim_fastgrid_params;
+
+/* BehaviorType subset on this path (include/sim/mesh_state.hpp:108-121). */
+enum {
+  YSIM_BEHAVIOR_FLOAT = 0,            /* static obstacle: target only, never moves */
+  YSIM_BEHAVIOR_FASTGRID_CLOTH = 1    /* grid cloth: query + target, integrated */
+};
+
+/* One mesh of a packed scene (the Scene::pack contract, scene.hpp:783-1182). */
+typedef struct ysim_mesh_desc {
+  uint32_t num_vertices;
+  uint32_t num_facets;
+  uint32_t behavior;                   /* YSIM_BEHAVIOR_* */
+  uint32_t collidable;                 /* 0: leaves the broad phase entirely */
+  uint32_t group;                      /* meshes interact only within one group (batching) */
+  uint32_t shape;                      /* ColliderKind lane carried into pair records; 0 = Mesh */
+  ysim_fastgrid_params cloth;          /* used when behavior == FASTGRID_CLOTH */
+} ysim_mesh_desc;
+
+/* Per-scene stepping parameters: SymplecticSystem fields
+ * (include/sim/symplectic_system.hpp:190-222) plus the Simulator collision knobs
+ * (include/sim/simulator.hpp:121-181, bvh.hpp:2322-2409). */
+typedef struct ysim_sim_params {
+  float subh;                /* substep dt */
+  float kair;                /* air drag coefficient (force += kair * v) */
+  float kd;                  /* spring damping */
+  float margin;              /* broad query half-width (Simulator::margin) */
+  float radius;              /* narrow radius (Simulator::radius) */
+  float narrow_thickness;    /* narrow band = radius + narrow_thickness, strict < */
+  uint32_t morton_bits;      /* 30 or 63 */
+  uint32_t rebuild_every;    /* 1: rebuild the dynamic LBVHs every substep; 0: build once then refit */
+  uint32_t self_collision;   /* query a cloth against its own tree */
+  uint32_t swept;            /* leaf = hull{x, x + v*subh} (enlargeTrajectory) */
+  uint32_t broad_mode;       /* 0: LBVH traversal, 1: spatial hash (same candidate set) */
+  uint32_t hash_slots_log2;  /* spatial hash table size = 2^hash_slots_log2 */
+  float hash_cell;           /* spatial hash cell edge (0 = 2 * margin) */
+  uint32_t contact_capacity; /* contacts kept per vertex in the fused path (0 = default 16) */
+} ysim_sim_params;
+
+/* Stages timed by ysim_hip_scene_stage_timings, in execution order. */
+enum {
+  YSIM_STAGE_BOX = 0,        /* S0 scene box */
+  YSIM_STAGE_MORTON = 1,     /* S1 Morton codes (+ sort digit histograms) */
+  YSIM_STAGE_SORT = 2,       /* S2 LSD radix sort (all passes) */
+  YSIM_STAGE_HIERARCHY = 3,  /* S3 Karras hierarchy */
+  YSIM_STAGE_REFIT = 4,      /* S4+S5 swept leaves + atomic bottom-up */
+  YSIM_STAGE_QUERY = 5,      /* S6+S7 traversal fused with the narrow phase */
+  YSIM_STAGE_INTEGRATE = 6,  /* S9 springs + integrate + contact projection */
+  YSIM_STAGE_COUNT = 7
+};
+
+typedef struct ysim_ctx ysim_ctx;
+typedef struct ysim_scene ysim_scene;
+typedef struct ysim_batch ysim_batch;
+
+/* ------------------------------------------------------------------ runtime
+ * Replaces MetalGlobalContext / GlobalAutoAllocator
+ * (include/sim/core_types.hpp:83-187, 497-529). */
+/* ctx flags: YSIM_CTX_FORMAL_REFIT publishes bottom-up boxes with acq_rel
+ * device-scope atomics (L2 writeback/invalidate) instead of device-coherent
+ * stores + wait; both give identical results, the default is faster. */
+#define YSIM_CTX_FORMAL_REFIT 1u
+const char* ysim_hip_last_error(void);
+int ysim_hip_version(void);
+int ysim_hip_device_count(int* out);
+int ysim_hip_ctx_create(int device, uint32_t flags, ysim_ctx** out);
+int ysim_hip_ctx_destroy(ysim_ctx* ctx);
+int ysim_hip_sync(ysim_ctx* ctx);
+int ysim_hip_alloc(ysim_ctx* ctx, size_t bytes, void** d_out);
+int ysim_hip_free(ysim_ctx* ctx, void* d_ptr);
+int ysim_hip_upload(ysim_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);
+int ysim_hip_download(ysim_ctx* ctx, void* h_dst, const void* d_src, size_t bytes);
+int ysim_hip_memset(ysim_ctx* ctx, void* d_dst, int value, size_t bytes);
+
+/* ------------------------------------------------------------------ stages
+ * Scene box over packed float3 positions: BVH::build's CPU loop
+ * (include/sim/bvh.hpp:1229-1231).  Writes one ysim_aabb4 with i0 = num_prims. */
+int ysim_hip_scene_box(ysim_ctx* ctx, const float* d_x, uint32_t num_vertices,
+                       uint32_t num_prims, ysim_aabb4* d_box);
+
+/* Morton codes of triangle centroids: fillMortons_Tri (src/metal/bvh.metal:26-63).
+ * bits == 30: d_keys is uint32_t[n]; bits == 63: d_keys is uint64_t[n].
+ * d_idx[i] = i. */
+int ysim_hip_morton(ysim_ctx* ctx, const float* d_x, const uint32_t* d_facets,
+                    uint32_t num_facets, const ysim_aabb4* d_box, uint32_t bits,
+                    void* d_keys, uint32_t* d_idx);
+
+/* Stable LSD radix sort of (key, value) pairs in place: RadixSorter<METAL>::sort
+ * (include/sim/spatial_hash.hpp:48-70, src/metal/radixsort.metal:21-112).
+ * key_bits <= 32: uint32_t keys; key_bits <= 64: uint64_t keys.  Ties keep
+ * the input order. */
+int ysim_hip_sort_pairs(ysim_ctx* ctx, void* d_keys, uint32_t* d_vals, uint32_t n,
+                        uint32_t key_bits);
+
+/* Karras hierarchy over sorted codes: buildTree_Tri (src/metal/bvh.metal:197-258,
+ * 352-393).  Writes child links of the 2n-1 node slots and d_parent
+ * (root parent = -1); leaf slots get child_a = -1, child_b = facet id.
+ * Boxes are left for ysim_hip_lbvh_refit. */
+int ysim_hip_lbvh_hierarchy(ysim_ctx* ctx, const void* d_sorted_keys,
+                            const uint32_t* d_sorted_idx, uint32_t n, uint32_t bits,
+                            ysim_bvh_node* d_tree, int32_t* d_parent);
+
+/* Leaf boxes + atomic bottom-up union: buildLeaf_Tri / buildSweptLeaf_Tri +
+ * bottomUpBoxes (src/metal/bvh.metal:260-351, 464-586).  swept != 0 unions each
+ * vertex with x + v*dt (enlargeTrajectory, bvh.hpp:1608-1682). */
+int ysim_hip_lbvh_refit(ysim_ctx* ctx, const float* d_x, const float* d_v, float dt,
+                        uint32_t swept, const uint32_t* d_facets, uint32_t num_facets,
+                        ysim_bvh_node* d_tree, const int32_t* d_parent);
+
+/* Vertex-vs-triangle broad query of one (query mesh, target tree) pair:
+ * queryPoints/queryAABB (src/metal/bvh.metal:1285-1412), host BVH::queryPoints
+ * (include/sim/bvh.hpp:1803-1841).  Appends to d_out starting at *d_count;
+ * never truncates silently: if the total exceeds capacity, returns
+ * YSIM_ERR_CAPACITY and *h_required receives the needed size. */
+int ysim_hip_broad_query(ysim_ctx* ctx, const float* d_xq, uint32_t num_query,
+                         uint32_t query_obj, uint32_t query_behavior, uint32_t query_shape,
+                         const ysim_bvh_node* d_tree, uint32_t num_target_facets,
+                         const uint32_t* d_target_facets, uint32_t target_obj,
+                         uint32_t target_behavior, uint32_t target_shape,
+                         float margin, ysim_broad_collision* d_out, uint32_t capacity,
+                         uint32_t* d_count, uint32_t* h_required);
+
+/* Swept point-triangle narrow phase: narrow_pt_tri (src/metal/bruteforce.metal:9-178),
+ * host BruteForce::narrow (include/sim/bruteforce.hpp:114-153).  Positions are
+ * the packed scene arrays indexed through d_states_offsets / d_facets_offsets. */
+int ysim_hip_narrow(ysim_ctx* ctx, const ysim_broad_collision* d_pairs, uint32_t num_pairs,
+                    const float* d_x, const float* d_xprev, const uint32_t* d_states_offsets,
+                    const uint32_t* d_facets, const uint32_t* d_facets_offsets,
+                    const uint32_t* d_adj_offsets, const uint32_t* d_adj_facets,
+                    float radius, float thickness,
+                    ysim_narrow_collision* d_out, uint32_t capacity, uint32_t* d_count,
+                    uint32_t* h_required);
+
+/* Per-vertex contact CSR: narrow_bucket_zero/count/scan/scatter
+ * (src/metal/physics.metal:546-616), CPU twin bruteforce.hpp:358-383.  Buckets
+ * are ordered by (target_obj, triangle) so the response is deterministic. */
+int ysim_hip_contact_csr(ysim_ctx* ctx, const ysim_narrow_collision* d_contacts,
+                         uint32_t num_contacts, const uint32_t* d_states_offsets,
+                         uint32_t num_points_total, uint32_t* d_offsets /* V+1 */,
+                         ysim_narrow_collision* d_sorted);
+
+/* FastGrid springs + symplectic integrate + contact projection for one cloth
+ * mesh: compute_cloth_grid_forces_fast + integrate_cloth_grid
+ * (src/metal/physics.metal:146-312).  Reads (d_x, d_v), writes (d_x_out, d_v_out)
+ * (distinct buffers), d_f_out optional (may be NULL).  d_anomaly: uint32 flag set
+ * when the world-bounds guard fires. */
+int ysim_hip_cloth_step(ysim_ctx* ctx, const ysim_fastgrid_params* h_params,
+                        uint32_t num_vertices, float subh, float kair, float kd,
+                        const float* d_x, const float* d_v, const float* d_ext,
+                        const float* d_mass, const float* d_fixed,
+                        const uint32_t* d_csr_offsets, const ysim_narrow_collision* d_csr,
+                        float* d_x_out, float* d_v_out, float* d_f_out,
+                        uint32_t* d_anomaly);
+
+/* ------------------------------------------------------------------ scene
+ * The Simulator::initialize / update substep loop (include/sim/simulator.hpp:
+ * 2942-3115, 3512-3815) over a packed scene resident in HBM. */
+int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_desc* h_meshes,
+                          const float* h_x, const float* h_v, const float* h_mass,
+                          const float* h_ext, const float* h_fixed, const uint32_t* h_facets,
+                          const ysim_sim_params* h_params, ysim_scene** out);
+int ysim_hip_scene_destroy(ysim_scene* scene);
+/* Run n substeps (eager launches, or the captured graph when use_graph != 0). */
+int ysim_hip_scene_step(ysim_scene* scene, uint32_t n, uint32_t use_graph);
+/* Copy current x, v (packed float3) and xPrev to host; any pointer may be NULL. */
+int ysim_hip_scene_get_state(ysim_scene* scene, float* h_x, float* h_v, float* h_xprev);
+int ysim_hip_scene_set_state(ysim_scene* scene, const float* h_x, const float* h_v,
+                             const float* h_xprev);
+/* Contacts of the last substep as a CSR ordered by (vertex, target_obj, triangle). */
+int ysim_hip_scene_get_contacts(ysim_scene* scene, uint32_t* h_offsets /* Vtot+1 */,
+                                ysim_narrow_collision* h_contacts, uint32_t capacity,
+                                uint32_t* h_count);
+/* Broad candidate pairs at the current state (collision pipeline only, no step),
+ * unordered; caller sorts to compare as a set. */
+int ysim_hip_scene_get_pairs(ysim_scene* scene, ysim_broad_collision* h_pairs,
+                             uint32_t capacity, uint32_t* h_count);
+/* Tree of one mesh (2T-1 nodes) and the sorted Morton keys (uint64 widened). */
+int ysim_hip_scene_get_tree(ysim_scene* scene, uint32_t mesh, ysim_bvh_node* h_nodes,
+                            int32_t* h_parent, uint64_t* h_sorted_keys, uint32_t* h_sorted_idx);
+/* Per-stage average device time (ms) over the last n_steps eager substeps run
+ * with timing enabled; out has YSIM_STAGE_COUNT entries. */
+int ysim_hip_scene_stage_timings(ysim_scene* scene, uint32_t n_steps, float* h_ms);
+/* Sticky flags: bit0 anomaly (world-bounds guard), bit1 contact overflow,
+ * bit2 traversal guard. */
+int ysim_hip_scene_flags(ysim_scene* scene, uint32_t* h_flags);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* YSIM_HIP_H */

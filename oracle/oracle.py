@@ -1,0 +1,276 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker.  The product path never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from ysim_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+u32p = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+f32p = C.POINTER(C.c_float)
+f64p = C.POINTER(C.c_double)
+u8p = C.POINTER(C.c_uint8)
+vp = C.c_void_p
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "all"], cwd=HERE, check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "yo_set_threads": (C.c_int, [C.c_int]),
+        "yo_scene_box": (None, [f32p, C.c_uint32, f32p]),
+        "yo_expand10": (C.c_uint32, [C.c_uint32]),
+        "yo_expand21": (C.c_uint64, [C.c_uint64]),
+        "yo_morton": (None, [f32p, u32p, C.c_uint32, f32p, C.c_uint32, u64p, u32p]),
+        "yo_sort_pairs": (None, [u64p, u32p, C.c_uint32, C.c_uint32]),
+        "yo_karras": (None, [u64p, u32p, C.c_uint32, C.c_uint32, vp, C.POINTER(C.c_int32)]),
+        "yo_refit": (None, [f32p, f32p, C.c_float, C.c_uint32, u32p, C.c_uint32, vp]),
+        "yo_broad_query": (C.c_uint32, [f32p, C.c_uint32, C.c_uint32, vp, C.c_uint32, u32p,
+                                        C.c_uint32, C.c_float, u32p, C.c_uint32]),
+        "yo_brute_candidates": (C.c_uint32, [f32p, C.c_uint32, C.c_uint32, f32p, C.c_uint32, u32p,
+                                             C.c_uint32, C.c_float, u32p, C.c_uint32]),
+        "yo_narrow_f32": (None, [u32p, C.c_uint32, f32p, f32p, f32p, u32p, C.c_uint32, C.c_float,
+                                 C.c_float, u8p, f32p]),
+        "yo_narrow_f64": (None, [u32p, C.c_uint32, f64p, f64p, f64p, u32p, C.c_uint32, C.c_double,
+                                 C.c_double, u8p, f64p]),
+        "yo_fastgrid_forces_f32": (None, [vp, C.c_uint32, C.c_float, C.c_float, f32p, f32p, f32p,
+                                          f32p, f32p]),
+        "yo_fastgrid_forces_f64": (None, [vp, C.c_uint32, C.c_double, C.c_double, f64p, f64p, f64p,
+                                          f64p, f64p]),
+        "yo_spring_count": (C.c_uint32, [C.c_uint32]),
+        "yo_integrate_f32": (C.c_uint32, [C.c_uint32, C.c_float, C.c_float, f32p, f32p, f32p, f32p,
+                                          f32p, u32p, vp]),
+        "yo_scene_create": (vp, [C.c_uint32, vp, f32p, f32p, f32p, f32p, f32p, u32p, vp]),
+        "yo_scene_destroy": (None, [vp]),
+        "yo_scene_enable_twin": (None, [vp]),
+        "yo_scene_set_kspring": (None, [vp, f32p, C.c_uint32]),
+        "yo_scene_step": (None, [vp, C.c_uint32]),
+        "yo_scene_pairs": (C.c_uint64, [vp, u32p, C.c_uint64]),
+        "yo_scene_get_state": (None, [vp, f32p, f32p, f32p]),
+        "yo_scene_set_state": (None, [vp, f32p, f32p, f32p]),
+        "yo_scene_get_twin": (None, [vp, f64p, f64p]),
+        "yo_scene_num_contacts": (C.c_uint32, [vp]),
+        "yo_scene_get_contacts": (None, [vp, u32p, vp]),
+        "yo_scene_get_tree": (None, [vp, C.c_uint32, vp, C.POINTER(C.c_int32), u64p, u32p]),
+        "yo_scene_anomaly": (C.c_uint32, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def P(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def set_threads(n: int) -> int:
+    return lib().yo_set_threads(n)
+
+
+# ------------------------------------------------------------------ stages
+def scene_box(x):
+    out = np.zeros(6, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1)
+    lib().yo_scene_box(P(x, f32p), x.size // 3, P(out, f32p))
+    return out
+
+
+def morton(x, facets, box6, bits):
+    x = np.ascontiguousarray(x, np.float32).reshape(-1)
+    facets = np.ascontiguousarray(facets, np.uint32).reshape(-1)
+    n = facets.size // 3
+    keys = np.zeros(n, np.uint64)
+    idx = np.zeros(n, np.uint32)
+    box6 = np.ascontiguousarray(box6, np.float32)
+    lib().yo_morton(P(x, f32p), P(facets, u32p), n, P(box6, f32p), bits, P(keys, u64p), P(idx, u32p))
+    return keys, idx
+
+
+def sort_pairs(keys, vals, key_bits):
+    k = np.ascontiguousarray(keys, np.uint64).copy()
+    v = np.ascontiguousarray(vals, np.uint32).copy()
+    lib().yo_sort_pairs(P(k, u64p), P(v, u32p), k.size, key_bits)
+    return k, v
+
+
+def karras(keys, idx, bits):
+    n = keys.size
+    tree = np.zeros(max(2 * n - 1, 0), abi.BVH_NODE)
+    parent = np.full(max(2 * n - 1, 0), -1, np.int32)
+    k = np.ascontiguousarray(keys, np.uint64)
+    i = np.ascontiguousarray(idx, np.uint32)
+    lib().yo_karras(P(k, u64p), P(i, u32p), n, bits, tree.ctypes.data, P(parent, C.POINTER(C.c_int32)))
+    return tree, parent
+
+
+def refit(tree, x, v, dt, swept, facets):
+    x = np.ascontiguousarray(x, np.float32).reshape(-1)
+    v = np.ascontiguousarray(v, np.float32).reshape(-1)
+    facets = np.ascontiguousarray(facets, np.uint32).reshape(-1)
+    n = facets.size // 3
+    lib().yo_refit(P(x, f32p), P(v, f32p), dt, swept, P(facets, u32p), n, tree.ctypes.data)
+    return tree
+
+
+def broad_query(xq, qobj, tree, tfacets, tobj, margin):
+    xq = np.ascontiguousarray(xq, np.float32).reshape(-1)
+    tf = np.ascontiguousarray(tfacets, np.uint32).reshape(-1)
+    nq = xq.size // 3
+    L = lib()
+    cnt = L.yo_broad_query(P(xq, f32p), nq, qobj, tree.ctypes.data, tf.size // 3, P(tf, u32p), tobj,
+                           margin, None, 0)
+    out = np.zeros(2 * cnt, np.uint32)
+    L.yo_broad_query(P(xq, f32p), nq, qobj, tree.ctypes.data, tf.size // 3, P(tf, u32p), tobj,
+                     margin, P(out, u32p), cnt)
+    return out.reshape(-1, 2)
+
+
+def brute_candidates(xq, qobj, leafbox6, tfacets, tobj, margin):
+    xq = np.ascontiguousarray(xq, np.float32).reshape(-1)
+    lb = np.ascontiguousarray(leafbox6, np.float32).reshape(-1)
+    tf = np.ascontiguousarray(tfacets, np.uint32).reshape(-1)
+    L = lib()
+    cnt = L.yo_brute_candidates(P(xq, f32p), xq.size // 3, qobj, P(lb, f32p), tf.size // 3,
+                                P(tf, u32p), tobj, margin, None, 0)
+    out = np.zeros(2 * cnt, np.uint32)
+    L.yo_brute_candidates(P(xq, f32p), xq.size // 3, qobj, P(lb, f32p), tf.size // 3, P(tf, u32p),
+                          tobj, margin, P(out, u32p), cnt)
+    return out.reshape(-1, 2)
+
+
+def narrow(pairs, xq, xqprev, xt, tfacets, same_obj, radius, thickness, f64=False):
+    pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1)
+    tf = np.ascontiguousarray(tfacets, np.uint32).reshape(-1)
+    n = pairs.size // 2
+    flags = np.zeros(n, np.uint8)
+    if f64:
+        arrs = [np.ascontiguousarray(a, np.float64).reshape(-1) for a in (xq, xqprev, xt)]
+        nd4 = np.zeros(4 * n, np.float64)
+        lib().yo_narrow_f64(P(pairs, u32p), n, P(arrs[0], f64p), P(arrs[1], f64p), P(arrs[2], f64p),
+                            P(tf, u32p), int(same_obj), radius, thickness, P(flags, u8p), P(nd4, f64p))
+    else:
+        arrs = [np.ascontiguousarray(a, np.float32).reshape(-1) for a in (xq, xqprev, xt)]
+        nd4 = np.zeros(4 * n, np.float32)
+        lib().yo_narrow_f32(P(pairs, u32p), n, P(arrs[0], f32p), P(arrs[1], f32p), P(arrs[2], f32p),
+                            P(tf, u32p), int(same_obj), radius, thickness, P(flags, u8p), P(nd4, f32p))
+    return flags.astype(bool), nd4.reshape(-1, 4)
+
+
+def fastgrid_forces(cloth: dict, x, v, ext, kair, kd, kspring=None, f64=False):
+    prm = abi.fastgrid_params(cloth)
+    dt = np.float64 if f64 else np.float32
+    tp = f64p if f64 else f32p
+    x, v, ext = (np.ascontiguousarray(a, dt).reshape(-1) for a in (x, v, ext))
+    ks = None if kspring is None else np.ascontiguousarray(kspring, dt)
+    f = np.zeros_like(x)
+    fn = lib().yo_fastgrid_forces_f64 if f64 else lib().yo_fastgrid_forces_f32
+    fn(C.byref(prm), x.size // 3, kair, kd, P(x, tp), P(v, tp), P(ext, tp), P(ks, tp), P(f, tp))
+    return f
+
+
+def spring_count(n: int) -> int:
+    return lib().yo_spring_count(n)
+
+
+# ------------------------------------------------------------------ scene
+class OracleScene:
+    """CPU twin of the HIP scene: same packed inputs, same substep."""
+
+    def __init__(self, scene, twin: bool = False, kspring=None):
+        self.scene = scene
+        pk = scene.packed()
+        self._keep = pk
+        self.descs = abi.mesh_descs(scene)
+        self.params = abi.sim_params(scene.params)
+        L = lib()
+        self.h = L.yo_scene_create(len(scene.meshes), C.cast(self.descs, vp), P(pk["x"], f32p),
+                                   P(pk["v"], f32p), P(pk["mass"], f32p), P(pk["ext"], f32p),
+                                   P(pk["fixed"], f32p), P(pk["facets"], u32p), C.byref(self.params))
+        if kspring is not None:
+            ks = np.ascontiguousarray(kspring, np.float32)
+            L.yo_scene_set_kspring(self.h, P(ks, f32p), ks.size)
+        if twin:
+            L.yo_scene_enable_twin(self.h)
+        self.V = scene.num_vertices
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().yo_scene_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def step(self, n: int = 1):
+        lib().yo_scene_step(self.h, n)
+
+    def state(self):
+        x = np.zeros(3 * self.V, np.float32)
+        v = np.zeros(3 * self.V, np.float32)
+        xp = np.zeros(3 * self.V, np.float32)
+        lib().yo_scene_get_state(self.h, P(x, f32p), P(v, f32p), P(xp, f32p))
+        return x.reshape(-1, 3), v.reshape(-1, 3), xp.reshape(-1, 3)
+
+    def set_state(self, x=None, v=None, xprev=None):
+        a = [None if t is None else np.ascontiguousarray(t, np.float32).reshape(-1) for t in (x, v, xprev)]
+        lib().yo_scene_set_state(self.h, P(a[0], f32p), P(a[1], f32p), P(a[2], f32p))
+
+    def twin_state(self):
+        x = np.zeros(3 * self.V, np.float64)
+        v = np.zeros(3 * self.V, np.float64)
+        lib().yo_scene_get_twin(self.h, P(x, f64p), P(v, f64p))
+        return x.reshape(-1, 3), v.reshape(-1, 3)
+
+    def contacts(self):
+        L = lib()
+        n = L.yo_scene_num_contacts(self.h)
+        off = np.zeros(self.V + 1, np.uint32)
+        rec = np.zeros(n, abi.NARROW)
+        L.yo_scene_get_contacts(self.h, P(off, u32p), rec.ctypes.data)
+        return off, rec
+
+    def pairs(self):
+        """Candidate set at the current state as an (n, 4) array (qobj, point, tobj, tri)."""
+        L = lib()
+        n = L.yo_scene_pairs(self.h, None, 0)
+        out = np.zeros(4 * n, np.uint32)
+        L.yo_scene_pairs(self.h, P(out, u32p), n)
+        return out.reshape(-1, 4)
+
+    def tree(self, mesh: int):
+        T = self.scene.meshes[mesh].facets.shape[0]
+        nodes = np.zeros(2 * T - 1, abi.BVH_NODE)
+        parent = np.zeros(2 * T - 1, np.int32)
+        keys = np.zeros(T, np.uint64)
+        idx = np.zeros(T, np.uint32)
+        lib().yo_scene_get_tree(self.h, mesh, nodes.ctypes.data, P(parent, C.POINTER(C.c_int32)),
+                                P(keys, u64p), P(idx, u32p))
+        return nodes, parent, keys, idx
+
+    def anomaly(self) -> int:
+        return lib().yo_scene_anomaly(self.h)

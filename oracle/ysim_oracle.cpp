@@ -1,0 +1,1035 @@
+// ysim_oracle.cpp — CPU oracle for the cloth collision + spring substep.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker: it is loaded by tests/,
+// by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, never by the
+// product path (ysim_amd/ and its HIP library must not link or call it).
+//
+// It restates, in strict IEEE arithmetic (built with -ffp-contract=off, no
+// fast-math), the semantics of the reference's Metal kernels and host loops.
+// The reference itself cannot be built here (metal-cpp needs objc/runtime.h,
+// Eigen3 and xcrun metal are absent), so this restatement *is* the reference
+// CPU path.  Each function cites the reference file:line it follows.
+// Templated on the real type: float is the parity path, double is the FP64
+// twin used for error bars and for classifying threshold ties.
+//
+// Parity pins (tests/test_oracle_*.py): Morton vs an independent bit loop
+// (exhaustive per axis), sort vs numpy stable argsort, Karras vs a top-down
+// radix-split construction, refit vs recursive union, candidate set vs brute
+// force O(V*T), narrow vs the FP64 twin (ties reported), forces vs the FP64
+// twin and closed-form free fall, CSR order vs lexsort.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/ysim_hip.h"
+
+namespace {
+
+// ---------------------------------------------------------------- Morton
+// expandBits, src/metal/bvh.metal:26-32 (10 bits -> 30 bits).
+inline uint32_t expand10(uint32_t v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+// 21 bits -> 63 bits (new in this build; the reference has only 30-bit codes).
+inline uint64_t expand21(uint64_t v) {
+  v &= 0x1fffffull;
+  v = (v | (v << 32)) & 0x001f00000000ffffull;
+  v = (v | (v << 16)) & 0x001f0000ff0000ffull;
+  v = (v | (v << 8)) & 0x100f00f00f00f00full;
+  v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+  v = (v | (v << 2)) & 0x1249249249249249ull;
+  return v;
+}
+
+inline float fmin_(float a, float b) { return std::fmin(a, b); }
+inline float fmax_(float a, float b) { return std::fmax(a, b); }
+
+// mortonCode + fillMortons_Tri, src/metal/bvh.metal:34-63.
+inline uint64_t morton_tri(const float* x, const uint32_t* f, const float* bmin,
+                           const float* bmax, uint32_t bits) {
+  uint64_t e[3];
+  for (int k = 0; k < 3; ++k) {
+    float x0 = x[3 * f[0] + k], x1 = x[3 * f[1] + k], x2 = x[3 * f[2] + k];
+    float center = ((x0 + x1) + x2) / 3.0f;
+    float width = fmax_(bmax[k] - bmin[k], 1e-8f);
+    float c = (center - bmin[k]) / width;
+    if (bits == 30) {
+      float p = fmin_(fmax_(c * 1024.0f, 0.0f), 1023.0f);
+      e[k] = expand10((uint32_t)p);
+    } else {
+      float p = fmin_(fmax_(c * 2097152.0f, 0.0f), 2097151.0f);
+      e[k] = expand21((uint64_t)(uint32_t)p);
+    }
+  }
+  if (bits == 30) return e[0] * 4 + e[1] * 2 + e[2];
+  return (e[0] << 2) | (e[1] << 1) | e[2];
+}
+
+inline int clz32(uint32_t v) { return v ? __builtin_clz(v) : 32; }
+inline int clz64(uint64_t v) { return v ? __builtin_clzll(v) : 64; }
+
+// delta / determineRange / findSplit, src/metal/bvh.metal:197-258, with the
+// 64-bit variant (clz64, 64 + clz(idx ^ idx) tie-break) for 63-bit codes.
+struct Karras {
+  const uint64_t* code;
+  const uint32_t* idx;
+  int n;
+  bool wide;
+  int delta(int i, int j) const {
+    if (j < 0 || j >= n) return -1;
+    uint64_t ci = code[i], cj = code[j];
+    if (ci != cj) return wide ? clz64(ci ^ cj) : clz32((uint32_t)(ci ^ cj));
+    return (wide ? 64 : 32) + clz32(idx[i] ^ idx[j]);
+  }
+  void range(int index, int& lo, int& hi) const {
+    int d = (delta(index, index + 1) - delta(index, index - 1) >= 0) ? 1 : -1;
+    int dmin = delta(index, index - d);
+    int lmax = 2;
+    while (delta(index, index + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+      if (delta(index, index + (l + t) * d) > dmin) l += t;
+    int j = index + l * d;
+    if (d < 0) { lo = j; hi = index; } else { lo = index; hi = j; }
+  }
+  uint32_t split(int i0, int i1) const {
+    uint64_t c0 = code[i0], c1 = code[i1];
+    if (c0 == c1) return (uint32_t)((i0 + i1) >> 1);
+    int common = wide ? clz64(c0 ^ c1) : clz32((uint32_t)(c0 ^ c1));
+    uint32_t s = (uint32_t)i0, step = (uint32_t)(i1 - i0);
+    do {
+      step = (step + 1) >> 1;
+      uint32_t ns = s + step;
+      if (ns < (uint32_t)i1) {
+        uint64_t sc = code[ns];
+        int pre = wide ? clz64(c0 ^ sc) : clz32((uint32_t)(c0 ^ sc));
+        if (pre > common) s = ns;
+      }
+    } while (step > 1);
+    return s;
+  }
+};
+
+template <class R>
+struct V3 {
+  R x, y, z;
+};
+template <class R>
+inline V3<R> ld(const R* a, size_t i) { return {a[3 * i], a[3 * i + 1], a[3 * i + 2]}; }
+template <class R>
+inline void st(R* a, size_t i, V3<R> v) { a[3 * i] = v.x; a[3 * i + 1] = v.y; a[3 * i + 2] = v.z; }
+template <class R>
+inline V3<R> operator+(V3<R> a, V3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class R>
+inline V3<R> operator-(V3<R> a, V3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class R>
+inline V3<R> operator*(V3<R> a, R s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class R>
+inline V3<R> operator*(R s, V3<R> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <class R>
+inline R dot(V3<R> a, V3<R> b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+template <class R>
+inline V3<R> cross(V3<R> a, V3<R> b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class R>
+inline R sqrt_(R v) { return std::sqrt(v); }
+
+// calc_spring, src/metal/physics.metal:146-153.
+template <class R>
+inline V3<R> calc_spring(V3<R> p0, V3<R> v0, V3<R> p1, V3<R> v1, R rest, R ks, R kd) {
+  V3<R> dx = p1 - p0;
+  V3<R> dv = v1 - v0;
+  R len = sqrt_(dot(dx, dx));
+  if (len < (R)1e-7) return {0, 0, 0};
+  V3<R> ndx = {dx.x / len, dx.y / len, dx.z / len};
+  R s = ks * (len - rest) + kd * dot(dv, ndx);
+  return s * ndx;
+}
+
+// Spring numbering of a FastGrid cloth with side N (this build's convention for
+// per-spring stiffness; the reference uses one k per spring family):
+//   struct-x (r,c)-(r,c+1): r*(N-1)+c              [N*(N-1)]
+//   struct-y (r,c)-(r+1,c): SY + r*N+c             [(N-1)*N]
+//   shear-a  (r,c)-(r+1,c+1): SA + r*(N-1)+c       [(N-1)^2]
+//   shear-b  (r,c+1)-(r+1,c): SB + r*(N-1)+c       [(N-1)^2]
+//   bend-x   (r,c)-(r,c+2): BX + r*(N-2)+c         [N*(N-2)]
+//   bend-y   (r,c)-(r+2,c): BY + r*N+c             [(N-2)*N]
+struct SpringIndex {
+  uint32_t N, SY, SA, SB, BX, BY, total;
+  explicit SpringIndex(uint32_t n) : N(n) {
+    SY = N * (N - 1);
+    SA = SY + (N - 1) * N;
+    SB = SA + (N - 1) * (N - 1);
+    BX = SB + (N - 1) * (N - 1);
+    BY = BX + N * (N >= 2 ? N - 2 : 0);
+    total = BY + (N >= 2 ? N - 2 : 0) * N;
+  }
+};
+
+// compute_cloth_grid_forces_fast, src/metal/physics.metal:171-224 (gather of 12
+// grid springs in this exact order).  kspring (optional) holds one stiffness per
+// spring in SpringIndex order and replaces the per-family k.
+template <class R>
+void fastgrid_forces(const ysim_fastgrid_params& P, uint32_t nV, R kair, R kd, const R* x,
+                     const R* v, const R* ext, const R* kspring, R* f) {
+  const uint32_t N = P.particle_num_1d;
+  SpringIndex S(N);
+  const R rX = P.stretch_rest_x, rY = P.stretch_rest_y, rA = P.shear_rest_a, rB = P.shear_rest_b,
+          bX = P.bend_rest_x, bY = P.bend_rest_y;
+  const R ks = P.kstretch, kh = P.kshear, kb = P.kbend;
+#pragma omp parallel for schedule(static)
+  for (int64_t id_ = 0; id_ < (int64_t)nV; ++id_) {
+    uint32_t id = (uint32_t)id_;
+    uint32_t row = id / N, col = id % N;
+    V3<R> pos = ld(x, id), vel = ld(v, id);
+    V3<R> force = ld(ext, id) + vel * kair;
+    auto nb = [&](uint32_t r, uint32_t c, R rest, R k, uint32_t sidx) {
+      uint32_t j = r * N + c;
+      R kk = kspring ? kspring[sidx] : k;
+      force = force + calc_spring(pos, vel, ld(x, j), ld(v, j), rest, kk, kd);
+    };
+    if (col > 0) nb(row, col - 1, rX, ks, row * (N - 1) + col - 1);
+    if (col < N - 1) nb(row, col + 1, rX, ks, row * (N - 1) + col);
+    if (row > 0) nb(row - 1, col, rY, ks, S.SY + (row - 1) * N + col);
+    if (row < N - 1) nb(row + 1, col, rY, ks, S.SY + row * N + col);
+    if (col > 0 && row > 0) nb(row - 1, col - 1, rA, kh, S.SA + (row - 1) * (N - 1) + col - 1);
+    if (col < N - 1 && row > 0) nb(row - 1, col + 1, rB, kh, S.SB + (row - 1) * (N - 1) + col);
+    if (col > 0 && row < N - 1) nb(row + 1, col - 1, rB, kh, S.SB + row * (N - 1) + col - 1);
+    if (col < N - 1 && row < N - 1) nb(row + 1, col + 1, rA, kh, S.SA + row * (N - 1) + col);
+    if (col > 1) nb(row, col - 2, bX, kb, S.BX + row * (N - 2) + col - 2);
+    if (col < N - 2) nb(row, col + 2, bX, kb, S.BX + row * (N - 2) + col);
+    if (row > 1) nb(row - 2, col, bY, kb, S.BY + (row - 2) * N + col);
+    if (row < N - 2) nb(row + 2, col, bY, kb, S.BY + row * N + col);
+    st(f, id, force);
+  }
+}
+
+// nonFinite3 / sanitizeIntegrateOutput, src/metal/physics.metal:40-74.
+template <class R>
+inline bool nonfinite(V3<R> p) {
+  return !std::isfinite(p.x) || !std::isfinite(p.y) || !std::isfinite(p.z);
+}
+template <class R>
+inline bool sanitize(V3<R>& pos, V3<R>& vel, V3<R> fb) {
+  const R B = (R)1.0e4f;
+  bool touched = false;
+  if (nonfinite(pos)) {
+    pos = nonfinite(fb) ? V3<R>{0, 0, 0} : fb;
+    vel = {0, 0, 0};
+    touched = true;
+  }
+  if (nonfinite(vel)) {
+    vel = {0, 0, 0};
+    touched = true;
+  }
+  V3<R> bx = {std::fmin(std::fmax(pos.x, -B), B), std::fmin(std::fmax(pos.y, -B), B),
+              std::fmin(std::fmax(pos.z, -B), B)};
+  if (bx.x != pos.x || bx.y != pos.y || bx.z != pos.z) {
+    pos = bx;
+    vel = {0, 0, 0};
+    touched = true;
+  }
+  return touched;
+}
+
+struct Contact {  // one narrow contact, bucketed per global query vertex
+  uint32_t gv;    // statesOffsets[q] + point
+  uint32_t tobj, tri, qobj, point;
+  float n[3], d;
+  uint32_t qbeh, tbeh, qshape, tshape;
+};
+
+template <class R>
+struct ContactR {
+  uint32_t tobj, tri;
+  R n[3], d;
+};
+
+// integrate_cloth_grid, src/metal/physics.metal:226-312 (one vertex).  cs is the
+// vertex's contact bucket in (target_obj, triangle) order.
+template <class R>
+inline bool integrate_vertex(V3<R>& pos, V3<R>& vel, V3<R> force, R m, R mask, R subh,
+                             R thickness, const ContactR<R>* cs, uint32_t nc) {
+  V3<R> xold = pos;
+  vel.x = vel.x + ((subh * force.x) / m) * mask;
+  vel.y = vel.y + ((subh * force.y) / m) * mask;
+  vel.z = vel.z + ((subh * force.z) / m) * mask;
+  for (uint32_t i = 0; i < nc; ++i) {
+    bool dup = false;
+    for (uint32_t j = 0; j < i; ++j)
+      if (cs[j].tobj == cs[i].tobj && cs[j].tri == cs[i].tri) { dup = true; break; }
+    if (dup) continue;
+    V3<R> n = {cs[i].n[0], cs[i].n[1], cs[i].n[2]};
+    R nlen2 = dot(n, n);
+    if (nlen2 < (R)1e-12f) continue;
+    R inv = (R)1 / sqrt_(nlen2);
+    n = n * inv;
+    R vn = dot(vel, n);
+    R dist = cs[i].d;
+    if (dist < thickness) {
+      if (vn < (R)0) vel = vel - vn * n;
+      pos = pos + (thickness - dist) * n;
+    }
+  }
+  pos = pos + (subh * vel) * mask;
+  return sanitize(pos, vel, xold);
+}
+
+// pointInTriangleBary, src/metal/bruteforce.metal:9-28.
+template <class R>
+inline bool point_in_tri(V3<R> p, V3<R> v0, V3<R> v1) {
+  R d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1), d20 = dot(p, v0), d21 = dot(p, v1);
+  R denom = d00 * d11 - d01 * d01;
+  if (std::fabs(denom) < (R)1e-12f) return false;
+  R b = (d11 * d20 - d01 * d21) / denom;
+  R c = (d00 * d21 - d01 * d20) / denom;
+  R a = (R)1 - b - c;
+  return a >= (R)0 && b >= (R)0 && c >= (R)0;
+}
+
+// narrow_pt_tri geometry, src/metal/bruteforce.metal:114-158.  Returns true on a
+// contact, with the oriented unit normal and the signed current distance.
+template <class R>
+inline bool narrow_test(V3<R> qcur, V3<R> qprev, V3<R> t0, V3<R> t1, V3<R> t2, R band,
+                        V3<R>& n_out, R& d_out) {
+  V3<R> v0 = t1 - t0, v1 = t2 - t0;
+  V3<R> n = cross(v0, v1);
+  R nlen2 = dot(n, n);
+  if (nlen2 < (R)1e-12f) return false;
+  R inv = (R)1 / sqrt_(nlen2);
+  n = n * inv;
+  R dprev = dot(n, qprev - t0);
+  R dcur = dot(n, qcur - t0);
+  R dref = (dprev != (R)0) ? dprev : dcur;
+  if (dref < (R)0) {
+    n = {-n.x, -n.y, -n.z};
+    dprev = -dprev;
+    dcur = -dcur;
+  }
+  bool crossed = dcur < (R)0;
+  bool inmargin = dcur < band;
+  if (!crossed && !inmargin) return false;
+  V3<R> cp;
+  if (crossed && dprev > (R)0) {
+    R t = dprev / (dprev - dcur);
+    cp = qprev + (qcur - qprev) * t;
+  } else {
+    cp = qcur;
+  }
+  V3<R> pc = cp - t0;
+  R l = dot(n, pc);
+  V3<R> inplane = pc - n * l;
+  if (!point_in_tri(inplane, v0, v1)) return false;
+  n_out = n;
+  d_out = dcur;
+  return true;
+}
+
+inline bool box_overlap(const float* qmin, const float* qmax, const ysim_bvh_node& nd) {
+  // intersectAABB, src/metal/bvh.metal:1285-1290 (closed intervals).
+  if (qmax[0] < nd.min[0] || qmin[0] > nd.max[0]) return false;
+  if (qmax[1] < nd.min[1] || qmin[1] > nd.max[1]) return false;
+  if (qmax[2] < nd.min[2] || qmin[2] > nd.max[2]) return false;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ------------------------------------------------------------------ threads
+int yo_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
+// ------------------------------------------------------------------ S0 box
+// BVH::build scene box, include/sim/bvh.hpp:1229-1231.
+void yo_scene_box(const float* x, uint32_t nV, float* out6) {
+  float mn[3] = {x[0], x[1], x[2]}, mx[3] = {x[0], x[1], x[2]};
+  for (uint32_t i = 1; i < nV; ++i)
+    for (int k = 0; k < 3; ++k) {
+      mn[k] = fmin_(x[3 * i + k], mn[k]);
+      mx[k] = fmax_(x[3 * i + k], mx[k]);
+    }
+  for (int k = 0; k < 3; ++k) { out6[k] = mn[k]; out6[3 + k] = mx[k]; }
+}
+
+// ------------------------------------------------------------------ S1 Morton
+uint32_t yo_expand10(uint32_t v) { return expand10(v); }
+uint64_t yo_expand21(uint64_t v) { return expand21(v); }
+
+void yo_morton(const float* x, const uint32_t* facets, uint32_t nT, const float* box6,
+               uint32_t bits, uint64_t* keys, uint32_t* idx) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)nT; ++i) {
+    keys[i] = morton_tri(x, facets + 3 * i, box6, box6 + 3, bits);
+    idx[i] = (uint32_t)i;
+  }
+}
+
+// ------------------------------------------------------------------ S2 sort
+// Stable LSD radix sort with 8-bit digits, count -> scan -> scatter per pass,
+// the algorithm of RadixSorter<METAL>::sort (include/sim/spatial_hash.hpp:48-70,
+// src/metal/radixsort.metal:21-112), widened to key_bits.
+void yo_sort_pairs(uint64_t* keys, uint32_t* vals, uint32_t n, uint32_t key_bits) {
+  std::vector<uint64_t> k2(n);
+  std::vector<uint32_t> v2(n);
+  uint64_t* ks = keys;
+  uint32_t* vs = vals;
+  uint64_t* kd = k2.data();
+  uint32_t* vd = v2.data();
+  uint32_t passes = (key_bits + 7) / 8;
+  for (uint32_t p = 0; p < passes; ++p) {
+    uint32_t shift = 8 * p;
+    uint32_t hist[257] = {0};
+    for (uint32_t i = 0; i < n; ++i) hist[((ks[i] >> shift) & 255u) + 1]++;
+    for (int b = 1; b < 257; ++b) hist[b] += hist[b - 1];
+    for (uint32_t i = 0; i < n; ++i) {
+      uint32_t b = (uint32_t)((ks[i] >> shift) & 255u);
+      uint32_t dst = hist[b]++;
+      kd[dst] = ks[i];
+      vd[dst] = vs[i];
+    }
+    std::swap(ks, kd);
+    std::swap(vs, vd);
+  }
+  if (ks != keys) {
+    std::memcpy(keys, ks, sizeof(uint64_t) * n);
+    std::memcpy(vals, vs, sizeof(uint32_t) * n);
+  }
+}
+
+// ------------------------------------------------------------------ S3 Karras
+// buildTree_Tri, src/metal/bvh.metal:352-393 (child links + parent), with
+// leaves at slot n-1+i (child_a = -1, child_b = facet id) and root parent -1.
+void yo_karras(const uint64_t* keys, const uint32_t* idx, uint32_t n, uint32_t bits,
+               ysim_bvh_node* tree, int32_t* parent) {
+  if (n == 0) return;
+  Karras K{keys, idx, (int)n, bits > 32};
+  for (uint32_t i = 0; i < n; ++i) {
+    ysim_bvh_node& lf = tree[n - 1 + i];
+    lf.child_a = -1;
+    lf.child_b = (int32_t)idx[i];
+  }
+  parent[0] = -1;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n - 1; ++i) {
+    int lo, hi;
+    K.range((int)i, lo, hi);
+    uint32_t s = K.split(lo, hi);
+    int a = ((int)s == lo) ? (int)(s + n - 1) : (int)s;
+    int b = ((int)s + 1 == hi) ? (int)(s + n) : (int)(s + 1);
+    tree[i].child_a = a;
+    tree[i].child_b = b;
+    parent[a] = (int32_t)i;
+    parent[b] = (int32_t)i;
+  }
+}
+
+// ------------------------------------------------------------------ S4/S5 refit
+// Leaf boxes (buildLeaf_Tri / buildSweptLeaf_Tri, src/metal/bvh.metal:260-351;
+// CPU enlarge bvh.hpp:1669-1680: s = x + v*dt with separate rounding) then exact
+// union of the children for every internal node (bottomUpCombine,
+// include/sim/bvh.hpp:1534-1553), in post-order.
+void yo_refit(const float* x, const float* v, float dt, uint32_t swept, const uint32_t* facets,
+              uint32_t n, ysim_bvh_node* tree) {
+  if (n == 0) return;
+  for (uint32_t i = 0; i < n; ++i) {
+    ysim_bvh_node& lf = tree[n - 1 + i];
+    const uint32_t* f = facets + 3 * (uint32_t)lf.child_b;
+    float lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) { lo[k] = x[3 * f[0] + k]; hi[k] = lo[k]; }
+    for (int c = 0; c < 3; ++c)
+      for (int k = 0; k < 3; ++k) {
+        float p = x[3 * f[c] + k];
+        lo[k] = fmin_(lo[k], p);
+        hi[k] = fmax_(hi[k], p);
+        if (swept) {
+          float s = p + v[3 * f[c] + k] * dt;
+          lo[k] = fmin_(lo[k], s);
+          hi[k] = fmax_(hi[k], s);
+        }
+      }
+    for (int k = 0; k < 3; ++k) { lf.min[k] = lo[k]; lf.max[k] = hi[k]; }
+  }
+  if (n == 1) return;
+  // iterative post-order from the root
+  std::vector<std::pair<int, int>> st;
+  st.push_back({0, 0});
+  while (!st.empty()) {
+    auto& top = st.back();
+    int nd = top.first;
+    if (tree[nd].child_a < 0) { st.pop_back(); continue; }
+    if (top.second == 0) { top.second = 1; st.push_back({tree[nd].child_a, 0}); continue; }
+    if (top.second == 1) { top.second = 2; st.push_back({tree[nd].child_b, 0}); continue; }
+    const ysim_bvh_node& A = tree[tree[nd].child_a];
+    const ysim_bvh_node& B = tree[tree[nd].child_b];
+    for (int k = 0; k < 3; ++k) {
+      tree[nd].min[k] = fmin_(A.min[k], B.min[k]);
+      tree[nd].max[k] = fmax_(A.max[k], B.max[k]);
+    }
+    st.pop_back();
+  }
+}
+
+// ------------------------------------------------------------------ S6 broad
+// queryPoints/queryAABB, src/metal/bvh.metal:1322-1412: for each query vertex p,
+// every leaf whose box overlaps [x_p - m, x_p + m] (closed), excluding facets
+// that contain p when the query mesh is the target mesh.  Unbounded stack; the
+// output order is DFS order, compared as a set.
+uint32_t yo_broad_query(const float* xq, uint32_t nq, uint32_t qobj, const ysim_bvh_node* tree,
+                        uint32_t nT, const uint32_t* tfacets, uint32_t tobj, float margin,
+                        uint32_t* out_pairs /* 2 per pair, may be NULL */, uint32_t cap) {
+  uint32_t count = 0;
+  std::vector<int> stack;
+  for (uint32_t p = 0; p < nq; ++p) {
+    float qmin[3], qmax[3];
+    for (int k = 0; k < 3; ++k) { qmin[k] = xq[3 * p + k] - margin; qmax[k] = xq[3 * p + k] + margin; }
+    stack.clear();
+    stack.push_back(0);
+    while (!stack.empty()) {
+      int nd = stack.back();
+      stack.pop_back();
+      const ysim_bvh_node& N = tree[nd];
+      if (!box_overlap(qmin, qmax, N)) continue;
+      if (N.child_a < 0) {
+        uint32_t fid = (uint32_t)N.child_b;
+        const uint32_t* f = tfacets + 3 * fid;
+        if (qobj == tobj && (p == f[0] || p == f[1] || p == f[2])) continue;
+        if (out_pairs && count < cap) { out_pairs[2 * count] = p; out_pairs[2 * count + 1] = fid; }
+        ++count;
+        continue;
+      }
+      stack.push_back(N.child_b);
+      stack.push_back(N.child_a);
+    }
+  }
+  (void)nT;
+  return count;
+}
+
+// Brute force O(nq * nT) over explicit leaf boxes (leaf boxes given per facet
+// id as 6 floats).  Independent of topology: pins the traversal.
+uint32_t yo_brute_candidates(const float* xq, uint32_t nq, uint32_t qobj, const float* leafbox6,
+                             uint32_t nT, const uint32_t* tfacets, uint32_t tobj, float margin,
+                             uint32_t* out_pairs, uint32_t cap) {
+  std::vector<uint32_t> counts(nq, 0);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t p_ = 0; p_ < (int64_t)nq; ++p_) {
+    uint32_t p = (uint32_t)p_;
+    float qmin[3], qmax[3];
+    for (int k = 0; k < 3; ++k) { qmin[k] = xq[3 * p + k] - margin; qmax[k] = xq[3 * p + k] + margin; }
+    uint32_t c = 0;
+    for (uint32_t t = 0; t < nT; ++t) {
+      ysim_bvh_node nd;
+      for (int k = 0; k < 3; ++k) { nd.min[k] = leafbox6[6 * t + k]; nd.max[k] = leafbox6[6 * t + 3 + k]; }
+      if (!box_overlap(qmin, qmax, nd)) continue;
+      const uint32_t* f = tfacets + 3 * t;
+      if (qobj == tobj && (p == f[0] || p == f[1] || p == f[2])) continue;
+      ++c;
+    }
+    counts[p] = c;
+  }
+  uint32_t total = 0;
+  std::vector<uint32_t> off(nq);
+  for (uint32_t p = 0; p < nq; ++p) { off[p] = total; total += counts[p]; }
+  if (out_pairs && total <= cap) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t p_ = 0; p_ < (int64_t)nq; ++p_) {
+      uint32_t p = (uint32_t)p_;
+      float qmin[3], qmax[3];
+      for (int k = 0; k < 3; ++k) { qmin[k] = xq[3 * p + k] - margin; qmax[k] = xq[3 * p + k] + margin; }
+      uint32_t o = off[p];
+      for (uint32_t t = 0; t < nT; ++t) {
+        ysim_bvh_node nd;
+        for (int k = 0; k < 3; ++k) { nd.min[k] = leafbox6[6 * t + k]; nd.max[k] = leafbox6[6 * t + 3 + k]; }
+        if (!box_overlap(qmin, qmax, nd)) continue;
+        const uint32_t* f = tfacets + 3 * t;
+        if (qobj == tobj && (p == f[0] || p == f[1] || p == f[2])) continue;
+        out_pairs[2 * o] = p;
+        out_pairs[2 * o + 1] = t;
+        ++o;
+      }
+    }
+  }
+  return total;
+}
+
+// ------------------------------------------------------------------ S7 narrow
+// narrow_pt_tri, src/metal/bruteforce.metal:41-178, for a list of pairs
+// (point, tri) of one (query mesh, target mesh).  x/xprev of the query mesh and
+// x of the target mesh are mesh-local packed arrays.  flags[i] = 1 if the pair
+// fires.  Writes normal (3) + distance (1) per pair.  Self pairs with tri
+// containing the point are skipped (the adjacency CSR check).
+void yo_narrow_f32(const uint32_t* pairs, uint32_t np, const float* xq, const float* xqprev,
+                   const float* xt, const uint32_t* tfacets, uint32_t same_obj, float radius,
+                   float thickness, uint8_t* flags, float* nd4) {
+  const float band = radius + thickness;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)np; ++i) {
+    uint32_t p = pairs[2 * i], t = pairs[2 * i + 1];
+    const uint32_t* f = tfacets + 3 * t;
+    flags[i] = 0;
+    if (same_obj && (p == f[0] || p == f[1] || p == f[2])) continue;
+    V3<float> n;
+    float d;
+    if (narrow_test(ld(xq, p), ld(xqprev, p), ld(xt, f[0]), ld(xt, f[1]), ld(xt, f[2]), band, n, d)) {
+      flags[i] = 1;
+      nd4[4 * i] = n.x; nd4[4 * i + 1] = n.y; nd4[4 * i + 2] = n.z; nd4[4 * i + 3] = d;
+    }
+  }
+}
+void yo_narrow_f64(const uint32_t* pairs, uint32_t np, const double* xq, const double* xqprev,
+                   const double* xt, const uint32_t* tfacets, uint32_t same_obj, double radius,
+                   double thickness, uint8_t* flags, double* nd4) {
+  const double band = radius + thickness;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)np; ++i) {
+    uint32_t p = pairs[2 * i], t = pairs[2 * i + 1];
+    const uint32_t* f = tfacets + 3 * t;
+    flags[i] = 0;
+    if (same_obj && (p == f[0] || p == f[1] || p == f[2])) continue;
+    V3<double> n;
+    double d;
+    if (narrow_test(ld(xq, p), ld(xqprev, p), ld(xt, f[0]), ld(xt, f[1]), ld(xt, f[2]), band, n, d)) {
+      flags[i] = 1;
+      nd4[4 * i] = n.x; nd4[4 * i + 1] = n.y; nd4[4 * i + 2] = n.z; nd4[4 * i + 3] = d;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ S9 springs
+void yo_fastgrid_forces_f32(const ysim_fastgrid_params* P, uint32_t nV, float kair, float kd,
+                            const float* x, const float* v, const float* ext, const float* kspring,
+                            float* f) {
+  fastgrid_forces<float>(*P, nV, kair, kd, x, v, ext, kspring, f);
+}
+void yo_fastgrid_forces_f64(const ysim_fastgrid_params* P, uint32_t nV, double kair, double kd,
+                            const double* x, const double* v, const double* ext,
+                            const double* kspring, double* f) {
+  fastgrid_forces<double>(*P, nV, kair, kd, x, v, ext, kspring, f);
+}
+uint32_t yo_spring_count(uint32_t N) { return SpringIndex(N).total; }
+
+// Integrate one FastGrid mesh in place given forces and a per-vertex contact
+// CSR (offsets nV+1, records ordered by (target_obj, triangle) per vertex).
+// Returns the number of vertices the world-bounds guard touched.
+uint32_t yo_integrate_f32(uint32_t nV, float subh, float thickness, float* x, float* v,
+                          const float* f, const float* mass, const float* fixed,
+                          const uint32_t* csr_off, const ysim_narrow_collision* csr) {
+  uint32_t touched = 0;
+  std::vector<ContactR<float>> cs;
+  for (uint32_t i = 0; i < nV; ++i) {
+    V3<float> pos = ld(x, i), vel = ld(v, i);
+    uint32_t b = csr_off ? csr_off[i] : 0, e = csr_off ? csr_off[i + 1] : 0;
+    cs.resize(e - b);
+    for (uint32_t k = b; k < e; ++k)
+      cs[k - b] = {csr[k].target_obj, csr[k].triangle,
+                   {csr[k].normal_distance[0], csr[k].normal_distance[1], csr[k].normal_distance[2]},
+                   csr[k].normal_distance[3]};
+    touched += integrate_vertex<float>(pos, vel, ld(f, i), mass[i], fixed[i], subh, thickness,
+                                       cs.data(), e - b);
+    st(x, i, pos);
+    st(v, i, vel);
+  }
+  return touched;
+}
+uint32_t yo_integrate_f64(uint32_t nV, double subh, double thickness, double* x, double* v,
+                          const double* f, const double* mass, const double* fixed,
+                          const uint32_t* csr_off, const uint32_t* csr_tobj,
+                          const uint32_t* csr_tri, const double* csr_nd4) {
+  uint32_t touched = 0;
+  std::vector<ContactR<double>> cs;
+  for (uint32_t i = 0; i < nV; ++i) {
+    V3<double> pos = ld(x, i), vel = ld(v, i);
+    uint32_t b = csr_off ? csr_off[i] : 0, e = csr_off ? csr_off[i + 1] : 0;
+    cs.resize(e - b);
+    for (uint32_t k = b; k < e; ++k)
+      cs[k - b] = {csr_tobj[k], csr_tri[k],
+                   {csr_nd4[4 * k], csr_nd4[4 * k + 1], csr_nd4[4 * k + 2]}, csr_nd4[4 * k + 3]};
+    touched += integrate_vertex<double>(pos, vel, ld(f, i), mass[i], fixed[i], subh, thickness,
+                                        cs.data(), e - b);
+    st(x, i, pos);
+    st(v, i, vel);
+  }
+  return touched;
+}
+
+}  // extern "C"
+
+// ====================================================================== scene
+// The substep of Simulator::update (include/sim/simulator.hpp:3512-3815) over a
+// packed scene: [build] -> refit(+enlarge) -> broad -> narrow -> CSR ->
+// xPrev := x -> forces -> integrate, restricted to FastGrid cloth (dynamic) and
+// Float (static) meshes.  Per-mesh LBVHs as in BVH<…,SCENE,OBJECT>::objTrees
+// (include/sim/bvh.hpp:2537-2592).  Static trees are built once.
+namespace {
+
+struct OMesh {
+  ysim_mesh_desc d;
+  uint32_t voff, foff;
+  std::vector<uint64_t> keys;
+  std::vector<uint32_t> idx;
+  std::vector<ysim_bvh_node> tree;
+  std::vector<int32_t> parent;
+  bool built = false;
+};
+
+struct OScene {
+  ysim_sim_params P;
+  std::vector<OMesh> meshes;
+  std::vector<float> x, xprev, v, f, mass, ext, fixed;
+  std::vector<uint32_t> facets;
+  uint32_t Vtot = 0, Ttot = 0;
+  uint64_t step = 0;
+  uint32_t anomaly = 0;
+  // last substep contacts as CSR
+  std::vector<uint32_t> csr_off;
+  std::vector<ysim_narrow_collision> csr;
+  // double twin of the dynamics (same contacts)
+  bool twin = false;
+  std::vector<double> xd, vd, xprevd, fd;
+  std::vector<double> twin_nd4;  // FP64 contact normals/distances, same records as csr
+  std::vector<double> kspring_d;
+  std::vector<float> kspring;    // optional per-spring k for mesh 0 (adjoint configs)
+  double timing[YSIM_STAGE_COUNT] = {0};
+};
+
+bool is_dynamic(const ysim_mesh_desc& d) { return d.behavior == YSIM_BEHAVIOR_FASTGRID_CLOTH; }
+
+void build_tree(OScene& S, OMesh& M) {
+  uint32_t n = M.d.num_facets;
+  M.keys.resize(n);
+  M.idx.resize(n);
+  M.tree.assign(n ? 2 * n - 1 : 0, ysim_bvh_node{});
+  M.parent.assign(n ? 2 * n - 1 : 0, -1);
+  if (!n) return;
+  const float* x = S.x.data() + 3 * M.voff;
+  const uint32_t* f = S.facets.data() + 3 * M.foff;
+  float box[6];
+  yo_scene_box(x, M.d.num_vertices, box);
+  yo_morton(x, f, n, box, S.P.morton_bits, M.keys.data(), M.idx.data());
+  yo_sort_pairs(M.keys.data(), M.idx.data(), n, S.P.morton_bits == 30 ? 30 : 63);
+  yo_karras(M.keys.data(), M.idx.data(), n, S.P.morton_bits, M.tree.data(), M.parent.data());
+  M.built = true;
+}
+
+}  // namespace
+
+extern "C" {
+
+void* yo_scene_create(uint32_t nm, const ysim_mesh_desc* meshes, const float* x, const float* v,
+                      const float* mass, const float* ext, const float* fixed,
+                      const uint32_t* facets, const ysim_sim_params* P) {
+  OScene* S = new OScene();
+  S->P = *P;
+  uint32_t vo = 0, fo = 0;
+  for (uint32_t m = 0; m < nm; ++m) {
+    OMesh M;
+    M.d = meshes[m];
+    M.voff = vo;
+    M.foff = fo;
+    vo += M.d.num_vertices;
+    fo += M.d.num_facets;
+    S->meshes.push_back(M);
+  }
+  S->Vtot = vo;
+  S->Ttot = fo;
+  S->x.assign(x, x + 3 * vo);
+  S->xprev = S->x;  // scene.hpp:1048
+  S->v.assign(v, v + 3 * vo);
+  S->f.assign(3 * vo, 0.f);
+  S->mass.assign(mass, mass + vo);
+  S->ext.assign(ext, ext + 3 * vo);
+  S->fixed.assign(fixed, fixed + vo);
+  S->facets.assign(facets, facets + 3 * fo);
+  for (auto& M : S->meshes) {
+    build_tree(*S, M);
+    yo_refit(S->x.data() + 3 * M.voff, S->v.data() + 3 * M.voff, 0.f, 0,
+             S->facets.data() + 3 * M.foff, M.d.num_facets, M.tree.data());
+  }
+  S->csr_off.assign(vo + 1, 0);
+  return S;
+}
+
+void yo_scene_destroy(void* s) { delete (OScene*)s; }
+
+void yo_scene_enable_twin(void* s) {
+  OScene* S = (OScene*)s;
+  S->twin = true;
+  S->xd.assign(S->x.begin(), S->x.end());
+  S->vd.assign(S->v.begin(), S->v.end());
+  S->xprevd.assign(S->xprev.begin(), S->xprev.end());
+  S->fd.assign(S->f.size(), 0.0);
+  S->kspring_d.assign(S->kspring.begin(), S->kspring.end());
+}
+
+// Per-spring stiffness for mesh `mesh` (SpringIndex order); NULL clears it.
+void yo_scene_set_kspring(void* s, const float* k, uint32_t n) {
+  OScene* S = (OScene*)s;
+  if (!k) { S->kspring.clear(); S->kspring_d.clear(); return; }
+  S->kspring.assign(k, k + n);
+  S->kspring_d.assign(k, k + n);
+}
+
+// Collision pipeline at the current state: returns the per-vertex contact CSR
+// (into S->csr_off / S->csr) and, optionally, the broad pair count.
+static void collide(OScene& S, uint64_t* npairs_out, std::vector<uint32_t>* pairs_out) {
+  const ysim_sim_params& P = S.P;
+  bool rebuild = (P.rebuild_every > 0) && (S.step % P.rebuild_every == 0);
+  for (auto& M : S.meshes) {
+    if (!is_dynamic(M.d)) continue;
+    if (rebuild || !M.built) build_tree(S, M);
+    yo_refit(S.x.data() + 3 * M.voff, S.v.data() + 3 * M.voff, P.subh, P.swept,
+             S.facets.data() + 3 * M.foff, M.d.num_facets, M.tree.data());
+  }
+  std::vector<Contact> contacts;
+  uint64_t npairs = 0;
+  for (uint32_t q = 0; q < S.meshes.size(); ++q) {
+    OMesh& Q = S.meshes[q];
+    if (!is_dynamic(Q.d) || !Q.d.collidable) continue;
+    // margin <= 0 selects a per-query-mesh margin and band equal to the cloth's
+    // thickness (batched instances of different resolutions).
+    const float margin = P.margin > 0.f ? P.margin : Q.d.cloth.thickness;
+    const float band = P.radius + (P.margin > 0.f ? P.narrow_thickness : Q.d.cloth.thickness);
+    for (uint32_t t = 0; t < S.meshes.size(); ++t) {
+      OMesh& T = S.meshes[t];
+      if (!T.d.collidable || T.d.group != Q.d.group || T.d.num_facets == 0) continue;
+      if (q == t && !P.self_collision) continue;
+      const float* xq = S.x.data() + 3 * Q.voff;
+      const float* xqp = S.xprev.data() + 3 * Q.voff;
+      const float* xt = S.x.data() + 3 * T.voff;
+      const uint32_t* tf = S.facets.data() + 3 * T.foff;
+      uint32_t nq = Q.d.num_vertices;
+      std::vector<std::vector<Contact>> per(nq);
+      std::vector<uint64_t> pc(nq, 0);
+      std::vector<std::vector<uint32_t>> pp(pairs_out ? nq : 0);
+#pragma omp parallel for schedule(dynamic, 256)
+      for (int64_t p_ = 0; p_ < (int64_t)nq; ++p_) {
+        uint32_t p = (uint32_t)p_;
+        float qmin[3], qmax[3];
+        for (int k = 0; k < 3; ++k) { qmin[k] = xq[3 * p + k] - margin; qmax[k] = xq[3 * p + k] + margin; }
+        std::vector<int> stack;
+        stack.reserve(64);
+        stack.push_back(0);
+        while (!stack.empty()) {
+          int nd = stack.back();
+          stack.pop_back();
+          const ysim_bvh_node& N = T.tree[nd];
+          if (!box_overlap(qmin, qmax, N)) continue;
+          if (N.child_a < 0) {
+            uint32_t fid = (uint32_t)N.child_b;
+            const uint32_t* f = tf + 3 * fid;
+            if (q == t && (p == f[0] || p == f[1] || p == f[2])) continue;
+            pc[p]++;
+            if (pairs_out) { pp[p].push_back(p); pp[p].push_back(fid); }
+            V3<float> n;
+            float d;
+            if (narrow_test(ld(xq, p), ld(xqp, p), ld(xt, f[0]), ld(xt, f[1]), ld(xt, f[2]), band, n, d)) {
+              Contact c;
+              c.gv = Q.voff + p; c.tobj = t; c.tri = fid; c.qobj = q; c.point = p;
+              c.n[0] = n.x; c.n[1] = n.y; c.n[2] = n.z; c.d = d;
+              c.qbeh = Q.d.behavior; c.tbeh = T.d.behavior; c.qshape = Q.d.shape; c.tshape = T.d.shape;
+              per[p].push_back(c);
+            }
+            continue;
+          }
+          stack.push_back(N.child_b);
+          stack.push_back(N.child_a);
+        }
+      }
+      for (uint32_t p = 0; p < nq; ++p) {
+        npairs += pc[p];
+        for (auto& c : per[p]) contacts.push_back(c);
+        if (pairs_out) {
+          for (size_t k = 0; k < pp[p].size(); k += 2) {
+            pairs_out->push_back(q); pairs_out->push_back(pp[p][k]);
+            pairs_out->push_back(t); pairs_out->push_back(pp[p][k + 1]);
+          }
+        }
+      }
+    }
+  }
+  // CSR ordered by (vertex, target_obj, triangle): bruteforce.hpp:358-383 with
+  // the deterministic in-bucket order of this build.
+  std::sort(contacts.begin(), contacts.end(), [](const Contact& a, const Contact& b) {
+    if (a.gv != b.gv) return a.gv < b.gv;
+    if (a.tobj != b.tobj) return a.tobj < b.tobj;
+    return a.tri < b.tri;
+  });
+  S.csr_off.assign(S.Vtot + 1, 0);
+  S.csr.resize(contacts.size());
+  for (size_t i = 0; i < contacts.size(); ++i) {
+    const Contact& c = contacts[i];
+    S.csr_off[c.gv + 1]++;
+    ysim_narrow_collision& r = S.csr[i];
+    r.point = c.point; r.triangle = c.tri; r.query_obj = c.qobj; r.target_obj = c.tobj;
+    r.normal_distance[0] = c.n[0]; r.normal_distance[1] = c.n[1]; r.normal_distance[2] = c.n[2];
+    r.normal_distance[3] = c.d;
+    r.query_behavior = c.qbeh; r.target_behavior = c.tbeh; r.query_shape = c.qshape; r.target_shape = c.tshape;
+  }
+  for (uint32_t i = 0; i < S.Vtot; ++i) S.csr_off[i + 1] += S.csr_off[i];
+  if (npairs_out) *npairs_out = npairs;
+}
+
+// FP64 twin of the contact geometry: recompute each contact record of the FP32
+// CSR with double positions (same pair set), for error bars only.
+static void twin_contacts(OScene& S) {
+  S.twin_nd4.assign(4 * S.csr.size(), 0.0);
+  for (size_t i = 0; i < S.csr.size(); ++i) {
+    const ysim_narrow_collision& r = S.csr[i];
+    const double band = 1.0;  // the FP32 decision stands; FP64 only re-evaluates the geometry
+    const OMesh& Q = S.meshes[r.query_obj];
+    const OMesh& T = S.meshes[r.target_obj];
+    const uint32_t* f = S.facets.data() + 3 * (T.foff + r.triangle);
+    V3<double> n;
+    double d;
+    const double* xq = S.xd.data() + 3 * Q.voff;
+    const double* xqp = S.xprevd.data() + 3 * Q.voff;
+    const double* xt = S.xd.data() + 3 * T.voff;
+    if (narrow_test(ld(xq, r.point), ld(xqp, r.point), ld(xt, f[0]), ld(xt, f[1]), ld(xt, f[2]),
+                    band, n, d)) {
+      S.twin_nd4[4 * i] = n.x; S.twin_nd4[4 * i + 1] = n.y; S.twin_nd4[4 * i + 2] = n.z;
+      S.twin_nd4[4 * i + 3] = d;
+    } else {
+      // geometry rejected in FP64 (a tie): keep the FP32 record's values
+      for (int k = 0; k < 4; ++k) S.twin_nd4[4 * i + k] = r.normal_distance[k];
+    }
+  }
+}
+
+static void dynamics(OScene& S) {
+  const ysim_sim_params& P = S.P;
+  // snapshot_xprev_range (physics.metal:520-529): xPrev := x for dynamic meshes
+  for (auto& M : S.meshes) {
+    if (!is_dynamic(M.d)) continue;
+    std::memcpy(S.xprev.data() + 3 * M.voff, S.x.data() + 3 * M.voff, 12 * (size_t)M.d.num_vertices);
+    if (S.twin)
+      std::memcpy(S.xprevd.data() + 3 * M.voff, S.xd.data() + 3 * M.voff, 24 * (size_t)M.d.num_vertices);
+  }
+  bool first_cloth = true;
+  for (auto& M : S.meshes) {
+    if (!is_dynamic(M.d)) continue;
+    const float* ks = (first_cloth && !S.kspring.empty()) ? S.kspring.data() : nullptr;
+    const double* ksd = (first_cloth && !S.kspring_d.empty()) ? S.kspring_d.data() : nullptr;
+    first_cloth = false;
+    size_t o = M.voff;
+    fastgrid_forces<float>(M.d.cloth, M.d.num_vertices, P.kair, P.kd, S.x.data() + 3 * o,
+                           S.v.data() + 3 * o, S.ext.data() + 3 * o, ks, S.f.data() + 3 * o);
+    if (S.twin) {
+      std::vector<double> extd(S.ext.begin() + 3 * o, S.ext.begin() + 3 * (o + M.d.num_vertices));
+      fastgrid_forces<double>(M.d.cloth, M.d.num_vertices, (double)P.kair, (double)P.kd,
+                              S.xd.data() + 3 * o, S.vd.data() + 3 * o, extd.data(), ksd,
+                              S.fd.data() + 3 * o);
+    }
+  }
+  std::vector<ContactR<float>> cs;
+  std::vector<ContactR<double>> csd;
+  for (auto& M : S.meshes) {
+    if (!is_dynamic(M.d)) continue;
+    for (uint32_t i = 0; i < M.d.num_vertices; ++i) {
+      uint32_t g = M.voff + i;
+      uint32_t b = S.csr_off[g], e = S.csr_off[g + 1];
+      cs.resize(e - b);
+      for (uint32_t k = b; k < e; ++k)
+        cs[k - b] = {S.csr[k].target_obj, S.csr[k].triangle,
+                     {S.csr[k].normal_distance[0], S.csr[k].normal_distance[1], S.csr[k].normal_distance[2]},
+                     S.csr[k].normal_distance[3]};
+      V3<float> pos = ld(S.x.data(), g), vel = ld(S.v.data(), g);
+      if (integrate_vertex<float>(pos, vel, ld(S.f.data(), g), S.mass[g], S.fixed[g], P.subh,
+                                  M.d.cloth.thickness, cs.data(), e - b))
+        S.anomaly |= 1u;
+      st(S.x.data(), g, pos);
+      st(S.v.data(), g, vel);
+      if (S.twin) {
+        csd.resize(e - b);
+        for (uint32_t k = b; k < e; ++k)
+          csd[k - b] = {S.csr[k].target_obj, S.csr[k].triangle,
+                        {S.twin_nd4[4 * k], S.twin_nd4[4 * k + 1], S.twin_nd4[4 * k + 2]},
+                        S.twin_nd4[4 * k + 3]};
+        V3<double> pd = ld(S.xd.data(), g), vd = ld(S.vd.data(), g);
+        integrate_vertex<double>(pd, vd, ld(S.fd.data(), g), (double)S.mass[g], (double)S.fixed[g],
+                                 (double)P.subh, (double)M.d.cloth.thickness, csd.data(), e - b);
+        st(S.xd.data(), g, pd);
+        st(S.vd.data(), g, vd);
+      }
+    }
+  }
+}
+
+void yo_scene_step(void* s, uint32_t n) {
+  OScene* S = (OScene*)s;
+  for (uint32_t i = 0; i < n; ++i) {
+    collide(*S, nullptr, nullptr);
+    if (S->twin) twin_contacts(*S);
+    dynamics(*S);
+    S->step++;
+  }
+}
+
+// Collision pipeline only (no step): candidate pairs as (qobj, point, tobj, tri)
+// quadruples.  Returns the pair count; writes up to cap quadruples.
+uint64_t yo_scene_pairs(void* s, uint32_t* out4, uint64_t cap) {
+  OScene* S = (OScene*)s;
+  std::vector<uint32_t> pairs;
+  uint64_t np = 0;
+  collide(*S, &np, &pairs);
+  if (out4) {
+    uint64_t n = std::min<uint64_t>(cap, pairs.size() / 4);
+    std::memcpy(out4, pairs.data(), 16 * n);
+  }
+  return np;
+}
+
+void yo_scene_get_state(void* s, float* x, float* v, float* xprev) {
+  OScene* S = (OScene*)s;
+  if (x) std::memcpy(x, S->x.data(), 4 * S->x.size());
+  if (v) std::memcpy(v, S->v.data(), 4 * S->v.size());
+  if (xprev) std::memcpy(xprev, S->xprev.data(), 4 * S->xprev.size());
+}
+void yo_scene_set_state(void* s, const float* x, const float* v, const float* xprev) {
+  OScene* S = (OScene*)s;
+  if (x) std::memcpy(S->x.data(), x, 4 * S->x.size());
+  if (v) std::memcpy(S->v.data(), v, 4 * S->v.size());
+  if (xprev) std::memcpy(S->xprev.data(), xprev, 4 * S->xprev.size());
+}
+void yo_scene_get_twin(void* s, double* x, double* v) {
+  OScene* S = (OScene*)s;
+  if (x) std::memcpy(x, S->xd.data(), 8 * S->xd.size());
+  if (v) std::memcpy(v, S->vd.data(), 8 * S->vd.size());
+}
+uint32_t yo_scene_num_contacts(void* s) { return (uint32_t)((OScene*)s)->csr.size(); }
+void yo_scene_get_contacts(void* s, uint32_t* off, ysim_narrow_collision* out) {
+  OScene* S = (OScene*)s;
+  if (off) std::memcpy(off, S->csr_off.data(), 4 * S->csr_off.size());
+  if (out) std::memcpy(out, S->csr.data(), sizeof(ysim_narrow_collision) * S->csr.size());
+}
+void yo_scene_get_tree(void* s, uint32_t mesh, ysim_bvh_node* nodes, int32_t* parent,
+                       uint64_t* keys, uint32_t* idx) {
+  OScene* S = (OScene*)s;
+  OMesh& M = S->meshes[mesh];
+  if (nodes) std::memcpy(nodes, M.tree.data(), sizeof(ysim_bvh_node) * M.tree.size());
+  if (parent) std::memcpy(parent, M.parent.data(), 4 * M.parent.size());
+  if (keys) std::memcpy(keys, M.keys.data(), 8 * M.keys.size());
+  if (idx) std::memcpy(idx, M.idx.data(), 4 * M.idx.size());
+}
+uint32_t yo_scene_anomaly(void* s) { return ((OScene*)s)->anomaly; }
+
+}  // extern "C"

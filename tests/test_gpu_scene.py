@@ -1,0 +1,72 @@
+"""Scene-level parity: the full substep on the GPU vs the CPU oracle."""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleScene
+from ysim_amd import scenes
+from ysim_amd.hip import HipScene
+
+pytestmark = pytest.mark.gpu
+
+
+def pair_set(a):
+    return set(map(tuple, np.asarray(a).tolist()))
+
+
+def assert_same_state(hs, os_):
+    xg, vg, pg = hs.state()
+    xo, vo, po = os_.state()
+    assert np.array_equal(xg, xo)
+    assert np.array_equal(vg, vo)
+    assert np.array_equal(pg, po)
+
+
+def assert_same_contacts(hs, os_):
+    og, cg = hs.contacts()
+    oo, co = os_.contacts()
+    assert np.array_equal(og, oo)
+    for f in ("point", "triangle", "query_obj", "target_obj", "normal_distance"):
+        assert np.array_equal(cg[f], co[f]), f
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_c1_bit_exact(ctx, oracle_threads, graph):
+    sc = scenes.config_c1()
+    hs, os_ = HipScene(ctx, sc), OracleScene(sc)
+    assert pair_set(hs.pairs()) == pair_set(os_.pairs())
+    hs.step(20, graph=graph)
+    os_.step(20)
+    assert_same_state(hs, os_)
+    assert_same_contacts(hs, os_)
+    assert hs.flags() == 0
+
+
+@pytest.mark.parametrize("rebuild", [0, 1])
+def test_drape_contacts_bit_exact(ctx, oracle_threads, rebuild):
+    sc = scenes.config_drape(n=32, height=0.8, rebuild_every=rebuild)
+    hs, os_ = HipScene(ctx, sc), OracleScene(sc)
+    total = 0
+    for _ in range(8):
+        hs.step(10, graph=True)
+        os_.step(10)
+        assert_same_state(hs, os_)
+        assert_same_contacts(hs, os_)
+        total += len(os_.contacts()[1])
+    assert total > 0, "scene produced no contacts"
+    assert pair_set(hs.pairs()) == pair_set(os_.pairs())
+
+
+def test_c2_small_fold_63bit(ctx, oracle_threads):
+    sc = scenes.config_c2(n=96)
+    hs, os_ = HipScene(ctx, sc), OracleScene(sc)
+    for _ in range(3):
+        hs.step(2)
+        os_.step(2)
+        assert_same_state(hs, os_)
+        ng, pg, kg, ig = hs.tree(0, keys=True)
+        no, po, ko, io = os_.tree(0)
+        assert np.array_equal(kg, ko) and np.array_equal(ig, io)
+        for f in ("min", "max", "child_a", "child_b"):
+            assert np.array_equal(ng[f], no[f]), f
+        assert np.array_equal(pg, po)
+    assert pair_set(hs.pairs()) == pair_set(os_.pairs())

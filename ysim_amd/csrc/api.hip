@@ -1,0 +1,1037 @@
+// api.hip — the C ABI (include/ysim_hip.h): runtime, stage entry points and the
+// resident scene with its substep loop.
+//
+// The scene mirrors Simulator::initialize / update (include/sim/simulator.hpp:
+// 2942-3115, 3512-3815) for FastGrid cloth + static (Float) obstacle meshes:
+//   [rebuild: box -> Morton -> sort -> Karras] -> leaf/bottom-up refit
+//   -> traversal fused with the narrow phase -> springs + integrate + contacts.
+// Positions and velocities are double-buffered: substep s reads buffer s%2 and
+// writes buffer (s+1)%2, whose old content (x before the previous integrate)
+// is exactly the xPrev the swept narrow phase needs.  A hipGraph captures a
+// whole period of substeps (even length, so buffer roles repeat).
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace ysim {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int Ctx::ensure_scratch(size_t bytes) {
+  if (scratch_bytes >= bytes) return YSIM_OK;
+  if (scratch) YSIM_HIP_TRY(hipFree(scratch));
+  scratch = nullptr;
+  scratch_bytes = 0;
+  YSIM_HIP_TRY(hipMalloc(&scratch, bytes));
+  scratch_bytes = bytes;
+  return YSIM_OK;
+}
+
+}  // namespace ysim
+
+using namespace ysim;
+
+#define YSIM_TRY(expr)              \
+  do {                              \
+    int rc_ = (expr);               \
+    if (rc_ != YSIM_OK) return rc_; \
+  } while (0)
+
+#define YSIM_ARG(cond, msg) \
+  do {                      \
+    if (!(cond)) return fail(YSIM_ERR_INVALID, msg); \
+  } while (0)
+
+struct ysim_ctx : ysim::Ctx {};
+
+namespace {
+
+uint32_t bits_for(uint32_t n) {  // ceil(log2(n)) for n >= 1
+  uint32_t b = 0;
+  while ((1ull << b) < n) ++b;
+  return b;
+}
+
+struct BuildGroup {
+  std::vector<uint32_t> meshes;
+  SegTable tsegs, vsegs;
+  uint32_t T = 0, V = 0;
+  uint32_t code_bits = 30;
+  bool key64 = false;
+  SortPlan plan;
+};
+
+constexpr int NEV = YSIM_STAGE_COUNT + 1;
+
+}  // namespace
+
+struct ysim_scene {
+  ysim_ctx* ctx = nullptr;
+  hipStream_t st = nullptr;
+  ysim_sim_params P{};
+  uint32_t nm = 0, Vtot = 0, Ttot = 0, Ntot = 0;
+  std::vector<ysim_mesh_desc> meshes;
+  std::vector<uint32_t> voff, foff, noff;
+  std::vector<uint32_t> dyn, stat;
+  std::vector<uint32_t> dyn_voff;        // host copy of dyn_vsegs offsets
+  uint32_t Vdyn = 0, Tdyn = 0;
+  float* x[2] = {nullptr, nullptr};
+  float* v[2] = {nullptr, nullptr};
+  float *mass = nullptr, *ext = nullptr, *fixed = nullptr, *kspring = nullptr;
+  uint32_t* facets = nullptr;
+  MeshTables mt;
+  ysim_fastgrid_params* d_cloth = nullptr;
+  uint32_t *d_nfacets = nullptr, *d_behavior = nullptr, *d_shape = nullptr;
+  float *d_margin = nullptr, *d_band = nullptr;
+  ysim_bvh_node* tree = nullptr;
+  int32_t* parent = nullptr;
+  uint32_t* visit = nullptr;
+  std::vector<BuildGroup> dyn_groups, stat_groups;
+  SegTable dyn_tsegs, dyn_vsegs;
+  uint32_t *d_tgt_off = nullptr, *d_tgt = nullptr;
+  void* keys[2] = {nullptr, nullptr};
+  uint32_t* vals[2] = {nullptr, nullptr};
+  uint32_t *hist = nullptr, *status = nullptr, *counters = nullptr, *box_acc = nullptr;
+  uint32_t cap = 16;
+  uint32_t* ccount = nullptr;
+  float4* cnd = nullptr;
+  uint2* cid = nullptr;
+  uint32_t* npairs = nullptr;  // [0] candidate statistics, [1] pair-emission counter
+  uint32_t* flags = nullptr;   // [0] sticky flags
+  uint32_t* pairs = nullptr;
+  uint32_t pair_cap = 0;
+  std::vector<void*> allocs;
+  uint32_t parity = 0;
+  uint64_t step = 0;
+  bool formal_refit = false;
+  uint32_t last_sorted = 0;
+  hipGraphExec_t graph = nullptr;
+  uint32_t graph_period = 0;
+  hipEvent_t ev[NEV] = {};
+  bool timing = false;
+  double stage_ms[YSIM_STAGE_COUNT] = {0};
+
+  template <typename T>
+  int alloc(T** p, size_t n) {
+    *p = nullptr;
+    YSIM_HIP_TRY(hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T)));
+    allocs.push_back((void*)*p);
+    return YSIM_OK;
+  }
+  template <typename T>
+  int put(T** p, const T* h, size_t n) {
+    YSIM_TRY(alloc(p, n));
+    if (n) YSIM_HIP_TRY(hipMemcpyAsync(*p, h, n * sizeof(T), hipMemcpyHostToDevice, st));
+    return YSIM_OK;
+  }
+  template <typename T>
+  int put(T** p, const std::vector<T>& h) {
+    return put(p, h.data(), h.size());
+  }
+  int zero(void* p, size_t words) {
+    if (!words) return YSIM_OK;
+    hipLaunchKernelGGL(k_zero_u32, dim3(std::min<uint32_t>(4096, ceil_div(words, 256))), dim3(256), 0, st,
+                       (uint32_t*)p, (uint32_t)words);
+    YSIM_CHECK_LAUNCH();
+    return YSIM_OK;
+  }
+  bool dynamic(uint32_t m) const { return meshes[m].behavior == YSIM_BEHAVIOR_FASTGRID_CLOTH; }
+
+  int make_seg(const std::vector<uint32_t>& ms, bool tris, SegTable& out) {
+    std::vector<uint32_t> off(1, 0);
+    for (uint32_t m : ms) off.push_back(off.back() + (tris ? meshes[m].num_facets : meshes[m].num_vertices));
+    uint32_t *d_off = nullptr, *d_mesh = nullptr;
+    YSIM_TRY(put(&d_off, off));
+    YSIM_TRY(put(&d_mesh, ms));
+    out.nseg = (uint32_t)ms.size();
+    out.off = d_off;
+    out.mesh = d_mesh;
+    return YSIM_OK;
+  }
+
+  // Groups of meshes whose keys (segment id + Morton code) fit in 64 bits.
+  int make_groups(const std::vector<uint32_t>& ms, std::vector<BuildGroup>& out) {
+    const uint32_t cb = P.morton_bits;
+    std::vector<std::vector<uint32_t>> lists;
+    std::vector<uint32_t> nonempty;
+    for (uint32_t m : ms)
+      if (meshes[m].num_facets) nonempty.push_back(m);
+    if (nonempty.empty()) return YSIM_OK;
+    if (nonempty.size() == 1 || bits_for((uint32_t)nonempty.size()) + cb <= 64) {
+      lists.push_back(nonempty);
+    } else {
+      for (uint32_t m : nonempty) lists.push_back({m});
+    }
+    for (auto& l : lists) {
+      BuildGroup g;
+      g.meshes = l;
+      YSIM_TRY(make_seg(l, true, g.tsegs));
+      YSIM_TRY(make_seg(l, false, g.vsegs));
+      for (uint32_t m : l) {
+        g.T += meshes[m].num_facets;
+        g.V += meshes[m].num_vertices;
+      }
+      g.code_bits = cb;
+      uint32_t key_bits = cb + (l.size() > 1 ? bits_for((uint32_t)l.size()) : 0);
+      g.key64 = key_bits > 32;
+      g.plan = sort_plan(g.T, key_bits);
+      out.push_back(g);
+    }
+    return YSIM_OK;
+  }
+
+  int enqueue_build(BuildGroup& g, const float* xc) {
+    if (!g.T) return YSIM_OK;
+    YSIM_TRY(launch_box(st, g.vsegs, mt, xc, g.V, box_acc));
+    if (timing) YSIM_HIP_TRY(hipEventRecord(ev[1], st));
+    YSIM_TRY(launch_morton(st, g.tsegs, mt, xc, facets, box_acc, g.code_bits, g.key64, keys[0], vals[0], g.T,
+                           g.plan, hist, status, counters));
+    if (timing) YSIM_HIP_TRY(hipEventRecord(ev[2], st));
+    YSIM_TRY(launch_sort_passes(st, g.plan, keys[0], vals[0], keys[1], vals[1], hist, status, counters, flags));
+    if (timing) YSIM_HIP_TRY(hipEventRecord(ev[3], st));
+    const uint32_t sorted = (g.plan.passes % 2 == 0) ? 0 : 1;
+    YSIM_TRY(launch_hierarchy(st, g.tsegs, mt, keys[sorted], g.key64, vals[sorted], g.code_bits, tree, parent,
+                              visit, g.T, box_acc, g.tsegs.nseg, hist, (uint32_t)g.plan.hist_words));
+    last_sorted = sorted;
+    return YSIM_OK;
+  }
+
+  QueryArgs query_args(uint32_t par) {
+    QueryArgs a;
+    a.qsegs = dyn_vsegs;
+    a.mt = mt;
+    a.tgt_off = d_tgt_off;
+    a.tgt = d_tgt;
+    a.mesh_nfacets = d_nfacets;
+    a.mesh_margin = d_margin;
+    a.mesh_band = d_band;
+    a.mesh_behavior = d_behavior;
+    a.mesh_shape = d_shape;
+    a.x = x[par];
+    a.xprev = x[par ^ 1];
+    a.facets = facets;
+    a.tree = tree;
+    a.parent = parent;
+    a.nitems = Vdyn;
+    a.cap = cap;
+    a.ccount = ccount;
+    a.cnd = cnd;
+    a.cid = cid;
+    a.npairs = npairs;
+    a.flags = flags;
+    return a;
+  }
+
+  int enqueue_collision(uint32_t par, bool rebuild) {
+    const float* xc = x[par];
+    if (timing) YSIM_HIP_TRY(hipEventRecord(ev[0], st));
+    if (rebuild) {
+      for (auto& g : dyn_groups) YSIM_TRY(enqueue_build(g, xc));
+    } else if (timing) {
+      for (int e = 1; e <= 3; ++e) YSIM_HIP_TRY(hipEventRecord(ev[e], st));
+    }
+    if (timing) YSIM_HIP_TRY(hipEventRecord(ev[4], st));
+    YSIM_TRY(launch_refit(st, dyn_tsegs, mt, xc, v[par], P.subh, P.swept != 0, facets, tree, parent, visit, Tdyn,
+                          flags, formal_refit));
+    if (timing) YSIM_HIP_TRY(hipEventRecord(ev[5], st));
+    return YSIM_OK;
+  }
+
+  int enqueue_substep(uint32_t par, bool rebuild) {
+    YSIM_TRY(enqueue_collision(par, rebuild));
+    QueryArgs qa = query_args(par);
+    YSIM_TRY(launch_query(st, qa, false));
+    if (timing) YSIM_HIP_TRY(hipEventRecord(ev[6], st));
+    ClothArgs ca;
+    ca.dsegs = dyn_vsegs;
+    ca.mt = mt;
+    ca.cloth = d_cloth;
+    ca.kspring = kspring;
+    ca.subh = P.subh;
+    ca.kair = P.kair;
+    ca.kd = P.kd;
+    ca.x = x[par];
+    ca.v = v[par];
+    ca.ext = ext;
+    ca.mass = mass;
+    ca.fixed = fixed;
+    ca.ccount = ccount;
+    ca.cnd = cnd;
+    ca.cid = cid;
+    ca.cap = cap;
+    ca.x_out = x[par ^ 1];
+    ca.v_out = v[par ^ 1];
+    ca.nitems = Vdyn;
+    ca.flags = flags;
+    YSIM_TRY(launch_cloth(st, ca));
+    if (timing) YSIM_HIP_TRY(hipEventRecord(ev[7], st));
+    return YSIM_OK;
+  }
+
+  bool rebuild_at(uint64_t s) const { return P.rebuild_every > 0 && (s % P.rebuild_every) == 0; }
+
+  int eager_steps(uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) {
+      YSIM_TRY(enqueue_substep(parity, rebuild_at(step)));
+      if (timing) {
+        YSIM_HIP_TRY(hipEventSynchronize(ev[7]));
+        for (int k = 0; k < YSIM_STAGE_COUNT; ++k) {
+          float ms = 0.f;
+          YSIM_HIP_TRY(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+          stage_ms[k] += ms;
+        }
+      }
+      parity ^= 1u;
+      ++step;
+    }
+    return YSIM_OK;
+  }
+
+  int capture_graph() {
+    if (graph) return YSIM_OK;
+    uint32_t period = 2;
+    if (P.rebuild_every > 1) period = (P.rebuild_every % 2 == 0) ? P.rebuild_every : 2 * P.rebuild_every;
+    hipGraph_t g = nullptr;
+    YSIM_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = YSIM_OK;
+    for (uint32_t i = 0; i < period && rc == YSIM_OK; ++i)
+      rc = enqueue_substep(i % 2, rebuild_at(i));
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc != YSIM_OK) return rc;
+    if (e != hipSuccess) return fail(YSIM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    YSIM_HIP_TRY(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0));
+    YSIM_HIP_TRY(hipGraphDestroy(g));
+    graph_period = period;
+    return YSIM_OK;
+  }
+
+  int steps(uint32_t n, bool use_graph) {
+    if (!use_graph) return eager_steps(n);
+    YSIM_TRY(capture_graph());
+    // align to the graph phase (step % period == 0 also implies parity 0)
+    while (n && (step % graph_period) != 0) {
+      YSIM_TRY(eager_steps(1));
+      --n;
+    }
+    while (n >= graph_period) {
+      YSIM_HIP_TRY(hipGraphLaunch(graph, st));
+      step += graph_period;
+      n -= graph_period;
+    }
+    return eager_steps(n);
+  }
+
+  ~ysim_scene() {
+    if (graph) hipGraphExecDestroy(graph);
+    for (auto& e : ev)
+      if (e) hipEventDestroy(e);
+    for (void* p : allocs) hipFree(p);
+  }
+};
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* ysim_hip_last_error(void) { return g_err.c_str(); }
+int ysim_hip_version(void) { return 10000; }
+
+int ysim_hip_device_count(int* out) {
+  YSIM_ARG(out, "null out");
+  YSIM_HIP_TRY(hipGetDeviceCount(out));
+  return YSIM_OK;
+}
+
+int ysim_hip_ctx_create(int device, uint32_t flags, ysim_ctx** out) {
+  YSIM_ARG(out, "null out");
+  *out = nullptr;
+  int n = 0;
+  YSIM_HIP_TRY(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(YSIM_ERR_INVALID, "no such HIP device");
+  YSIM_HIP_TRY(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  YSIM_HIP_TRY(hipGetDeviceProperties(&prop, device));
+  if (std::string(prop.gcnArchName).rfind("gfx950", 0) != 0)
+    return fail(YSIM_ERR_UNSUPPORTED, std::string("library built for gfx950, device is ") + prop.gcnArchName);
+  ysim_ctx* c = new ysim_ctx();
+  c->device = device;
+  c->num_cus = prop.multiProcessorCount;
+  c->formal_refit = (flags & YSIM_CTX_FORMAL_REFIT) != 0;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(YSIM_ERR_HIP, "hipStreamCreate failed");
+  }
+  *out = c;
+  return YSIM_OK;
+}
+
+int ysim_hip_ctx_destroy(ysim_ctx* ctx) {
+  if (!ctx) return YSIM_OK;
+  hipStreamSynchronize(ctx->stream);
+  if (ctx->scratch) hipFree(ctx->scratch);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return YSIM_OK;
+}
+
+int ysim_hip_sync(ysim_ctx* ctx) {
+  YSIM_ARG(ctx, "null ctx");
+  YSIM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return YSIM_OK;
+}
+
+int ysim_hip_alloc(ysim_ctx* ctx, size_t bytes, void** d_out) {
+  YSIM_ARG(ctx && d_out, "null argument");
+  YSIM_HIP_TRY(hipMalloc(d_out, std::max<size_t>(bytes, 1)));
+  return YSIM_OK;
+}
+int ysim_hip_free(ysim_ctx* ctx, void* d_ptr) {
+  YSIM_ARG(ctx, "null ctx");
+  if (d_ptr) {
+    YSIM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    YSIM_HIP_TRY(hipFree(d_ptr));
+  }
+  return YSIM_OK;
+}
+int ysim_hip_upload(ysim_ctx* ctx, void* d_dst, const void* h_src, size_t bytes) {
+  YSIM_ARG(ctx && (bytes == 0 || (d_dst && h_src)), "null argument");
+  if (bytes) YSIM_HIP_TRY(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  YSIM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return YSIM_OK;
+}
+int ysim_hip_download(ysim_ctx* ctx, void* h_dst, const void* d_src, size_t bytes) {
+  YSIM_ARG(ctx && (bytes == 0 || (d_src && h_dst)), "null argument");
+  if (bytes) YSIM_HIP_TRY(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  YSIM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return YSIM_OK;
+}
+int ysim_hip_memset(ysim_ctx* ctx, void* d_dst, int value, size_t bytes) {
+  YSIM_ARG(ctx && (bytes == 0 || d_dst), "null argument");
+  if (bytes) YSIM_HIP_TRY(hipMemsetAsync(d_dst, value, bytes, ctx->stream));
+  return YSIM_OK;
+}
+
+// ------------------------------------------------------------------ stages
+namespace {
+// single-segment tables for stage entry points: off = {0, n}, mesh = {0},
+// voff = foff = noff = {0, ...}
+struct Unit {
+  uint32_t* d = nullptr;  // [0]=0 [1]=n [2]=0 (mesh id) [3]=0 [4]=0 [5]=0
+  SegTable seg() const { return SegTable{1, d, d + 2}; }
+  MeshTables mt() const { return MeshTables{d + 3, d + 4, d + 5}; }
+};
+int unit_table(ysim_ctx* ctx, uint32_t n, Unit& u) {
+  YSIM_TRY(ctx->ensure_scratch(1 << 20));
+  u.d = (uint32_t*)ctx->scratch;
+  uint32_t h[8] = {0, n, 0, 0, 0, 0, 0, 0};
+  YSIM_HIP_TRY(hipMemcpyAsync(u.d, h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
+  return YSIM_OK;
+}
+uint32_t* scratch_words(ysim_ctx* ctx, size_t off_words) { return (uint32_t*)ctx->scratch + off_words; }
+}  // namespace
+
+int ysim_hip_scene_box(ysim_ctx* ctx, const float* d_x, uint32_t num_vertices, uint32_t num_prims,
+                       ysim_aabb4* d_box) {
+  YSIM_ARG(ctx && d_x && d_box && num_vertices > 0, "scene_box: bad argument");
+  Unit u;
+  YSIM_TRY(unit_table(ctx, num_vertices, u));
+  uint32_t* acc = scratch_words(ctx, 64);
+  uint32_t init[6] = {~0u, ~0u, ~0u, 0u, 0u, 0u};
+  YSIM_HIP_TRY(hipMemcpyAsync(acc, init, sizeof(init), hipMemcpyHostToDevice, ctx->stream));
+  YSIM_TRY(launch_box(ctx->stream, u.seg(), u.mt(), d_x, num_vertices, acc));
+  uint32_t h[6];
+  YSIM_HIP_TRY(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  YSIM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ysim_aabb4 b;
+  auto un = [](uint32_t k) {
+    uint32_t uu = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    float f;
+    std::memcpy(&f, &uu, 4);
+    return f;
+  };
+  for (int k = 0; k < 3; ++k) {
+    b.min[k] = un(h[k]);
+    b.max[k] = un(h[3 + k]);
+  }
+  b.i0 = (int32_t)num_prims;
+  b.i1 = 0;
+  YSIM_HIP_TRY(hipMemcpyAsync(d_box, &b, sizeof(b), hipMemcpyHostToDevice, ctx->stream));
+  YSIM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return YSIM_OK;
+}
+
+int ysim_hip_morton(ysim_ctx* ctx, const float* d_x, const uint32_t* d_facets, uint32_t num_facets,
+                    const ysim_aabb4* d_box, uint32_t bits, void* d_keys, uint32_t* d_idx) {
+  YSIM_ARG(ctx && d_x && d_facets && d_box && d_keys && d_idx, "morton: null argument");
+  YSIM_ARG(bits == 30 || bits == 63, "morton: bits must be 30 or 63");
+  if (!num_facets) return YSIM_OK;
+  Unit u;
+  YSIM_TRY(unit_table(ctx, num_facets, u));
+  // box in ordered-key form
+  ysim_aabb4 hb;
+  YSIM_HIP_TRY(hipMemcpyAsync(&hb, d_box, sizeof(hb), hipMemcpyDeviceToHost, ctx->stream));
+  YSIM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  auto key = [](float f) {
+    uint32_t uu;
+    std::memcpy(&uu, &f, 4);
+    return (uu & 0x80000000u) ? ~uu : (uu | 0x80000000u);
+  };
+  uint32_t acc_h[6] = {key(hb.min[0]), key(hb.min[1]), key(hb.min[2]), key(hb.max[0]), key(hb.max[1]), key(hb.max[2])};
+  uint32_t* acc = scratch_words(ctx, 64);
+  uint32_t* hist = scratch_words(ctx, 128);  // 2048 words
+  YSIM_HIP_TRY(hipMemcpyAsync(acc, acc_h, sizeof(acc_h), hipMemcpyHostToDevice, ctx->stream));
+  SortPlan none = sort_plan(0, bits);
+  none.passes = 0;
+  // vertex count is not needed: facets index into d_x directly
+  YSIM_TRY(launch_morton(ctx->stream, u.seg(), u.mt(), d_x, d_facets, acc, bits, bits > 32, d_keys, d_idx,
+                         num_facets, none, hist, nullptr, scratch_words(ctx, 4096)));
+  YSIM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return YSIM_OK;
+}
+
+int ysim_hip_sort_pairs(ysim_ctx* ctx, void* d_keys, uint32_t* d_vals, uint32_t n, uint32_t key_bits) {
+  YSIM_ARG(ctx && (n == 0 || (d_keys && d_vals)), "sort_pairs: null argument");
+  YSIM_ARG(key_bits >= 1 && key_bits <= 64, "sort_pairs: key_bits in [1, 64]");
+  if (n == 0) return YSIM_OK;
+  YSIM_ARG(n < (1u << 30), "sort_pairs: n must be < 2^30");
+  SortPlan p = sort_plan(n, key_bits);
+  const size_t kb = (p.wide ? 8 : 4);
+  const size_t words = 4096 + p.hist_words + p.counter_words + 16 + p.status_words + (n * (kb + 4) + 3) / 4 + 64;
+  YSIM_TRY(ctx->ensure_scratch(words * 4));
+  uint32_t* base = (uint32_t*)ctx->scratch;
+  uint32_t* hist = base + 4096;
+  uint32_t* counters = hist + p.hist_words;
+  uint32_t* err = counters + p.counter_words;
+  uint32_t* status = err + 16;
+  char* tmp = (char*)(status + p.status_words);
+  tmp = (char*)(((uintptr_t)tmp + 255) & ~(uintptr_t)255);
+  void* keys_b = tmp;
+  uint32_t* vals_b = (uint32_t*)(tmp + ((n * kb + 255) & ~(size_t)255));
+  YSIM_TRY(launch_sort_reset(ctx->stream, p, hist, status, counters, true));
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, err, 16u);
+  YSIM_TRY(launch_sort_histogram(ctx->stream, p, d_keys, hist));
+  YSIM_TRY(launch_sort_passes(ctx->stream, p, d_keys, d_vals, keys_b, vals_b, hist, status, counters, err));
+  if (p.passes % 2 == 1) {
+    YSIM_HIP_TRY(hipMemcpyAsync(d_keys, keys_b, n * kb, hipMemcpyDeviceToDevice, ctx->stream));
+    YSIM_HIP_TRY(hipMemcpyAsync(d_vals, vals_b, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  uint32_t herr = 0;
+  YSIM_HIP_TRY(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, ctx->stream));
+  YSIM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (herr) return fail(YSIM_ERR_HIP, "sort_pairs: look-back guard fired");
+  return YSIM_OK;
+}
+
+int ysim_hip_lbvh_hierarchy(ysim_ctx* ctx, const void* d_sorted_keys, const uint32_t* d_sorted_idx, uint32_t n,
+                            uint32_t bits, ysim_bvh_node* d_tree, int32_t* d_parent) {
+  YSIM_ARG(ctx && d_sorted_keys && d_sorted_idx && d_tree && d_parent, "lbvh_hierarchy: null argument");
+  YSIM_ARG(bits == 30 || bits == 63 || bits == 32 || bits == 64, "lbvh_hierarchy: bits");
+  if (!n) return YSIM_OK;
+  YSIM_TRY(ctx->ensure_scratch((size_t)(1 << 20) + (size_t)n * 8 + 4096));
+  Unit u;
+  YSIM_TRY(unit_table(ctx, n, u));
+  uint32_t* visit = (uint32_t*)((char*)ctx->scratch + (1 << 20));
+  YSIM_TRY(launch_hierarchy(ctx->stream, u.seg(), u.mt(), d_sorted_keys, bits > 32, d_sorted_idx, bits, d_tree,
+                            d_parent, visit, n, nullptr, 0, nullptr, 0));
+  YSIM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return YSIM_OK;
+}
+
+int ysim_hip_lbvh_refit(ysim_ctx* ctx, const float* d_x, const float* d_v, float dt, uint32_t swept,
+                        const uint32_t* d_facets, uint32_t num_facets, ysim_bvh_node* d_tree,
+                        const int32_t* d_parent) {
+  YSIM_ARG(ctx && d_x && d_facets && d_tree && d_parent, "lbvh_refit: null argument");
+  YSIM_ARG(!swept || d_v, "lbvh_refit: swept needs velocities");
+  if (!num_facets) return YSIM_OK;
+  YSIM_TRY(ctx->ensure_scratch((size_t)(1 << 20) + (size_t)num_facets * 4 + 4096));
+  Unit u;
+  YSIM_TRY(unit_table(ctx, num_facets, u));
+  uint32_t* visit = (uint32_t*)((char*)ctx->scratch + (1 << 20));
+  uint32_t* err = scratch_words(ctx, 64);
+  hipLaunchKernelGGL(k_zero_u32, dim3(std::max<uint32_t>(1, ceil_div(num_facets, 256))), dim3(256), 0, ctx->stream,
+                     visit, num_facets);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, err, 1u);
+  YSIM_TRY(launch_refit(ctx->stream, u.seg(), u.mt(), d_x, d_v ? d_v : d_x, dt, swept != 0, d_facets, d_tree,
+                        d_parent, visit, num_facets, err, ctx->formal_refit));
+  uint32_t herr = 0;
+  YSIM_HIP_TRY(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, ctx->stream));
+  YSIM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (herr) return fail(YSIM_ERR_INVALID, "lbvh_refit: corrupted topology (guard fired)");
+  return YSIM_OK;
+}
+
+int ysim_hip_broad_query(ysim_ctx* ctx, const float* d_xq, uint32_t num_query, uint32_t query_obj,
+                         uint32_t query_behavior, uint32_t query_shape, const ysim_bvh_node* d_tree,
+                         uint32_t num_target_facets, const uint32_t* d_target_facets, uint32_t target_obj,
+                         uint32_t target_behavior, uint32_t target_shape, float margin,
+                         ysim_broad_collision* d_out, uint32_t capacity, uint32_t* d_count, uint32_t* h_required) {
+  YSIM_ARG(ctx && d_xq && d_tree && d_target_facets && d_count, "broad_query: null argument");
+  YSIM_ARG(capacity == 0 || d_out, "broad_query: null output");
+  if (!num_query || !num_target_facets) return YSIM_OK;
+  // parent links are needed by the traversal: rebuild them from child links
+  const uint32_t nn = 2 * num_target_facets - 1;
+  YSIM_TRY(ctx->ensure_scratch((size_t)(1 << 20) + (size_t)nn * 4 + 4096));
+  int32_t* par = (int32_t*)((char*)ctx->scratch + (1 << 20));
+  uint32_t* flags = scratch_words(ctx, 64);
+  std::vector<ysim_bvh_node> h(nn);
+  YSIM_HIP_TRY(hipMemcpyAsync(h.data(), d_tree, nn * sizeof(ysim_bvh_node), hipMemcpyDeviceToHost, ctx->stream));
+  YSIM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  std::vector<int32_t> hp(nn, -1);
+  for (uint32_t i = 0; i + 1 < num_target_facets; ++i) {
+    int a = h[i].child_a, b = h[i].child_b;
+    if (a < 0 || b < 0 || a >= (int)nn || b >= (int)nn) return fail(YSIM_ERR_INVALID, "broad_query: bad tree");
+    hp[a] = (int32_t)i;
+    hp[b] = (int32_t)i;
+  }
+  YSIM_HIP_TRY(hipMemcpyAsync(par, hp.data(), nn * 4, hipMemcpyHostToDevice, ctx->stream));
+  uint32_t start = 0;
+  YSIM_HIP_TRY(hipMemcpyAsync(&start, d_count, 4, hipMemcpyDeviceToHost, ctx->stream));
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, flags, 1u);
+  YSIM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  YSIM_TRY(launch_broad_pairs(ctx->stream, d_xq, num_query, query_obj, query_behavior, query_shape, d_tree, par,
+                              num_target_facets, d_target_facets, target_obj, target_behavior, target_shape, margin,
+                              d_out, capacity, d_count, flags));
+  uint32_t res[2] = {0, 0};
+  YSIM_HIP_TRY(hipMemcpyAsync(&res[0], d_count, 4, hipMemcpyDeviceToHost, ctx->stream));
+  YSIM_HIP_TRY(hipMemcpyAsync(&res[1], flags, 4, hipMemcpyDeviceToHost, ctx->stream));
+  YSIM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (h_required) *h_required = res[0];
+  if (res[1]) return fail(YSIM_ERR_INVALID, "broad_query: traversal guard fired");
+  if (res[0] > capacity) return fail(YSIM_ERR_CAPACITY, "broad_query: pair capacity exceeded");
+  (void)start;
+  return YSIM_OK;
+}
+
+int ysim_hip_narrow(ysim_ctx* ctx, const ysim_broad_collision* d_pairs, uint32_t num_pairs, const float* d_x,
+                    const float* d_xprev, const uint32_t* d_states_offsets, const uint32_t* d_facets,
+                    const uint32_t* d_facets_offsets, const uint32_t* d_adj_offsets, const uint32_t* d_adj_facets,
+                    float radius, float thickness, ysim_narrow_collision* d_out, uint32_t capacity,
+                    uint32_t* d_count, uint32_t* h_required) {
+  YSIM_ARG(ctx && d_x && d_xprev && d_states_offsets && d_facets && d_facets_offsets && d_count,
+           "narrow: null argument");
+  YSIM_ARG(num_pairs == 0 || d_pairs, "narrow: null pairs");
+  YSIM_ARG(capacity == 0 || d_out, "narrow: null output");
+  YSIM_TRY(launch_narrow_pairs(ctx->stream, d_pairs, num_pairs, d_x, d_xprev, d_states_offsets, d_facets,
+                               d_facets_offsets, d_adj_offsets, d_adj_facets, radius + thickness, d_out, capacity,
+                               d_count));
+  uint32_t cnt = 0;
+  YSIM_HIP_TRY(hipMemcpyAsync(&cnt, d_count, 4, hipMemcpyDeviceToHost, ctx->stream));
+  YSIM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (h_required) *h_required = cnt;
+  if (cnt > capacity) return fail(YSIM_ERR_CAPACITY, "narrow: contact capacity exceeded");
+  return YSIM_OK;
+}
+
+int ysim_hip_contact_csr(ysim_ctx* ctx, const ysim_narrow_collision* d_contacts, uint32_t num_contacts,
+                         const uint32_t* d_states_offsets, uint32_t num_points_total, uint32_t* d_offsets,
+                         ysim_narrow_collision* d_sorted) {
+  YSIM_ARG(ctx && d_states_offsets && d_offsets, "contact_csr: null argument");
+  YSIM_ARG(num_contacts == 0 || (d_contacts && d_sorted), "contact_csr: null contacts");
+  YSIM_TRY(ctx->ensure_scratch(((size_t)num_points_total + 2048) * 4));
+  YSIM_TRY(launch_contact_csr(ctx->stream, d_contacts, num_contacts, d_states_offsets, num_points_total, d_offsets,
+                              d_sorted, (uint32_t*)ctx->scratch));
+  YSIM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return YSIM_OK;
+}
+
+int ysim_hip_cloth_step(ysim_ctx* ctx, const ysim_fastgrid_params* h_params, uint32_t num_vertices, float subh,
+                        float kair, float kd, const float* d_x, const float* d_v, const float* d_ext,
+                        const float* d_mass, const float* d_fixed, const uint32_t* d_csr_offsets,
+                        const ysim_narrow_collision* d_csr, float* d_x_out, float* d_v_out, float* d_f_out,
+                        uint32_t* d_anomaly) {
+  YSIM_ARG(ctx && h_params && d_x && d_v && d_ext && d_mass && d_fixed && d_x_out && d_v_out && d_anomaly,
+           "cloth_step: null argument");
+  YSIM_ARG(d_x_out != d_x && d_v_out != d_v, "cloth_step: outputs must not alias inputs");
+  YSIM_ARG(h_params->particle_num_1d >= 2 &&
+               (uint64_t)h_params->particle_num_1d * h_params->particle_num_1d == num_vertices,
+           "cloth_step: num_vertices must be particle_num_1d^2");
+  Unit u;
+  YSIM_TRY(unit_table(ctx, num_vertices, u));
+  ysim_fastgrid_params* dp = (ysim_fastgrid_params*)scratch_words(ctx, 64);
+  YSIM_HIP_TRY(hipMemcpyAsync(dp, h_params, sizeof(*h_params), hipMemcpyHostToDevice, ctx->stream));
+  ClothArgs a;
+  a.dsegs = u.seg();
+  a.mt = u.mt();
+  a.cloth = dp;
+  a.subh = subh;
+  a.kair = kair;
+  a.kd = kd;
+  a.x = d_x;
+  a.v = d_v;
+  a.ext = d_ext;
+  a.mass = d_mass;
+  a.fixed = d_fixed;
+  a.csr_off = d_csr_offsets;
+  a.csr = d_csr;
+  a.x_out = d_x_out;
+  a.v_out = d_v_out;
+  a.f_out = d_f_out;
+  a.nitems = num_vertices;
+  a.flags = d_anomaly;
+  if (!d_csr_offsets) a.cap = 0;
+  YSIM_TRY(launch_cloth(ctx->stream, a));
+  YSIM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return YSIM_OK;
+}
+
+// ------------------------------------------------------------------ scene
+int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_desc* h_meshes, const float* h_x,
+                          const float* h_v, const float* h_mass, const float* h_ext, const float* h_fixed,
+                          const uint32_t* h_facets, const ysim_sim_params* h_params, ysim_scene** out) {
+  YSIM_ARG(ctx && h_meshes && h_x && h_v && h_mass && h_ext && h_fixed && h_facets && h_params && out,
+           "scene_create: null argument");
+  YSIM_ARG(num_meshes > 0, "scene_create: no meshes");
+  const ysim_sim_params& P = *h_params;
+  YSIM_ARG(P.morton_bits == 30 || P.morton_bits == 63, "scene_create: morton_bits must be 30 or 63");
+  YSIM_ARG(P.broad_mode == 0, "scene_create: broad_mode 1 (spatial hash) is served by ysim_hip_hash_*");
+  *out = nullptr;
+  std::unique_ptr<ysim_scene> S(new ysim_scene());
+  S->ctx = ctx;
+  S->st = ctx->stream;
+  S->P = P;
+  S->formal_refit = ctx->formal_refit;
+  S->nm = num_meshes;
+  S->meshes.assign(h_meshes, h_meshes + num_meshes);
+  S->cap = P.contact_capacity ? P.contact_capacity : 16;
+  S->voff.assign(num_meshes + 1, 0);
+  S->foff.assign(num_meshes + 1, 0);
+  S->noff.assign(num_meshes + 1, 0);
+  for (uint32_t m = 0; m < num_meshes; ++m) {
+    const ysim_mesh_desc& d = S->meshes[m];
+    YSIM_ARG(d.behavior == YSIM_BEHAVIOR_FLOAT || d.behavior == YSIM_BEHAVIOR_FASTGRID_CLOTH,
+             "scene_create: unsupported behavior");
+    if (d.behavior == YSIM_BEHAVIOR_FASTGRID_CLOTH) {
+      YSIM_ARG(d.cloth.particle_num_1d >= 3 &&
+                   (uint64_t)d.cloth.particle_num_1d * d.cloth.particle_num_1d == d.num_vertices,
+               "scene_create: FastGrid cloth needs num_vertices = N^2, N >= 3");
+      S->dyn.push_back(m);
+    } else {
+      S->stat.push_back(m);
+    }
+    S->voff[m + 1] = S->voff[m] + d.num_vertices;
+    S->foff[m + 1] = S->foff[m] + d.num_facets;
+    S->noff[m + 1] = S->noff[m] + (d.num_facets ? 2 * d.num_facets - 1 : 0);
+  }
+  S->Vtot = S->voff[num_meshes];
+  S->Ttot = S->foff[num_meshes];
+  S->Ntot = S->noff[num_meshes];
+  YSIM_ARG((uint64_t)S->Vtot < (1ull << 31) && (uint64_t)S->Ntot < (1ull << 31), "scene_create: scene too large");
+  for (uint32_t i = 0; i < 3 * S->Ttot; ++i) {
+    uint32_t m = (uint32_t)(std::upper_bound(S->foff.begin(), S->foff.end(), i / 3) - S->foff.begin()) - 1;
+    YSIM_ARG(h_facets[i] < S->meshes[m].num_vertices, "scene_create: facet index out of range");
+  }
+  ysim_scene& s = *S;
+  const size_t V3 = 3 * (size_t)s.Vtot;
+  for (int b = 0; b < 2; ++b) {
+    YSIM_TRY(s.put(&s.x[b], h_x, V3));
+    YSIM_TRY(s.put(&s.v[b], h_v, V3));
+  }
+  YSIM_TRY(s.put(&s.mass, h_mass, s.Vtot));
+  YSIM_TRY(s.put(&s.ext, h_ext, V3));
+  YSIM_TRY(s.put(&s.fixed, h_fixed, s.Vtot));
+  YSIM_TRY(s.put(&s.facets, h_facets, 3 * (size_t)s.Ttot));
+  uint32_t *dv, *df, *dn;
+  YSIM_TRY(s.put(&dv, s.voff));
+  YSIM_TRY(s.put(&df, s.foff));
+  YSIM_TRY(s.put(&dn, s.noff));
+  s.mt = MeshTables{dv, df, dn};
+  std::vector<ysim_fastgrid_params> cloth(num_meshes);
+  std::vector<uint32_t> nf(num_meshes), beh(num_meshes), shp(num_meshes);
+  std::vector<float> margin(num_meshes), band(num_meshes);
+  for (uint32_t m = 0; m < num_meshes; ++m) {
+    cloth[m] = s.meshes[m].cloth;
+    nf[m] = s.meshes[m].num_facets;
+    beh[m] = s.meshes[m].behavior;
+    shp[m] = s.meshes[m].shape;
+    // margin <= 0: per-mesh margin and band = the cloth's thickness (batching)
+    margin[m] = P.margin > 0.f ? P.margin : s.meshes[m].cloth.thickness;
+    band[m] = P.radius + (P.margin > 0.f ? P.narrow_thickness : s.meshes[m].cloth.thickness);
+  }
+  YSIM_TRY(s.put(&s.d_cloth, cloth));
+  YSIM_TRY(s.put(&s.d_nfacets, nf));
+  YSIM_TRY(s.put(&s.d_behavior, beh));
+  YSIM_TRY(s.put(&s.d_shape, shp));
+  YSIM_TRY(s.put(&s.d_margin, margin));
+  YSIM_TRY(s.put(&s.d_band, band));
+  YSIM_TRY(s.alloc(&s.tree, s.Ntot));
+  YSIM_TRY(s.alloc(&s.parent, s.Ntot));
+  YSIM_TRY(s.alloc(&s.visit, s.Ntot));
+  YSIM_TRY(s.zero(s.tree, 8 * (size_t)s.Ntot));
+  YSIM_TRY(s.zero(s.visit, s.Ntot));
+  // segment tables
+  YSIM_TRY(s.make_groups(s.dyn, s.dyn_groups));
+  YSIM_TRY(s.make_groups(s.stat, s.stat_groups));
+  YSIM_TRY(s.make_seg(s.dyn, true, s.dyn_tsegs));
+  YSIM_TRY(s.make_seg(s.dyn, false, s.dyn_vsegs));
+  s.dyn_voff.assign(1, 0);
+  for (uint32_t m : s.dyn) {
+    s.Vdyn += s.meshes[m].num_vertices;
+    s.Tdyn += s.meshes[m].num_facets;
+    s.dyn_voff.push_back(s.Vdyn);
+  }
+  // targets per query (dynamic) mesh: collidable meshes of the same group
+  std::vector<uint32_t> toff(1, 0), tg;
+  for (uint32_t q : s.dyn) {
+    const ysim_mesh_desc& Q = s.meshes[q];
+    if (Q.collidable) {
+      for (uint32_t t = 0; t < num_meshes; ++t) {
+        const ysim_mesh_desc& T = s.meshes[t];
+        if (!T.collidable || T.group != Q.group || T.num_facets == 0) continue;
+        if (t == q && !P.self_collision) continue;
+        tg.push_back(t);
+      }
+    }
+    toff.push_back((uint32_t)tg.size());
+  }
+  YSIM_TRY(s.put(&s.d_tgt_off, toff));
+  YSIM_TRY(s.put(&s.d_tgt, tg));
+  // sort buffers sized for the largest group
+  size_t maxT = 0, maxStatus = 0, maxSeg = 1;
+  for (auto* gs : {&s.dyn_groups, &s.stat_groups})
+    for (auto& g : *gs) {
+      maxT = std::max<size_t>(maxT, g.T);
+      maxStatus = std::max(maxStatus, g.plan.status_words);
+      maxSeg = std::max<size_t>(maxSeg, g.tsegs.nseg);
+    }
+  for (int b = 0; b < 2; ++b) {
+    uint64_t* k = nullptr;
+    YSIM_TRY(s.alloc(&k, maxT));
+    s.keys[b] = k;
+    YSIM_TRY(s.alloc(&s.vals[b], maxT));
+  }
+  YSIM_TRY(s.alloc(&s.hist, SORT_MAX_PASSES * 256));
+  YSIM_TRY(s.alloc(&s.status, maxStatus));
+  YSIM_TRY(s.alloc(&s.counters, SORT_MAX_PASSES));
+  YSIM_TRY(s.alloc(&s.box_acc, 6 * maxSeg));
+  YSIM_TRY(s.zero(s.hist, SORT_MAX_PASSES * 256));
+  {
+    std::vector<uint32_t> init(6 * maxSeg);
+    for (size_t i = 0; i < init.size(); ++i) init[i] = (i % 6) < 3 ? 0xffffffffu : 0u;
+    YSIM_HIP_TRY(hipMemcpyAsync(s.box_acc, init.data(), init.size() * 4, hipMemcpyHostToDevice, s.st));
+  }
+  YSIM_TRY(s.alloc(&s.ccount, s.Vdyn));
+  YSIM_TRY(s.alloc(&s.cnd, (size_t)s.Vdyn * s.cap));
+  YSIM_TRY(s.alloc(&s.cid, (size_t)s.Vdyn * s.cap));
+  YSIM_TRY(s.zero(s.ccount, s.Vdyn));
+  YSIM_TRY(s.alloc(&s.npairs, 4));
+  YSIM_TRY(s.alloc(&s.flags, 4));
+  YSIM_TRY(s.zero(s.npairs, 4));
+  YSIM_TRY(s.zero(s.flags, 4));
+  for (auto& e : s.ev) YSIM_HIP_TRY(hipEventCreate(&e));
+  // initial build of every tree (unswept), Simulator::initialize -> broadPhase.build
+  for (auto* gs : {&s.stat_groups, &s.dyn_groups})
+    for (auto& g : *gs) {
+      YSIM_TRY(s.enqueue_build(g, s.x[0]));
+      YSIM_TRY(launch_refit(s.st, g.tsegs, s.mt, s.x[0], s.v[0], 0.f, false, s.facets, s.tree, s.parent, s.visit,
+                            g.T, s.flags, s.formal_refit));
+    }
+  YSIM_HIP_TRY(hipStreamSynchronize(s.st));
+  uint32_t fl = 0;
+  YSIM_HIP_TRY(hipMemcpy(&fl, s.flags, 4, hipMemcpyDeviceToHost));
+  if (fl) return fail(YSIM_ERR_STATE, "scene_create: initial build raised flags " + std::to_string(fl));
+  *out = S.release();
+  return YSIM_OK;
+}
+
+int ysim_hip_scene_destroy(ysim_scene* scene) {
+  if (!scene) return YSIM_OK;
+  hipStreamSynchronize(scene->st);
+  delete scene;
+  return YSIM_OK;
+}
+
+int ysim_hip_scene_step(ysim_scene* scene, uint32_t n, uint32_t use_graph) {
+  YSIM_ARG(scene, "null scene");
+  scene->timing = false;
+  return scene->steps(n, use_graph != 0);
+}
+
+int ysim_hip_scene_get_state(ysim_scene* s, float* h_x, float* h_v, float* h_xprev) {
+  YSIM_ARG(s, "null scene");
+  const size_t b = 12 * (size_t)s->Vtot;
+  if (h_x) YSIM_HIP_TRY(hipMemcpyAsync(h_x, s->x[s->parity], b, hipMemcpyDeviceToHost, s->st));
+  if (h_v) YSIM_HIP_TRY(hipMemcpyAsync(h_v, s->v[s->parity], b, hipMemcpyDeviceToHost, s->st));
+  if (h_xprev) YSIM_HIP_TRY(hipMemcpyAsync(h_xprev, s->x[s->parity ^ 1], b, hipMemcpyDeviceToHost, s->st));
+  YSIM_HIP_TRY(hipStreamSynchronize(s->st));
+  return YSIM_OK;
+}
+
+int ysim_hip_scene_set_state(ysim_scene* s, const float* h_x, const float* h_v, const float* h_xprev) {
+  YSIM_ARG(s, "null scene");
+  const size_t b = 12 * (size_t)s->Vtot;
+  if (h_x) YSIM_HIP_TRY(hipMemcpyAsync(s->x[s->parity], h_x, b, hipMemcpyHostToDevice, s->st));
+  if (h_v) YSIM_HIP_TRY(hipMemcpyAsync(s->v[s->parity], h_v, b, hipMemcpyHostToDevice, s->st));
+  if (h_xprev) YSIM_HIP_TRY(hipMemcpyAsync(s->x[s->parity ^ 1], h_xprev, b, hipMemcpyHostToDevice, s->st));
+  YSIM_HIP_TRY(hipStreamSynchronize(s->st));
+  return YSIM_OK;
+}
+
+int ysim_hip_scene_get_contacts(ysim_scene* s, uint32_t* h_offsets, ysim_narrow_collision* h_contacts,
+                                uint32_t capacity, uint32_t* h_count) {
+  YSIM_ARG(s && h_offsets && h_count, "scene_get_contacts: null argument");
+  std::vector<uint32_t> cnt(s->Vdyn);
+  std::vector<float4> nd((size_t)s->Vdyn * s->cap);
+  std::vector<uint2> id((size_t)s->Vdyn * s->cap);
+  YSIM_HIP_TRY(hipMemcpyAsync(cnt.data(), s->ccount, 4 * (size_t)s->Vdyn, hipMemcpyDeviceToHost, s->st));
+  YSIM_HIP_TRY(hipMemcpyAsync(nd.data(), s->cnd, 16 * nd.size(), hipMemcpyDeviceToHost, s->st));
+  YSIM_HIP_TRY(hipMemcpyAsync(id.data(), s->cid, 8 * id.size(), hipMemcpyDeviceToHost, s->st));
+  uint32_t fl = 0;
+  YSIM_HIP_TRY(hipMemcpyAsync(&fl, s->flags, 4, hipMemcpyDeviceToHost, s->st));
+  YSIM_HIP_TRY(hipStreamSynchronize(s->st));
+  if (fl & 2u) return fail(YSIM_ERR_CAPACITY, "scene_get_contacts: a vertex exceeded contact_capacity");
+  std::fill(h_offsets, h_offsets + s->Vtot + 1, 0u);
+  uint32_t total = 0;
+  for (size_t k = 0; k < s->dyn.size(); ++k) {
+    const uint32_t m = s->dyn[k];
+    for (uint32_t p = 0; p < s->meshes[m].num_vertices; ++p) {
+      const uint32_t j = s->dyn_voff[k] + p;
+      const uint32_t g = s->voff[m] + p;
+      h_offsets[g + 1] = std::min(cnt[j], s->cap);
+    }
+  }
+  for (uint32_t g = 0; g < s->Vtot; ++g) h_offsets[g + 1] += h_offsets[g];
+  total = h_offsets[s->Vtot];
+  *h_count = total;
+  if (!h_contacts) return YSIM_OK;
+  if (total > capacity) return fail(YSIM_ERR_CAPACITY, "scene_get_contacts: capacity too small");
+  for (size_t k = 0; k < s->dyn.size(); ++k) {
+    const uint32_t m = s->dyn[k];
+    for (uint32_t p = 0; p < s->meshes[m].num_vertices; ++p) {
+      const uint32_t j = s->dyn_voff[k] + p;
+      const uint32_t g = s->voff[m] + p;
+      const uint32_t c = std::min(cnt[j], s->cap);
+      std::vector<uint32_t> order(c);
+      std::iota(order.begin(), order.end(), 0u);
+      std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        const uint2 A = id[(size_t)j * s->cap + a], B = id[(size_t)j * s->cap + b];
+        return A.x != B.x ? A.x < B.x : A.y < B.y;
+      });
+      for (uint32_t r = 0; r < c; ++r) {
+        const size_t src = (size_t)j * s->cap + order[r];
+        ysim_narrow_collision& o = h_contacts[h_offsets[g] + r];
+        o.point = p;
+        o.triangle = id[src].y;
+        o.query_obj = m;
+        o.target_obj = id[src].x;
+        o.normal_distance[0] = nd[src].x;
+        o.normal_distance[1] = nd[src].y;
+        o.normal_distance[2] = nd[src].z;
+        o.normal_distance[3] = nd[src].w;
+        o.query_behavior = s->meshes[m].behavior;
+        o.target_behavior = s->meshes[id[src].x].behavior;
+        o.query_shape = s->meshes[m].shape;
+        o.target_shape = s->meshes[id[src].x].shape;
+      }
+    }
+  }
+  return YSIM_OK;
+}
+
+int ysim_hip_scene_get_pairs(ysim_scene* s, ysim_broad_collision* h_pairs, uint32_t capacity, uint32_t* h_count) {
+  YSIM_ARG(s && h_count, "scene_get_pairs: null argument");
+  s->timing = false;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    YSIM_TRY(s->enqueue_collision(s->parity, s->rebuild_at(s->step)));
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->st, s->npairs + 1, 1u);
+    QueryArgs qa = s->query_args(s->parity);
+    qa.npairs = s->npairs + 1;
+    qa.pairs = s->pairs;
+    qa.pair_cap = s->pair_cap;
+    YSIM_TRY(launch_query(s->st, qa, true));
+    uint32_t n = 0;
+    YSIM_HIP_TRY(hipMemcpyAsync(&n, s->npairs + 1, 4, hipMemcpyDeviceToHost, s->st));
+    YSIM_HIP_TRY(hipStreamSynchronize(s->st));
+    if (n <= s->pair_cap) {
+      *h_count = n;
+      if (!h_pairs) return YSIM_OK;
+      if (n > capacity) return fail(YSIM_ERR_CAPACITY, "scene_get_pairs: capacity too small");
+      std::vector<uint32_t> q(4 * (size_t)n);
+      if (n) YSIM_HIP_TRY(hipMemcpy(q.data(), s->pairs, 16 * (size_t)n, hipMemcpyDeviceToHost));
+      for (uint32_t i = 0; i < n; ++i) {
+        ysim_broad_collision& r = h_pairs[i];
+        r.query_obj = q[4 * i];
+        r.point = q[4 * i + 1];
+        r.target_obj = q[4 * i + 2];
+        r.triangle = q[4 * i + 3];
+        r.query_behavior = s->meshes[r.query_obj].behavior;
+        r.target_behavior = s->meshes[r.target_obj].behavior;
+        r.query_shape = s->meshes[r.query_obj].shape;
+        r.target_shape = s->meshes[r.target_obj].shape;
+      }
+      return YSIM_OK;
+    }
+    // grow the pair buffer and run again
+    if (s->pairs) {
+      YSIM_HIP_TRY(hipFree(s->pairs));
+      s->allocs.erase(std::remove(s->allocs.begin(), s->allocs.end(), (void*)s->pairs), s->allocs.end());
+    }
+    s->pair_cap = n + n / 4 + 1024;
+    YSIM_TRY(s->alloc(&s->pairs, 4 * (size_t)s->pair_cap));
+  }
+  return fail(YSIM_ERR_STATE, "scene_get_pairs: pair count unstable");
+}
+
+int ysim_hip_scene_get_tree(ysim_scene* s, uint32_t mesh, ysim_bvh_node* h_nodes, int32_t* h_parent,
+                            uint64_t* h_sorted_keys, uint32_t* h_sorted_idx) {
+  YSIM_ARG(s && mesh < s->nm, "scene_get_tree: bad mesh");
+  const uint32_t T = s->meshes[mesh].num_facets;
+  if (!T) return YSIM_OK;
+  const size_t nn = 2 * (size_t)T - 1;
+  if (h_nodes) YSIM_HIP_TRY(hipMemcpyAsync(h_nodes, s->tree + s->noff[mesh], nn * 32, hipMemcpyDeviceToHost, s->st));
+  if (h_parent) YSIM_HIP_TRY(hipMemcpyAsync(h_parent, s->parent + s->noff[mesh], nn * 4, hipMemcpyDeviceToHost, s->st));
+  YSIM_HIP_TRY(hipStreamSynchronize(s->st));
+  if (h_sorted_keys || h_sorted_idx) {
+    // the sort buffers hold the last build of the first dynamic group
+    if (s->dyn_groups.empty()) return fail(YSIM_ERR_STATE, "scene_get_tree: no dynamic build");
+    const BuildGroup& g = s->dyn_groups[0];
+    auto it = std::find(g.meshes.begin(), g.meshes.end(), mesh);
+    if (it == g.meshes.end()) return fail(YSIM_ERR_STATE, "scene_get_tree: keys of this mesh are not resident");
+    uint32_t start = 0;
+    for (auto m = g.meshes.begin(); m != it; ++m) start += s->meshes[*m].num_facets;
+    const uint64_t mask = (g.code_bits >= 64) ? ~0ull : ((1ull << g.code_bits) - 1);
+    if (h_sorted_keys) {
+      if (g.key64) {
+        YSIM_HIP_TRY(hipMemcpy(h_sorted_keys, (uint64_t*)s->keys[s->last_sorted] + start, 8 * (size_t)T,
+                               hipMemcpyDeviceToHost));
+      } else {
+        std::vector<uint32_t> k(T);
+        YSIM_HIP_TRY(hipMemcpy(k.data(), (uint32_t*)s->keys[s->last_sorted] + start, 4 * (size_t)T,
+                               hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < T; ++i) h_sorted_keys[i] = k[i];
+      }
+      for (uint32_t i = 0; i < T; ++i) h_sorted_keys[i] &= mask;
+    }
+    if (h_sorted_idx)
+      YSIM_HIP_TRY(hipMemcpy(h_sorted_idx, s->vals[s->last_sorted] + start, 4 * (size_t)T, hipMemcpyDeviceToHost));
+  }
+  return YSIM_OK;
+}
+
+int ysim_hip_scene_stage_timings(ysim_scene* s, uint32_t n_steps, float* h_ms) {
+  YSIM_ARG(s && h_ms, "scene_stage_timings: null argument");
+  for (double& d : s->stage_ms) d = 0.0;
+  s->timing = true;
+  int rc = s->eager_steps(n_steps);
+  s->timing = false;
+  if (rc != YSIM_OK) return rc;
+  for (int k = 0; k < YSIM_STAGE_COUNT; ++k) h_ms[k] = n_steps ? (float)(s->stage_ms[k] / n_steps) : 0.f;
+  return YSIM_OK;
+}
+
+int ysim_hip_scene_flags(ysim_scene* s, uint32_t* h_flags) {
+  YSIM_ARG(s && h_flags, "scene_flags: null argument");
+  YSIM_HIP_TRY(hipMemcpyAsync(h_flags, s->flags, 4, hipMemcpyDeviceToHost, s->st));
+  YSIM_HIP_TRY(hipStreamSynchronize(s->st));
+  return YSIM_OK;
+}
+
+}  // extern "C"

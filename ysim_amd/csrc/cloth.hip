@@ -1,0 +1,146 @@
+// cloth.hip — FastGrid springs + symplectic integrate + contact projection,
+// fused into one pass per vertex.
+//
+// compute_cloth_grid_forces_fast (src/metal/physics.metal:171-224) gathers 12
+// grid springs; integrate_cloth_grid (physics.metal:226-312) then updates v,
+// applies the contact bucket (dedup by target facet, gate distance < thickness,
+// drop the inward normal velocity, push out to thickness), updates x and runs
+// the world-bounds guard.  Forces read (x, v) and the results go to separate
+// buffers (x_out, v_out), so the two reference passes fuse without a race and
+// the old x buffer becomes the next substep's xPrev (snapshot_xprev_range,
+// physics.metal:520-529) without a copy.
+#include "kernels.hpp"
+
+namespace ysim {
+
+namespace {
+// Per-spring index of a FastGrid cloth of side N (families in the order
+// struct-x, struct-y, shear-a, shear-b, bend-x, bend-y; see the adjoint API).
+struct SpringIdx {
+  uint32_t N, SY, SA, SB, BX, BY;
+  __device__ explicit SpringIdx(uint32_t n) : N(n) {
+    SY = N * (N - 1);
+    SA = SY + (N - 1) * N;
+    SB = SA + (N - 1) * (N - 1);
+    BX = SB + (N - 1) * (N - 1);
+    BY = BX + N * (N - 2);
+  }
+};
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_cloth(ClothArgs a) {
+  const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= a.nitems) return;
+  const uint32_t sg = seg_of(a.dsegs, j);
+  const uint32_t m = a.dsegs.mesh[sg];
+  const uint32_t i = j - a.dsegs.off[sg];
+  const uint32_t vb = a.mt.voff[m];
+  const uint32_t g = vb + i;
+  const ysim_fastgrid_params P = a.cloth[m];
+  const uint32_t N = P.particle_num_1d;
+  const uint32_t row = i / N, col = i % N;
+  const float* X = a.x;
+  const float* V = a.v;
+  const f3 pos0 = ld3(X, g), vel0 = ld3(V, g);
+  const float kd = a.kd;
+  const float* KS = (a.kspring && sg == 0) ? a.kspring : nullptr;
+  const SpringIdx S(N);
+
+  f3 force = ld3(a.ext, g) + vel0 * a.kair;
+  auto nb = [&](uint32_t r, uint32_t c, float rest, float k, uint32_t sidx) {
+    const uint32_t q = vb + r * N + c;
+    const float kk = KS ? KS[sidx] : k;
+    force = force + calc_spring(pos0, vel0, ld3(X, q), ld3(V, q), rest, kk, kd);
+  };
+  // stretch, shear, bend in the reference's fixed order (physics.metal:195-218)
+  if (col > 0) nb(row, col - 1, P.stretch_rest_x, P.kstretch, row * (N - 1) + col - 1);
+  if (col < N - 1) nb(row, col + 1, P.stretch_rest_x, P.kstretch, row * (N - 1) + col);
+  if (row > 0) nb(row - 1, col, P.stretch_rest_y, P.kstretch, S.SY + (row - 1) * N + col);
+  if (row < N - 1) nb(row + 1, col, P.stretch_rest_y, P.kstretch, S.SY + row * N + col);
+  if (col > 0 && row > 0) nb(row - 1, col - 1, P.shear_rest_a, P.kshear, S.SA + (row - 1) * (N - 1) + col - 1);
+  if (col < N - 1 && row > 0) nb(row - 1, col + 1, P.shear_rest_b, P.kshear, S.SB + (row - 1) * (N - 1) + col);
+  if (col > 0 && row < N - 1) nb(row + 1, col - 1, P.shear_rest_b, P.kshear, S.SB + row * (N - 1) + col - 1);
+  if (col < N - 1 && row < N - 1) nb(row + 1, col + 1, P.shear_rest_a, P.kshear, S.SA + row * (N - 1) + col);
+  if (col > 1) nb(row, col - 2, P.bend_rest_x, P.kbend, S.BX + row * (N - 2) + col - 2);
+  if (col < N - 2) nb(row, col + 2, P.bend_rest_x, P.kbend, S.BX + row * (N - 2) + col);
+  if (row > 1) nb(row - 2, col, P.bend_rest_y, P.kbend, S.BY + (row - 2) * N + col);
+  if (row < N - 2) nb(row + 2, col, P.bend_rest_y, P.kbend, S.BY + row * N + col);
+  if (a.f_out) st3(a.f_out, g, force);
+
+  // integrate_cloth_grid
+  const float mask = a.fixed[g];
+  const float mass = a.mass[g];
+  const float subh = a.subh;
+  f3 pos = pos0, vel = vel0;
+  vel.x = vel.x + ((subh * force.x) / mass) * mask;
+  vel.y = vel.y + ((subh * force.y) / mass) * mask;
+  vel.z = vel.z + ((subh * force.z) / mass) * mask;
+
+  const float thickness = P.thickness;
+  auto apply = [&](float4 nd) {
+    f3 n = {nd.x, nd.y, nd.z};
+    float nlen2 = dot(n, n);
+    if (nlen2 < 1e-12f) return;
+    float inv = 1.0f / sqrtf(nlen2);
+    n = n * inv;
+    float vn = dot(vel, n);
+    float dist = nd.w;
+    if (dist < thickness) {
+      if (vn < 0.0f) vel = vel - vn * n;
+      pos = pos + (thickness - dist) * n;
+    }
+  };
+  if (a.csr_off) {
+    // bucket already ordered; skip repeats of a (target, facet) (physics.metal:257-271)
+    const uint32_t b = a.csr_off[g], e = a.csr_off[g + 1];
+    for (uint32_t c = b; c < e; ++c) {
+      bool dup = false;
+      for (uint32_t d = b; d < c; ++d)
+        if (a.csr[d].target_obj == a.csr[c].target_obj && a.csr[d].triangle == a.csr[c].triangle) {
+          dup = true;
+          break;
+        }
+      if (dup) continue;
+      const float* q = a.csr[c].normal_distance;
+      apply(make_float4(q[0], q[1], q[2], q[3]));
+    }
+  } else {
+    uint32_t cnt = a.ccount ? a.ccount[j] : 0u;
+    if (cnt > a.cap) cnt = a.cap;
+    const float4* ND = a.cnd + (size_t)j * a.cap;
+    const uint2* ID = a.cid + (size_t)j * a.cap;
+    // visit contacts in ascending (target mesh, facet) order; equal keys are
+    // duplicates and are applied once (physics.metal:257-271)
+    uint64_t last = 0;
+    bool first = true;
+    for (uint32_t it = 0; it < cnt; ++it) {
+      uint64_t best = ~0ull;
+      uint32_t bi = 0;
+      for (uint32_t c = 0; c < cnt; ++c) {
+        uint2 id = ID[c];
+        uint64_t key = ((uint64_t)id.x << 32) | id.y;
+        if ((first || key > last) && key < best) {
+          best = key;
+          bi = c;
+        }
+      }
+      if (best == ~0ull) break;
+      first = false;
+      last = best;
+      apply(ND[bi]);
+    }
+  }
+  pos = pos + (subh * vel) * mask;
+  if (sanitize(pos, vel, pos0)) atomicOr(a.flags, 1u);
+  st3(a.v_out, g, vel);
+  st3(a.x_out, g, pos);
+}
+
+int launch_cloth(hipStream_t s, const ClothArgs& a) {
+  if (!a.nitems) return YSIM_OK;
+  hipLaunchKernelGGL(k_cloth, dim3(ceil_div(a.nitems, 256)), dim3(256), 0, s, a);
+  YSIM_CHECK_LAUNCH();
+  return YSIM_OK;
+}
+
+}  // namespace ysim

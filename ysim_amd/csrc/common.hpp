@@ -1,0 +1,248 @@
+// common.hpp — device helpers shared by every kernel of the gfx950 cloth
+// library, plus the host-side context and error plumbing.
+//
+// Arithmetic contract: every translation unit is compiled with
+// -ffp-contract=off and without fast-math; IEEE division and sqrt are the HIP
+// defaults.  The FP32 results are therefore bit-identical to a strict-IEEE CPU
+// evaluation of the same operation sequence.  Each helper states the reference
+// kernel it restates.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/ysim_hip.h"
+
+static_assert(sizeof(ysim_aabb4) == 32, "AABB4 layout");
+static_assert(sizeof(ysim_bvh_node) == 32, "BVHNode layout");
+static_assert(sizeof(ysim_broad_collision) == 32, "BroadCollision layout");
+static_assert(sizeof(ysim_narrow_collision) == 48, "NarrowCollision layout");
+static_assert(sizeof(ysim_fastgrid_params) == 44, "FastGridClothBehaviorParams layout");
+
+namespace ysim {
+
+// ------------------------------------------------------------------ errors
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define YSIM_HIP_TRY(expr)                                                        \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return ::ysim::fail(YSIM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define YSIM_CHECK_LAUNCH() YSIM_HIP_TRY(hipGetLastError())
+
+// ------------------------------------------------------------------ context
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 256;
+  bool formal_refit = false;  // YSIM_CTX_FORMAL_REFIT
+  // scratch reused by stage entry points (sort, counters)
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  int ensure_scratch(size_t bytes);
+};
+
+// ------------------------------------------------------------------ vectors
+struct f3 {
+  float x, y, z;
+};
+struct __attribute__((packed, aligned(4))) pf3 {
+  float x, y, z;
+};
+
+__device__ __forceinline__ f3 ld3(const float* a, uint32_t i) {
+  pf3 v = reinterpret_cast<const pf3*>(a)[i];
+  return {v.x, v.y, v.z};
+}
+__device__ __forceinline__ void st3(float* a, uint32_t i, f3 v) {
+  reinterpret_cast<pf3*>(a)[i] = pf3{v.x, v.y, v.z};
+}
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 operator*(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ f3 vmin(f3 a, f3 b) { return {fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)}; }
+__device__ __forceinline__ f3 vmax(f3 a, f3 b) { return {fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)}; }
+
+// ------------------------------------------------------------------ ordered float keys
+// Monotone float <-> uint32 map: a < b (with -0 < +0) iff key(a) < key(b), so
+// atomicMin/atomicMax on keys give the exact float min/max in any order.
+__device__ __forceinline__ uint32_t fkey(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float funkey(uint32_t k) {
+  uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+  return __uint_as_float(u);
+}
+
+// ------------------------------------------------------------------ Morton
+// expandBits, src/metal/bvh.metal:26-32.
+__device__ __forceinline__ uint32_t expand10(uint32_t v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+// 21 -> 63 bit spread (the 63-bit variant of this build).
+__device__ __forceinline__ uint64_t expand21(uint64_t v) {
+  v &= 0x1fffffull;
+  v = (v | (v << 32)) & 0x001f00000000ffffull;
+  v = (v | (v << 16)) & 0x001f0000ff0000ffull;
+  v = (v | (v << 8)) & 0x100f00f00f00f00full;
+  v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+  v = (v | (v << 2)) & 0x1249249249249249ull;
+  return v;
+}
+// mortonCode + fillMortons_Tri normalisation, src/metal/bvh.metal:34-63.
+__device__ __forceinline__ uint64_t morton_of(f3 x0, f3 x1, f3 x2, f3 bmin, f3 bmax, bool wide) {
+  float cen[3] = {((x0.x + x1.x) + x2.x) / 3.0f, ((x0.y + x1.y) + x2.y) / 3.0f,
+                  ((x0.z + x1.z) + x2.z) / 3.0f};
+  float lo[3] = {bmin.x, bmin.y, bmin.z}, hi[3] = {bmax.x, bmax.y, bmax.z};
+  uint64_t e[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float width = fmaxf(hi[k] - lo[k], 1e-8f);
+    float c = (cen[k] - lo[k]) / width;
+    if (!wide) {
+      float p = fminf(fmaxf(c * 1024.0f, 0.0f), 1023.0f);
+      e[k] = expand10((uint32_t)p);
+    } else {
+      float p = fminf(fmaxf(c * 2097152.0f, 0.0f), 2097151.0f);
+      e[k] = expand21((uint64_t)(uint32_t)p);
+    }
+  }
+  if (!wide) return e[0] * 4 + e[1] * 2 + e[2];
+  return (e[0] << 2) | (e[1] << 1) | e[2];
+}
+
+// ------------------------------------------------------------------ narrow
+// pointInTriangleBary, src/metal/bruteforce.metal:9-28.
+__device__ __forceinline__ bool point_in_tri(f3 p, f3 v0, f3 v1) {
+  float d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1), d20 = dot(p, v0), d21 = dot(p, v1);
+  float denom = d00 * d11 - d01 * d01;
+  if (fabsf(denom) < 1e-12f) return false;
+  float b = (d11 * d20 - d01 * d21) / denom;
+  float c = (d00 * d21 - d01 * d20) / denom;
+  float a = 1.0f - b - c;
+  return a >= 0.0f && b >= 0.0f && c >= 0.0f;
+}
+
+// narrow_pt_tri geometry, src/metal/bruteforce.metal:114-158: swept segment
+// [qprev, qcur] against the triangle plane; n oriented toward qprev's side;
+// fires when crossed or d_cur < band; contact point projected into the plane
+// must lie inside (barycentric).  rsqrt is restated as 1/sqrt (IEEE).
+__device__ __forceinline__ bool narrow_test(f3 qcur, f3 qprev, f3 t0, f3 t1, f3 t2, float band,
+                                            f3& n_out, float& d_out) {
+  f3 v0 = t1 - t0, v1 = t2 - t0;
+  f3 n = cross(v0, v1);
+  float nlen2 = dot(n, n);
+  if (nlen2 < 1e-12f) return false;
+  float inv = 1.0f / sqrtf(nlen2);
+  n = n * inv;
+  float dprev = dot(n, qprev - t0);
+  float dcur = dot(n, qcur - t0);
+  float dref = (dprev != 0.0f) ? dprev : dcur;
+  if (dref < 0.0f) {
+    n = f3{-n.x, -n.y, -n.z};
+    dprev = -dprev;
+    dcur = -dcur;
+  }
+  bool crossed = dcur < 0.0f;
+  bool inmargin = dcur < band;
+  if (!crossed && !inmargin) return false;
+  f3 cp;
+  if (crossed && dprev > 0.0f) {
+    float t = dprev / (dprev - dcur);
+    cp = qprev + (qcur - qprev) * t;
+  } else {
+    cp = qcur;
+  }
+  f3 pc = cp - t0;
+  float l = dot(n, pc);
+  f3 inplane = pc - n * l;
+  if (!point_in_tri(inplane, v0, v1)) return false;
+  n_out = n;
+  d_out = dcur;
+  return true;
+}
+
+// intersectAABB, src/metal/bvh.metal:1285-1290 (closed intervals).
+__device__ __forceinline__ bool overlap(f3 qmin, f3 qmax, f3 nmin, f3 nmax) {
+  return !(qmax.x < nmin.x || qmin.x > nmax.x || qmax.y < nmin.y || qmin.y > nmax.y ||
+           qmax.z < nmin.z || qmin.z > nmax.z);
+}
+
+// ------------------------------------------------------------------ springs
+// calc_spring, src/metal/physics.metal:146-153.
+__device__ __forceinline__ f3 calc_spring(f3 p0, f3 v0, f3 p1, f3 v1, float rest, float ks, float kd) {
+  f3 dx = p1 - p0;
+  f3 dv = v1 - v0;
+  float len = sqrtf(dot(dx, dx));
+  if (len < 1e-7f) return f3{0.f, 0.f, 0.f};
+  f3 ndx = {dx.x / len, dx.y / len, dx.z / len};
+  float s = ks * (len - rest) + kd * dot(dv, ndx);
+  return s * ndx;
+}
+
+// nonFinite3 / sanitizeIntegrateOutput, src/metal/physics.metal:40-74.
+__device__ __forceinline__ bool nonfinite3(f3 p) {
+  return ((__float_as_uint(p.x) & 0x7F800000u) == 0x7F800000u) ||
+         ((__float_as_uint(p.y) & 0x7F800000u) == 0x7F800000u) ||
+         ((__float_as_uint(p.z) & 0x7F800000u) == 0x7F800000u);
+}
+__device__ __forceinline__ bool sanitize(f3& pos, f3& vel, f3 fb) {
+  const float B = 1.0e4f;
+  bool touched = false;
+  if (nonfinite3(pos)) {
+    pos = nonfinite3(fb) ? f3{0.f, 0.f, 0.f} : fb;
+    vel = f3{0.f, 0.f, 0.f};
+    touched = true;
+  }
+  if (nonfinite3(vel)) {
+    vel = f3{0.f, 0.f, 0.f};
+    touched = true;
+  }
+  f3 bx = {fminf(fmaxf(pos.x, -B), B), fminf(fmaxf(pos.y, -B), B), fminf(fmaxf(pos.z, -B), B)};
+  if (bx.x != pos.x || bx.y != pos.y || bx.z != pos.z) {
+    pos = bx;
+    vel = f3{0.f, 0.f, 0.f};
+    touched = true;
+  }
+  return touched;
+}
+
+// ------------------------------------------------------------------ coherent access
+// Device-scope (cross-XCD) coherent loads/stores used by the atomic bottom-up
+// and the decoupled look-back: relaxed agent-scope atomics compile to sc1
+// accesses that bypass the non-coherent L2 state of other XCDs.
+__device__ __forceinline__ void st_agent(float* p, float v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float ld_agent(const float* p) {
+  return __hip_atomic_load(const_cast<float*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t ld_agent_u32(const uint32_t* p) {
+  return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent_u32(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Wait until every outstanding memory operation of this wave has completed.
+__device__ __forceinline__ void wait_all_mem() { __builtin_amdgcn_s_waitcnt(0); }
+
+inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace ysim

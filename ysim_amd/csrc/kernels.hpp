@@ -1,0 +1,130 @@
+// kernels.hpp — device-side tables and host launchers of the LBVH, query and
+// cloth kernels.  Every kernel is "segmented": one launch covers all meshes of
+// a scene (one segment per mesh), so the plain single-cloth substep and the
+// batched multi-instance substep run the same code.
+#pragma once
+#include "common.hpp"
+#include "sort.hpp"
+
+namespace ysim {
+
+// A ragged range table: items [off[k], off[k+1]) belong to segment k, whose
+// mesh is mesh[k].
+struct SegTable {
+  uint32_t nseg = 0;
+  const uint32_t* off = nullptr;   // device, nseg + 1
+  const uint32_t* mesh = nullptr;  // device, nseg
+};
+
+__device__ __forceinline__ uint32_t seg_of(const SegTable& s, uint32_t j) {
+  if (s.nseg <= 1) return 0;
+  uint32_t lo = 0, hi = s.nseg;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (s.off[mid] <= j) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Per-mesh offsets into the packed scene arrays (device, nm + 1 each).
+struct MeshTables {
+  const uint32_t* voff = nullptr;  // vertices
+  const uint32_t* foff = nullptr;  // facets
+  const uint32_t* noff = nullptr;  // BVH node windows (2T - 1 per mesh)
+};
+
+// ---------------------------------------------------------------- lbvh.hip
+// S0: per-segment exact min/max of vertex positions into box_acc[seg*6] (ordered keys).
+int launch_box(hipStream_t s, SegTable vsegs, MeshTables mt, const float* x, uint32_t nitems,
+               uint32_t* box_acc);
+// S1: Morton keys of every triangle of tsegs (segment id prefixed above the code
+// bits when nseg > 1), values = mesh-local facet id, plus the digit histograms
+// of all sort passes; also zeroes the sort look-back state.
+int launch_morton(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x,
+                  const uint32_t* facets, const uint32_t* box_acc, uint32_t code_bits,
+                  bool key64, void* keys, uint32_t* vals, uint32_t nitems, const SortPlan& plan,
+                  uint32_t* hist, uint32_t* status, uint32_t* counters);
+// S3: Karras hierarchy per segment; leaf child links; zeroes the refit visit
+// counters; resets box_acc (nbox segments) and the sort histogram for the next build.
+int launch_hierarchy(hipStream_t s, SegTable tsegs, MeshTables mt, const void* keys, bool key64,
+                     const uint32_t* vals, uint32_t code_bits, ysim_bvh_node* tree, int32_t* parent,
+                     uint32_t* visit, uint32_t nitems, uint32_t* box_acc, uint32_t nbox,
+                     uint32_t* hist, uint32_t hist_words);
+// S4+S5: leaf boxes (swept optional) + atomic bottom-up union per segment.
+int launch_refit(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x, const float* v,
+                 float dt, bool swept, const uint32_t* facets, ysim_bvh_node* tree,
+                 const int32_t* parent, uint32_t* visit, uint32_t nitems, uint32_t* err, bool formal);
+
+// ---------------------------------------------------------------- query.hip
+struct QueryArgs {
+  SegTable qsegs;                  // query vertex segments (one per dynamic mesh)
+  MeshTables mt;
+  const uint32_t* tgt_off = nullptr;  // per query segment: [tgt_off[k], tgt_off[k+1]) into tgt
+  const uint32_t* tgt = nullptr;      // target mesh ids
+  const uint32_t* mesh_nfacets = nullptr;
+  const float* mesh_margin = nullptr; // per query mesh broad half-width
+  const float* mesh_band = nullptr;   // per query mesh narrow band (radius + thickness)
+  const uint32_t* mesh_behavior = nullptr;
+  const uint32_t* mesh_shape = nullptr;
+  const float* x = nullptr;
+  const float* xprev = nullptr;
+  const uint32_t* facets = nullptr;
+  const ysim_bvh_node* tree = nullptr;
+  const int32_t* parent = nullptr;
+  uint32_t nitems = 0;                // total query vertices
+  uint32_t cap = 16;                  // contact slots per vertex
+  uint32_t* ccount = nullptr;         // per query vertex contact count
+  float4* cnd = nullptr;              // [nitems * cap] normal + distance
+  uint2* cid = nullptr;               // [nitems * cap] (target mesh, facet)
+  uint32_t* npairs = nullptr;         // running candidate-pair counter (stats)
+  uint32_t* flags = nullptr;          // bit1 contact overflow, bit2 traversal guard
+  // pair-emission mode (parity / drop-in broad phase)
+  uint32_t* pairs = nullptr;          // 4 words per pair (qobj, point, tobj, tri)
+  uint32_t pair_cap = 0;
+};
+int launch_query(hipStream_t s, const QueryArgs& a, bool emit_pairs);
+// Drop-in broad query of one (query mesh, target tree): BroadCollision records.
+int launch_broad_pairs(hipStream_t s, const float* xq, uint32_t nq, uint32_t qobj, uint32_t qbeh,
+                       uint32_t qshape, const ysim_bvh_node* tree, const int32_t* parent, uint32_t nT,
+                       const uint32_t* tfacets, uint32_t tobj, uint32_t tbeh, uint32_t tshape,
+                       float margin, ysim_broad_collision* out, uint32_t cap, uint32_t* count,
+                       uint32_t* flags);
+// Drop-in narrow phase over broad pairs (NarrowCollision records, atomic append).
+int launch_narrow_pairs(hipStream_t s, const ysim_broad_collision* pairs, uint32_t np, const float* x,
+                        const float* xprev, const uint32_t* states_off, const uint32_t* facets,
+                        const uint32_t* facets_off, const uint32_t* adj_off, const uint32_t* adj,
+                        float band, ysim_narrow_collision* out, uint32_t cap, uint32_t* count);
+// Contact CSR: per-vertex buckets ordered by (target_obj, triangle).
+int launch_contact_csr(hipStream_t s, const ysim_narrow_collision* c, uint32_t nc,
+                       const uint32_t* states_off, uint32_t nv, uint32_t* offsets,
+                       ysim_narrow_collision* sorted, uint32_t* scratch /* nv + 1024 words */);
+
+// ---------------------------------------------------------------- cloth.hip
+struct ClothArgs {
+  SegTable dsegs;                     // dynamic vertex segments (== query segments)
+  MeshTables mt;
+  const ysim_fastgrid_params* cloth = nullptr;  // per mesh
+  const float* kspring = nullptr;     // optional per-spring k of the first cloth
+  float subh = 0, kair = 0, kd = 0;
+  const float* x = nullptr;
+  const float* v = nullptr;
+  const float* ext = nullptr;
+  const float* mass = nullptr;
+  const float* fixed = nullptr;
+  const uint32_t* ccount = nullptr;
+  const float4* cnd = nullptr;
+  const uint2* cid = nullptr;
+  uint32_t cap = 16;
+  // alternative contact source: a CSR ordered by (target mesh, facet) per
+  // vertex (the drop-in cloth_step entry point); used when csr_off != nullptr
+  const uint32_t* csr_off = nullptr;
+  const ysim_narrow_collision* csr = nullptr;
+  float* x_out = nullptr;
+  float* v_out = nullptr;
+  float* f_out = nullptr;             // optional
+  uint32_t nitems = 0;
+  uint32_t* flags = nullptr;          // bit0 anomaly
+};
+int launch_cloth(hipStream_t s, const ClothArgs& a);
+
+}  // namespace ysim

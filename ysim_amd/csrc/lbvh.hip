@@ -1,0 +1,392 @@
+// lbvh.hip — LBVH construction and refit for gfx950.
+//
+//   k_box       S0  scene box per mesh (BVH::build CPU loop, bvh.hpp:1229-1231)
+//   k_morton    S1  Morton keys (fillMortons_Tri, bvh.metal:42-63) + sort histograms
+//   k_hierarchy S3  Karras hierarchy (buildTree_Tri, bvh.metal:197-258, 352-393)
+//   k_refit     S4+S5 leaf boxes (buildLeaf/buildSweptLeaf_Tri, bvh.metal:260-351)
+//                   + atomic bottom-up union (bottomUpBoxes, bvh.metal:464-586)
+//
+// All kernels are segmented (one segment per mesh) and write each mesh's tree
+// into its own node window with mesh-local child indices, so every tree is
+// identical to a standalone per-mesh tree of the reference.
+#include <algorithm>
+#include <cmath>
+
+#include "kernels.hpp"
+
+namespace ysim {
+
+namespace {
+
+constexpr uint32_t BOX_ITEMS = 8;
+constexpr uint32_t MORTON_ITEMS = 8;
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__device__ __forceinline__ void flush_box(uint32_t* acc, uint32_t seg, f3 lo, f3 hi) {
+  uint32_t* a = acc + seg * 6;
+  atomicMin(a + 0, fkey(lo.x));
+  atomicMin(a + 1, fkey(lo.y));
+  atomicMin(a + 2, fkey(lo.z));
+  atomicMax(a + 3, fkey(hi.x));
+  atomicMax(a + 4, fkey(hi.y));
+  atomicMax(a + 5, fkey(hi.z));
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ S0 box
+__global__ __launch_bounds__(256) void k_box(SegTable vs, MeshTables mt, const float* __restrict__ x,
+                                             uint32_t n, uint32_t* __restrict__ acc) {
+  __shared__ float s_red[6][4];
+  const uint32_t start = blockIdx.x * (256 * BOX_ITEMS);
+  if (start >= n) return;
+  const uint32_t end = min(start + 256 * BOX_ITEMS, n);
+  const uint32_t s0 = seg_of(vs, start), s1 = seg_of(vs, end - 1);
+  const float INF = __uint_as_float(0x7f800000u);
+  if (s0 == s1) {
+    const uint32_t gbase = mt.voff[vs.mesh[s0]] - vs.off[s0];
+    f3 lo = {INF, INF, INF}, hi = {-INF, -INF, -INF};
+#pragma unroll
+    for (uint32_t i = 0; i < BOX_ITEMS; ++i) {
+      uint32_t j = start + threadIdx.x + 256 * i;
+      if (j < end) {
+        f3 p = ld3(x, gbase + j);
+        lo = vmin(lo, p);
+        hi = vmax(hi, p);
+      }
+    }
+    float r[6] = {wave_min(lo.x), wave_min(lo.y), wave_min(lo.z), wave_max(hi.x), wave_max(hi.y), wave_max(hi.z)};
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+      for (int k = 0; k < 6; ++k) s_red[k][w] = r[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      f3 L = {s_red[0][0], s_red[1][0], s_red[2][0]}, H = {s_red[3][0], s_red[4][0], s_red[5][0]};
+      for (int ww = 1; ww < 4; ++ww) {
+        L = vmin(L, f3{s_red[0][ww], s_red[1][ww], s_red[2][ww]});
+        H = vmax(H, f3{s_red[3][ww], s_red[4][ww], s_red[5][ww]});
+      }
+      flush_box(acc, s0, L, H);
+    }
+    return;
+  }
+  // block spans several meshes: per-thread running accumulator, flushed on change
+  uint32_t cur = 0xffffffffu;
+  f3 lo = {INF, INF, INF}, hi = {-INF, -INF, -INF};
+  for (uint32_t i = 0; i < BOX_ITEMS; ++i) {
+    uint32_t j = start + threadIdx.x + 256 * i;
+    if (j >= end) break;
+    uint32_t sg = seg_of(vs, j);
+    if (sg != cur) {
+      if (cur != 0xffffffffu) flush_box(acc, cur, lo, hi);
+      cur = sg;
+      lo = f3{INF, INF, INF};
+      hi = f3{-INF, -INF, -INF};
+    }
+    f3 p = ld3(x, mt.voff[vs.mesh[sg]] + (j - vs.off[sg]));
+    lo = vmin(lo, p);
+    hi = vmax(hi, p);
+  }
+  if (cur != 0xffffffffu) flush_box(acc, cur, lo, hi);
+}
+
+int launch_box(hipStream_t s, SegTable vsegs, MeshTables mt, const float* x, uint32_t nitems,
+               uint32_t* box_acc) {
+  if (!nitems) return YSIM_OK;
+  hipLaunchKernelGGL(k_box, dim3(ceil_div(nitems, 256 * BOX_ITEMS)), dim3(256), 0, s, vsegs, mt, x,
+                     nitems, box_acc);
+  YSIM_CHECK_LAUNCH();
+  return YSIM_OK;
+}
+
+// ------------------------------------------------------------------ S1 Morton
+template <typename K>
+__global__ __launch_bounds__(256) void k_morton(SegTable ts, MeshTables mt, const float* __restrict__ x,
+                                                const uint32_t* __restrict__ facets,
+                                                const uint32_t* __restrict__ acc, uint32_t code_bits,
+                                                K* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                uint32_t n, uint32_t passes, uint32_t* __restrict__ hist,
+                                                uint32_t* __restrict__ status, uint32_t status_words,
+                                                uint32_t* __restrict__ counters) {
+  __shared__ uint32_t s_h[SORT_MAX_PASSES * 256];
+  for (uint32_t i = threadIdx.x; i < passes * 256; i += 256) s_h[i] = 0;
+  // zero the look-back state of this build's sort passes
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < status_words; i += gridDim.x * 256) status[i] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < SORT_MAX_PASSES) counters[threadIdx.x] = 0;
+  __syncthreads();
+  const bool wide = code_bits > 32;
+  const uint32_t start = blockIdx.x * (256 * MORTON_ITEMS);
+  for (uint32_t i = 0; i < MORTON_ITEMS; ++i) {
+    uint32_t j = start + threadIdx.x + 256 * i;
+    if (j >= n) break;
+    uint32_t sg = seg_of(ts, j);
+    uint32_t m = ts.mesh[sg];
+    uint32_t fid = j - ts.off[sg];
+    const uint32_t* f = facets + 3 * (size_t)(mt.foff[m] + fid);
+    uint32_t vb = mt.voff[m];
+    f3 x0 = ld3(x, vb + f[0]), x1 = ld3(x, vb + f[1]), x2 = ld3(x, vb + f[2]);
+    const uint32_t* a = acc + sg * 6;
+    f3 bmin = {funkey(a[0]), funkey(a[1]), funkey(a[2])};
+    f3 bmax = {funkey(a[3]), funkey(a[4]), funkey(a[5])};
+    uint64_t code = morton_of(x0, x1, x2, bmin, bmax, wide);
+    K key = (ts.nseg > 1) ? (K)(((uint64_t)sg << code_bits) | code) : (K)code;
+    keys[j] = key;
+    vals[j] = fid;
+    for (uint32_t p = 0; p < passes; ++p) atomicAdd(&s_h[p * 256 + (uint32_t)((key >> (8 * p)) & 255u)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < passes * 256; i += 256)
+    if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
+}
+
+int launch_morton(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x,
+                  const uint32_t* facets, const uint32_t* box_acc, uint32_t code_bits, bool key64,
+                  void* keys, uint32_t* vals, uint32_t nitems, const SortPlan& plan, uint32_t* hist,
+                  uint32_t* status, uint32_t* counters) {
+  if (!nitems) return YSIM_OK;
+  uint32_t blocks = ceil_div(nitems, 256 * MORTON_ITEMS);
+  if (key64)
+    hipLaunchKernelGGL(k_morton<uint64_t>, dim3(blocks), dim3(256), 0, s, tsegs, mt, x, facets, box_acc,
+                       code_bits, (uint64_t*)keys, vals, nitems, plan.passes, hist, status,
+                       (uint32_t)plan.status_words, counters);
+  else
+    hipLaunchKernelGGL(k_morton<uint32_t>, dim3(blocks), dim3(256), 0, s, tsegs, mt, x, facets, box_acc,
+                       code_bits, (uint32_t*)keys, vals, nitems, plan.passes, hist, status,
+                       (uint32_t)plan.status_words, counters);
+  YSIM_CHECK_LAUNCH();
+  return YSIM_OK;
+}
+
+// ------------------------------------------------------------------ S3 Karras
+namespace {
+template <typename K>
+struct KarrasSeg {
+  const K* keys;
+  const uint32_t* idx;
+  int n;
+  uint64_t mask;
+  bool wide;
+  __device__ __forceinline__ uint64_t code(int i) const { return (uint64_t)keys[i] & mask; }
+  __device__ __forceinline__ int clzw(uint64_t v) const {
+    return wide ? __clzll((long long)v) : __clz((int)(uint32_t)v);
+  }
+  // delta, bvh.metal:226-238
+  __device__ __forceinline__ int delta(int i, int j) const {
+    if (j < 0 || j >= n) return -1;
+    uint64_t ci = code(i), cj = code(j);
+    if (ci != cj) return clzw(ci ^ cj);
+    return (wide ? 64 : 32) + __clz((int)(idx[i] ^ idx[j]));
+  }
+};
+}  // namespace
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_hierarchy(SegTable ts, MeshTables mt, const K* __restrict__ keys,
+                                                   const uint32_t* __restrict__ vals, uint32_t code_bits,
+                                                   ysim_bvh_node* __restrict__ tree,
+                                                   int32_t* __restrict__ parent,
+                                                   uint32_t* __restrict__ visit, uint32_t n,
+                                                   uint32_t* __restrict__ box_acc, uint32_t nbox,
+                                                   uint32_t* __restrict__ hist, uint32_t hist_words) {
+  const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+  // reset accumulators for the next build (their readers ran in earlier kernels)
+  if (j < nbox * 6) box_acc[j] = ((j % 6) < 3) ? 0xffffffffu : 0u;
+  if (j < hist_words) hist[j] = 0;
+  if (j >= n) return;
+  const uint32_t sg = seg_of(ts, j);
+  const uint32_t m = ts.mesh[sg];
+  const uint32_t s = ts.off[sg];
+  const int nn = (int)(ts.off[sg + 1] - s);
+  const int i = (int)(j - s);
+  const uint32_t base = mt.noff[m];
+  ysim_bvh_node* T = tree + base;
+  int32_t* P = parent + base;
+  const bool wide = code_bits > 32;
+  const uint64_t mask = (code_bits >= 64) ? ~0ull : ((1ull << code_bits) - 1);
+  KarrasSeg<K> K_{keys + s, vals + s, nn, mask, wide};
+  // leaf slot nn-1+i: child_a = -1, child_b = facet id
+  T[nn - 1 + i].child_a = -1;
+  T[nn - 1 + i].child_b = (int32_t)K_.idx[i];
+  if (i == 0) P[0] = -1;
+  if (i >= nn - 1) return;
+  visit[base + i] = 0;
+  // determineRange, bvh.metal:240-258
+  int d = (K_.delta(i, i + 1) - K_.delta(i, i - 1) >= 0) ? 1 : -1;
+  int dmin = K_.delta(i, i - d);
+  int lmax = 2;
+  while (K_.delta(i, i + lmax * d) > dmin) lmax <<= 1;
+  int l = 0;
+  for (int t = lmax >> 1; t >= 1; t >>= 1)
+    if (K_.delta(i, i + (l + t) * d) > dmin) l += t;
+  int jj = i + l * d;
+  int lo = d < 0 ? jj : i, hi = d < 0 ? i : jj;
+  // findSplit, bvh.metal:197-224
+  uint32_t split;
+  uint64_t c0 = K_.code(lo), c1 = K_.code(hi);
+  if (c0 == c1) {
+    split = (uint32_t)((lo + hi) >> 1);
+  } else {
+    int common = K_.clzw(c0 ^ c1);
+    uint32_t sp = (uint32_t)lo, step = (uint32_t)(hi - lo);
+    do {
+      step = (step + 1) >> 1;
+      uint32_t ns = sp + step;
+      if (ns < (uint32_t)hi) {
+        if (K_.clzw(c0 ^ K_.code((int)ns)) > common) sp = ns;
+      }
+    } while (step > 1);
+    split = sp;
+  }
+  int ca = ((int)split == lo) ? (int)split + nn - 1 : (int)split;
+  int cb = ((int)split + 1 == hi) ? (int)split + nn : (int)split + 1;
+  T[i].child_a = ca;
+  T[i].child_b = cb;
+  P[ca] = i;
+  P[cb] = i;
+}
+
+int launch_hierarchy(hipStream_t s, SegTable tsegs, MeshTables mt, const void* keys, bool key64,
+                     const uint32_t* vals, uint32_t code_bits, ysim_bvh_node* tree, int32_t* parent,
+                     uint32_t* visit, uint32_t nitems, uint32_t* box_acc, uint32_t nbox, uint32_t* hist,
+                     uint32_t hist_words) {
+  uint32_t work = std::max<uint32_t>(nitems, std::max<uint32_t>(nbox * 6, hist_words));
+  if (!work) return YSIM_OK;
+  uint32_t blocks = ceil_div(work, 256);
+  if (key64)
+    hipLaunchKernelGGL(k_hierarchy<uint64_t>, dim3(blocks), dim3(256), 0, s, tsegs, mt,
+                       (const uint64_t*)keys, vals, code_bits, tree, parent, visit, nitems, box_acc, nbox,
+                       hist, hist_words);
+  else
+    hipLaunchKernelGGL(k_hierarchy<uint32_t>, dim3(blocks), dim3(256), 0, s, tsegs, mt,
+                       (const uint32_t*)keys, vals, code_bits, tree, parent, visit, nitems, box_acc, nbox,
+                       hist, hist_words);
+  YSIM_CHECK_LAUNCH();
+  return YSIM_OK;
+}
+
+// ------------------------------------------------------------------ S4+S5 refit
+// One thread per leaf.  The leaf box is the hull of the triangle's vertices
+// (and of x + v*dt when swept).  Each thread then climbs toward the root; at a
+// parent the first arrival stops and the second (odd counter value: counters
+// advance by two per refit, so they need no clearing between refits) unions
+// its own box with the sibling's and continues.  Publication of a box to the
+// sibling thread on another XCD:
+//   formal: acq_rel device-scope RMW (L2 writeback / invalidate), plain boxes;
+//   fast:   boxes written and read with device-coherent (sc1) accesses and the
+//           writer waits for its stores to complete before the RMW.
+template <bool FORMAL>
+__global__ __launch_bounds__(256) void k_refit(SegTable ts, MeshTables mt, const float* __restrict__ x,
+                                               const float* __restrict__ v, float dt, int swept,
+                                               const uint32_t* __restrict__ facets,
+                                               ysim_bvh_node* tree, const int32_t* __restrict__ parent,
+                                               uint32_t* visit, uint32_t n, uint32_t* err) {
+  const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t sg = seg_of(ts, j);
+  const uint32_t m = ts.mesh[sg];
+  const uint32_t s = ts.off[sg];
+  const int nn = (int)(ts.off[sg + 1] - s);
+  const int i = (int)(j - s);
+  const uint32_t base = mt.noff[m];
+  ysim_bvh_node* T = tree + base;
+  const int32_t* P = parent + base;
+  uint32_t* C = visit + base;
+
+  int child = nn - 1 + i;
+  const uint32_t fid = (uint32_t)T[child].child_b;
+  const uint32_t* f = facets + 3 * (size_t)(mt.foff[m] + fid);
+  const uint32_t vb = mt.voff[m];
+  f3 lo = ld3(x, vb + f[0]), hi = lo;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    f3 p = ld3(x, vb + f[c]);
+    lo = vmin(lo, p);
+    hi = vmax(hi, p);
+    if (swept) {
+      f3 vv = ld3(v, vb + f[c]);
+      f3 q = p + vv * dt;
+      lo = vmin(lo, q);
+      hi = vmax(hi, q);
+    }
+  }
+  auto store_box = [&](int node, f3 a, f3 b) {
+    float* q = reinterpret_cast<float*>(T + node);
+    if (FORMAL) {
+      q[0] = a.x; q[1] = a.y; q[2] = a.z; q[4] = b.x; q[5] = b.y; q[6] = b.z;
+    } else {
+      st_agent(q + 0, a.x); st_agent(q + 1, a.y); st_agent(q + 2, a.z);
+      st_agent(q + 4, b.x); st_agent(q + 5, b.y); st_agent(q + 6, b.z);
+    }
+  };
+  store_box(child, lo, hi);
+  if (nn <= 1) return;
+  const int nnodes = 2 * nn - 1;
+  for (int step = 0; step < 2 * nn; ++step) {
+    int p = P[child];
+    if (p < 0 || p >= nn - 1) {  // index guard (corrupted topology)
+      atomicOr(err, 4u);
+      return;
+    }
+    uint32_t old;
+    if (FORMAL) {
+      old = __hip_atomic_fetch_add(C + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      wait_all_mem();
+      old = __hip_atomic_fetch_add(C + p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if ((old & 1u) == 0u) return;  // first arrival: the sibling's thread combines
+    const int ca = T[p].child_a, cb = T[p].child_b;
+    if (ca < 0 || ca >= nnodes || cb < 0 || cb >= nnodes) {
+      atomicOr(err, 4u);
+      return;
+    }
+    const int sib = (ca == child) ? cb : ca;
+    const float* q = reinterpret_cast<const float*>(T + sib);
+    f3 smin, smax;
+    if (FORMAL) {
+      smin = f3{q[0], q[1], q[2]};
+      smax = f3{q[4], q[5], q[6]};
+    } else {
+      smin = f3{ld_agent(q + 0), ld_agent(q + 1), ld_agent(q + 2)};
+      smax = f3{ld_agent(q + 4), ld_agent(q + 5), ld_agent(q + 6)};
+    }
+    // union in (child_a, child_b) order, bvh.metal:567-573
+    if (sib == cb) {
+      lo = vmin(lo, smin);
+      hi = vmax(hi, smax);
+    } else {
+      lo = vmin(smin, lo);
+      hi = vmax(smax, hi);
+    }
+    store_box(p, lo, hi);
+    child = p;
+    if (p == 0) return;
+  }
+  atomicOr(err, 4u);
+}
+
+int launch_refit(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x, const float* v, float dt,
+                 bool swept, const uint32_t* facets, ysim_bvh_node* tree, const int32_t* parent,
+                 uint32_t* visit, uint32_t nitems, uint32_t* err, bool formal) {
+  if (!nitems) return YSIM_OK;
+  if (formal)
+    hipLaunchKernelGGL(k_refit<true>, dim3(ceil_div(nitems, 256)), dim3(256), 0, s, tsegs, mt, x, v, dt,
+                       (int)swept, facets, tree, parent, visit, nitems, err);
+  else
+    hipLaunchKernelGGL(k_refit<false>, dim3(ceil_div(nitems, 256)), dim3(256), 0, s, tsegs, mt, x, v, dt,
+                       (int)swept, facets, tree, parent, visit, nitems, err);
+  YSIM_CHECK_LAUNCH();
+  return YSIM_OK;
+}
+
+}  // namespace ysim

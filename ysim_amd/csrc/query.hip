@@ -1,0 +1,183 @@
+// query.hip — BVH traversal fused with the point-triangle narrow phase.
+//
+// One thread per query vertex (queryPoints, src/metal/bvh.metal:1393-1412).  For
+// every target tree of its collision group the thread runs a short-stack DFS
+// that tests both children at once and keeps only "both children hit" siblings
+// on a per-thread LDS stack; if that stack fills, the walk continues
+// stacklessly through parent links (exactly the same visit set).  Every
+// candidate (leaf box overlapping [x - m, x + m], closed; facets containing the
+// vertex skipped on self queries, bvh.metal:1366-1371) is either emitted as a
+// pair record (broad-phase mode) or immediately tested with the swept
+// point-triangle narrow phase (narrow_pt_tri, src/metal/bruteforce.metal:41-178),
+// and contacts are kept in per-vertex slots.  Contacts are applied later in
+// (target mesh, facet) order, so the result does not depend on traversal order.
+#include "kernels.hpp"
+
+namespace ysim {
+
+namespace {
+constexpr int QB = 128;      // threads per block
+constexpr int SDEPTH = 16;   // LDS stack entries per thread
+
+struct NodeV {
+  f3 lo, hi;
+  int a, b;
+};
+__device__ __forceinline__ NodeV load_node(const ysim_bvh_node* T, int i) {
+  const float4* q = reinterpret_cast<const float4*>(T + i);
+  float4 u = q[0], w = q[1];
+  return NodeV{{u.x, u.y, u.z}, {w.x, w.y, w.z}, __float_as_int(u.w), __float_as_int(w.w)};
+}
+
+// Visits every leaf whose box overlaps [qmin, qmax]; on_leaf(facet) per leaf.
+template <class F>
+__device__ __forceinline__ void traverse(const ysim_bvh_node* T, const int32_t* P, uint32_t nT, f3 qmin,
+                                         f3 qmax, int* stk, F&& on_leaf, uint32_t* flags) {
+  NodeV r = load_node(T, 0);
+  if (!overlap(qmin, qmax, r.lo, r.hi)) return;
+  if (r.a < 0) {
+    on_leaf(r.b);
+    return;
+  }
+  int c = 0, ca = r.a, cb = r.b, sp = 0;
+  bool stackless = false;
+  const uint32_t guard_max = 4 * nT + 16;
+  uint32_t guard = 0;
+  while (true) {
+    if (++guard > guard_max) {
+      atomicOr(flags, 4u);
+      return;
+    }
+    NodeV A = load_node(T, ca), B = load_node(T, cb);
+    const bool hA = overlap(qmin, qmax, A.lo, A.hi), hB = overlap(qmin, qmax, B.lo, B.hi);
+    if (hA && A.a < 0) on_leaf(A.b);
+    if (hB && B.a < 0) on_leaf(B.b);
+    const bool goA = hA && A.a >= 0, goB = hB && B.a >= 0;
+    if (goA) {
+      if (goB && !stackless) {
+        if (sp < SDEPTH) stk[(sp++) * QB] = cb;
+        else stackless = true;
+      }
+      c = ca; ca = A.a; cb = A.b;
+      continue;
+    }
+    if (goB) {
+      c = cb; ca = B.a; cb = B.b;
+      continue;
+    }
+    if (!stackless) {
+      if (sp == 0) return;
+      c = stk[(--sp) * QB];
+      NodeV N = load_node(T, c);
+      ca = N.a; cb = N.b;
+      continue;
+    }
+    // stackless continuation: climb until an unvisited right sibling overlaps
+    bool found = false;
+    while (!found) {
+      if (c == 0) return;
+      if (++guard > guard_max) {
+        atomicOr(flags, 4u);
+        return;
+      }
+      const int p = P[c];
+      if (p < 0 || p >= (int)nT - 1) {
+        atomicOr(flags, 4u);
+        return;
+      }
+      NodeV PN = load_node(T, p);
+      if (PN.a == c) {
+        NodeV S = load_node(T, PN.b);
+        if (S.a >= 0 && overlap(qmin, qmax, S.lo, S.hi)) {
+          c = PN.b; ca = S.a; cb = S.b;
+          found = true;
+        } else {
+          c = PN.b;  // leaf siblings were handled with their parent
+        }
+      } else {
+        c = p;
+      }
+    }
+  }
+}
+}  // namespace
+
+template <bool EMIT_PAIRS>
+__global__ __launch_bounds__(QB) void k_query(QueryArgs a) {
+  __shared__ int s_stack[SDEPTH * QB];
+  const uint32_t j = blockIdx.x * QB + threadIdx.x;
+  uint32_t npair = 0;
+  if (j < a.nitems) {
+    const uint32_t sg = seg_of(a.qsegs, j);
+    const uint32_t m = a.qsegs.mesh[sg];
+    const uint32_t p = j - a.qsegs.off[sg];
+    const uint32_t g = a.mt.voff[m] + p;
+    const f3 qcur = ld3(a.x, g);
+    const f3 qprev = ld3(a.xprev, g);
+    const float margin = a.mesh_margin[m];
+    const float band = a.mesh_band[m];
+    const f3 qmin = {qcur.x - margin, qcur.y - margin, qcur.z - margin};
+    const f3 qmax = {qcur.x + margin, qcur.y + margin, qcur.z + margin};
+    uint32_t cnt = 0;
+    const uint32_t cap = a.cap;
+    for (uint32_t ti = a.tgt_off[sg]; ti < a.tgt_off[sg + 1]; ++ti) {
+      const uint32_t t = a.tgt[ti];
+      const uint32_t nT = a.mesh_nfacets[t];
+      if (nT == 0) continue;
+      const ysim_bvh_node* T = a.tree + a.mt.noff[t];
+      const int32_t* P = a.parent + a.mt.noff[t];
+      const uint32_t* F = a.facets + 3 * (size_t)a.mt.foff[t];
+      const uint32_t tvb = a.mt.voff[t];
+      const bool self = (t == m);
+      traverse(T, P, nT, qmin, qmax, s_stack + threadIdx.x,
+               [&](int fid) {
+                 const uint32_t* f = F + 3 * (size_t)fid;
+                 const uint32_t f0 = f[0], f1 = f[1], f2 = f[2];
+                 if (self && (p == f0 || p == f1 || p == f2)) return;
+                 ++npair;
+                 if (EMIT_PAIRS) {
+                   uint32_t slot = atomicAdd(a.npairs, 1u);
+                   if (slot < a.pair_cap) {
+                     uint4* o = reinterpret_cast<uint4*>(a.pairs) + slot;
+                     *o = make_uint4(m, p, t, (uint32_t)fid);
+                   }
+                   return;
+                 }
+                 f3 nrm;
+                 float d;
+                 if (narrow_test(qcur, qprev, ld3(a.x, tvb + f0), ld3(a.x, tvb + f1), ld3(a.x, tvb + f2),
+                                 band, nrm, d)) {
+                   if (cnt < cap) {
+                     a.cnd[(size_t)j * cap + cnt] = make_float4(nrm.x, nrm.y, nrm.z, d);
+                     a.cid[(size_t)j * cap + cnt] = make_uint2(t, (uint32_t)fid);
+                   }
+                   ++cnt;
+                 }
+               },
+               a.flags);
+    }
+    if (!EMIT_PAIRS) {
+      a.ccount[j] = cnt;
+      if (cnt > cap) atomicOr(a.flags, 2u);
+    }
+  }
+  if (!EMIT_PAIRS && a.npairs) {
+    // candidate-pair statistics: one atomic per wave
+    uint32_t s = npair;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(a.npairs, s);
+  }
+}
+
+int launch_query(hipStream_t s, const QueryArgs& a, bool emit_pairs) {
+  if (!a.nitems) return YSIM_OK;
+  if (emit_pairs)
+    hipLaunchKernelGGL(k_query<true>, dim3(ceil_div(a.nitems, QB)), dim3(QB), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_query<false>, dim3(ceil_div(a.nitems, QB)), dim3(QB), 0, s, a);
+  YSIM_CHECK_LAUNCH();
+  return YSIM_OK;
+}
+
+}  // namespace ysim

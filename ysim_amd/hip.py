@@ -1,0 +1,257 @@
+"""Host-side HIP backend: device context, device arrays and the resident scene.
+
+These classes are the Python face of the `HIP` backend tag; each method names
+the reference operation it stands in for.  All compute runs in libysim_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import abi
+from ._lib import ERR_CAPACITY, YsimError, call, load
+
+vp = C.c_void_p
+
+
+class HipContext:
+    """One device + one HIP stream (MetalGlobalContext, core_types.hpp:83-187)."""
+
+    def __init__(self, device: int = 0):
+        load()
+        h = vp()
+        call("ysim_hip_ctx_create", device, 0, C.byref(h))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().ysim_hip_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        call("ysim_hip_sync", self.h)
+
+    # ---------------------------------------------------------- device arrays
+    def alloc(self, nbytes: int) -> "DeviceArray":
+        return DeviceArray(self, nbytes)
+
+    def to_device(self, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(self, a.nbytes, dtype=a.dtype, shape=a.shape)
+        if a.nbytes:
+            call("ysim_hip_upload", self.h, d.ptr, a.ctypes.data, a.nbytes)
+        return d
+
+    def zeros(self, shape, dtype) -> "DeviceArray":
+        a = np.zeros(shape, dtype)
+        return self.to_device(a)
+
+
+class DeviceArray:
+    """A device allocation owned by Python (VectorBase<HIP, T> view + storage)."""
+
+    def __init__(self, ctx: HipContext, nbytes: int, dtype=np.uint8, shape=None):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        self.dtype = np.dtype(dtype)
+        self.shape = shape if shape is not None else (self.nbytes // self.dtype.itemsize,)
+        p = vp()
+        call("ysim_hip_alloc", ctx.h, max(self.nbytes, 1), C.byref(p))
+        self.ptr = p
+
+    def get(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            call("ysim_hip_download", self.ctx.h, out.ctypes.data, self.ptr, self.nbytes)
+        return out
+
+    def set(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes <= self.nbytes
+        call("ysim_hip_upload", self.ctx.h, self.ptr, a.ctypes.data, a.nbytes)
+
+    def __del__(self):
+        try:
+            if getattr(self, "ptr", None) and self.ctx.h:
+                load().ysim_hip_free(self.ctx.h, self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- stage ops
+def scene_box(ctx: HipContext, x: np.ndarray, num_prims: int = 0) -> np.ndarray:
+    """BVH::build scene box (bvh.hpp:1229-1231) -> (min[3], max[3])."""
+    dx = ctx.to_device(np.asarray(x, np.float32).reshape(-1))
+    box = ctx.zeros(8, np.float32)
+    call("ysim_hip_scene_box", ctx.h, dx.ptr, dx.nbytes // 12, num_prims, box.ptr)
+    b = box.get()
+    return np.concatenate([b[0:3], b[4:7]])
+
+
+def morton(ctx: HipContext, x, facets, box6, bits: int):
+    """fillMortons_Tri (bvh.metal:42-63): keys (uint64 view) and identity indices."""
+    dx = ctx.to_device(np.asarray(x, np.float32).reshape(-1))
+    df = ctx.to_device(np.asarray(facets, np.uint32).reshape(-1))
+    n = df.nbytes // 12
+    b = np.zeros(8, np.float32)
+    b[0:3] = box6[0:3]
+    b[4:7] = box6[3:6]
+    db = ctx.to_device(b)
+    kd = np.uint64 if bits > 32 else np.uint32
+    keys = ctx.zeros(n, kd)
+    idx = ctx.zeros(n, np.uint32)
+    call("ysim_hip_morton", ctx.h, dx.ptr, df.ptr, n, db.ptr, bits, keys.ptr, idx.ptr)
+    return keys.get().astype(np.uint64), idx.get()
+
+
+class RadixSorter:
+    """RadixSorter<HIP, E>::sort (spatial_hash.hpp:48-70): stable LSD radix sort."""
+
+    def __init__(self, ctx: HipContext):
+        self.ctx = ctx
+
+    def sort(self, keys: np.ndarray, vals: np.ndarray, key_bits: int):
+        kd = np.uint64 if key_bits > 32 else np.uint32
+        dk = self.ctx.to_device(np.asarray(keys, kd))
+        dv = self.ctx.to_device(np.asarray(vals, np.uint32))
+        call("ysim_hip_sort_pairs", self.ctx.h, dk.ptr, dv.ptr, len(keys), key_bits)
+        return dk.get().astype(np.uint64), dv.get()
+
+
+def lbvh_hierarchy(ctx: HipContext, sorted_keys, sorted_idx, bits: int):
+    """buildTree_Tri child links (bvh.metal:352-393) -> (nodes, parent)."""
+    n = len(sorted_keys)
+    kd = np.uint64 if bits > 32 else np.uint32
+    dk = ctx.to_device(np.asarray(sorted_keys, kd))
+    di = ctx.to_device(np.asarray(sorted_idx, np.uint32))
+    tree = ctx.zeros(2 * n - 1, abi.BVH_NODE)
+    parent = ctx.to_device(np.full(2 * n - 1, -1, np.int32))
+    call("ysim_hip_lbvh_hierarchy", ctx.h, dk.ptr, di.ptr, n, bits, tree.ptr, parent.ptr)
+    return tree.get(), parent.get()
+
+
+def lbvh_refit(ctx: HipContext, tree, parent, x, v, dt, swept, facets):
+    """buildLeaf/buildSweptLeaf + bottomUpBoxes (bvh.metal:260-586) -> nodes."""
+    dt_ = ctx.to_device(np.ascontiguousarray(tree))
+    dp = ctx.to_device(np.asarray(parent, np.int32))
+    dx = ctx.to_device(np.asarray(x, np.float32).reshape(-1))
+    dv = ctx.to_device(np.asarray(v, np.float32).reshape(-1))
+    df = ctx.to_device(np.asarray(facets, np.uint32).reshape(-1))
+    call("ysim_hip_lbvh_refit", ctx.h, dx.ptr, dv.ptr, float(dt), int(swept), df.ptr, df.nbytes // 12,
+         dt_.ptr, dp.ptr)
+    return dt_.get()
+
+
+def broad_query(ctx: HipContext, xq, qobj, tree, tfacets, tobj, margin, capacity=None):
+    """queryPoints (bvh.metal:1393-1412) for one (query mesh, target tree)."""
+    dx = ctx.to_device(np.asarray(xq, np.float32).reshape(-1))
+    dtree = ctx.to_device(np.ascontiguousarray(tree))
+    df = ctx.to_device(np.asarray(tfacets, np.uint32).reshape(-1))
+    nT = df.nbytes // 12
+    cap = capacity if capacity is not None else 16 * (dx.nbytes // 12) + 1024
+    for _ in range(2):
+        out = ctx.zeros(max(cap, 1), abi.BROAD)
+        cnt = ctx.zeros(1, np.uint32)
+        req = C.c_uint32(0)
+        rc = load().ysim_hip_broad_query(ctx.h, dx.ptr, dx.nbytes // 12, qobj, 1, 0, dtree.ptr, nT, df.ptr, tobj,
+                                         0, 0, float(margin), out.ptr, cap, cnt.ptr, C.byref(req))
+        if rc == ERR_CAPACITY:
+            cap = req.value
+            continue
+        if rc:
+            raise YsimError(rc, load().ysim_hip_last_error().decode())
+        return out.get()[: req.value]
+    raise RuntimeError("broad_query: capacity retry failed")
+
+
+# ---------------------------------------------------------------- the scene
+class HipScene:
+    """A packed scene resident in HBM plus its substep (Simulator::update)."""
+
+    def __init__(self, ctx: HipContext, scene, kspring: Optional[np.ndarray] = None):
+        self.ctx = ctx
+        self.scene = scene
+        pk = scene.packed()
+        self.descs = abi.mesh_descs(scene)
+        self.params = abi.sim_params(scene.params)
+        h = vp()
+        call("ysim_hip_scene_create", ctx.h, len(scene.meshes), C.cast(self.descs, vp), pk["x"].ctypes.data,
+             pk["v"].ctypes.data, pk["mass"].ctypes.data, pk["ext"].ctypes.data, pk["fixed"].ctypes.data,
+             pk["facets"].ctypes.data, C.byref(self.params), C.byref(h))
+        self.h = h
+        self.V = scene.num_vertices
+        if kspring is not None:
+            raise NotImplementedError("per-spring stiffness is served by the adjoint API")
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().ysim_hip_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n: int = 1, graph: bool = False):
+        call("ysim_hip_scene_step", self.h, n, int(graph))
+
+    def sync(self):
+        self.ctx.sync()
+
+    def state(self):
+        x = np.empty((self.V, 3), np.float32)
+        v = np.empty((self.V, 3), np.float32)
+        xp = np.empty((self.V, 3), np.float32)
+        call("ysim_hip_scene_get_state", self.h, x.ctypes.data, v.ctypes.data, xp.ctypes.data)
+        return x, v, xp
+
+    def set_state(self, x=None, v=None, xprev=None):
+        a = [None if t is None else np.ascontiguousarray(t, np.float32).reshape(-1) for t in (x, v, xprev)]
+        call("ysim_hip_scene_set_state", self.h, *(None if t is None else t.ctypes.data for t in a))
+
+    def contacts(self):
+        off = np.zeros(self.V + 1, np.uint32)
+        n = C.c_uint32(0)
+        call("ysim_hip_scene_get_contacts", self.h, off.ctypes.data, None, 0, C.byref(n))
+        rec = np.zeros(n.value, abi.NARROW)
+        call("ysim_hip_scene_get_contacts", self.h, off.ctypes.data, rec.ctypes.data, n.value, C.byref(n))
+        return off, rec
+
+    def pairs(self):
+        """Candidate set at the current state as (n, 4): (qobj, point, tobj, tri)."""
+        n = C.c_uint32(0)
+        call("ysim_hip_scene_get_pairs", self.h, None, 0, C.byref(n))
+        rec = np.zeros(n.value, abi.BROAD)
+        call("ysim_hip_scene_get_pairs", self.h, rec.ctypes.data, n.value, C.byref(n))
+        return np.stack([rec["query_obj"], rec["point"], rec["target_obj"], rec["triangle"]], 1)
+
+    def tree(self, mesh: int, keys: bool = False):
+        T = self.scene.meshes[mesh].facets.shape[0]
+        nodes = np.zeros(2 * T - 1, abi.BVH_NODE)
+        parent = np.zeros(2 * T - 1, np.int32)
+        k = np.zeros(T, np.uint64) if keys else None
+        i = np.zeros(T, np.uint32) if keys else None
+        call("ysim_hip_scene_get_tree", self.h, mesh, nodes.ctypes.data, parent.ctypes.data,
+             None if k is None else k.ctypes.data, None if i is None else i.ctypes.data)
+        return (nodes, parent, k, i) if keys else (nodes, parent)
+
+    def stage_timings(self, n: int):
+        out = (C.c_float * len(abi.STAGES))()
+        call("ysim_hip_scene_stage_timings", self.h, n, out)
+        return dict(zip(abi.STAGES, list(out)))
+
+    def flags(self) -> int:
+        f = C.c_uint32(0)
+        call("ysim_hip_scene_flags", self.h, C.byref(f))
+        return f.value

@@ -1,0 +1,358 @@
+"""Synthetic scene generators and the packed-scene layout.
+
+The packed layout follows the reference's Scene::pack contract
+(include/sim/scene.hpp:783-1182): per-mesh vertex and facet ranges concatenated
+into packed float3 arrays (x, v, ext), per-vertex mass and pin mask, and facets
+as uint3 with mesh-local vertex ids.
+
+Generators (numpy, float32 arithmetic so the inputs are bit-identical for the
+HIP path and the CPU oracle):
+  * grid cloth — MeshGridInitializer on the XZ plane
+    (include/sim/initializers.hpp:40-127): vertex id = row*N + col, facet order
+    per quad alternating the diagonal with (row+col) parity;
+  * optional fold of the grid along x = 0.5 into two layers `gap` apart;
+  * UV sphere with independent slice and stack counts (the reference's
+    primitive is t x t, include/primitive_geometry.hpp:57-140);
+  * icosphere of a given subdivision level.
+Rest lengths follow recomputeFastGridRest (include/sim/collision_pod.hpp:418-442).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List, Optional
+
+import numpy as np
+
+F32 = np.float32
+GRAVITY = (0.0, -9.81, 0.0)  # SceneEnvironment.gravity, include/sim/scene.hpp:187
+
+BEHAVIOR_FLOAT = 0
+BEHAVIOR_FASTGRID_CLOTH = 1
+
+
+@dataclasses.dataclass
+class Mesh:
+    x: np.ndarray            # (V, 3) float32
+    facets: np.ndarray       # (T, 3) uint32, mesh-local ids
+    behavior: int
+    cloth: Optional[dict] = None   # FastGrid params when behavior == cloth
+    mass: float = 1.0        # per-vertex mass
+    collidable: bool = True
+    group: int = 0
+    shape: int = 0
+    fixed: Optional[np.ndarray] = None   # (V,) float32 pin mask, 1 = free
+
+
+@dataclasses.dataclass
+class Scene:
+    meshes: List[Mesh]
+    params: dict
+    name: str = ""
+    info: dict = dataclasses.field(default_factory=dict)
+
+    # -------------------------------------------------------------- packing
+    def packed(self):
+        xs, vs, ms, es, fx, fs = [], [], [], [], [], []
+        g = np.array(GRAVITY, dtype=F32)
+        for m in self.meshes:
+            V = m.x.shape[0]
+            xs.append(np.ascontiguousarray(m.x, dtype=F32))
+            vs.append(np.zeros((V, 3), F32))
+            mass = np.full(V, m.mass, F32)
+            ms.append(mass)
+            if m.behavior == BEHAVIOR_FASTGRID_CLOTH:
+                # applyEnvironmentForces: ext = gravity * m (simulator.hpp:3148-3164)
+                es.append((g[None, :] * mass[:, None]).astype(F32))
+            else:
+                es.append(np.zeros((V, 3), F32))
+            fx.append(np.ones(V, F32) if m.fixed is None else m.fixed.astype(F32))
+            fs.append(np.ascontiguousarray(m.facets, dtype=np.uint32))
+        return dict(
+            x=np.concatenate(xs).reshape(-1), v=np.concatenate(vs).reshape(-1),
+            mass=np.concatenate(ms), ext=np.concatenate(es).reshape(-1),
+            fixed=np.concatenate(fx), facets=np.concatenate(fs).reshape(-1))
+
+    def offsets(self):
+        vo = np.zeros(len(self.meshes) + 1, np.uint32)
+        fo = np.zeros(len(self.meshes) + 1, np.uint32)
+        for i, m in enumerate(self.meshes):
+            vo[i + 1] = vo[i] + m.x.shape[0]
+            fo[i + 1] = fo[i] + m.facets.shape[0]
+        return vo, fo
+
+    @property
+    def num_vertices(self):
+        return int(sum(m.x.shape[0] for m in self.meshes))
+
+    @property
+    def num_facets(self):
+        return int(sum(m.facets.shape[0] for m in self.meshes))
+
+
+# ------------------------------------------------------------------ geometry
+def grid_facets(n: int) -> np.ndarray:
+    """Facets of an n x n grid, MeshGridInitializer order (initializers.hpp:95-122)."""
+    r, c = np.meshgrid(np.arange(n - 1), np.arange(n - 1), indexing="ij")
+    r = r.reshape(-1).astype(np.int64)
+    c = c.reshape(-1).astype(np.int64)
+    p00 = r * n + c
+    p10 = r * n + c + 1
+    p01 = (r + 1) * n + c
+    p11 = (r + 1) * n + c + 1
+    even = ((r + c) & 1) == 0
+    f = np.empty((r.size, 2, 3), np.int64)
+    f[even, 0] = np.stack([p00, p01, p11], 1)[even]
+    f[even, 1] = np.stack([p00, p11, p10], 1)[even]
+    f[~even, 0] = np.stack([p00, p01, p10], 1)[~even]
+    f[~even, 1] = np.stack([p10, p01, p11], 1)[~even]
+    return f.reshape(-1, 3).astype(np.uint32)
+
+
+def grid_positions(n: int, size: float, origin, rng: np.random.Generator,
+                   jitter_rel: float = 1e-3, fold_gap: float = 0.0) -> np.ndarray:
+    """XZ-plane grid: x = origin.x + col*l, z = origin.z + row*l, y = origin.y.
+
+    Jitter ~ N(0, (jitter_rel*l)^2) is added along the plane normal (y), in the
+    slot the reference's applyClothJiggle uses (initializers.hpp:209-222).
+    With fold_gap > 0 the sheet is folded along x = origin.x + size/2 into two
+    layers fold_gap apart, joined by a half-cylinder of radius fold_gap/2 whose
+    arc length equals the flat length it replaces (springs start near rest).
+    """
+    l = size / (n - 1)
+    col = np.arange(n, dtype=np.float64)
+    xf = col * l
+    y = np.zeros(n)
+    if fold_gap > 0:
+        r = fold_gap / 2.0
+        L = math.pi * r
+        a = size / 2 - L / 2
+        b = size / 2 + L / 2
+        xs = np.empty(n)
+        for i, u in enumerate(xf):
+            if u <= a:
+                xs[i], y[i] = u, 0.0
+            elif u >= b:
+                xs[i], y[i] = a - (u - b), 2 * r
+            else:
+                th = (u - a) / r
+                xs[i], y[i] = a + r * math.sin(th), r * (1 - math.cos(th))
+        xf = xs
+    X = np.empty((n, n, 3), np.float64)
+    X[:, :, 0] = origin[0] + xf[None, :]
+    X[:, :, 1] = origin[1] + y[None, :]
+    X[:, :, 2] = origin[2] + (np.arange(n) * l)[:, None]
+    X = X.reshape(-1, 3).astype(F32)
+    if jitter_rel > 0:
+        jit = (rng.standard_normal(n * n) * (jitter_rel * l)).astype(F32)
+        X[:, 1] = (X[:, 1] + jit).astype(F32)
+    return X
+
+
+def fastgrid_rest(x: np.ndarray, n: int, defaults: dict) -> dict:
+    """recomputeFastGridRest (include/sim/collision_pod.hpp:418-442), float32."""
+    out = dict(defaults)
+    V = x.shape[0]
+
+    def dist(a, b):
+        d = (x[a] - x[b]).astype(F32)
+        return F32(np.sqrt(F32(F32(d[0] * d[0] + d[1] * d[1]) + d[2] * d[2])))
+
+    def setv(key, a, b):
+        if a < V and b < V:
+            d = dist(a, b)
+            if d > F32(1e-9):
+                out[key] = float(d)
+
+    setv("stretch_rest_x", 0, 1)
+    setv("stretch_rest_y", 0, n)
+    setv("shear_rest_a", 0, n + 1)
+    setv("shear_rest_b", 1, n)
+    setv("bend_rest_x", 0, 2)
+    setv("bend_rest_y", 0, 2 * n)
+    return out
+
+
+def cloth_mesh(n: int, size: float, origin, rng, k=(500.0, 50.0, 5.0), mass=1.0,
+               thickness=None, fold_gap=0.0, group=0, jitter_rel=1e-3) -> Mesh:
+    x = grid_positions(n, size, origin, rng, jitter_rel=jitter_rel, fold_gap=fold_gap)
+    l = size / (n - 1)
+    defaults = dict(stretch_rest_x=l, stretch_rest_y=l, shear_rest_a=l * math.sqrt(2),
+                    shear_rest_b=l * math.sqrt(2), bend_rest_x=2 * l, bend_rest_y=2 * l)
+    rest = fastgrid_rest(x, n, defaults)
+    cloth = dict(particle_num_1d=n, **rest, kstretch=float(k[0]), kshear=float(k[1]),
+                 kbend=float(k[2]), thickness=float(0.5 * l if thickness is None else thickness))
+    return Mesh(x=x, facets=grid_facets(n), behavior=BEHAVIOR_FASTGRID_CLOTH, cloth=cloth,
+                mass=mass, group=group)
+
+
+def uv_sphere(slices: int, stacks: int, radius: float, center) -> tuple:
+    """UV sphere: poles + (stacks-1) rings of `slices` vertices.
+
+    V = slices*(stacks-1) + 2, F = 2*slices*(stacks-1) (the closed forms the
+    reference's primitive test checks for slices == stacks,
+    test/primitive_test.cpp:44-94).
+    """
+    verts = [(0.0, radius, 0.0)]
+    for i in range(1, stacks):
+        phi = math.pi * i / stacks
+        for j in range(slices):
+            th = 2 * math.pi * j / slices
+            verts.append((radius * math.sin(phi) * math.cos(th), radius * math.cos(phi),
+                          radius * math.sin(phi) * math.sin(th)))
+    verts.append((0.0, -radius, 0.0))
+    V = np.array(verts, np.float64) + np.asarray(center, np.float64)[None, :]
+    fac = []
+    bot = len(verts) - 1
+
+    def ring(i, j):
+        return 1 + (i - 1) * slices + (j % slices)
+
+    for j in range(slices):
+        fac.append((0, ring(1, j + 1), ring(1, j)))
+    for i in range(1, stacks - 1):
+        for j in range(slices):
+            a, b = ring(i, j), ring(i, j + 1)
+            c, d = ring(i + 1, j), ring(i + 1, j + 1)
+            fac.append((a, b, d))
+            fac.append((a, d, c))
+    for j in range(slices):
+        fac.append((bot, ring(stacks - 1, j), ring(stacks - 1, j + 1)))
+    return V.astype(F32), np.array(fac, np.uint32)
+
+
+def icosphere(level: int, radius: float, center) -> tuple:
+    """Icosahedron subdivided `level` times: V = 10*4^L + 2, F = 20*4^L."""
+    t = (1 + math.sqrt(5)) / 2
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t),
+         (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(p, np.float64) / np.linalg.norm(p) for p in v]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+             (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+             (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(level):
+        cache = {}
+        nf = []
+
+        def mid(a, b):
+            key = (a, b) if a < b else (b, a)
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    V = np.array(verts) * radius + np.asarray(center, np.float64)[None, :]
+    return V.astype(F32), np.array(faces, np.uint32)
+
+
+def obstacle(x, f, group=0) -> Mesh:
+    return Mesh(x=x, facets=f, behavior=BEHAVIOR_FLOAT, mass=1.0, group=group)
+
+
+# ------------------------------------------------------------------ params
+def sim_params(subh, margin, morton_bits=30, rebuild_every=0, self_collision=1, swept=1,
+               broad_mode=0, hash_slots_log2=22, hash_cell=0.0, radius=0.0, kd=0.01,
+               kair=-0.001, contact_capacity=16):
+    """SymplecticSystem + Simulator knobs.  thickness = margin = band (radius 0)."""
+    return dict(subh=float(subh), kair=float(kair), kd=float(kd), margin=float(margin),
+                radius=float(radius), narrow_thickness=float(margin), morton_bits=morton_bits,
+                rebuild_every=rebuild_every, self_collision=self_collision, swept=swept,
+                broad_mode=broad_mode, hash_slots_log2=hash_slots_log2, hash_cell=float(hash_cell),
+                contact_capacity=contact_capacity)
+
+
+# Per-vertex mass.  The reference's `mass` knob is per vertex
+# (simulator.hpp:706-713).  A total mass of 1 spread over the grid makes
+# dt*sqrt(k/m) = 9.5 at 512^2, dt = 1/1200, k = 500, far beyond the symplectic
+# Euler stability bound of 2; the sheet explodes within a few substeps.  The
+# configs therefore use mass 1 per vertex (see DESIGN.md for the measured
+# blow-up at the literal setting, reproducible with vertex_mass="total").
+def vertex_mass(n_vertices: int, mode: str = "per_vertex") -> float:
+    return 1.0 / n_vertices if mode == "total" else 1.0
+
+
+# ------------------------------------------------------------------ configs
+def config_c1(seed: int = 1, n: int = 64, mass_mode: str = "per_vertex", slices=32, stacks=16):
+    """Plain CPU-runnable case: 64x64 cloth over a UV sphere, 30-bit LBVH, refit."""
+    rng = np.random.default_rng(seed)
+    l = 1.0 / (n - 1)
+    cl = cloth_mesh(n, 1.0, (0.0, 1.0, 0.0), rng, mass=vertex_mass(n * n, mass_mode))
+    sx, sf = uv_sphere(slices, stacks, 0.3, (0.5, 0.5, 0.5))
+    p = sim_params(subh=1 / 600, margin=0.5 * l, morton_bits=30, rebuild_every=0)
+    return Scene([cl, obstacle(sx, sf)], p, name="c1-64x64-cloth-uv-sphere",
+                 info=dict(substeps=20))
+
+
+def config_c2(seed: int = 2, n: int = 512, mass_mode: str = "per_vertex", gap=0.02):
+    """512x512 self-colliding folded cloth, 63-bit LBVH rebuilt every substep."""
+    rng = np.random.default_rng(seed)
+    l = 1.0 / (n - 1)
+    cl = cloth_mesh(n, 1.0, (0.0, 1.0, 0.0), rng, mass=vertex_mass(n * n, mass_mode),
+                    fold_gap=gap)
+    p = sim_params(subh=1 / 1200, margin=0.5 * l, morton_bits=63, rebuild_every=1)
+    return Scene([cl], p, name=f"c2-{n}x{n}-folded-self-collision", info=dict(substeps=100))
+
+
+def config_drape(seed: int = 3, n: int = 32, height=0.82, subh=1 / 600, bits=30, rebuild_every=0,
+                 slices=24, stacks=12, radius=0.3):
+    """Small cloth placed just above a sphere so contacts fire within a few substeps."""
+    rng = np.random.default_rng(seed)
+    l = 1.0 / (n - 1)
+    cl = cloth_mesh(n, 1.0, (0.0, height, 0.0), rng, mass=1.0)
+    sx, sf = uv_sphere(slices, stacks, radius, (0.5, 0.5, 0.5))
+    p = sim_params(subh=subh, margin=0.5 * l, morton_bits=bits, rebuild_every=rebuild_every)
+    return Scene([cl, obstacle(sx, sf)], p, name=f"drape-{n}")
+
+
+def config_c3(seed: int = 3, count: int = 256, lo=16, hi=256, mass_mode="per_vertex"):
+    """Batched: `count` cloth instances, side ~ log-uniform[lo, hi], each with its own sphere."""
+    rng = np.random.default_rng(seed)
+    meshes = []
+    for i in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        n = max(lo, min(hi, n))
+        r = float(rng.uniform(0.15, 0.4))
+        c = (float(rng.uniform(0.2, 0.8)), float(rng.uniform(0.3, 0.7)), float(rng.uniform(0.2, 0.8)))
+        ks = float(rng.uniform(200.0, 1000.0))
+        l = 1.0 / (n - 1)
+        cl = cloth_mesh(n, 1.0, (0.0, 1.0, 0.0), rng, k=(ks, 50.0, 5.0),
+                        mass=vertex_mass(n * n, mass_mode), group=i)
+        cl.cloth["thickness"] = float(0.5 * l)
+        sx, sf = uv_sphere(32, 16, r, c)
+        meshes += [cl, obstacle(sx, sf, group=i)]
+    # one margin for the whole batch: the finest instance's half spacing
+    nmax = max(m.cloth["particle_num_1d"] for m in meshes if m.cloth)
+    p = sim_params(subh=1 / 600, margin=0.5 / (nmax - 1), morton_bits=30, rebuild_every=0)
+    return Scene(meshes, p, name=f"c3-batched-{count}", info=dict(substeps=50))
+
+
+def config_c4(seed: int = 4, n: int = 128):
+    """Differentiation: 128x128 cloth over the UV sphere, 60 steps at 1/600."""
+    rng = np.random.default_rng(seed)
+    l = 1.0 / (n - 1)
+    cl = cloth_mesh(n, 1.0, (0.0, 0.85, 0.0), rng, mass=1.0)
+    sx, sf = uv_sphere(32, 16, 0.3, (0.5, 0.5, 0.5))
+    p = sim_params(subh=1 / 600, margin=0.5 * l, morton_bits=30, rebuild_every=0)
+    return Scene([cl, obstacle(sx, sf)], p, name="c4-128x128-adjoint", info=dict(substeps=60))
+
+
+def config_c5(seed: int = 5, n: int = 1024, level: int = 6, broad_mode: int = 0):
+    """1024x1024 self-colliding cloth over 4 icospheres (level 6), 63-bit LBVH or hash."""
+    rng = np.random.default_rng(seed)
+    l = 1.0 / (n - 1)
+    cl = cloth_mesh(n, 1.0, (0.0, 1.0, 0.0), rng, mass=1.0)
+    meshes = [cl]
+    for _ in range(4):
+        r = float(rng.uniform(0.1, 0.25))
+        c = (float(rng.uniform(0.2, 0.8)), float(rng.uniform(0.3, 0.7)), float(rng.uniform(0.2, 0.8)))
+        ix, ifc = icosphere(level, r, c)
+        meshes.append(obstacle(ix, ifc))
+    thick = 0.5 * l
+    p = sim_params(subh=1 / 2400, margin=thick, morton_bits=63, rebuild_every=1,
+                   broad_mode=broad_mode, hash_slots_log2=22, hash_cell=2 * thick)
+    return Scene(meshes, p, name=f"c5-{n}x{n}-4-icospheres", info=dict(substeps=50))

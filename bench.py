@@ -1,0 +1,181 @@
+"""Benchmark: cloth substeps/s of the plain LBVH + narrow + springs substep on one MI355X.
+
+Workload (N=1): the 512 x 512 self-colliding cloth folded into two layers
+0.02 apart, 63-bit Morton LBVH rebuilt every substep, vertex-vs-all-triangles
+self query (incident facets excluded), swept point-triangle narrow phase,
+FastGrid springs + symplectic integrate, dt = 1/1200.  Inputs are generated
+with a fixed seed and are resident in HBM before the timed region; the timed
+region replays the captured hipGraph of the substep.
+
+Reported alongside the throughput:
+  roofline      the dominant stage's algorithmic bytes per launch / its average
+                HIP-event duration (eager substeps with per-stage events),
+                against the 8 TB/s HBM3E peak;
+  stages        every stage's average time and algorithmic GB/s;
+  cpu_baseline  the strict-IEEE CPU oracle of the same substep on the same
+                inputs, timed on this host (bounded sample), single thread and
+                all cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "cloth substeps/s (LBVH+narrow+springs) + HBM GB/s; plain vs batched, 1 GPU"
+
+
+def stage_bytes(V: int, T: int, k: int, passes: int, C: int) -> dict:
+    """Compulsory (algorithmic) bytes per substep of each stage: every input read
+    once and every output written once.  k = key bytes, C = contacts."""
+    return {
+        # positions read
+        "box": 12 * V,
+        # facets + positions read, keys + values written
+        "morton": 12 * T + 12 * V + (k + 4) * T,
+        # per pass: keys + values read and written
+        "sort": passes * 2 * (k + 4) * T,
+        # sorted keys + values read, child links + parent written
+        "hierarchy": (k + 4) * T + 8 * (T - 1) + 4 * (2 * T - 1),
+        # S4 leaves: facet id, facets, x and v read, leaf node written;
+        # S5 bottom-up: every node read, internal nodes written, parent links read,
+        # one 12-byte visit record per internal node
+        "refit": (4 * T + 12 * T + 24 * V + 32 * T)
+                 + (32 * (2 * T - 1) + 32 * (T - 1) + 4 * (2 * T - 1) + 12 * (T - 1)),
+        # x, xPrev of queries, tree, facets read once; counts + contact slots written
+        "query": 24 * V + 32 * (2 * T - 1) + 12 * T + 4 * V + 24 * C,
+        # x, v, ext, mass, pin mask, contact count read; x, v written; contacts read
+        "integrate": 24 * V + 12 * V + 4 * V + 4 * V + 4 * V + 24 * V + 24 * C,
+    }
+
+
+def cpu_baseline(scene, sample_steps: int, warm: int = 1):
+    from oracle import oracle as O
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    res = {}
+    for label, threads in (("single", 1), ("all", cores)):
+        O.set_threads(threads)
+        S = O.OracleScene(scene)
+        S.step(warm)
+        t0 = time.perf_counter()
+        S.step(sample_steps)
+        dt = time.perf_counter() - t0
+        res[label] = (sample_steps / dt, threads)
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=512, help="cloth side (512 = the benchmark workload)")
+    ap.add_argument("--cpu-steps", type=int, default=12, help="oracle substeps timed for cpu_baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on exactly one GPU (single-GPU scope); "
+              "--gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+    if args.steps % 2:
+        args.steps += 1  # the captured graph advances two substeps per launch
+
+    from ysim_amd import scenes
+    from ysim_amd.hip import HipContext, HipScene
+
+    scene = scenes.config_c2(n=args.n)
+    V, T = scene.num_vertices, scene.num_facets
+    ctx = HipContext(0)
+    hs = HipScene(ctx, scene)
+    use_graph = not args.no_graph
+
+    # warmup (also captures and instantiates the graph)
+    hs.step(args.warmup, graph=use_graph)
+    ctx.sync()
+    hs.counters(reset=True)
+
+    # timed region: exactly `steps` substeps, synchronised on both sides
+    ctx.sync()
+    t0 = time.perf_counter()
+    hs.step(args.steps, graph=use_graph)
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    cnt = hs.counters(reset=True)
+    flags = hs.flags()
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = args.steps / elapsed
+
+    # per-stage device time with HIP events on the scene's stream (eager substeps)
+    nst = max(20, min(200, args.steps // 2))
+    stages_ms = hs.stage_timings(nst)
+    cnt2 = hs.counters(reset=True)
+    contacts = cnt2["contacts"]
+    pairs_per_step = cnt["pairs"] / max(1, cnt["steps"])
+    passes = 8  # 63-bit keys, 8-bit digits
+    sb = stage_bytes(V, T, 8, passes, contacts)
+    launches = {"box": 1, "morton": 1, "sort": passes, "hierarchy": 1, "refit": 1, "query": 1, "integrate": 1}
+    kernels = {"box": "k_box", "morton": "k_morton<uint64_t>", "sort": "k_onesweep<uint64_t, 8>",
+               "hierarchy": "k_hierarchy<uint64_t>", "refit": "k_refit<false>", "query": "k_query<false>",
+               "integrate": "k_cloth"}
+    stages = {}
+    for s, ms in stages_ms.items():
+        gbs = sb[s] / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        stages[s] = {"ms": round(ms, 5), "algorithmic_bytes": int(sb[s]), "GBps": round(gbs, 1),
+                     "frac_hbm": round(gbs / HBM_PEAK_GBS, 4), "launches": launches[s]}
+    dom = max(stages_ms, key=lambda s: stages_ms[s])
+    per_launch_bytes = sb[dom] / launches[dom]
+    per_launch_ms = stages_ms[dom] / launches[dom]
+    achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator; inputs resident in HBM before timing)",
+        "config": {
+            "workload": f"{scene.name}: {args.n}x{args.n} cloth folded 0.02 apart, self-collision, "
+                        "63-bit Morton LBVH rebuilt every substep, dt=1/1200",
+            "vertices": V, "triangles": T, "morton_bits": 63, "rebuild_every": 1,
+            "sort": {"digit_bits": 8, "passes": passes},
+            "hipgraph": use_graph, "vertex_mass": 1.0,
+            "candidate_pairs_per_substep": round(pairs_per_step, 1), "contacts_last_substep": contacts,
+            "flags": flags,
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": kernels[dom], "stage": dom,
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": int(per_launch_bytes), "ms_per_launch": round(per_launch_ms, 5),
+        },
+        "stages": stages,
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(scene, args.cpu_steps)
+        out["cpu_baseline"] = {
+            "value": round(cb["single"][0], 3), "unit": "substeps/s", "cores": 1, "kind": "port",
+            "sample": f"{args.cpu_steps} substeps of the same {args.n}x{args.n} workload after 1 warm-up "
+                      "substep, strict-IEEE C++ oracle (-O3 -ffp-contract=off), single thread",
+            "all_cores": {"value": round(cb["all"][0], 3), "cores": cb["all"][1]},
+        }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -102,19 +102,19 @@ struct Karras {
     int j = index + l * d;
     if (d < 0) { lo = j; hi = index; } else { lo = index; hi = j; }
   }
+  // findSplit, bvh.metal:197-224, with one deliberate change: the common
+  // prefix is measured with delta() (code, then index tie-break) in every
+  // case.  The reference returns the midpoint when the end codes are equal,
+  // which disagrees with determineRange's index tie-break for runs of >= 3
+  // equal codes and yields nodes that claim the same child.  When the end codes
+  // differ the two formulations pick the same split.
   uint32_t split(int i0, int i1) const {
-    uint64_t c0 = code[i0], c1 = code[i1];
-    if (c0 == c1) return (uint32_t)((i0 + i1) >> 1);
-    int common = wide ? clz64(c0 ^ c1) : clz32((uint32_t)(c0 ^ c1));
+    int common = delta(i0, i1);
     uint32_t s = (uint32_t)i0, step = (uint32_t)(i1 - i0);
     do {
       step = (step + 1) >> 1;
       uint32_t ns = s + step;
-      if (ns < (uint32_t)i1) {
-        uint64_t sc = code[ns];
-        int pre = wide ? clz64(c0 ^ sc) : clz32((uint32_t)(c0 ^ sc));
-        if (pre > common) s = ns;
-      }
+      if (ns < (uint32_t)i1 && delta(i0, (int)ns) > common) s = ns;
     } while (step > 1);
     return s;
   }

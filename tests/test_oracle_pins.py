@@ -111,6 +111,21 @@ def topdown_tree(keys, idx, bits):
     return child
 
 
+def test_karras_long_duplicate_runs():
+    """Runs of many equal codes: the split follows the index tie-break."""
+    n = 4097
+    k = np.repeat(np.arange(0, 64, dtype=np.uint64), n // 64 + 1)[:n]
+    i = np.arange(n, dtype=np.uint32)
+    tree, parent = O.karras(k, i, 30)
+    child = topdown_tree(k, i, 30)
+    for node, (ca, cb) in child.items():
+        assert (tree[node]["child_a"], tree[node]["child_b"]) == (ca, cb)
+        assert parent[ca] == node and parent[cb] == node
+    # every slot except the root has exactly one parent
+    refs = np.concatenate([tree[: n - 1]["child_a"], tree[: n - 1]["child_b"]])
+    assert np.array_equal(np.sort(refs), np.arange(1, 2 * n - 1))
+
+
 @pytest.mark.parametrize("bits,dups", [(30, False), (63, False), (30, True)])
 def test_karras_matches_topdown(bits, dups):
     sc = scenes.config_c1(n=40)

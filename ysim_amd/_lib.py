@@ -63,6 +63,7 @@ _SIG = {
     "ysim_hip_scene_get_tree": (C.c_int, [vp, u32, vp, vp, vp, vp]),
     "ysim_hip_scene_stage_timings": (C.c_int, [vp, u32, f32p]),
     "ysim_hip_scene_flags": (C.c_int, [vp, u32p]),
+    "ysim_hip_scene_counters": (C.c_int, [vp, u64p, u32]),
 }
 
 _lib = None

@@ -118,6 +118,8 @@ struct ysim_scene {
   hipGraphExec_t graph = nullptr;
   uint32_t graph_period = 0;
   hipEvent_t ev[NEV] = {};
+  uint64_t pairs_hi = 0;        // reserved for a widened pair counter
+  uint64_t counter_step0 = 0;
   bool timing = false;
   double stage_ms[YSIM_STAGE_COUNT] = {0};
 
@@ -505,7 +507,9 @@ int ysim_hip_sort_pairs(ysim_ctx* ctx, void* d_keys, uint32_t* d_vals, uint32_t 
   YSIM_ARG(n < (1u << 30), "sort_pairs: n must be < 2^30");
   SortPlan p = sort_plan(n, key_bits);
   const size_t kb = (p.wide ? 8 : 4);
-  const size_t words = 4096 + p.hist_words + p.counter_words + 16 + p.status_words + (n * (kb + 4) + 3) / 4 + 64;
+  // layout: [4096 words misc][hist][counters][err 16][status][pad to 256 B][keys_b, 256-B rounded][vals_b]
+  const size_t tmp_bytes = 256 + (((size_t)n * kb + 255) & ~(size_t)255) + (size_t)n * 4 + 256;
+  const size_t words = 4096 + p.hist_words + p.counter_words + 16 + p.status_words + tmp_bytes / 4 + 1;
   YSIM_TRY(ctx->ensure_scratch(words * 4));
   uint32_t* base = (uint32_t*)ctx->scratch;
   uint32_t* hist = base + 4096;
@@ -1024,6 +1028,27 @@ int ysim_hip_scene_stage_timings(ysim_scene* s, uint32_t n_steps, float* h_ms) {
   s->timing = false;
   if (rc != YSIM_OK) return rc;
   for (int k = 0; k < YSIM_STAGE_COUNT; ++k) h_ms[k] = n_steps ? (float)(s->stage_ms[k] / n_steps) : 0.f;
+  return YSIM_OK;
+}
+
+int ysim_hip_scene_counters(ysim_scene* s, uint64_t* h_out, uint32_t reset) {
+  YSIM_ARG(s && h_out, "scene_counters: null argument");
+  uint32_t np = 0;
+  YSIM_HIP_TRY(hipMemcpyAsync(&np, s->npairs, 4, hipMemcpyDeviceToHost, s->st));
+  std::vector<uint32_t> cnt(s->Vdyn);
+  if (s->Vdyn) YSIM_HIP_TRY(hipMemcpyAsync(cnt.data(), s->ccount, 4 * (size_t)s->Vdyn, hipMemcpyDeviceToHost, s->st));
+  YSIM_HIP_TRY(hipStreamSynchronize(s->st));
+  uint64_t c = 0;
+  for (uint32_t k : cnt) c += std::min(k, s->cap);
+  h_out[0] = s->pairs_hi + np;
+  h_out[1] = s->step - s->counter_step0;
+  h_out[2] = c;
+  if (reset) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->st, s->npairs, 1u);
+    YSIM_CHECK_LAUNCH();
+    s->pairs_hi = 0;
+    s->counter_step0 = s->step;
+  }
   return YSIM_OK;
 }
 

@@ -230,23 +230,18 @@ __global__ __launch_bounds__(256) void k_hierarchy(SegTable ts, MeshTables mt, c
     if (K_.delta(i, i + (l + t) * d) > dmin) l += t;
   int jj = i + l * d;
   int lo = d < 0 ? jj : i, hi = d < 0 ? i : jj;
-  // findSplit, bvh.metal:197-224
-  uint32_t split;
-  uint64_t c0 = K_.code(lo), c1 = K_.code(hi);
-  if (c0 == c1) {
-    split = (uint32_t)((lo + hi) >> 1);
-  } else {
-    int common = K_.clzw(c0 ^ c1);
-    uint32_t sp = (uint32_t)lo, step = (uint32_t)(hi - lo);
-    do {
-      step = (step + 1) >> 1;
-      uint32_t ns = sp + step;
-      if (ns < (uint32_t)hi) {
-        if (K_.clzw(c0 ^ K_.code((int)ns)) > common) sp = ns;
-      }
-    } while (step > 1);
-    split = sp;
-  }
+  // findSplit, bvh.metal:197-224, measuring the common prefix with delta()
+  // (code, then index tie-break) in every case; the reference's midpoint
+  // shortcut for equal end codes contradicts that tie-break for runs of >= 3
+  // equal codes.  Identical split whenever the end codes differ.
+  const int common = K_.delta(lo, hi);
+  uint32_t sp = (uint32_t)lo, step = (uint32_t)(hi - lo);
+  do {
+    step = (step + 1) >> 1;
+    uint32_t ns = sp + step;
+    if (ns < (uint32_t)hi && K_.delta(lo, (int)ns) > common) sp = ns;
+  } while (step > 1);
+  const uint32_t split = sp;
   int ca = ((int)split == lo) ? (int)split + nn - 1 : (int)split;
   int cb = ((int)split + 1 == hi) ? (int)split + nn : (int)split + 1;
   T[i].child_a = ca;

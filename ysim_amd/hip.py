@@ -251,6 +251,11 @@ class HipScene:
         call("ysim_hip_scene_stage_timings", self.h, n, out)
         return dict(zip(abi.STAGES, list(out)))
 
+    def counters(self, reset: bool = False):
+        out = (C.c_uint64 * 3)()
+        call("ysim_hip_scene_counters", self.h, out, int(reset))
+        return dict(pairs=int(out[0]), steps=int(out[1]), contacts=int(out[2]))
+
     def flags(self) -> int:
         f = C.c_uint32(0)
         call("ysim_hip_scene_flags", self.h, C.byref(f))

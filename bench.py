@@ -3,9 +3,14 @@
 Workload (N=1): the 512 x 512 self-colliding cloth folded into two layers
 0.02 apart, 63-bit Morton LBVH rebuilt every substep, vertex-vs-all-triangles
 self query (incident facets excluded), swept point-triangle narrow phase,
-FastGrid springs + symplectic integrate, dt = 1/1200.  Inputs are generated
-with a fixed seed and are resident in HBM before the timed region; the timed
-region replays the captured hipGraph of the substep.
+FastGrid springs + symplectic integrate, dt = 1/1200, run for 100 substeps.
+Inputs are generated with a fixed seed and are resident in HBM before the
+timed region.  The timed region replays the captured hipGraph of the substep in
+episodes of the configured 100-substep run, each restored on the device from
+the initial snapshot (the restore copies are inside the timed region).  Runs
+much longer than the configured 100 substeps are not representative: past
+~330 substeps the fold collapses into itself and the candidate set explodes
+(same in the CPU oracle).
 
 Reported alongside the throughput:
   roofline      the dominant stage's algorithmic bytes per launch / its average
@@ -81,6 +86,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=12, help="oracle substeps timed for cpu_baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--episode", type=int, default=100, help="substeps per restored episode (config: 100)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on exactly one GPU (single-GPU scope); "
@@ -88,36 +94,56 @@ def main():
         sys.exit(2)
     if args.steps % 2:
         args.steps += 1  # the captured graph advances two substeps per launch
+    if args.episode % 2:
+        args.episode += 1
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
     from ysim_amd import scenes
     from ysim_amd.hip import HipContext, HipScene
 
+    log(f"generating the {args.n}x{args.n} folded cloth")
     scene = scenes.config_c2(n=args.n)
     V, T = scene.num_vertices, scene.num_facets
     ctx = HipContext(0)
     hs = HipScene(ctx, scene)
+    hs.snapshot()
     use_graph = not args.no_graph
 
-    # warmup (also captures and instantiates the graph)
-    hs.step(args.warmup, graph=use_graph)
+    def run(n):
+        """n substeps as restored episodes of args.episode substeps."""
+        left = n
+        while left > 0:
+            e = min(args.episode, left)
+            hs.restore()
+            hs.step(e, graph=use_graph)
+            left -= e
+
+    log(f"warmup {args.warmup} substeps (graph capture: {use_graph})")
+    run(args.warmup)
     ctx.sync()
     hs.counters(reset=True)
 
     # timed region: exactly `steps` substeps, synchronised on both sides
+    log(f"timed region: {args.steps} substeps")
     ctx.sync()
     t0 = time.perf_counter()
-    hs.step(args.steps, graph=use_graph)
+    run(args.steps)
     ctx.sync()
     elapsed = time.perf_counter() - t0
     cnt = hs.counters(reset=True)
     flags = hs.flags()
     ms_per_step = 1e3 * elapsed / args.steps
     value = args.steps / elapsed
+    log(f"{value:.1f} substeps/s, flags={flags}")
 
     # per-stage device time with HIP events on the scene's stream (eager substeps)
-    nst = max(20, min(200, args.steps // 2))
+    nst = min(args.episode, 100)
+    hs.restore()
     stages_ms = hs.stage_timings(nst)
     cnt2 = hs.counters(reset=True)
+    log("stage timings: " + ", ".join(f"{k}={v:.4f}ms" for k, v in stages_ms.items()))
     contacts = cnt2["contacts"]
     pairs_per_step = cnt["pairs"] / max(1, cnt["steps"])
     passes = 8  # 63-bit keys, 8-bit digits
@@ -154,7 +180,7 @@ def main():
                         "63-bit Morton LBVH rebuilt every substep, dt=1/1200",
             "vertices": V, "triangles": T, "morton_bits": 63, "rebuild_every": 1,
             "sort": {"digit_bits": 8, "passes": passes},
-            "hipgraph": use_graph, "vertex_mass": 1.0,
+            "hipgraph": use_graph, "vertex_mass": 1.0, "episode_substeps": args.episode,
             "candidate_pairs_per_substep": round(pairs_per_step, 1), "contacts_last_substep": contacts,
             "flags": flags,
         },
@@ -167,6 +193,7 @@ def main():
         "stages": stages,
     }
     if not args.no_cpu:
+        log(f"cpu baseline: {args.cpu_steps} oracle substeps, 1 thread and all cores")
         cb = cpu_baseline(scene, args.cpu_steps)
         out["cpu_baseline"] = {
             "value": round(cb["single"][0], 3), "unit": "substeps/s", "cores": 1, "kind": "port",

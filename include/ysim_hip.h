@@ -260,6 +260,11 @@ int ysim_hip_scene_stage_timings(ysim_scene* scene, uint32_t n_steps, float* h_m
 /* Sticky flags: bit0 anomaly (world-bounds guard), bit1 contact overflow,
  * bit2 traversal guard. */
 int ysim_hip_scene_flags(ysim_scene* scene, uint32_t* h_flags);
+/* Device-side snapshot of the dynamic state (x, v, xPrev, substep counter)
+ * and its restore: episodes of a fixed-length run without host transfers.
+ * Both are stream-ordered device copies. */
+int ysim_hip_scene_snapshot(ysim_scene* scene);
+int ysim_hip_scene_restore(ysim_scene* scene);
 /* Counters since the last reset: out[0] candidate pairs tested by the fused
  * query, out[1] substeps run, out[2] contacts of the last substep. */
 int ysim_hip_scene_counters(ysim_scene* scene, uint64_t* h_out, uint32_t reset);

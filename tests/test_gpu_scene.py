@@ -70,3 +70,33 @@ def test_c2_small_fold_63bit(ctx, oracle_threads):
             assert np.array_equal(ng[f], no[f]), f
         assert np.array_equal(pg, po)
     assert pair_set(hs.pairs()) == pair_set(os_.pairs())
+
+
+def test_c2_full_size_bit_exact(ctx, oracle_threads):
+    """512 x 512 fold, 63-bit rebuild every substep: state, trees and candidate set."""
+    sc = scenes.config_c2()
+    hs, os_ = HipScene(ctx, sc), OracleScene(sc)
+    assert pair_set(hs.pairs()) == pair_set(os_.pairs())
+    hs.step(4, graph=True)
+    os_.step(4)
+    assert_same_state(hs, os_)
+    ng, pg, kg, ig = hs.tree(0, keys=True)
+    no, po, ko, io = os_.tree(0)
+    assert np.array_equal(kg, ko) and np.array_equal(ig, io)
+    assert np.array_equal(pg, po)
+    for f in ("min", "max", "child_a", "child_b"):
+        assert np.array_equal(ng[f], no[f]), f
+    assert hs.flags() == 0
+
+
+def test_snapshot_restore_replays_identically(ctx):
+    sc = scenes.config_drape(n=24, height=0.8, rebuild_every=1)
+    hs = HipScene(ctx, sc)
+    hs.snapshot()
+    hs.step(30, graph=True)
+    a = hs.state()
+    hs.restore()
+    hs.step(30, graph=True)
+    b = hs.state()
+    for u, w in zip(a, b):
+        assert np.array_equal(u, w)

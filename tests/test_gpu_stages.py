@@ -115,3 +115,55 @@ def test_broad_query_matches_brute_force(ctx):
         b = set(map(tuple, bp.tolist()))
         assert len(gp) == len(a), "duplicate pairs"
         assert a == b
+
+
+# ------------------------------------------------------------------ full size (512 x 512 fold)
+@pytest.fixture(scope="module")
+def c2_full():
+    sc = scenes.config_c2()
+    m = sc.meshes[0]
+    rng = np.random.default_rng(11)
+    v = (rng.standard_normal(m.x.shape) * 0.05).astype(np.float32)
+    box = O.scene_box(m.x)
+    k, i = O.morton(m.x, m.facets, box, 63)
+    ks, is_ = O.sort_pairs(k, i, 63)
+    tree, parent = O.karras(ks, is_, 63)
+    swept = O.refit(tree.copy(), m.x, v, np.float32(1 / 1200), 1, m.facets)
+    return dict(sc=sc, m=m, v=v, box=box, k=k, i=i, ks=ks, is_=is_, tree=tree, parent=parent, swept=swept)
+
+
+def test_full_morton_and_sort(ctx, c2_full):
+    d = c2_full
+    kg, ig = hip.morton(ctx, d["m"].x, d["m"].facets, d["box"], 63)
+    assert np.array_equal(kg, d["k"]) and np.array_equal(ig, d["i"])
+    ks, vs = hip.RadixSorter(ctx).sort(d["k"], d["i"], 63)
+    assert np.array_equal(ks, d["ks"]) and np.array_equal(vs, d["is_"])
+
+
+def test_full_hierarchy(ctx, c2_full):
+    d = c2_full
+    tg, pg = hip.lbvh_hierarchy(ctx, d["ks"], d["is_"], 63)
+    assert np.array_equal(tg["child_a"], d["tree"]["child_a"])
+    assert np.array_equal(tg["child_b"], d["tree"]["child_b"])
+    assert np.array_equal(pg, d["parent"])
+
+
+@pytest.mark.parametrize("formal", [False, True])
+def test_full_refit(ctx, ctx_formal, c2_full, formal):
+    d = c2_full
+    c = ctx_formal if formal else ctx
+    got = hip.lbvh_refit(c, d["tree"], d["parent"], d["m"].x, d["v"], 1 / 1200, 1, d["m"].facets)
+    for f in ("min", "max", "child_a", "child_b"):
+        assert np.array_equal(got[f], d["swept"][f]), f
+
+
+def test_full_broad_query(ctx, c2_full):
+    d = c2_full
+    margin = d["sc"].params["margin"]
+    got = hip.broad_query(ctx, d["m"].x, 0, d["swept"], d["m"].facets, 0, margin)
+    exp = O.broad_query(d["m"].x, 0, d["swept"], d["m"].facets, 0, margin)
+    g = np.stack([got["point"], got["triangle"]], 1).astype(np.uint64)
+    e = exp.astype(np.uint64)
+    gk = np.sort(g[:, 0] << np.uint64(32) | g[:, 1])
+    ek = np.sort(e[:, 0] << np.uint64(32) | e[:, 1])
+    assert np.array_equal(gk, ek)

@@ -64,6 +64,8 @@ _SIG = {
     "ysim_hip_scene_stage_timings": (C.c_int, [vp, u32, f32p]),
     "ysim_hip_scene_flags": (C.c_int, [vp, u32p]),
     "ysim_hip_scene_counters": (C.c_int, [vp, u64p, u32]),
+    "ysim_hip_scene_snapshot": (C.c_int, [vp]),
+    "ysim_hip_scene_restore": (C.c_int, [vp]),
 }
 
 _lib = None

@@ -119,6 +119,9 @@ struct ysim_scene {
   uint32_t graph_period = 0;
   hipEvent_t ev[NEV] = {};
   uint64_t pairs_hi = 0;        // reserved for a widened pair counter
+  float *snap_x = nullptr, *snap_v = nullptr, *snap_xp = nullptr;
+  uint64_t snap_step = 0;
+  bool snap_valid = false;
   uint64_t counter_step0 = 0;
   bool timing = false;
   double stage_ms[YSIM_STAGE_COUNT] = {0};
@@ -1028,6 +1031,36 @@ int ysim_hip_scene_stage_timings(ysim_scene* s, uint32_t n_steps, float* h_ms) {
   s->timing = false;
   if (rc != YSIM_OK) return rc;
   for (int k = 0; k < YSIM_STAGE_COUNT; ++k) h_ms[k] = n_steps ? (float)(s->stage_ms[k] / n_steps) : 0.f;
+  return YSIM_OK;
+}
+
+int ysim_hip_scene_snapshot(ysim_scene* s) {
+  YSIM_ARG(s, "null scene");
+  const size_t b = 12 * (size_t)s->Vtot;
+  if (!s->snap_x) {
+    YSIM_TRY(s->alloc(&s->snap_x, 3 * (size_t)s->Vtot));
+    YSIM_TRY(s->alloc(&s->snap_v, 3 * (size_t)s->Vtot));
+    YSIM_TRY(s->alloc(&s->snap_xp, 3 * (size_t)s->Vtot));
+  }
+  YSIM_HIP_TRY(hipMemcpyAsync(s->snap_x, s->x[s->parity], b, hipMemcpyDeviceToDevice, s->st));
+  YSIM_HIP_TRY(hipMemcpyAsync(s->snap_v, s->v[s->parity], b, hipMemcpyDeviceToDevice, s->st));
+  YSIM_HIP_TRY(hipMemcpyAsync(s->snap_xp, s->x[s->parity ^ 1], b, hipMemcpyDeviceToDevice, s->st));
+  s->snap_step = s->step;
+  s->snap_valid = true;
+  return YSIM_OK;
+}
+
+int ysim_hip_scene_restore(ysim_scene* s) {
+  YSIM_ARG(s, "null scene");
+  if (!s->snap_valid) return fail(YSIM_ERR_STATE, "scene_restore: no snapshot");
+  const size_t b = 12 * (size_t)s->Vtot;
+  // restore into the buffers of the snapshot's parity so graph phases line up
+  const uint32_t par = (uint32_t)(s->snap_step & 1u);
+  YSIM_HIP_TRY(hipMemcpyAsync(s->x[par], s->snap_x, b, hipMemcpyDeviceToDevice, s->st));
+  YSIM_HIP_TRY(hipMemcpyAsync(s->v[par], s->snap_v, b, hipMemcpyDeviceToDevice, s->st));
+  YSIM_HIP_TRY(hipMemcpyAsync(s->x[par ^ 1], s->snap_xp, b, hipMemcpyDeviceToDevice, s->st));
+  s->parity = par;
+  s->step = s->snap_step;
   return YSIM_OK;
 }
 

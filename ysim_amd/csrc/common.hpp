@@ -16,6 +16,10 @@
 
 #include "../../include/ysim_hip.h"
 
+// Helpers marked YSIM_HD are pure functions usable on the device and, for
+// CPU checks of the kernel logic, on the host.
+#define YSIM_HD __host__ __device__ __forceinline__
+
 static_assert(sizeof(ysim_aabb4) == 32, "AABB4 layout");
 static_assert(sizeof(ysim_bvh_node) == 32, "BVHNode layout");
 static_assert(sizeof(ysim_broad_collision) == 32, "BroadCollision layout");
@@ -64,16 +68,16 @@ __device__ __forceinline__ f3 ld3(const float* a, uint32_t i) {
 __device__ __forceinline__ void st3(float* a, uint32_t i, f3 v) {
   reinterpret_cast<pf3*>(a)[i] = pf3{v.x, v.y, v.z};
 }
-__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
-__device__ __forceinline__ f3 operator*(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
-__device__ __forceinline__ float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
-__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+YSIM_HD f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+YSIM_HD f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+YSIM_HD f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+YSIM_HD f3 operator*(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
+YSIM_HD float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+YSIM_HD f3 cross(f3 a, f3 b) {
   return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
-__device__ __forceinline__ f3 vmin(f3 a, f3 b) { return {fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)}; }
-__device__ __forceinline__ f3 vmax(f3 a, f3 b) { return {fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)}; }
+YSIM_HD f3 vmin(f3 a, f3 b) { return {fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)}; }
+YSIM_HD f3 vmax(f3 a, f3 b) { return {fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)}; }
 
 // ------------------------------------------------------------------ ordered float keys
 // Monotone float <-> uint32 map: a < b (with -0 < +0) iff key(a) < key(b), so
@@ -89,7 +93,7 @@ __device__ __forceinline__ float funkey(uint32_t k) {
 
 // ------------------------------------------------------------------ Morton
 // expandBits, src/metal/bvh.metal:26-32.
-__device__ __forceinline__ uint32_t expand10(uint32_t v) {
+YSIM_HD uint32_t expand10(uint32_t v) {
   v = (v * 0x00010001u) & 0xFF0000FFu;
   v = (v * 0x00000101u) & 0x0F00F00Fu;
   v = (v * 0x00000011u) & 0xC30C30C3u;
@@ -97,7 +101,7 @@ __device__ __forceinline__ uint32_t expand10(uint32_t v) {
   return v;
 }
 // 21 -> 63 bit spread (the 63-bit variant of this build).
-__device__ __forceinline__ uint64_t expand21(uint64_t v) {
+YSIM_HD uint64_t expand21(uint64_t v) {
   v &= 0x1fffffull;
   v = (v | (v << 32)) & 0x001f00000000ffffull;
   v = (v | (v << 16)) & 0x001f0000ff0000ffull;
@@ -107,7 +111,7 @@ __device__ __forceinline__ uint64_t expand21(uint64_t v) {
   return v;
 }
 // mortonCode + fillMortons_Tri normalisation, src/metal/bvh.metal:34-63.
-__device__ __forceinline__ uint64_t morton_of(f3 x0, f3 x1, f3 x2, f3 bmin, f3 bmax, bool wide) {
+YSIM_HD uint64_t morton_of(f3 x0, f3 x1, f3 x2, f3 bmin, f3 bmax, bool wide) {
   float cen[3] = {((x0.x + x1.x) + x2.x) / 3.0f, ((x0.y + x1.y) + x2.y) / 3.0f,
                   ((x0.z + x1.z) + x2.z) / 3.0f};
   float lo[3] = {bmin.x, bmin.y, bmin.z}, hi[3] = {bmax.x, bmax.y, bmax.z};
@@ -130,7 +134,7 @@ __device__ __forceinline__ uint64_t morton_of(f3 x0, f3 x1, f3 x2, f3 bmin, f3 b
 
 // ------------------------------------------------------------------ narrow
 // pointInTriangleBary, src/metal/bruteforce.metal:9-28.
-__device__ __forceinline__ bool point_in_tri(f3 p, f3 v0, f3 v1) {
+YSIM_HD bool point_in_tri(f3 p, f3 v0, f3 v1) {
   float d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1), d20 = dot(p, v0), d21 = dot(p, v1);
   float denom = d00 * d11 - d01 * d01;
   if (fabsf(denom) < 1e-12f) return false;
@@ -144,7 +148,7 @@ __device__ __forceinline__ bool point_in_tri(f3 p, f3 v0, f3 v1) {
 // [qprev, qcur] against the triangle plane; n oriented toward qprev's side;
 // fires when crossed or d_cur < band; contact point projected into the plane
 // must lie inside (barycentric).  rsqrt is restated as 1/sqrt (IEEE).
-__device__ __forceinline__ bool narrow_test(f3 qcur, f3 qprev, f3 t0, f3 t1, f3 t2, float band,
+YSIM_HD bool narrow_test(f3 qcur, f3 qprev, f3 t0, f3 t1, f3 t2, float band,
                                             f3& n_out, float& d_out) {
   f3 v0 = t1 - t0, v1 = t2 - t0;
   f3 n = cross(v0, v1);
@@ -180,14 +184,14 @@ __device__ __forceinline__ bool narrow_test(f3 qcur, f3 qprev, f3 t0, f3 t1, f3 
 }
 
 // intersectAABB, src/metal/bvh.metal:1285-1290 (closed intervals).
-__device__ __forceinline__ bool overlap(f3 qmin, f3 qmax, f3 nmin, f3 nmax) {
+YSIM_HD bool overlap(f3 qmin, f3 qmax, f3 nmin, f3 nmax) {
   return !(qmax.x < nmin.x || qmin.x > nmax.x || qmax.y < nmin.y || qmin.y > nmax.y ||
            qmax.z < nmin.z || qmin.z > nmax.z);
 }
 
 // ------------------------------------------------------------------ springs
 // calc_spring, src/metal/physics.metal:146-153.
-__device__ __forceinline__ f3 calc_spring(f3 p0, f3 v0, f3 p1, f3 v1, float rest, float ks, float kd) {
+YSIM_HD f3 calc_spring(f3 p0, f3 v0, f3 p1, f3 v1, float rest, float ks, float kd) {
   f3 dx = p1 - p0;
   f3 dv = v1 - v0;
   float len = sqrtf(dot(dx, dx));

@@ -12,6 +12,7 @@
 // and contacts are kept in per-vertex slots.  Contacts are applied later in
 // (target mesh, facet) order, so the result does not depend on traversal order.
 #include "kernels.hpp"
+#include "traverse.hpp"
 
 namespace ysim {
 
@@ -19,87 +20,6 @@ namespace {
 constexpr int QB = 128;      // threads per block
 constexpr int SDEPTH = 16;   // LDS stack entries per thread
 
-struct NodeV {
-  f3 lo, hi;
-  int a, b;
-};
-__device__ __forceinline__ NodeV load_node(const ysim_bvh_node* T, int i) {
-  const float4* q = reinterpret_cast<const float4*>(T + i);
-  float4 u = q[0], w = q[1];
-  return NodeV{{u.x, u.y, u.z}, {w.x, w.y, w.z}, __float_as_int(u.w), __float_as_int(w.w)};
-}
-
-// Visits every leaf whose box overlaps [qmin, qmax]; on_leaf(facet) per leaf.
-template <class F>
-__device__ __forceinline__ void traverse(const ysim_bvh_node* T, const int32_t* P, uint32_t nT, f3 qmin,
-                                         f3 qmax, int* stk, F&& on_leaf, uint32_t* flags) {
-  NodeV r = load_node(T, 0);
-  if (!overlap(qmin, qmax, r.lo, r.hi)) return;
-  if (r.a < 0) {
-    on_leaf(r.b);
-    return;
-  }
-  int c = 0, ca = r.a, cb = r.b, sp = 0;
-  bool stackless = false;
-  const uint32_t guard_max = 4 * nT + 16;
-  uint32_t guard = 0;
-  while (true) {
-    if (++guard > guard_max) {
-      atomicOr(flags, 4u);
-      return;
-    }
-    NodeV A = load_node(T, ca), B = load_node(T, cb);
-    const bool hA = overlap(qmin, qmax, A.lo, A.hi), hB = overlap(qmin, qmax, B.lo, B.hi);
-    if (hA && A.a < 0) on_leaf(A.b);
-    if (hB && B.a < 0) on_leaf(B.b);
-    const bool goA = hA && A.a >= 0, goB = hB && B.a >= 0;
-    if (goA) {
-      if (goB && !stackless) {
-        if (sp < SDEPTH) stk[(sp++) * QB] = cb;
-        else stackless = true;
-      }
-      c = ca; ca = A.a; cb = A.b;
-      continue;
-    }
-    if (goB) {
-      c = cb; ca = B.a; cb = B.b;
-      continue;
-    }
-    if (!stackless) {
-      if (sp == 0) return;
-      c = stk[(--sp) * QB];
-      NodeV N = load_node(T, c);
-      ca = N.a; cb = N.b;
-      continue;
-    }
-    // stackless continuation: climb until an unvisited right sibling overlaps
-    bool found = false;
-    while (!found) {
-      if (c == 0) return;
-      if (++guard > guard_max) {
-        atomicOr(flags, 4u);
-        return;
-      }
-      const int p = P[c];
-      if (p < 0 || p >= (int)nT - 1) {
-        atomicOr(flags, 4u);
-        return;
-      }
-      NodeV PN = load_node(T, p);
-      if (PN.a == c) {
-        NodeV S = load_node(T, PN.b);
-        if (S.a >= 0 && overlap(qmin, qmax, S.lo, S.hi)) {
-          c = PN.b; ca = S.a; cb = S.b;
-          found = true;
-        } else {
-          c = PN.b;  // leaf siblings were handled with their parent
-        }
-      } else {
-        c = p;
-      }
-    }
-  }
-}
 }  // namespace
 
 template <bool EMIT_PAIRS>
@@ -129,7 +49,7 @@ __global__ __launch_bounds__(QB) void k_query(QueryArgs a) {
       const uint32_t* F = a.facets + 3 * (size_t)a.mt.foff[t];
       const uint32_t tvb = a.mt.voff[t];
       const bool self = (t == m);
-      traverse(T, P, nT, qmin, qmax, s_stack + threadIdx.x,
+      const int it = traverse<SDEPTH>(T, P, nT, qmin, qmax, s_stack + threadIdx.x, QB,
                [&](int fid) {
                  const uint32_t* f = F + 3 * (size_t)fid;
                  const uint32_t f0 = f[0], f1 = f[1], f2 = f[2];
@@ -153,8 +73,8 @@ __global__ __launch_bounds__(QB) void k_query(QueryArgs a) {
                    }
                    ++cnt;
                  }
-               },
-               a.flags);
+               });
+      if (it < 0) atomicOr(a.flags, 4u);
     }
     if (!EMIT_PAIRS) {
       a.ccount[j] = cnt;

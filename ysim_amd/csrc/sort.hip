@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
       uint32_t s = ld_agent_u32(status + (size_t)j * 256 + d);
       uint32_t flag = s & ~COUNT_MASK;
       if (flag == 0) {
-        if (++spins > (1u << 26)) {  // wedge guard: a predecessor never published
+        if (++spins > (1u << 22)) {  // wedge guard: a predecessor never published
           atomicOr(err, 1u);
           break;
         }

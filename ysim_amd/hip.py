@@ -251,6 +251,12 @@ class HipScene:
         call("ysim_hip_scene_stage_timings", self.h, n, out)
         return dict(zip(abi.STAGES, list(out)))
 
+    def snapshot(self):
+        call("ysim_hip_scene_snapshot", self.h)
+
+    def restore(self):
+        call("ysim_hip_scene_restore", self.h)
+
     def counters(self, reset: bool = False):
         out = (C.c_uint64 * 3)()
         call("ysim_hip_scene_counters", self.h, out, int(reset))

@@ -123,6 +123,7 @@ struct ysim_scene {
   uint64_t snap_step = 0;
   bool snap_valid = false;
   uint64_t counter_step0 = 0;
+  uint64_t steps_run = 0;       // substeps executed (monotonic, unaffected by restore)
   bool timing = false;
   double stage_ms[YSIM_STAGE_COUNT] = {0};
 
@@ -298,6 +299,7 @@ struct ysim_scene {
       }
       parity ^= 1u;
       ++step;
+      ++steps_run;
     }
     return YSIM_OK;
   }
@@ -331,6 +333,7 @@ struct ysim_scene {
     while (n >= graph_period) {
       YSIM_HIP_TRY(hipGraphLaunch(graph, st));
       step += graph_period;
+      steps_run += graph_period;
       n -= graph_period;
     }
     return eager_steps(n);
@@ -1074,13 +1077,13 @@ int ysim_hip_scene_counters(ysim_scene* s, uint64_t* h_out, uint32_t reset) {
   uint64_t c = 0;
   for (uint32_t k : cnt) c += std::min(k, s->cap);
   h_out[0] = s->pairs_hi + np;
-  h_out[1] = s->step - s->counter_step0;
+  h_out[1] = s->steps_run - s->counter_step0;
   h_out[2] = c;
   if (reset) {
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->st, s->npairs, 1u);
     YSIM_CHECK_LAUNCH();
     s->pairs_hi = 0;
-    s->counter_step0 = s->step;
+    s->counter_step0 = s->steps_run;
   }
   return YSIM_OK;
 }

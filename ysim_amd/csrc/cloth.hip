@@ -29,7 +29,7 @@ struct SpringIdx {
 }  // namespace
 
 __global__ __launch_bounds__(256) void k_cloth(ClothArgs a) {
-  const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t j = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (j >= a.nitems) return;
   const uint32_t sg = seg_of(a.dsegs, j);
   const uint32_t m = a.dsegs.mesh[sg];

@@ -244,9 +244,32 @@ __device__ __forceinline__ uint32_t ld_agent_u32(const uint32_t* p) {
 __device__ __forceinline__ void st_agent_u32(uint32_t* p, uint32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ void st_agent_u64(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t ld_agent_u64(const uint64_t* p) {
+  return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t pack2(float a, float b) {
+  return (uint64_t)__float_as_uint(a) | ((uint64_t)__float_as_uint(b) << 32);
+}
+__device__ __forceinline__ float lo_f(uint64_t v) { return __uint_as_float((uint32_t)v); }
+__device__ __forceinline__ float hi_f(uint64_t v) { return __uint_as_float((uint32_t)(v >> 32)); }
 // Wait until every outstanding memory operation of this wave has completed.
 __device__ __forceinline__ void wait_all_mem() { __builtin_amdgcn_s_waitcnt(0); }
 
 inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// XCD-aware workgroup order.  MI355X deals workgroup b of a launch to XCD
+// (b mod 8), and each XCD has its own L2.  Returning a logical block index so
+// that XCD x processes one contiguous chunk of the work keeps spatially
+// coherent items (and the tree nodes they touch) in one L2.  A bijection on
+// [0, nblocks).
+constexpr uint32_t NUM_XCD = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblocks) {
+  const uint32_t x = b % NUM_XCD, k = b / NUM_XCD;
+  const uint32_t per = nblocks / NUM_XCD, rem = nblocks % NUM_XCD;
+  return x * per + min(x, rem) + k;
+}
 
 }  // namespace ysim

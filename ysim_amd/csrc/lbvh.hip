@@ -285,7 +285,7 @@ __global__ __launch_bounds__(256) void k_refit(SegTable ts, MeshTables mt, const
                                                const uint32_t* __restrict__ facets,
                                                ysim_bvh_node* tree, const int32_t* __restrict__ parent,
                                                uint32_t* visit, uint32_t n, uint32_t* err) {
-  const uint32_t j = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t j = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (j >= n) return;
   const uint32_t sg = seg_of(ts, j);
   const uint32_t m = ts.mesh[sg];
@@ -319,8 +319,11 @@ __global__ __launch_bounds__(256) void k_refit(SegTable ts, MeshTables mt, const
     if (FORMAL) {
       q[0] = a.x; q[1] = a.y; q[2] = a.z; q[4] = b.x; q[5] = b.y; q[6] = b.z;
     } else {
-      st_agent(q + 0, a.x); st_agent(q + 1, a.y); st_agent(q + 2, a.z);
-      st_agent(q + 4, b.x); st_agent(q + 5, b.y); st_agent(q + 6, b.z);
+      // nodes are 32-byte aligned: (min.x, min.y) and (max.x, max.y) are 8-byte words
+      st_agent_u64(reinterpret_cast<uint64_t*>(q + 0), pack2(a.x, a.y));
+      st_agent(q + 2, a.z);
+      st_agent_u64(reinterpret_cast<uint64_t*>(q + 4), pack2(b.x, b.y));
+      st_agent(q + 6, b.z);
     }
   };
   store_box(child, lo, hi);
@@ -352,8 +355,10 @@ __global__ __launch_bounds__(256) void k_refit(SegTable ts, MeshTables mt, const
       smin = f3{q[0], q[1], q[2]};
       smax = f3{q[4], q[5], q[6]};
     } else {
-      smin = f3{ld_agent(q + 0), ld_agent(q + 1), ld_agent(q + 2)};
-      smax = f3{ld_agent(q + 4), ld_agent(q + 5), ld_agent(q + 6)};
+      const uint64_t m01 = ld_agent_u64(reinterpret_cast<const uint64_t*>(q + 0));
+      const uint64_t x01 = ld_agent_u64(reinterpret_cast<const uint64_t*>(q + 4));
+      smin = f3{lo_f(m01), hi_f(m01), ld_agent(q + 2)};
+      smax = f3{lo_f(x01), hi_f(x01), ld_agent(q + 6)};
     }
     // union in (child_a, child_b) order, bvh.metal:567-573
     if (sib == cb) {

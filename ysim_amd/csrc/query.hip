@@ -25,7 +25,7 @@ constexpr int SDEPTH = 16;   // LDS stack entries per thread
 template <bool EMIT_PAIRS>
 __global__ __launch_bounds__(QB) void k_query(QueryArgs a) {
   __shared__ int s_stack[SDEPTH * QB];
-  const uint32_t j = blockIdx.x * QB + threadIdx.x;
+  const uint32_t j = xcd_block(blockIdx.x, gridDim.x) * QB + threadIdx.x;
   uint32_t npair = 0;
   if (j < a.nitems) {
     const uint32_t sg = seg_of(a.qsegs, j);

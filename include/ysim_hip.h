@@ -239,6 +239,17 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
 int ysim_hip_scene_destroy(ysim_scene* scene);
 /* Run n substeps (eager launches, or the captured graph when use_graph != 0). */
 int ysim_hip_scene_step(ysim_scene* scene, uint32_t n, uint32_t use_graph);
+/* The substep as its three reference stages, for callers that drive them one by
+ * one (BVH::refit/refitSwept, detectCollisions + narrowAndSortByVertices,
+ * SymplecticSystem::update; simulator.hpp:3624-3805).  They must be called in
+ * this order once per substep (YSIM_ERR_STATE otherwise); scene_step runs all
+ * three.  refit rebuilds the dynamic trees when the rebuild cadence says so;
+ * query runs the traversal fused with the narrow phase and leaves the
+ * per-vertex contacts; integrate runs springs + integrate + contacts and
+ * advances the substep (xPrev := x is implicit). */
+int ysim_hip_scene_refit(ysim_scene* scene);
+int ysim_hip_scene_query(ysim_scene* scene);
+int ysim_hip_scene_integrate(ysim_scene* scene);
 /* Copy current x, v (packed float3) and xPrev to host; any pointer may be NULL. */
 int ysim_hip_scene_get_state(ysim_scene* scene, float* h_x, float* h_v, float* h_xprev);
 int ysim_hip_scene_set_state(ysim_scene* scene, const float* h_x, const float* h_v,

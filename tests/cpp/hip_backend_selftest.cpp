@@ -1,0 +1,122 @@
+// C++ self-test of the HIP backend tag (ysim_amd/host/ysim_hip_backend.hpp),
+// in the style of the reference's headless self-test (test/self_test_inline.hpp):
+// PASS/FAIL lines on stderr, exit code 0 when everything passes.
+//   Block 6  (drape, :1413-1517): mean y falls, contacts fire, no tunnelling
+//   Block 11 (determinism, :1971-2053): two runs are bit-identical
+//   Block 13 (contacts, :2168-2250): contacts for overlap, none when separated
+//   staged driving (refit/query/integrate) == one-call scene_step
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../ysim_amd/host/ysim_hip_backend.hpp"
+
+static int failures = 0;
+static void report(bool ok, const char* name) {
+  std::fprintf(stderr, "%s %s\n", ok ? "PASS" : "FAIL", name);
+  if (!ok) ++failures;
+}
+
+static void uv_sphere(int slices, int stacks, float r, const float c[3], std::vector<float>& x,
+                      std::vector<uint32_t>& f) {
+  const double PI = 3.14159265358979323846;
+  x = {c[0], c[1] + r, c[2]};
+  for (int i = 1; i < stacks; ++i)
+    for (int j = 0; j < slices; ++j) {
+      double ph = PI * i / stacks, th = 2 * PI * j / slices;
+      x.push_back((float)(c[0] + r * std::sin(ph) * std::cos(th)));
+      x.push_back((float)(c[1] + r * std::cos(ph)));
+      x.push_back((float)(c[2] + r * std::sin(ph) * std::sin(th)));
+    }
+  x.insert(x.end(), {c[0], c[1] - r, c[2]});
+  auto ring = [&](int i, int j) { return (uint32_t)(1 + (i - 1) * slices + (j % slices)); };
+  uint32_t bot = (uint32_t)(x.size() / 3 - 1);
+  for (int j = 0; j < slices; ++j) f.insert(f.end(), {0u, ring(1, j + 1), ring(1, j)});
+  for (int i = 1; i < stacks - 1; ++i)
+    for (int j = 0; j < slices; ++j)
+      f.insert(f.end(), {ring(i, j), ring(i, j + 1), ring(i + 1, j + 1), ring(i, j), ring(i + 1, j + 1), ring(i + 1, j)});
+  for (int j = 0; j < slices; ++j) f.insert(f.end(), {bot, ring(stacks - 1, j), ring(stacks - 1, j + 1)});
+}
+
+static void build(Simulator<HIP, float>& sim, float height, uint32_t n) {
+  const float center[3] = {0.5f, height, 0.5f};
+  const float l = 1.0f / (n - 1);
+  std::vector<float> jitter(n * n);
+  for (uint32_t i = 0; i < n * n; ++i) jitter[i] = 1e-3f * l * std::sin(0.7f * i);  // deterministic
+  sim.scene.addClothGridFastAt(n, 1.0f, center, 500.f, 50.f, 5.f, 0.5f * l, 1.0f, &jitter);
+  std::vector<float> sx;
+  std::vector<uint32_t> sf;
+  const float sc[3] = {0.5f, 0.5f, 0.5f};
+  uv_sphere(24, 12, 0.3f, sc, sx, sf);
+  sim.scene.addFloatMesh(sx, sf);
+  sim.margin = 0.5f * l;
+  sim.radius = 0.f;
+  sim.system.kd = 0.01f;
+  sim.initialize();
+}
+
+int main() {
+  int ndev = 0;
+  if (ysim_hip_device_count(&ndev) != YSIM_OK || ndev == 0) {
+    std::fprintf(stderr, "SKIP no HIP device\n");
+    return 0;
+  }
+  const uint32_t n = 24;
+  SymplecticSystem<HIP, float> sysA(1.0f / 60.f, 10), sysB(1.0f / 60.f, 10);
+  Simulator<HIP, float> A(sysA), B(sysB);
+  build(A, 0.8f, n);
+  build(B, 0.8f, n);
+  std::vector<float> x0 = A.scene.positions();
+  double y0 = 0;
+  for (uint32_t i = 0; i < n * n; ++i) y0 += x0[3 * i + 1];
+  size_t fired = 0;
+  for (int frame = 0; frame < 8; ++frame) {
+    A.update();
+    std::vector<uint32_t> off;
+    std::vector<ysim_narrow_collision> c;
+    A.narrowPhase.contacts(off, c);
+    fired += c.size();
+  }
+  std::vector<float> x1 = A.scene.positions();
+  double y1 = 0, rmin = 1e9;
+  for (uint32_t i = 0; i < n * n; ++i) {
+    y1 += x1[3 * i + 1];
+    double dx = x1[3 * i] - 0.5, dy = x1[3 * i + 1] - 0.5, dz = x1[3 * i + 2] - 0.5;
+    rmin = std::fmin(rmin, std::sqrt(dx * dx + dy * dy + dz * dz));
+  }
+  report(y1 < y0, "Block 6 / mean y falls");
+  report(fired > 0, "Block 6 / contacts fire");
+  const double sagitta = 0.3 * (1 - std::cos(3.14159265358979 / 12));
+  report(rmin > 0.3 - sagitta - 2 * (0.5 / (n - 1)), "Block 6 / no tunnelling into the sphere");
+  report(A.system.anomalyFlag == 0, "NAN-GUARD / no anomaly on a sane drape");
+
+  // Block 11: B driven with the one-call substep must equal A's staged run
+  ysim_hip_detail::check(ysim_hip_scene_step(B.scene.handle, 8 * 10, 1), "scene_step");
+  std::vector<float> xb = B.scene.positions();
+  report(std::memcmp(x1.data(), xb.data(), 4 * x1.size()) == 0, "Block 11 / staged == graph substeps, bit-identical");
+
+  // Block 13: a cloth far above the sphere has no contacts
+  SymplecticSystem<HIP, float> sysC(1.0f / 60.f, 10);
+  Simulator<HIP, float> Cfar(sysC);
+  build(Cfar, 1.6f, n);
+  Cfar.update();
+  std::vector<uint32_t> off;
+  std::vector<ysim_narrow_collision> c;
+  Cfar.narrowPhase.contacts(off, c);
+  report(c.empty(), "Block 13 / separated objects have no contacts");
+
+  // RadixSorter<HIP> through VectorBase<HIP>
+  std::vector<uint64_t> keys = {5, 3, 5, 1, 3, 0};
+  std::vector<uint32_t> vals = {0, 1, 2, 3, 4, 5};
+  VectorBase<HIP, uint64_t> dk(keys.size());
+  VectorBase<HIP, uint32_t> dv(vals.size());
+  dk.upload(keys);
+  dv.upload(vals);
+  RadixSorter<HIP, uint64_t>().sort(dk, dv, 63);
+  std::vector<uint32_t> sv = dv.download();
+  report(sv == std::vector<uint32_t>({5, 3, 1, 4, 0, 2}), "RadixSorter<HIP> / stable order");
+
+  std::fprintf(stderr, "%s: %d failure(s)\n", failures ? "FAILED" : "ALL PASSED", failures);
+  return failures ? 1 : 0;
+}

@@ -56,6 +56,9 @@ _SIG = {
     "ysim_hip_scene_create": (C.c_int, [vp, u32, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(vp)]),
     "ysim_hip_scene_destroy": (C.c_int, [vp]),
     "ysim_hip_scene_step": (C.c_int, [vp, u32, u32]),
+    "ysim_hip_scene_refit": (C.c_int, [vp]),
+    "ysim_hip_scene_query": (C.c_int, [vp]),
+    "ysim_hip_scene_integrate": (C.c_int, [vp]),
     "ysim_hip_scene_get_state": (C.c_int, [vp, vp, vp, vp]),
     "ysim_hip_scene_set_state": (C.c_int, [vp, vp, vp, vp]),
     "ysim_hip_scene_get_contacts": (C.c_int, [vp, vp, vp, u32, u32p]),

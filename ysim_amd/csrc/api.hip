@@ -124,6 +124,7 @@ struct ysim_scene {
   bool snap_valid = false;
   uint64_t counter_step0 = 0;
   uint64_t steps_run = 0;       // substeps executed (monotonic, unaffected by restore)
+  int stage_phase = 0;          // 0: next is refit, 1: query, 2: integrate (staged driving)
   bool timing = false;
   double stage_ms[YSIM_STAGE_COUNT] = {0};
 
@@ -255,9 +256,18 @@ struct ysim_scene {
 
   int enqueue_substep(uint32_t par, bool rebuild) {
     YSIM_TRY(enqueue_collision(par, rebuild));
+    YSIM_TRY(enqueue_query(par));
+    return enqueue_integrate(par);
+  }
+
+  int enqueue_query(uint32_t par) {
     QueryArgs qa = query_args(par);
     YSIM_TRY(launch_query(st, qa, false));
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[6], st));
+    return YSIM_OK;
+  }
+
+  int enqueue_integrate(uint32_t par) {
     ClothArgs ca;
     ca.dsegs = dyn_vsegs;
     ca.mt = mt;
@@ -861,8 +871,37 @@ int ysim_hip_scene_destroy(ysim_scene* scene) {
 
 int ysim_hip_scene_step(ysim_scene* scene, uint32_t n, uint32_t use_graph) {
   YSIM_ARG(scene, "null scene");
+  if (scene->stage_phase != 0) return fail(YSIM_ERR_STATE, "scene_step: a staged substep is in progress");
   scene->timing = false;
   return scene->steps(n, use_graph != 0);
+}
+
+int ysim_hip_scene_refit(ysim_scene* s) {
+  YSIM_ARG(s, "null scene");
+  if (s->stage_phase != 0) return fail(YSIM_ERR_STATE, "scene_refit: previous substep not integrated");
+  s->timing = false;
+  YSIM_TRY(s->enqueue_collision(s->parity, s->rebuild_at(s->step)));
+  s->stage_phase = 1;
+  return YSIM_OK;
+}
+
+int ysim_hip_scene_query(ysim_scene* s) {
+  YSIM_ARG(s, "null scene");
+  if (s->stage_phase != 1) return fail(YSIM_ERR_STATE, "scene_query: call scene_refit first");
+  YSIM_TRY(s->enqueue_query(s->parity));
+  s->stage_phase = 2;
+  return YSIM_OK;
+}
+
+int ysim_hip_scene_integrate(ysim_scene* s) {
+  YSIM_ARG(s, "null scene");
+  if (s->stage_phase != 2) return fail(YSIM_ERR_STATE, "scene_integrate: call scene_query first");
+  YSIM_TRY(s->enqueue_integrate(s->parity));
+  s->parity ^= 1u;
+  ++s->step;
+  ++s->steps_run;
+  s->stage_phase = 0;
+  return YSIM_OK;
 }
 
 int ysim_hip_scene_get_state(ysim_scene* s, float* h_x, float* h_v, float* h_xprev) {

@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "hash.hpp"
 #include "kernels.hpp"
 
 namespace ysim {
@@ -125,6 +126,8 @@ struct ysim_scene {
   uint64_t counter_step0 = 0;
   uint64_t steps_run = 0;       // substeps executed (monotonic, unaffected by restore)
   int stage_phase = 0;          // 0: next is refit, 1: query, 2: integrate (staged driving)
+  bool hash_mode = false;       // broad_mode 1: spatial hash instead of LBVH traversal
+  HashArgs hargs;
   bool timing = false;
   double stage_ms[YSIM_STAGE_COUNT] = {0};
 
@@ -242,6 +245,16 @@ struct ysim_scene {
   int enqueue_collision(uint32_t par, bool rebuild) {
     const float* xc = x[par];
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[0], st));
+    if (hash_mode) {
+      if (timing)
+        for (int e = 1; e <= 4; ++e) YSIM_HIP_TRY(hipEventRecord(ev[e], st));
+      HashArgs h = hargs;
+      h.x = xc;
+      h.v = v[par];
+      YSIM_TRY(launch_hash_build(st, h));
+      if (timing) YSIM_HIP_TRY(hipEventRecord(ev[5], st));
+      return YSIM_OK;
+    }
     if (rebuild) {
       for (auto& g : dyn_groups) YSIM_TRY(enqueue_build(g, xc));
     } else if (timing) {
@@ -262,7 +275,13 @@ struct ysim_scene {
 
   int enqueue_query(uint32_t par) {
     QueryArgs qa = query_args(par);
-    YSIM_TRY(launch_query(st, qa, false));
+    if (hash_mode) {
+      HashArgs h = hargs;
+      h.x = x[par];
+      YSIM_TRY(launch_hash_query(st, h, qa, false));
+    } else {
+      YSIM_TRY(launch_query(st, qa, false));
+    }
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[6], st));
     return YSIM_OK;
   }
@@ -712,7 +731,9 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
   YSIM_ARG(num_meshes > 0, "scene_create: no meshes");
   const ysim_sim_params& P = *h_params;
   YSIM_ARG(P.morton_bits == 30 || P.morton_bits == 63, "scene_create: morton_bits must be 30 or 63");
-  YSIM_ARG(P.broad_mode == 0, "scene_create: broad_mode 1 (spatial hash) is served by ysim_hip_hash_*");
+  YSIM_ARG(P.broad_mode == 0 || P.broad_mode == 1, "scene_create: broad_mode must be 0 (LBVH) or 1 (hash)");
+  YSIM_ARG(P.broad_mode == 0 || (P.hash_slots_log2 >= 10 && P.hash_slots_log2 <= 26),
+           "scene_create: hash_slots_log2 must be in [10, 26]");
   *out = nullptr;
   std::unique_ptr<ysim_scene> S(new ysim_scene());
   S->ctx = ctx;
@@ -847,6 +868,65 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
   YSIM_TRY(s.zero(s.npairs, 4));
   YSIM_TRY(s.zero(s.flags, 4));
   for (auto& e : s.ev) YSIM_HIP_TRY(hipEventCreate(&e));
+  if (P.broad_mode == 1) {
+    s.hash_mode = true;
+    std::vector<uint32_t> targets, dynf(num_meshes), grp(num_meshes);
+    float max_margin = 0.f;
+    for (uint32_t m = 0; m < num_meshes; ++m) {
+      dynf[m] = s.dynamic(m) ? 1u : 0u;
+      grp[m] = s.meshes[m].group;
+      if (s.meshes[m].collidable && s.meshes[m].num_facets) targets.push_back(m);
+      if (s.dynamic(m)) max_margin = std::max(max_margin, margin[m]);
+    }
+    HashArgs& h = s.hargs;
+    YSIM_TRY(s.make_seg(targets, true, h.tsegs));
+    h.mt = s.mt;
+    h.dt = P.subh;
+    h.swept = (int)P.swept;
+    h.facets = s.facets;
+    uint32_t *dd = nullptr, *dg = nullptr;
+    YSIM_TRY(s.put(&dd, dynf));
+    YSIM_TRY(s.put(&dg, grp));
+    h.mesh_dynamic = dd;
+    h.mesh_group = dg;
+    for (uint32_t m : targets) h.ntri += s.meshes[m].num_facets;
+    const float cell = P.hash_cell > 0.f ? P.hash_cell : 2.f * max_margin;
+    YSIM_ARG(cell > 0.f, "scene_create: hash cell must be positive");
+    h.inv_h = 1.0f / cell;
+    h.origin = f3{0.f, 0.f, 0.f};
+    h.mask = (1u << P.hash_slots_log2) - 1u;
+    h.self_collision = P.self_collision;
+    h.flags = s.flags;
+    // entry capacity from the initial geometry (cells touched per triangle box),
+    // doubled; overflow raises flag bit 3
+    uint64_t cells = 0;
+    for (uint32_t m : targets)
+      for (uint32_t t = 0; t < s.meshes[m].num_facets; ++t) {
+        const uint32_t* f = h_facets + 3 * ((size_t)s.foff[m] + t);
+        float lo[3], hi[3];
+        for (int k = 0; k < 3; ++k) {
+          lo[k] = hi[k] = h_x[3 * ((size_t)s.voff[m] + f[0]) + k];
+          for (int c = 1; c < 3; ++c) {
+            float q = h_x[3 * ((size_t)s.voff[m] + f[c]) + k];
+            lo[k] = std::min(lo[k], q);
+            hi[k] = std::max(hi[k], q);
+          }
+        }
+        uint64_t n = 1;
+        for (int k = 0; k < 3; ++k)
+          n *= (uint64_t)(std::floor(hi[k] * h.inv_h) - std::floor(lo[k] * h.inv_h) + 2);
+        cells += n;
+      }
+    cells = std::min<uint64_t>(2 * cells + (1u << 20), (1ull << 31) - 1);
+    h.entry_cap = (uint32_t)cells;
+    YSIM_TRY(s.alloc(&h.entries, h.entry_cap));
+    YSIM_TRY(s.alloc(&h.boxes, 6 * (size_t)h.ntri));
+    YSIM_TRY(s.alloc(&h.ncells, h.ntri));
+    YSIM_TRY(s.alloc(&h.count, (size_t)h.mask + 1));
+    YSIM_TRY(s.alloc(&h.cursor, (size_t)h.mask + 1));
+    YSIM_TRY(s.alloc(&h.offsets, (size_t)h.mask + 2));
+    YSIM_TRY(s.alloc(&h.scan_tmp, 1025));
+  }
   // initial build of every tree (unswept), Simulator::initialize -> broadPhase.build
   for (auto* gs : {&s.stat_groups, &s.dyn_groups})
     for (auto& g : *gs) {
@@ -995,7 +1075,13 @@ int ysim_hip_scene_get_pairs(ysim_scene* s, ysim_broad_collision* h_pairs, uint3
     qa.npairs = s->npairs + 1;
     qa.pairs = s->pairs;
     qa.pair_cap = s->pair_cap;
-    YSIM_TRY(launch_query(s->st, qa, true));
+    if (s->hash_mode) {
+      HashArgs h = s->hargs;
+      h.x = s->x[s->parity];
+      YSIM_TRY(launch_hash_query(s->st, h, qa, true));
+    } else {
+      YSIM_TRY(launch_query(s->st, qa, true));
+    }
     uint32_t n = 0;
     YSIM_HIP_TRY(hipMemcpyAsync(&n, s->npairs + 1, 4, hipMemcpyDeviceToHost, s->st));
     YSIM_HIP_TRY(hipStreamSynchronize(s->st));

@@ -341,11 +341,11 @@ def config_c4(seed: int = 4, n: int = 128):
     return Scene([cl, obstacle(sx, sf)], p, name="c4-128x128-adjoint", info=dict(substeps=60))
 
 
-def config_c5(seed: int = 5, n: int = 1024, level: int = 6, broad_mode: int = 0):
+def config_c5(seed: int = 5, n: int = 1024, level: int = 6, broad_mode: int = 0, height: float = 1.0):
     """1024x1024 self-colliding cloth over 4 icospheres (level 6), 63-bit LBVH or hash."""
     rng = np.random.default_rng(seed)
     l = 1.0 / (n - 1)
-    cl = cloth_mesh(n, 1.0, (0.0, 1.0, 0.0), rng, mass=1.0)
+    cl = cloth_mesh(n, 1.0, (0.0, height, 0.0), rng, mass=1.0)
     meshes = [cl]
     for _ in range(4):
         r = float(rng.uniform(0.1, 0.25))

@@ -77,6 +77,55 @@ def cpu_baseline(scene, sample_steps: int, warm: int = 1):
     return res
 
 
+def batched_vs_sequential(ctx, substeps: int, log):
+    """256 ragged cloth+sphere instances: one batched scene (every stage one launch
+    over all instances, hipGraph) vs each instance as its own plain scene stepped
+    in turn (hipGraph each), same substeps; results must be bit-identical."""
+    from ysim_amd import scenes
+    from ysim_amd.hip import HipScene
+    sc = scenes.config_c3(substeps=substeps)
+    nI = len(sc.meshes) // 2
+    log(f"batched: {nI} instances, {sc.num_vertices} vertices, {sc.num_facets} triangles")
+    b = HipScene(ctx, sc)
+    b.snapshot()
+    b.step(2, graph=True)
+    b.restore()
+    ctx.sync()
+    t0 = time.perf_counter()
+    b.step(substeps, graph=True)
+    ctx.sync()
+    tb = time.perf_counter() - t0
+    xb = b.state()[0]
+    flags = b.flags()
+    plains = [HipScene(ctx, scenes.instance_scene(sc, i)) for i in range(nI)]
+    for p in plains:
+        p.snapshot()
+        p.step(2, graph=True)
+        p.restore()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for p in plains:
+        p.step(substeps, graph=True)
+    ctx.sync()
+    ts = time.perf_counter() - t0
+    vo, _ = sc.offsets()
+    identical = True
+    for i, p in enumerate(plains):
+        s0, s1 = int(vo[2 * i]), int(vo[2 * i + 1])
+        identical &= bool(np.array_equal(xb[s0:s1], p.state()[0][: s1 - s0]))
+    log(f"batched {substeps / tb:.1f} substeps/s vs sequential {substeps / ts:.1f}: {ts / tb:.1f}x, "
+        f"identical={identical}")
+    return {
+        "workload": f"{sc.name}: {nI} instances, side log-uniform [16, 256], own UV sphere and k_struct, "
+                    f"{substeps} substeps at dt=1/600, 30-bit LBVH refit",
+        "instances": nI, "vertices": sc.num_vertices, "triangles": sc.num_facets,
+        "batched_substeps_per_s": round(substeps / tb, 2),
+        "sequential_plain_substeps_per_s": round(substeps / ts, 2),
+        "instance_substeps_per_s": round(nI * substeps / tb, 1),
+        "speedup": round(ts / tb, 2), "identical_results": identical, "flags": flags,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +136,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--episode", type=int, default=100, help="substeps per restored episode (config: 100)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-sequential comparison")
+    ap.add_argument("--batched-substeps", type=int, default=50)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on exactly one GPU (single-GPU scope); "
@@ -192,6 +243,8 @@ def main():
         },
         "stages": stages,
     }
+    if not args.no_batched:
+        out["batched"] = batched_vs_sequential(ctx, args.batched_substeps, log)
     if not args.no_cpu:
         log(f"cpu baseline: {args.cpu_steps} oracle substeps, 1 thread and all cores")
         cb = cpu_baseline(scene, args.cpu_steps)

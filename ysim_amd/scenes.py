@@ -309,8 +309,11 @@ def config_drape(seed: int = 3, n: int = 32, height=0.82, subh=1 / 600, bits=30,
     return Scene([cl, obstacle(sx, sf)], p, name=f"drape-{n}")
 
 
-def config_c3(seed: int = 3, count: int = 256, lo=16, hi=256, mass_mode="per_vertex"):
-    """Batched: `count` cloth instances, side ~ log-uniform[lo, hi], each with its own sphere."""
+def config_c3(seed: int = 3, count: int = 256, lo=16, hi=256, mass_mode="per_vertex", substeps=50):
+    """Batched: `count` cloth instances, side ~ log-uniform[lo, hi], each with its own
+    UV sphere (r ~ U(0.15, 0.4), centre ~ U([0.2, 0.8]^2) x U(0.3, 0.7)) and its own
+    k_struct ~ U(200, 1000).  Instance i is meshes (2i, 2i+1) in collision group i;
+    margin <= 0 gives every instance its own margin = band = thickness (0.5 spacing)."""
     rng = np.random.default_rng(seed)
     meshes = []
     for i in range(count):
@@ -319,16 +322,22 @@ def config_c3(seed: int = 3, count: int = 256, lo=16, hi=256, mass_mode="per_ver
         r = float(rng.uniform(0.15, 0.4))
         c = (float(rng.uniform(0.2, 0.8)), float(rng.uniform(0.3, 0.7)), float(rng.uniform(0.2, 0.8)))
         ks = float(rng.uniform(200.0, 1000.0))
-        l = 1.0 / (n - 1)
         cl = cloth_mesh(n, 1.0, (0.0, 1.0, 0.0), rng, k=(ks, 50.0, 5.0),
                         mass=vertex_mass(n * n, mass_mode), group=i)
-        cl.cloth["thickness"] = float(0.5 * l)
         sx, sf = uv_sphere(32, 16, r, c)
         meshes += [cl, obstacle(sx, sf, group=i)]
-    # one margin for the whole batch: the finest instance's half spacing
-    nmax = max(m.cloth["particle_num_1d"] for m in meshes if m.cloth)
-    p = sim_params(subh=1 / 600, margin=0.5 / (nmax - 1), morton_bits=30, rebuild_every=0)
-    return Scene(meshes, p, name=f"c3-batched-{count}", info=dict(substeps=50))
+    p = sim_params(subh=1 / 600, margin=-1.0, morton_bits=30, rebuild_every=0)
+    return Scene(meshes, p, name=f"c3-batched-{count}", info=dict(substeps=substeps))
+
+
+def instance_scene(batched: Scene, i: int) -> Scene:
+    """Instance i of a batched scene as a plain two-mesh scene (same params)."""
+    import copy
+    cl = copy.copy(batched.meshes[2 * i])
+    sp = copy.copy(batched.meshes[2 * i + 1])
+    cl.group = 0
+    sp.group = 0
+    return Scene([cl, sp], dict(batched.params), name=f"{batched.name}-instance-{i}")
 
 
 def config_c4(seed: int = 4, n: int = 128):

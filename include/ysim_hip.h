@@ -280,6 +280,31 @@ int ysim_hip_scene_restore(ysim_scene* scene);
  * query, out[1] substeps run, out[2] contacts of the last substep. */
 int ysim_hip_scene_counters(ysim_scene* scene, uint64_t* h_out, uint32_t reset);
 
+/* ---- differentiation ------------------------------------------------------
+ * No counterpart in the reference (it has no adjoint).  The differentiated map
+ * is its substep (Simulator::update, include/sim/simulator.hpp:3512-3815) for a
+ * scene with exactly one FastGrid cloth whose contacts are against static
+ * meshes (self_collision = 0): springs with one stiffness per spring
+ * (k = exp(log_k), spring order struct-x, struct-y, shear-a, shear-b, bend-x,
+ * bend-y, each row-major over its first endpoint), damping kd, symplectic
+ * integration and contact projection (src/metal/physics.metal:171-312).  The
+ * derivative holds each step's contact set and its gates (distance <
+ * thickness, v.n < 0) fixed.  Device pointers throughout; x/v arrays are the
+ * cloth's own 3 * num_vertices floats. */
+int ysim_hip_adj_num_springs(ysim_scene* scene, uint32_t* h_out);
+/* Run `steps` substeps from the scene's current state with k = exp(d_log_k)
+ * and damping kd (both stay in effect for later scene_step calls), keeping
+ * (x, v, xPrev) checkpoints every `checkpoint_every` substeps on the device. */
+int ysim_hip_adj_forward(ysim_scene* scene, const float* d_log_k, float kd, uint32_t steps,
+                         uint32_t checkpoint_every);
+/* Given dL/dx_N (and dL/dv_N, may be NULL = 0) at the end of the last
+ * adj_forward, write dL/dlog_k (num_springs floats), dL/dkd (1 float),
+ * dL/dx_0 and dL/dv_0 (3 * num_vertices floats each); any output may be NULL.
+ * Each segment is recomputed from its checkpoint; the scene is left in the
+ * state the forward ended in. */
+int ysim_hip_adj_backward(ysim_scene* scene, const float* d_x_bar, const float* d_v_bar, float* d_g_log_k,
+                          float* d_g_kd, float* d_g_x0, float* d_g_v0);
+
 #ifdef __cplusplus
 }
 #endif

@@ -75,6 +75,18 @@ def lib():
         "yo_scene_get_contacts": (None, [vp, u32p, vp]),
         "yo_scene_get_tree": (None, [vp, C.c_uint32, vp, C.POINTER(C.c_int32), u64p, u32p]),
         "yo_scene_anomaly": (C.c_uint32, [vp]),
+        "yo_adj_create": (vp, [C.c_uint32, vp, vp, f32p, f32p, f32p, f32p, f32p, C.c_uint32, f32p, u32p,
+                               u32p]),
+        "yo_adj_destroy": (None, [C.c_uint32, vp]),
+        "yo_adj_num_springs": (C.c_uint32, [C.c_uint32]),
+        "yo_adj_forward": (None, [C.c_uint32, vp, f64p, C.c_double, C.c_uint32, C.c_uint32, C.c_uint32, f64p,
+                                  f64p]),
+        "yo_adj_contact_hist": (None, [C.c_uint32, vp, u32p]),
+        "yo_adj_copy_record": (None, [C.c_uint32, vp, C.c_uint32, vp]),
+        "yo_adj_set_initial": (None, [C.c_uint32, vp, f64p, f64p]),
+        "yo_adj_plant": (None, [C.c_uint32, vp, C.c_uint32]),
+        "yo_adj_backward": (None, [C.c_uint32, vp, f64p, f64p, f64p, f64p, f64p, f64p]),
+        "yo_adj_contacts": (None, [C.c_uint32, vp, u64p, u64p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -274,3 +286,99 @@ class OracleScene:
 
     def anomaly(self) -> int:
         return lib().yo_scene_anomaly(self.h)
+
+
+# ------------------------------------------------------------------ adjoint
+class AdjointRollout:
+    """CPU rollout of the scene's first cloth (mesh 0) against its static meshes,
+    with the hand-derived adjoint (checkpoints every K steps, recompute per
+    segment).  precision 32 follows the device's operation order; precision 64
+    is the finite-difference-checked restatement."""
+
+    def __init__(self, scene, precision: int = 64):
+        from ysim_amd.scenes import BEHAVIOR_FASTGRID_CLOTH
+        assert scene.meshes[0].behavior == BEHAVIOR_FASTGRID_CLOTH
+        self.precision = int(precision)
+        self.scene = scene
+        cl = scene.meshes[0]
+        pk = scene.packed()
+        V = cl.x.shape[0]
+        self.V = V
+        self.N = int(cl.cloth["particle_num_1d"])
+        self.S = lib().yo_adj_num_springs(self.N)
+        self.descs = abi.mesh_descs(scene)
+        self.desc = self.descs[0]
+        self.params = abi.sim_params(scene.params)
+        tri, obj, fid = [], [], []
+        for mi, m in enumerate(scene.meshes[1:], start=1):
+            if m.behavior == BEHAVIOR_FASTGRID_CLOTH or m.group != cl.group:
+                continue
+            tri.append(np.asarray(m.x, np.float32)[np.asarray(m.facets, np.int64)].reshape(-1, 9))
+            obj.append(np.full(m.facets.shape[0], mi, np.uint32))
+            fid.append(np.arange(m.facets.shape[0], dtype=np.uint32))
+        obs = np.ascontiguousarray(np.concatenate(tri) if tri else np.zeros((0, 9), np.float32))
+        oo = np.ascontiguousarray(np.concatenate(obj) if obj else np.zeros(0, np.uint32))
+        of = np.ascontiguousarray(np.concatenate(fid) if fid else np.zeros(0, np.uint32))
+        self._keep = [np.ascontiguousarray(pk[k][: (3 * V if k in ("x", "v", "ext") else V)])
+                      for k in ("x", "v", "mass", "fixed", "ext")] + [obs, oo, of]
+        x0, v0, mass, fixed, ext = self._keep[:5]
+        self.h = lib().yo_adj_create(self.precision, C.byref(self.desc), C.byref(self.params), P(x0, f32p),
+                                     P(v0, f32p), P(mass, f32p), P(fixed, f32p), P(ext, f32p), oo.size,
+                                     P(obs, f32p), P(oo, u32p), P(of, u32p))
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().yo_adj_destroy(self.precision, self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def forward(self, theta, kd: float, steps: int, K: int = 10, mode: str = "detect"):
+        """mode "detect", "record" (detect and keep every step's contacts and
+        gates) or "frozen" (replay the recorded contacts and gates)."""
+        th = np.ascontiguousarray(theta, np.float64)
+        assert th.size == self.S
+        x = np.zeros(3 * self.V, np.float64)
+        v = np.zeros(3 * self.V, np.float64)
+        m = {"detect": 0, "record": 1, "frozen": 2}[mode]
+        lib().yo_adj_forward(self.precision, self.h, P(th, f64p), float(kd), steps, K, m, P(x, f64p), P(v, f64p))
+        return x.reshape(-1, 3), v.reshape(-1, 3)
+
+    def adopt_record(self, other: "AdjointRollout"):
+        """Take `other`'s recorded contacts and gates and switch to frozen mode
+        (forward and backward then follow `other`'s piece of the map)."""
+        lib().yo_adj_copy_record(other.precision, other.h, self.precision, self.h)
+
+    def set_initial(self, x0=None, v0=None):
+        """Initial state of later forwards (xPrev_0 = x_0); None keeps a part."""
+        a = [None if t is None else np.ascontiguousarray(t, np.float64).reshape(-1) for t in (x0, v0)]
+        lib().yo_adj_set_initial(self.precision, self.h, P(a[0], f64p), P(a[1], f64p))
+
+    def plant(self, bits: int):
+        """Plant a known mistake in the backward pass (tests of the checker)."""
+        lib().yo_adj_plant(self.precision, self.h, int(bits))
+
+    def contact_hist(self):
+        out = np.zeros(self.V, np.uint32)
+        lib().yo_adj_contact_hist(self.precision, self.h, P(out, u32p))
+        return out
+
+    def backward(self, xbar, vbar=None):
+        """Adjoint of the last forward.  After a "frozen" forward (or adopt_record)
+        the recorded contacts and gates are used."""
+        xb = np.ascontiguousarray(xbar, np.float64).reshape(-1)
+        vb = None if vbar is None else np.ascontiguousarray(vbar, np.float64).reshape(-1)
+        g = np.zeros(self.S, np.float64)
+        gkd = np.zeros(1, np.float64)
+        gx = np.zeros(3 * self.V, np.float64)
+        gv = np.zeros(3 * self.V, np.float64)
+        lib().yo_adj_backward(self.precision, self.h, P(xb, f64p), P(vb, f64p), P(g, f64p), P(gkd, f64p),
+                              P(gx, f64p), P(gv, f64p))
+        return g, float(gkd[0]), gx.reshape(-1, 3), gv.reshape(-1, 3)
+
+    def contacts(self):
+        c = np.zeros(1, np.uint64)
+        s = np.zeros(1, np.uint64)
+        lib().yo_adj_contacts(self.precision, self.h, P(c, u64p), P(s, u64p))
+        return int(c[0]), int(s[0])

@@ -130,6 +130,54 @@ struct ysim_scene {
   HashArgs hargs;
   bool timing = false;
   double stage_ms[YSIM_STAGE_COUNT] = {0};
+  // differentiation: checkpoints of the cloth's (x, v, xPrev) every K substeps
+  // of the last adjoint forward, the per-segment tape and adjoint buffers
+  struct Adj {
+    bool valid = false;
+    uint32_t mesh = 0, V = 0, voff = 0, S = 0;
+    uint32_t steps = 0, K = 0, nseg = 0;
+    uint64_t step0 = 0;
+    uint32_t par_end = 0;
+    float kd = 0.f;
+    float *ck = nullptr, *tape = nullptr, *xb = nullptr, *vb = nullptr, *fb = nullptr, *gk = nullptr,
+          *kdv = nullptr;
+    size_t ck_floats = 0, tape_floats = 0;
+  } adj;
+  float graph_kd = 0.f;
+  const float* graph_ks = nullptr;
+
+  int grow(float** p, size_t& have, size_t need) {
+    if (need <= have && *p) return YSIM_OK;
+    if (*p) {
+      YSIM_HIP_TRY(hipStreamSynchronize(st));
+      allocs.erase(std::find(allocs.begin(), allocs.end(), (void*)*p));
+      YSIM_HIP_TRY(hipFree(*p));
+      *p = nullptr;
+    }
+    YSIM_TRY(alloc(p, need));
+    have = need;
+    return YSIM_OK;
+  }
+  // copy the cloth range of (x[par], v[par], x[par ^ 1]) to/from 3 consecutive
+  // 3V-float blocks at dst
+  int save_state(float* dst, uint32_t par) {
+    const size_t n = 3 * (size_t)adj.V, o = 3 * (size_t)adj.voff;
+    YSIM_HIP_TRY(hipMemcpyAsync(dst, x[par] + o, 4 * n, hipMemcpyDeviceToDevice, st));
+    YSIM_HIP_TRY(hipMemcpyAsync(dst + n, v[par] + o, 4 * n, hipMemcpyDeviceToDevice, st));
+    YSIM_HIP_TRY(hipMemcpyAsync(dst + 2 * n, x[par ^ 1] + o, 4 * n, hipMemcpyDeviceToDevice, st));
+    return YSIM_OK;
+  }
+  int load_state(const float* src, uint32_t par) {
+    const size_t n = 3 * (size_t)adj.V, o = 3 * (size_t)adj.voff;
+    YSIM_HIP_TRY(hipMemcpyAsync(x[par] + o, src, 4 * n, hipMemcpyDeviceToDevice, st));
+    YSIM_HIP_TRY(hipMemcpyAsync(v[par] + o, src + n, 4 * n, hipMemcpyDeviceToDevice, st));
+    YSIM_HIP_TRY(hipMemcpyAsync(x[par ^ 1] + o, src + 2 * n, 4 * n, hipMemcpyDeviceToDevice, st));
+    return YSIM_OK;
+  }
+  void drop_graph() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    graph = nullptr;
+  }
 
   template <typename T>
   int alloc(T** p, size_t n) {
@@ -348,6 +396,8 @@ struct ysim_scene {
     YSIM_HIP_TRY(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0));
     YSIM_HIP_TRY(hipGraphDestroy(g));
     graph_period = period;
+    graph_kd = P.kd;
+    graph_ks = kspring;
     return YSIM_OK;
   }
 
@@ -1217,6 +1267,141 @@ int ysim_hip_scene_flags(ysim_scene* s, uint32_t* h_flags) {
   YSIM_ARG(s && h_flags, "scene_flags: null argument");
   YSIM_HIP_TRY(hipMemcpyAsync(h_flags, s->flags, 4, hipMemcpyDeviceToHost, s->st));
   YSIM_HIP_TRY(hipStreamSynchronize(s->st));
+  return YSIM_OK;
+}
+
+}  // extern "C"
+
+// ====================================================================== adjoint
+namespace {
+int adj_check(ysim_scene* s) {
+  if (s->dyn.size() != 1)
+    return fail(YSIM_ERR_UNSUPPORTED, "adjoint: the scene must hold exactly one FastGrid cloth");
+  if (s->P.self_collision)
+    return fail(YSIM_ERR_UNSUPPORTED, "adjoint: contacts must be against static meshes (self_collision = 0)");
+  if (s->cap > 64) return fail(YSIM_ERR_UNSUPPORTED, "adjoint: contact_capacity above 64");
+  if (s->stage_phase != 0) return fail(YSIM_ERR_STATE, "adjoint: a staged substep is in progress");
+  const ysim_mesh_desc& d = s->meshes[s->dyn[0]];
+  if (d.cloth.particle_num_1d < 3 || (uint64_t)d.cloth.particle_num_1d * d.cloth.particle_num_1d != d.num_vertices)
+    return fail(YSIM_ERR_INVALID, "adjoint: cloth is not an N x N grid with N >= 3");
+  return YSIM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int ysim_hip_adj_num_springs(ysim_scene* s, uint32_t* h_out) {
+  YSIM_ARG(s && h_out, "adj_num_springs: null argument");
+  YSIM_TRY(adj_check(s));
+  *h_out = SpringIdx(s->meshes[s->dyn[0]].cloth.particle_num_1d).total;
+  return YSIM_OK;
+}
+
+int ysim_hip_adj_forward(ysim_scene* s, const float* d_log_k, float kd, uint32_t steps, uint32_t checkpoint_every) {
+  YSIM_ARG(s && d_log_k, "adj_forward: null argument");
+  YSIM_ARG(steps > 0 && checkpoint_every > 0, "adj_forward: steps and checkpoint_every must be positive");
+  YSIM_TRY(adj_check(s));
+  auto& A = s->adj;
+  A.valid = false;
+  A.mesh = s->dyn[0];
+  A.V = s->meshes[A.mesh].num_vertices;
+  A.voff = s->voff[A.mesh];
+  A.S = SpringIdx(s->meshes[A.mesh].cloth.particle_num_1d).total;
+  if (!s->kspring) YSIM_TRY(s->alloc(&s->kspring, A.S));
+  YSIM_TRY(launch_adj_exp(s->st, d_log_k, s->kspring, A.S));
+  s->P.kd = kd;
+  if (s->graph && (s->graph_kd != kd || s->graph_ks != s->kspring)) s->drop_graph();
+  A.steps = steps;
+  A.K = checkpoint_every;
+  A.nseg = (steps + A.K - 1) / A.K;
+  const size_t blk = 9 * (size_t)A.V;
+  YSIM_TRY(s->grow(&A.ck, A.ck_floats, blk * (A.nseg + 1)));
+  A.step0 = s->step;
+  for (uint32_t sg = 0; sg < A.nseg; ++sg) {
+    YSIM_TRY(s->save_state(A.ck + blk * sg, s->parity));
+    YSIM_TRY(s->steps(std::min(A.K, steps - sg * A.K), true));
+  }
+  YSIM_TRY(s->save_state(A.ck + blk * A.nseg, s->parity));
+  A.par_end = s->parity;
+  A.kd = kd;
+  A.valid = true;
+  return YSIM_OK;
+}
+
+int ysim_hip_adj_backward(ysim_scene* s, const float* d_x_bar, const float* d_v_bar, float* d_g_log_k, float* d_g_kd,
+                          float* d_g_x0, float* d_g_v0) {
+  YSIM_ARG(s && d_x_bar, "adj_backward: null argument");
+  YSIM_TRY(adj_check(s));
+  auto& A = s->adj;
+  if (!A.valid) return fail(YSIM_ERR_STATE, "adj_backward: no adjoint forward to differentiate");
+  if (s->P.kd != A.kd) return fail(YSIM_ERR_STATE, "adj_backward: damping changed since adj_forward");
+  const size_t n3 = 3 * (size_t)A.V, blk = 9 * (size_t)A.V;
+  size_t have = 0;
+  if (!A.xb) {
+    have = 0; YSIM_TRY(s->grow(&A.xb, have, n3));
+    have = 0; YSIM_TRY(s->grow(&A.vb, have, n3));
+    have = 0; YSIM_TRY(s->grow(&A.fb, have, n3));
+    have = 0; YSIM_TRY(s->grow(&A.kdv, have, A.V));
+    have = 0; YSIM_TRY(s->grow(&A.gk, have, A.S));
+  }
+  YSIM_TRY(s->grow(&A.tape, A.tape_floats, blk * A.K));
+  YSIM_HIP_TRY(hipMemcpyAsync(A.xb, d_x_bar, 4 * n3, hipMemcpyDeviceToDevice, s->st));
+  if (d_v_bar) YSIM_HIP_TRY(hipMemcpyAsync(A.vb, d_v_bar, 4 * n3, hipMemcpyDeviceToDevice, s->st));
+  else YSIM_TRY(s->zero(A.vb, n3));
+  YSIM_TRY(s->zero(A.gk, A.S));
+  YSIM_TRY(s->zero(A.kdv, A.V));
+
+  AdjArgs a;
+  a.P = s->meshes[A.mesh].cloth;
+  a.V = A.V;
+  a.voff = A.voff;
+  a.x = s->x[0];
+  a.v = s->v[0];
+  a.ext = s->ext;
+  a.mass = s->mass;
+  a.fixed = s->fixed;
+  a.ks = s->kspring;
+  a.subh = s->P.subh;
+  a.kair = s->P.kair;
+  a.kd = s->P.kd;
+  a.ccount = s->ccount;
+  a.cnd = s->cnd;
+  a.cid = s->cid;
+  a.cap = s->cap;
+  a.xb = A.xb;
+  a.vb = A.vb;
+  a.fb = A.fb;
+  a.gk = A.gk;
+  a.kdv = A.kdv;
+  s->timing = false;
+  for (uint32_t sg = A.nseg; sg-- > 0;) {
+    const uint32_t n0 = sg * A.K, n1 = std::min(A.steps, n0 + A.K);
+    // recompute the segment from its checkpoint, taping each step's input state
+    YSIM_TRY(s->load_state(A.ck + blk * sg, 0));
+    s->parity = 0;
+    s->step = A.step0 + n0;
+    for (uint32_t n = n0; n < n1; ++n) {
+      YSIM_TRY(s->save_state(A.tape + blk * (n - n0), s->parity));
+      YSIM_TRY(s->eager_steps(1));
+    }
+    // reverse through the segment: contacts of step n at its input state, then
+    // the two adjoint passes
+    for (uint32_t n = n1; n-- > n0;) {
+      YSIM_TRY(s->load_state(A.tape + blk * (n - n0), 0));
+      s->parity = 0;
+      s->step = A.step0 + n;
+      YSIM_TRY(s->enqueue_collision(0, false));
+      YSIM_TRY(s->enqueue_query(0));
+      YSIM_TRY(launch_adj_step(s->st, a));
+    }
+  }
+  YSIM_TRY(launch_adj_finish(s->st, A.gk, s->kspring, d_g_log_k, A.S, A.kdv, A.V, d_g_kd));
+  if (d_g_x0) YSIM_HIP_TRY(hipMemcpyAsync(d_g_x0, A.xb, 4 * n3, hipMemcpyDeviceToDevice, s->st));
+  if (d_g_v0) YSIM_HIP_TRY(hipMemcpyAsync(d_g_v0, A.vb, 4 * n3, hipMemcpyDeviceToDevice, s->st));
+  // leave the scene where the forward left it
+  YSIM_TRY(s->load_state(A.ck + blk * A.nseg, A.par_end));
+  s->parity = A.par_end;
+  s->step = A.step0 + A.steps;
   return YSIM_OK;
 }
 

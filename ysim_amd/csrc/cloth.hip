@@ -13,21 +13,6 @@
 
 namespace ysim {
 
-namespace {
-// Per-spring index of a FastGrid cloth of side N (families in the order
-// struct-x, struct-y, shear-a, shear-b, bend-x, bend-y; see the adjoint API).
-struct SpringIdx {
-  uint32_t N, SY, SA, SB, BX, BY;
-  __device__ explicit SpringIdx(uint32_t n) : N(n) {
-    SY = N * (N - 1);
-    SA = SY + (N - 1) * N;
-    SB = SA + (N - 1) * (N - 1);
-    BX = SB + (N - 1) * (N - 1);
-    BY = BX + N * (N - 2);
-  }
-};
-}  // namespace
-
 __global__ __launch_bounds__(256) void k_cloth(ClothArgs a) {
   const uint32_t j = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (j >= a.nitems) return;
@@ -37,34 +22,9 @@ __global__ __launch_bounds__(256) void k_cloth(ClothArgs a) {
   const uint32_t vb = a.mt.voff[m];
   const uint32_t g = vb + i;
   const ysim_fastgrid_params P = a.cloth[m];
-  const uint32_t N = P.particle_num_1d;
-  const uint32_t row = i / N, col = i % N;
-  const float* X = a.x;
-  const float* V = a.v;
-  const f3 pos0 = ld3(X, g), vel0 = ld3(V, g);
-  const float kd = a.kd;
+  const f3 pos0 = ld3(a.x, g), vel0 = ld3(a.v, g);
   const float* KS = (a.kspring && sg == 0) ? a.kspring : nullptr;
-  const SpringIdx S(N);
-
-  f3 force = ld3(a.ext, g) + vel0 * a.kair;
-  auto nb = [&](uint32_t r, uint32_t c, float rest, float k, uint32_t sidx) {
-    const uint32_t q = vb + r * N + c;
-    const float kk = KS ? KS[sidx] : k;
-    force = force + calc_spring(pos0, vel0, ld3(X, q), ld3(V, q), rest, kk, kd);
-  };
-  // stretch, shear, bend in the reference's fixed order (physics.metal:195-218)
-  if (col > 0) nb(row, col - 1, P.stretch_rest_x, P.kstretch, row * (N - 1) + col - 1);
-  if (col < N - 1) nb(row, col + 1, P.stretch_rest_x, P.kstretch, row * (N - 1) + col);
-  if (row > 0) nb(row - 1, col, P.stretch_rest_y, P.kstretch, S.SY + (row - 1) * N + col);
-  if (row < N - 1) nb(row + 1, col, P.stretch_rest_y, P.kstretch, S.SY + row * N + col);
-  if (col > 0 && row > 0) nb(row - 1, col - 1, P.shear_rest_a, P.kshear, S.SA + (row - 1) * (N - 1) + col - 1);
-  if (col < N - 1 && row > 0) nb(row - 1, col + 1, P.shear_rest_b, P.kshear, S.SB + (row - 1) * (N - 1) + col);
-  if (col > 0 && row < N - 1) nb(row + 1, col - 1, P.shear_rest_b, P.kshear, S.SB + row * (N - 1) + col - 1);
-  if (col < N - 1 && row < N - 1) nb(row + 1, col + 1, P.shear_rest_a, P.kshear, S.SA + row * (N - 1) + col);
-  if (col > 1) nb(row, col - 2, P.bend_rest_x, P.kbend, S.BX + row * (N - 2) + col - 2);
-  if (col < N - 2) nb(row, col + 2, P.bend_rest_x, P.kbend, S.BX + row * (N - 2) + col);
-  if (row > 1) nb(row - 2, col, P.bend_rest_y, P.kbend, S.BY + (row - 2) * N + col);
-  if (row < N - 2) nb(row + 2, col, P.bend_rest_y, P.kbend, S.BY + row * N + col);
+  const f3 force = grid_force(P, i, vb, a.x, a.v, pos0, vel0, ld3(a.ext, g), a.kair, a.kd, KS);
   if (a.f_out) st3(a.f_out, g, force);
 
   // integrate_cloth_grid

@@ -100,6 +100,58 @@ int launch_contact_csr(hipStream_t s, const ysim_narrow_collision* c, uint32_t n
                        ysim_narrow_collision* sorted, uint32_t* scratch /* nv + 1024 words */);
 
 // ---------------------------------------------------------------- cloth.hip
+// Per-spring index of a FastGrid cloth of side N, families in the order
+// struct-x (r,c)-(r,c+1), struct-y (r,c)-(r+1,c), shear-a (r,c)-(r+1,c+1),
+// shear-b (r,c+1)-(r+1,c), bend-x (r,c)-(r,c+2), bend-y (r,c)-(r+2,c), each
+// row-major over its first endpoint.
+struct SpringIdx {
+  uint32_t N, SY, SA, SB, BX, BY, total;
+  YSIM_HD explicit SpringIdx(uint32_t n) : N(n) {
+    SY = N * (N - 1);
+    SA = SY + (N - 1) * N;
+    SB = SA + (N - 1) * (N - 1);
+    BX = SB + (N - 1) * (N - 1);
+    BY = BX + N * (N - 2);
+    total = BY + (N - 2) * N;
+  }
+};
+
+// Visit the (up to) 12 grid springs of local vertex i in the gather order of
+// compute_cloth_grid_forces_fast (src/metal/physics.metal:195-218):
+// fn(local neighbour, rest length, family k, spring index).
+template <class Fn>
+__device__ __forceinline__ void for_grid_springs(const ysim_fastgrid_params& P, uint32_t i, Fn&& fn) {
+  const uint32_t N = P.particle_num_1d;
+  const SpringIdx S(N);
+  const uint32_t row = i / N, col = i % N;
+  if (col > 0) fn(row * N + col - 1, P.stretch_rest_x, P.kstretch, row * (N - 1) + col - 1);
+  if (col < N - 1) fn(row * N + col + 1, P.stretch_rest_x, P.kstretch, row * (N - 1) + col);
+  if (row > 0) fn((row - 1) * N + col, P.stretch_rest_y, P.kstretch, S.SY + (row - 1) * N + col);
+  if (row < N - 1) fn((row + 1) * N + col, P.stretch_rest_y, P.kstretch, S.SY + row * N + col);
+  if (col > 0 && row > 0) fn((row - 1) * N + col - 1, P.shear_rest_a, P.kshear, S.SA + (row - 1) * (N - 1) + col - 1);
+  if (col < N - 1 && row > 0) fn((row - 1) * N + col + 1, P.shear_rest_b, P.kshear, S.SB + (row - 1) * (N - 1) + col);
+  if (col > 0 && row < N - 1) fn((row + 1) * N + col - 1, P.shear_rest_b, P.kshear, S.SB + row * (N - 1) + col - 1);
+  if (col < N - 1 && row < N - 1) fn((row + 1) * N + col + 1, P.shear_rest_a, P.kshear, S.SA + row * (N - 1) + col);
+  if (col > 1) fn(row * N + col - 2, P.bend_rest_x, P.kbend, S.BX + row * (N - 2) + col - 2);
+  if (col < N - 2) fn(row * N + col + 2, P.bend_rest_x, P.kbend, S.BX + row * (N - 2) + col);
+  if (row > 1) fn((row - 2) * N + col, P.bend_rest_y, P.kbend, S.BY + (row - 2) * N + col);
+  if (row < N - 2) fn((row + 2) * N + col, P.bend_rest_y, P.kbend, S.BY + row * N + col);
+}
+
+// Spring + damping + air force on local vertex i (global offset vb): ext + kair v
+// + the 12 calc_spring terms in gather order.  KS (optional) holds one
+// stiffness per spring and replaces the family constants.
+__device__ __forceinline__ f3 grid_force(const ysim_fastgrid_params& P, uint32_t i, uint32_t vb, const float* X,
+                                         const float* V, f3 pos0, f3 vel0, f3 ext, float kair, float kd,
+                                         const float* KS) {
+  f3 force = ext + vel0 * kair;
+  for_grid_springs(P, i, [&](uint32_t q, float rest, float k, uint32_t sidx) {
+    const float kk = KS ? KS[sidx] : k;
+    force = force + calc_spring(pos0, vel0, ld3(X, vb + q), ld3(V, vb + q), rest, kk, kd);
+  });
+  return force;
+}
+
 struct ClothArgs {
   SegTable dsegs;                     // dynamic vertex segments (== query segments)
   MeshTables mt;
@@ -126,5 +178,34 @@ struct ClothArgs {
   uint32_t* flags = nullptr;          // bit0 anomaly
 };
 int launch_cloth(hipStream_t s, const ClothArgs& a);
+
+// ---------------------------------------------------------------- adjoint.hip
+// Adjoint of springs + integrate + contact projection for one FastGrid cloth
+// whose contacts are against static meshes (contact slots indexed by the
+// cloth-local vertex).  Adjoint arrays (xb, vb, fb, kdv) are cloth-local.
+struct AdjArgs {
+  ysim_fastgrid_params P{};
+  uint32_t V = 0, voff = 0;
+  const float* x = nullptr;     // scene arrays (global vertex index)
+  const float* v = nullptr;
+  const float* ext = nullptr;
+  const float* mass = nullptr;
+  const float* fixed = nullptr;
+  const float* ks = nullptr;    // per-spring stiffness
+  float subh = 0, kair = 0, kd = 0;
+  const uint32_t* ccount = nullptr;
+  const float4* cnd = nullptr;
+  const uint2* cid = nullptr;
+  uint32_t cap = 16;
+  float* xb = nullptr;
+  float* vb = nullptr;
+  float* fb = nullptr;
+  float* gk = nullptr;          // per-spring d/dk accumulator
+  float* kdv = nullptr;         // per-vertex d/dkd accumulator
+};
+int launch_adj_exp(hipStream_t s, const float* theta, float* k, uint32_t n);
+int launch_adj_step(hipStream_t s, const AdjArgs& a);
+int launch_adj_finish(hipStream_t s, const float* gk, const float* k, float* g_log_k, uint32_t n,
+                      const float* kdv, uint32_t V, float* g_kd);
 
 }  // namespace ysim

@@ -266,3 +266,47 @@ class HipScene:
         f = C.c_uint32(0)
         call("ysim_hip_scene_flags", self.h, C.byref(f))
         return f.value
+
+    # ------------------------------------------------------------ differentiation
+    # Adjoint of the substep for a one-cloth scene with static obstacles
+    # (include/ysim_hip.h, "differentiation").  The *_ptr forms take device
+    # pointers (e.g. torch tensors' data_ptr()); the array forms stage numpy
+    # data through DeviceArrays.
+    def num_springs(self) -> int:
+        n = C.c_uint32(0)
+        call("ysim_hip_adj_num_springs", self.h, C.byref(n))
+        return n.value
+
+    def cloth_vertices(self) -> int:
+        return int(self.scene.meshes[0].x.shape[0])
+
+    def adj_forward_ptr(self, d_log_k: int, kd: float, steps: int, checkpoint_every: int = 10):
+        call("ysim_hip_adj_forward", self.h, vp(d_log_k), float(kd), int(steps), int(checkpoint_every))
+
+    def adj_backward_ptr(self, d_x_bar: int, d_v_bar=None, d_g_log_k=None, d_g_kd=None, d_g_x0=None,
+                         d_g_v0=None):
+        call("ysim_hip_adj_backward", self.h, vp(d_x_bar), vp(d_v_bar), vp(d_g_log_k), vp(d_g_kd), vp(d_g_x0),
+             vp(d_g_v0))
+
+    def adj_forward(self, log_k: np.ndarray, kd: float, steps: int, checkpoint_every: int = 10):
+        lk = np.ascontiguousarray(log_k, np.float32)
+        if lk.size != self.num_springs():
+            raise ValueError(f"log_k has {lk.size} entries, the cloth has {self.num_springs()} springs")
+        self._adj_lk = self.ctx.to_device(lk)
+        self.adj_forward_ptr(self._adj_lk.ptr.value, kd, steps, checkpoint_every)
+
+    def adj_backward(self, x_bar: np.ndarray, v_bar: Optional[np.ndarray] = None):
+        """-> (dL/dlog_k, dL/dkd, dL/dx_0, dL/dv_0) as numpy arrays."""
+        Vc = self.cloth_vertices()
+        xb = self.ctx.to_device(np.ascontiguousarray(x_bar, np.float32).reshape(-1))
+        if xb.nbytes != 12 * Vc:
+            raise ValueError("x_bar must hold 3 floats per cloth vertex")
+        vb = None if v_bar is None else self.ctx.to_device(np.ascontiguousarray(v_bar, np.float32).reshape(-1))
+        S = self.num_springs()
+        g = self.ctx.zeros(S, np.float32)
+        gkd = self.ctx.zeros(1, np.float32)
+        gx = self.ctx.zeros(3 * Vc, np.float32)
+        gv = self.ctx.zeros(3 * Vc, np.float32)
+        self.adj_backward_ptr(xb.ptr.value, None if vb is None else vb.ptr.value, g.ptr.value, gkd.ptr.value,
+                              gx.ptr.value, gv.ptr.value)
+        return g.get(), float(gkd.get()[0]), gx.get().reshape(-1, 3), gv.get().reshape(-1, 3)

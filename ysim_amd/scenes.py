@@ -340,14 +340,30 @@ def instance_scene(batched: Scene, i: int) -> Scene:
     return Scene([cl, sp], dict(batched.params), name=f"{batched.name}-instance-{i}")
 
 
-def config_c4(seed: int = 4, n: int = 128):
-    """Differentiation: 128x128 cloth over the UV sphere, 60 steps at 1/600."""
+def config_c4(seed: int = 4, n: int = 128, height: float = 0.81, steps: int = 60, checkpoint_every: int = 10):
+    """Differentiation: n x n cloth dropped onto a UV sphere (top at y = 0.8) from
+    `height`, `steps` substeps at 1/600 with checkpoints every `checkpoint_every`.
+    Contacts are against the static sphere only (no self-collision), the case the
+    adjoint covers.  Hidden per-spring log-stiffness ~ N(log 500, 0.3^2) defines
+    the target drape; the loss is 0.5 |x_N - x_target|^2."""
     rng = np.random.default_rng(seed)
     l = 1.0 / (n - 1)
-    cl = cloth_mesh(n, 1.0, (0.0, 0.85, 0.0), rng, mass=1.0)
+    cl = cloth_mesh(n, 1.0, (0.0, height, 0.0), rng, mass=1.0)
     sx, sf = uv_sphere(32, 16, 0.3, (0.5, 0.5, 0.5))
-    p = sim_params(subh=1 / 600, margin=0.5 * l, morton_bits=30, rebuild_every=0)
-    return Scene([cl, obstacle(sx, sf)], p, name="c4-128x128-adjoint", info=dict(substeps=60))
+    p = sim_params(subh=1 / 600, margin=0.5 * l, morton_bits=30, rebuild_every=0, self_collision=0)
+    return Scene([cl, obstacle(sx, sf)], p, name=f"c4-{n}x{n}-adjoint",
+                 info=dict(substeps=steps, checkpoint_every=checkpoint_every, log_k_mean=math.log(500.0),
+                           log_k_std=0.3, fd_eps=1e-3, fd_springs=64))
+
+
+def c4_theta(scene, seed: int = 44):
+    """(theta_target, theta_0): hidden per-spring log-k and the uniform start."""
+    n = int(scene.meshes[0].cloth["particle_num_1d"])
+    S = n * (n - 1) * 2 + 2 * (n - 1) * (n - 1) + 2 * n * (n - 2)
+    rng = np.random.default_rng(seed)
+    mu, sd = scene.info["log_k_mean"], scene.info["log_k_std"]
+    tgt = (mu + sd * rng.standard_normal(S)).astype(F32)
+    return tgt, np.full(S, mu, F32)
 
 
 def config_c5(seed: int = 5, n: int = 1024, level: int = 6, broad_mode: int = 0, height: float = 1.0):

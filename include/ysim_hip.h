@@ -280,6 +280,26 @@ int ysim_hip_scene_restore(ysim_scene* scene);
  * query, out[1] substeps run, out[2] contacts of the last substep. */
 int ysim_hip_scene_counters(ysim_scene* scene, uint64_t* h_out, uint32_t reset);
 
+/* ---- tuning -----------------------------------------------------------------
+ * No counterpart in the reference (its kernels use fixed threadgroup sizes and
+ * 8-bit digits, radixsort.metal:21-112).  Launch parameters of the scene's
+ * kernels; every setting gives bit-identical results. */
+typedef struct ysim_tuning {
+  uint32_t sort_digit_bits;  /* radix digit width: 8 or 11 */
+  uint32_t sort_items;       /* keys per thread of a sort tile (tile = 256 x items): 8 or 16 */
+  uint32_t query_block;      /* traversal + narrow threads per block: 64, 128 or 256 */
+  uint32_t cloth_block;      /* springs + integrate threads per block: 128, 256 or 512 */
+  float ms_per_substep;      /* autotune: the chosen setting's measured time; else 0 */
+  float ms_per_substep_default; /* autotune: the default setting's measured time */
+} ysim_tuning;
+int ysim_hip_scene_get_tuning(ysim_scene* scene, ysim_tuning* h_out);
+int ysim_hip_scene_set_tuning(ysim_scene* scene, const ysim_tuning* h_in);
+/* Time `trial_substeps` substeps (hipGraph) of the scene from its current state
+ * under each candidate setting (coordinate descent over the four parameters),
+ * apply the fastest, and restore the state (positions, velocities, step count,
+ * flags and counters) so the results of later substeps are unchanged. */
+int ysim_hip_scene_autotune(ysim_scene* scene, uint32_t trial_substeps, ysim_tuning* h_out);
+
 /* ---- differentiation ------------------------------------------------------
  * No counterpart in the reference (it has no adjoint).  The differentiated map
  * is its substep (Simulator::update, include/sim/simulator.hpp:3512-3815) for a

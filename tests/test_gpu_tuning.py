@@ -1,0 +1,106 @@
+"""Launch tuning: every sort digit width / tile size and every query and cloth
+block size gives bit-identical results; the autotuner leaves the state
+untouched and its choice keeps results identical."""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleScene
+from ysim_amd import scenes
+from ysim_amd._lib import YsimError
+from ysim_amd.hip import HipScene
+
+pytestmark = pytest.mark.gpu
+
+SORT_CONFIGS = [(8, 8), (8, 16), (11, 8), (11, 16)]
+
+
+def _same_state(a, b):
+    for u, w in zip(a, b):
+        assert np.array_equal(u, w)
+
+
+@pytest.mark.parametrize("db,items", SORT_CONFIGS)
+def test_sort_configs_bit_exact_63bit(ctx, oracle_threads, db, items):
+    print(f"sort config digit_bits={db} items={items} on the 96x96 fold (63-bit)", flush=True)
+    sc = scenes.config_c2(n=96)
+    hs, os_ = HipScene(ctx, sc), OracleScene(sc)
+    hs.set_tuning(sort_digit_bits=db, sort_items=items)
+    assert hs.tuning()["sort_digit_bits"] == db
+    for graph in (False, True):
+        hs.step(2, graph=graph)
+        os_.step(2)
+        _same_state(hs.state(), os_.state())
+        ng, pg, kg, ig = hs.tree(0, keys=True)
+        no, po, ko, io = os_.tree(0)
+        assert np.array_equal(kg, ko) and np.array_equal(ig, io)
+        assert np.array_equal(pg, po)
+        for f in ("min", "max", "child_a", "child_b"):
+            assert np.array_equal(ng[f], no[f]), f
+    assert hs.flags() == 0
+
+
+@pytest.mark.parametrize("db,items", SORT_CONFIGS)
+def test_sort_configs_bit_exact_30bit_rebuild(ctx, oracle_threads, db, items):
+    sc = scenes.config_drape(n=40, height=0.8, rebuild_every=1)
+    hs, os_ = HipScene(ctx, sc), OracleScene(sc)
+    hs.set_tuning(sort_digit_bits=db, sort_items=items)
+    hs.step(12, graph=True)
+    os_.step(12)
+    _same_state(hs.state(), os_.state())
+    ng, pg, kg, ig = hs.tree(0, keys=True)
+    no, po, ko, io = os_.tree(0)
+    assert np.array_equal(kg, ko) and np.array_equal(ig, io)
+
+
+@pytest.mark.parametrize("qb", [64, 128, 256])
+@pytest.mark.parametrize("cb", [128, 256, 512])
+def test_block_sizes_bit_exact(ctx, oracle_threads, qb, cb):
+    sc = scenes.config_drape(n=32, height=0.8)
+    hs, os_ = HipScene(ctx, sc), OracleScene(sc)
+    hs.set_tuning(query_block=qb, cloth_block=cb)
+    hs.step(20, graph=True)
+    os_.step(20)
+    _same_state(hs.state(), os_.state())
+    og, cg = hs.contacts()
+    oo, co = os_.contacts()
+    assert np.array_equal(og, oo) and len(co) > 0
+
+
+def test_autotune_keeps_results(ctx, oracle_threads):
+    print("autotune on the 128x128 fold (63-bit, rebuild every substep)", flush=True)
+    sc = scenes.config_c2(n=128)
+    a, b = HipScene(ctx, sc), HipScene(ctx, sc)
+    a.step(3, graph=True)
+    b.step(3, graph=True)
+    before = a.state()
+    c0 = a.counters()
+    t = a.autotune(6)
+    print("autotune:", t, flush=True)
+    _same_state(a.state(), before)
+    assert a.counters()["steps"] == c0["steps"]
+    assert t["ms_per_substep"] > 0 and t["ms_per_substep"] <= t["ms_per_substep_default"]
+    assert a.tuning()["query_block"] == t["query_block"]
+    a.step(8, graph=True)
+    b.step(8, graph=True)
+    _same_state(a.state(), b.state())
+    os_ = OracleScene(sc)
+    os_.step(11)
+    _same_state(a.state(), os_.state())
+
+
+def test_autotune_batched_keeps_results(ctx):
+    sc = scenes.config_c3(count=24, hi=96)
+    a, b = HipScene(ctx, sc), HipScene(ctx, sc)
+    t = a.autotune(4)
+    print("autotune batched:", t, flush=True)
+    a.step(10, graph=True)
+    b.step(10, graph=True)
+    _same_state(a.state(), b.state())
+
+
+def test_invalid_tuning_rejected(ctx):
+    hs = HipScene(ctx, scenes.config_drape(n=16))
+    for kw in ({"sort_digit_bits": 9}, {"sort_items": 4}, {"query_block": 96}, {"cloth_block": 1024}):
+        with pytest.raises(YsimError):
+            hs.set_tuning(**kw)
+    hs.step(2)

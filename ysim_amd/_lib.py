@@ -69,6 +69,9 @@ _SIG = {
     "ysim_hip_scene_counters": (C.c_int, [vp, u64p, u32]),
     "ysim_hip_scene_snapshot": (C.c_int, [vp]),
     "ysim_hip_scene_restore": (C.c_int, [vp]),
+    "ysim_hip_scene_get_tuning": (C.c_int, [vp, vp]),
+    "ysim_hip_scene_set_tuning": (C.c_int, [vp, vp]),
+    "ysim_hip_scene_autotune": (C.c_int, [vp, u32, vp]),
     "ysim_hip_adj_num_springs": (C.c_int, [vp, u32p]),
     "ysim_hip_adj_forward": (C.c_int, [vp, vp, C.c_float, u32, u32]),
     "ysim_hip_adj_backward": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),

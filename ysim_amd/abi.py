@@ -34,6 +34,15 @@ class SimParams(C.Structure):
                 ("hash_cell", C.c_float), ("contact_capacity", C.c_uint32)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("sort_digit_bits", C.c_uint32), ("sort_items", C.c_uint32), ("query_block", C.c_uint32),
+                ("cloth_block", C.c_uint32), ("ms_per_substep", C.c_float),
+                ("ms_per_substep_default", C.c_float)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 assert C.sizeof(Aabb4) == 32
 assert C.sizeof(FastGridParams) == 44
 assert C.sizeof(MeshDesc) == 24 + 44

@@ -102,7 +102,9 @@ struct ysim_scene {
   uint32_t *d_tgt_off = nullptr, *d_tgt = nullptr;
   void* keys[2] = {nullptr, nullptr};
   uint32_t* vals[2] = {nullptr, nullptr};
-  uint32_t *hist = nullptr, *status = nullptr, *counters = nullptr, *box_acc = nullptr;
+  uint32_t *hist = nullptr, *counters = nullptr, *box_acc = nullptr;
+  uint64_t* status = nullptr;   // sort look-back words (epoch-tagged, never cleared)
+  uint32_t* epoch = nullptr;    // [0] look-back epoch of the current sort
   uint32_t cap = 16;
   uint32_t* ccount = nullptr;
   float4* cnd = nullptr;
@@ -145,6 +147,8 @@ struct ysim_scene {
   } adj;
   float graph_kd = 0.f;
   const float* graph_ks = nullptr;
+  ysim_tuning tune{8, 8, 128, 256, 0.f, 0.f};
+  float* tune_buf = nullptr;   // autotune state save: x, v, xPrev of the whole scene
 
   int grow(float** p, size_t& have, size_t need) {
     if (need <= have && *p) return YSIM_OK;
@@ -177,6 +181,124 @@ struct ysim_scene {
   void drop_graph() {
     if (graph) (void)hipGraphExecDestroy(graph);
     graph = nullptr;
+  }
+
+  int apply_tuning(const ysim_tuning& t) {
+    if (!sort_config_ok(t.sort_digit_bits, t.sort_items))
+      return fail(YSIM_ERR_INVALID, "tuning: sort_digit_bits must be 8 or 11 and sort_items 8 or 16");
+    if (!query_block_ok(t.query_block)) return fail(YSIM_ERR_INVALID, "tuning: query_block must be 64, 128 or 256");
+    if (!cloth_block_ok(t.cloth_block)) return fail(YSIM_ERR_INVALID, "tuning: cloth_block must be 128, 256 or 512");
+    const bool replan = t.sort_digit_bits != tune.sort_digit_bits || t.sort_items != tune.sort_items;
+    tune.sort_digit_bits = t.sort_digit_bits;
+    tune.sort_items = t.sort_items;
+    tune.query_block = t.query_block;
+    tune.cloth_block = t.cloth_block;
+    if (replan) {
+      for (auto& g : dyn_groups) g.plan = sort_plan(g.T, g.plan.key_bits, t.sort_digit_bits, t.sort_items);
+      YSIM_TRY(zero(hist, SORT_HIST_WORDS));
+    }
+    drop_graph();
+    return YSIM_OK;
+  }
+
+  // whole-scene state save/restore for autotune trials
+  int tune_save(uint32_t* fl_np) {
+    const size_t n = 3 * (size_t)Vtot;
+    if (!tune_buf) YSIM_TRY(alloc(&tune_buf, 3 * n + 4));
+    YSIM_HIP_TRY(hipMemcpyAsync(tune_buf, x[parity], 4 * n, hipMemcpyDeviceToDevice, st));
+    YSIM_HIP_TRY(hipMemcpyAsync(tune_buf + n, v[parity], 4 * n, hipMemcpyDeviceToDevice, st));
+    YSIM_HIP_TRY(hipMemcpyAsync(tune_buf + 2 * n, x[parity ^ 1], 4 * n, hipMemcpyDeviceToDevice, st));
+    YSIM_HIP_TRY(hipMemcpyAsync(fl_np, flags, 4, hipMemcpyDeviceToHost, st));
+    YSIM_HIP_TRY(hipMemcpyAsync(fl_np + 1, npairs, 4, hipMemcpyDeviceToHost, st));
+    YSIM_HIP_TRY(hipStreamSynchronize(st));
+    return YSIM_OK;
+  }
+  int tune_restore(uint32_t par, uint64_t stp, const uint32_t* fl_np) {
+    const size_t n = 3 * (size_t)Vtot;
+    YSIM_HIP_TRY(hipMemcpyAsync(x[par], tune_buf, 4 * n, hipMemcpyDeviceToDevice, st));
+    YSIM_HIP_TRY(hipMemcpyAsync(v[par], tune_buf + n, 4 * n, hipMemcpyDeviceToDevice, st));
+    YSIM_HIP_TRY(hipMemcpyAsync(x[par ^ 1], tune_buf + 2 * n, 4 * n, hipMemcpyDeviceToDevice, st));
+    YSIM_HIP_TRY(hipMemcpyAsync(flags, fl_np, 4, hipMemcpyHostToDevice, st));
+    YSIM_HIP_TRY(hipMemcpyAsync(npairs, fl_np + 1, 4, hipMemcpyHostToDevice, st));
+    YSIM_HIP_TRY(hipStreamSynchronize(st));
+    parity = par;
+    step = stp;
+    return YSIM_OK;
+  }
+
+  int autotune(uint32_t trial, ysim_tuning* out) {
+    if (stage_phase != 0) return fail(YSIM_ERR_STATE, "autotune: a staged substep is in progress");
+    if (trial == 0) trial = 10;
+    uint32_t saved[2] = {0, 0};
+    YSIM_TRY(tune_save(saved));
+    const uint32_t par0 = parity;
+    const uint64_t step0 = step;
+    const uint64_t run0 = steps_run;
+    const bool timing0 = timing;
+    timing = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    YSIM_HIP_TRY(hipEventCreate(&e0));
+    YSIM_HIP_TRY(hipEventCreate(&e1));
+    auto measure = [&](const ysim_tuning& t, float& ms) -> int {
+      YSIM_TRY(apply_tuning(t));
+      YSIM_TRY(tune_restore(par0, step0, saved));
+      YSIM_TRY(steps(2, true));  // capture + warm
+      YSIM_TRY(tune_restore(par0, step0, saved));
+      YSIM_HIP_TRY(hipEventRecord(e0, st));
+      YSIM_TRY(steps(trial, true));
+      YSIM_HIP_TRY(hipEventRecord(e1, st));
+      YSIM_HIP_TRY(hipEventSynchronize(e1));
+      YSIM_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+      ms /= (float)trial;
+      return YSIM_OK;
+    };
+    ysim_tuning best = tune;
+    float best_ms = 0.f;
+    int rc = measure(best, best_ms);
+    const float default_ms = best_ms;
+    const bool rebuilds = P.rebuild_every > 0 && !hash_mode;
+    struct Cand { uint32_t a, b; };
+    // coordinate descent: sort (only when the substep rebuilds), query block, cloth block
+    if (rc == YSIM_OK && rebuilds) {
+      for (Cand c : {Cand{8, 8}, Cand{8, 16}, Cand{11, 8}, Cand{11, 16}}) {
+        ysim_tuning t = best;
+        t.sort_digit_bits = c.a;
+        t.sort_items = c.b;
+        if (t.sort_digit_bits == best.sort_digit_bits && t.sort_items == best.sort_items) continue;
+        float ms = 0.f;
+        if ((rc = measure(t, ms)) != YSIM_OK) break;
+        if (ms < best_ms) { best_ms = ms; best = t; }
+      }
+    }
+    if (rc == YSIM_OK)
+      for (uint32_t qb : {64u, 128u, 256u}) {
+        if (qb == best.query_block) continue;
+        ysim_tuning t = best;
+        t.query_block = qb;
+        float ms = 0.f;
+        if ((rc = measure(t, ms)) != YSIM_OK) break;
+        if (ms < best_ms) { best_ms = ms; best = t; }
+      }
+    if (rc == YSIM_OK)
+      for (uint32_t cb : {128u, 256u, 512u}) {
+        if (cb == best.cloth_block) continue;
+        ysim_tuning t = best;
+        t.cloth_block = cb;
+        float ms = 0.f;
+        if ((rc = measure(t, ms)) != YSIM_OK) break;
+        if (ms < best_ms) { best_ms = ms; best = t; }
+      }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    YSIM_TRY(rc);
+    YSIM_TRY(apply_tuning(best));
+    YSIM_TRY(tune_restore(par0, step0, saved));
+    steps_run = run0;
+    timing = timing0;
+    tune.ms_per_substep = best_ms;
+    tune.ms_per_substep_default = default_ms;
+    if (out) *out = tune;
+    return YSIM_OK;
   }
 
   template <typename T>
@@ -242,7 +364,7 @@ struct ysim_scene {
       g.code_bits = cb;
       uint32_t key_bits = cb + (l.size() > 1 ? bits_for((uint32_t)l.size()) : 0);
       g.key64 = key_bits > 32;
-      g.plan = sort_plan(g.T, key_bits);
+      g.plan = sort_plan(g.T, key_bits, tune.sort_digit_bits, tune.sort_items);
       out.push_back(g);
     }
     return YSIM_OK;
@@ -253,9 +375,10 @@ struct ysim_scene {
     YSIM_TRY(launch_box(st, g.vsegs, mt, xc, g.V, box_acc));
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[1], st));
     YSIM_TRY(launch_morton(st, g.tsegs, mt, xc, facets, box_acc, g.code_bits, g.key64, keys[0], vals[0], g.T,
-                           g.plan, hist, status, counters));
+                           g.plan, hist, counters, epoch));
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[2], st));
-    YSIM_TRY(launch_sort_passes(st, g.plan, keys[0], vals[0], keys[1], vals[1], hist, status, counters, flags));
+    YSIM_TRY(launch_sort_passes(st, g.plan, keys[0], vals[0], keys[1], vals[1], hist, status, counters, epoch,
+                                flags));
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[3], st));
     const uint32_t sorted = (g.plan.passes % 2 == 0) ? 0 : 1;
     YSIM_TRY(launch_hierarchy(st, g.tsegs, mt, keys[sorted], g.key64, vals[sorted], g.code_bits, tree, parent,
@@ -287,6 +410,7 @@ struct ysim_scene {
     a.cid = cid;
     a.npairs = npairs;
     a.flags = flags;
+    a.block = tune.query_block;
     return a;
   }
 
@@ -356,6 +480,7 @@ struct ysim_scene {
     ca.v_out = v[par ^ 1];
     ca.nitems = Vdyn;
     ca.flags = flags;
+    ca.block = tune.cloth_block;
     YSIM_TRY(launch_cloth(st, ca));
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[7], st));
     return YSIM_OK;
@@ -574,13 +699,13 @@ int ysim_hip_morton(ysim_ctx* ctx, const float* d_x, const uint32_t* d_facets, u
   };
   uint32_t acc_h[6] = {key(hb.min[0]), key(hb.min[1]), key(hb.min[2]), key(hb.max[0]), key(hb.max[1]), key(hb.max[2])};
   uint32_t* acc = scratch_words(ctx, 64);
-  uint32_t* hist = scratch_words(ctx, 128);  // 2048 words
+  uint32_t* hist = scratch_words(ctx, 128);  // unused: no histogram passes
   YSIM_HIP_TRY(hipMemcpyAsync(acc, acc_h, sizeof(acc_h), hipMemcpyHostToDevice, ctx->stream));
   SortPlan none = sort_plan(0, bits);
   none.passes = 0;
   // vertex count is not needed: facets index into d_x directly
   YSIM_TRY(launch_morton(ctx->stream, u.seg(), u.mt(), d_x, d_facets, acc, bits, bits > 32, d_keys, d_idx,
-                         num_facets, none, hist, nullptr, scratch_words(ctx, 4096)));
+                         num_facets, none, hist, nullptr, nullptr));
   YSIM_HIP_TRY(hipStreamSynchronize(ctx->stream));
   return YSIM_OK;
 }
@@ -592,23 +717,25 @@ int ysim_hip_sort_pairs(ysim_ctx* ctx, void* d_keys, uint32_t* d_vals, uint32_t 
   YSIM_ARG(n < (1u << 30), "sort_pairs: n must be < 2^30");
   SortPlan p = sort_plan(n, key_bits);
   const size_t kb = (p.wide ? 8 : 4);
-  // layout: [4096 words misc][hist][counters][err 16][status][pad to 256 B][keys_b, 256-B rounded][vals_b]
+  // layout: [4096 words misc][hist][counters][err 16][epoch 16][status (64-bit words)][pad to 256 B]
+  //         [keys_b, 256-B rounded][vals_b]
   const size_t tmp_bytes = 256 + (((size_t)n * kb + 255) & ~(size_t)255) + (size_t)n * 4 + 256;
-  const size_t words = 4096 + p.hist_words + p.counter_words + 16 + p.status_words + tmp_bytes / 4 + 1;
+  const size_t words = 4096 + p.hist_words + p.counter_words + 32 + 2 * p.status_words + tmp_bytes / 4 + 2;
   YSIM_TRY(ctx->ensure_scratch(words * 4));
   uint32_t* base = (uint32_t*)ctx->scratch;
   uint32_t* hist = base + 4096;
   uint32_t* counters = hist + p.hist_words;
   uint32_t* err = counters + p.counter_words;
-  uint32_t* status = err + 16;
+  uint32_t* epoch = err + 16;
+  uint64_t* status = (uint64_t*)(((uintptr_t)(epoch + 16) + 7) & ~(uintptr_t)7);
   char* tmp = (char*)(status + p.status_words);
   tmp = (char*)(((uintptr_t)tmp + 255) & ~(uintptr_t)255);
   void* keys_b = tmp;
   uint32_t* vals_b = (uint32_t*)(tmp + ((n * kb + 255) & ~(size_t)255));
-  YSIM_TRY(launch_sort_reset(ctx->stream, p, hist, status, counters, true));
+  YSIM_TRY(launch_sort_reset(ctx->stream, p, hist, status, counters, epoch, true));
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, err, 16u);
   YSIM_TRY(launch_sort_histogram(ctx->stream, p, d_keys, hist));
-  YSIM_TRY(launch_sort_passes(ctx->stream, p, d_keys, d_vals, keys_b, vals_b, hist, status, counters, err));
+  YSIM_TRY(launch_sort_passes(ctx->stream, p, d_keys, d_vals, keys_b, vals_b, hist, status, counters, epoch, err));
   if (p.passes % 2 == 1) {
     YSIM_HIP_TRY(hipMemcpyAsync(d_keys, keys_b, n * kb, hipMemcpyDeviceToDevice, ctx->stream));
     YSIM_HIP_TRY(hipMemcpyAsync(d_vals, vals_b, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
@@ -890,7 +1017,7 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
   for (auto* gs : {&s.dyn_groups, &s.stat_groups})
     for (auto& g : *gs) {
       maxT = std::max<size_t>(maxT, g.T);
-      maxStatus = std::max(maxStatus, g.plan.status_words);
+      maxStatus = std::max(maxStatus, sort_status_words_max(g.T, g.plan.key_bits));
       maxSeg = std::max<size_t>(maxSeg, g.tsegs.nseg);
     }
   for (int b = 0; b < 2; ++b) {
@@ -899,11 +1026,14 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
     s.keys[b] = k;
     YSIM_TRY(s.alloc(&s.vals[b], maxT));
   }
-  YSIM_TRY(s.alloc(&s.hist, SORT_MAX_PASSES * 256));
+  YSIM_TRY(s.alloc(&s.hist, SORT_HIST_WORDS));
   YSIM_TRY(s.alloc(&s.status, maxStatus));
   YSIM_TRY(s.alloc(&s.counters, SORT_MAX_PASSES));
+  YSIM_TRY(s.alloc(&s.epoch, 16));
   YSIM_TRY(s.alloc(&s.box_acc, 6 * maxSeg));
-  YSIM_TRY(s.zero(s.hist, SORT_MAX_PASSES * 256));
+  YSIM_TRY(s.zero(s.hist, SORT_HIST_WORDS));
+  YSIM_TRY(s.zero(s.status, 2 * maxStatus));
+  YSIM_TRY(s.zero(s.epoch, 16));
   {
     std::vector<uint32_t> init(6 * maxSeg);
     for (size_t i = 0; i < init.size(); ++i) init[i] = (i % 6) < 3 ? 0xffffffffu : 0u;
@@ -1403,6 +1533,31 @@ int ysim_hip_adj_backward(ysim_scene* s, const float* d_x_bar, const float* d_v_
   s->parity = A.par_end;
   s->step = A.step0 + A.steps;
   return YSIM_OK;
+}
+
+}  // extern "C"
+
+// ====================================================================== tuning
+extern "C" {
+
+int ysim_hip_scene_get_tuning(ysim_scene* s, ysim_tuning* h_out) {
+  YSIM_ARG(s && h_out, "scene_get_tuning: null argument");
+  *h_out = s->tune;
+  return YSIM_OK;
+}
+
+int ysim_hip_scene_set_tuning(ysim_scene* s, const ysim_tuning* h_in) {
+  YSIM_ARG(s && h_in, "scene_set_tuning: null argument");
+  if (s->stage_phase != 0) return fail(YSIM_ERR_STATE, "scene_set_tuning: a staged substep is in progress");
+  YSIM_TRY(s->apply_tuning(*h_in));
+  s->tune.ms_per_substep = 0.f;
+  s->tune.ms_per_substep_default = 0.f;
+  return YSIM_OK;
+}
+
+int ysim_hip_scene_autotune(ysim_scene* s, uint32_t trial_substeps, ysim_tuning* h_out) {
+  YSIM_ARG(s, "scene_autotune: null scene");
+  return s->autotune(trial_substeps, h_out);
 }
 
 }  // extern "C"

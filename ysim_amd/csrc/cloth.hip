@@ -13,8 +13,10 @@
 
 namespace ysim {
 
-__global__ __launch_bounds__(256) void k_cloth(ClothArgs a) {
-  const uint32_t j = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+// BS threads per block (128, 256 or 512; chosen by the autotuner).
+template <int BS>
+__global__ __launch_bounds__(BS) void k_cloth(ClothArgs a) {
+  const uint32_t j = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
   if (j >= a.nitems) return;
   const uint32_t sg = seg_of(a.dsegs, j);
   const uint32_t m = a.dsegs.mesh[sg];
@@ -96,9 +98,14 @@ __global__ __launch_bounds__(256) void k_cloth(ClothArgs a) {
   st3(a.x_out, g, pos);
 }
 
+bool cloth_block_ok(uint32_t b) { return b == 128 || b == 256 || b == 512; }
+
 int launch_cloth(hipStream_t s, const ClothArgs& a) {
   if (!a.nitems) return YSIM_OK;
-  hipLaunchKernelGGL(k_cloth, dim3(ceil_div(a.nitems, 256)), dim3(256), 0, s, a);
+  if (!cloth_block_ok(a.block)) return YSIM_ERR_INVALID;
+#define YSIM_C(B) hipLaunchKernelGGL((k_cloth<B>), dim3(ceil_div(a.nitems, B)), dim3(B), 0, s, a)
+  if (a.block == 128) YSIM_C(128); else if (a.block == 256) YSIM_C(256); else YSIM_C(512);
+#undef YSIM_C
   YSIM_CHECK_LAUNCH();
   return YSIM_OK;
 }

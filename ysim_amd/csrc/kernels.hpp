@@ -39,11 +39,12 @@ int launch_box(hipStream_t s, SegTable vsegs, MeshTables mt, const float* x, uin
                uint32_t* box_acc);
 // S1: Morton keys of every triangle of tsegs (segment id prefixed above the code
 // bits when nseg > 1), values = mesh-local facet id, plus the digit histograms
-// of all sort passes; also zeroes the sort look-back state.
+// of all sort passes of `plan`; zeroes the sort's block counters and advances
+// its look-back epoch (either may be null when no sort follows).
 int launch_morton(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x,
                   const uint32_t* facets, const uint32_t* box_acc, uint32_t code_bits,
                   bool key64, void* keys, uint32_t* vals, uint32_t nitems, const SortPlan& plan,
-                  uint32_t* hist, uint32_t* status, uint32_t* counters);
+                  uint32_t* hist, uint32_t* counters, uint32_t* epoch);
 // S3: Karras hierarchy per segment; leaf child links; zeroes the refit visit
 // counters; resets box_acc (nbox segments) and the sort histogram for the next build.
 int launch_hierarchy(hipStream_t s, SegTable tsegs, MeshTables mt, const void* keys, bool key64,
@@ -81,7 +82,9 @@ struct QueryArgs {
   // pair-emission mode (parity / drop-in broad phase)
   uint32_t* pairs = nullptr;          // 4 words per pair (qobj, point, tobj, tri)
   uint32_t pair_cap = 0;
+  uint32_t block = 128;               // threads per block: 64, 128 or 256
 };
+bool query_block_ok(uint32_t b);
 int launch_query(hipStream_t s, const QueryArgs& a, bool emit_pairs);
 // Drop-in broad query of one (query mesh, target tree): BroadCollision records.
 int launch_broad_pairs(hipStream_t s, const float* xq, uint32_t nq, uint32_t qobj, uint32_t qbeh,
@@ -176,7 +179,9 @@ struct ClothArgs {
   float* f_out = nullptr;             // optional
   uint32_t nitems = 0;
   uint32_t* flags = nullptr;          // bit0 anomaly
+  uint32_t block = 256;               // threads per block: 128, 256 or 512
 };
+bool cloth_block_ok(uint32_t b);
 int launch_cloth(hipStream_t s, const ClothArgs& a);
 
 // ---------------------------------------------------------------- adjoint.hip

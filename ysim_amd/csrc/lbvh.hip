@@ -110,19 +110,21 @@ int launch_box(hipStream_t s, SegTable vsegs, MeshTables mt, const float* x, uin
 }
 
 // ------------------------------------------------------------------ S1 Morton
-template <typename K>
+template <typename K, int DB>
 __global__ __launch_bounds__(256) void k_morton(SegTable ts, MeshTables mt, const float* __restrict__ x,
                                                 const uint32_t* __restrict__ facets,
                                                 const uint32_t* __restrict__ acc, uint32_t code_bits,
                                                 K* __restrict__ keys, uint32_t* __restrict__ vals,
                                                 uint32_t n, uint32_t passes, uint32_t* __restrict__ hist,
-                                                uint32_t* __restrict__ status, uint32_t status_words,
-                                                uint32_t* __restrict__ counters) {
-  __shared__ uint32_t s_h[SORT_MAX_PASSES * 256];
-  for (uint32_t i = threadIdx.x; i < passes * 256; i += 256) s_h[i] = 0;
-  // zero the look-back state of this build's sort passes
-  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < status_words; i += gridDim.x * 256) status[i] = 0;
-  if (blockIdx.x == 0 && threadIdx.x < SORT_MAX_PASSES) counters[threadIdx.x] = 0;
+                                                uint32_t* __restrict__ counters, uint32_t* __restrict__ epoch) {
+  constexpr uint32_t DIGITS = 1u << DB;
+  constexpr uint32_t HW = (DB == 8) ? SORT_MAX_PASSES * 256 : SORT_HIST_WORDS;
+  __shared__ uint32_t s_h[HW];
+  for (uint32_t i = threadIdx.x; i < passes * DIGITS; i += 256) s_h[i] = 0;
+  // the next sort's block counters start at zero and its look-back words carry
+  // a fresh epoch (the sort passes run after this kernel completes)
+  if (blockIdx.x == 0 && threadIdx.x < SORT_MAX_PASSES && counters) counters[threadIdx.x] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && epoch) epoch[0] = epoch[0] + 1u;
   __syncthreads();
   const bool wide = code_bits > 32;
   const uint32_t start = blockIdx.x * (256 * MORTON_ITEMS);
@@ -142,27 +144,29 @@ __global__ __launch_bounds__(256) void k_morton(SegTable ts, MeshTables mt, cons
     K key = (ts.nseg > 1) ? (K)(((uint64_t)sg << code_bits) | code) : (K)code;
     keys[j] = key;
     vals[j] = fid;
-    for (uint32_t p = 0; p < passes; ++p) atomicAdd(&s_h[p * 256 + (uint32_t)((key >> (8 * p)) & 255u)], 1u);
+    for (uint32_t p = 0; p < passes; ++p)
+      atomicAdd(&s_h[p * DIGITS + (uint32_t)((key >> (DB * p)) & (DIGITS - 1))], 1u);
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < passes * 256; i += 256)
+  for (uint32_t i = threadIdx.x; i < passes * DIGITS; i += 256)
     if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
 }
 
 int launch_morton(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x,
                   const uint32_t* facets, const uint32_t* box_acc, uint32_t code_bits, bool key64,
                   void* keys, uint32_t* vals, uint32_t nitems, const SortPlan& plan, uint32_t* hist,
-                  uint32_t* status, uint32_t* counters) {
+                  uint32_t* counters, uint32_t* epoch) {
   if (!nitems) return YSIM_OK;
   uint32_t blocks = ceil_div(nitems, 256 * MORTON_ITEMS);
-  if (key64)
-    hipLaunchKernelGGL(k_morton<uint64_t>, dim3(blocks), dim3(256), 0, s, tsegs, mt, x, facets, box_acc,
-                       code_bits, (uint64_t*)keys, vals, nitems, plan.passes, hist, status,
-                       (uint32_t)plan.status_words, counters);
-  else
-    hipLaunchKernelGGL(k_morton<uint32_t>, dim3(blocks), dim3(256), 0, s, tsegs, mt, x, facets, box_acc,
-                       code_bits, (uint32_t*)keys, vals, nitems, plan.passes, hist, status,
-                       (uint32_t)plan.status_words, counters);
+#define YSIM_MORTON(KT, DBV)                                                                                  \
+  hipLaunchKernelGGL((k_morton<KT, DBV>), dim3(blocks), dim3(256), 0, s, tsegs, mt, x, facets, box_acc,      \
+                     code_bits, (KT*)keys, vals, nitems, plan.passes, hist, counters, epoch)
+  if (plan.digit_bits == 11) {
+    if (key64) YSIM_MORTON(uint64_t, 11); else YSIM_MORTON(uint32_t, 11);
+  } else {
+    if (key64) YSIM_MORTON(uint64_t, 8); else YSIM_MORTON(uint32_t, 8);
+  }
+#undef YSIM_MORTON
   YSIM_CHECK_LAUNCH();
   return YSIM_OK;
 }

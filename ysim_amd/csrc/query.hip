@@ -17,12 +17,11 @@
 namespace ysim {
 
 namespace {
-constexpr int QB = 128;      // threads per block
 constexpr int SDEPTH = 16;   // LDS stack entries per thread
-
 }  // namespace
 
-template <bool EMIT_PAIRS>
+// QB threads per block (64, 128 or 256; chosen by the autotuner).
+template <bool EMIT_PAIRS, int QB>
 __global__ __launch_bounds__(QB) void k_query(QueryArgs a) {
   __shared__ int s_stack[SDEPTH * QB];
   const uint32_t j = xcd_block(blockIdx.x, gridDim.x) * QB + threadIdx.x;
@@ -90,12 +89,19 @@ __global__ __launch_bounds__(QB) void k_query(QueryArgs a) {
   }
 }
 
+bool query_block_ok(uint32_t b) { return b == 64 || b == 128 || b == 256; }
+
 int launch_query(hipStream_t s, const QueryArgs& a, bool emit_pairs) {
   if (!a.nitems) return YSIM_OK;
-  if (emit_pairs)
-    hipLaunchKernelGGL(k_query<true>, dim3(ceil_div(a.nitems, QB)), dim3(QB), 0, s, a);
-  else
-    hipLaunchKernelGGL(k_query<false>, dim3(ceil_div(a.nitems, QB)), dim3(QB), 0, s, a);
+  const uint32_t qb = a.block;
+  if (!query_block_ok(qb)) return YSIM_ERR_INVALID;
+#define YSIM_Q(E, B) hipLaunchKernelGGL((k_query<E, B>), dim3(ceil_div(a.nitems, B)), dim3(B), 0, s, a)
+  if (emit_pairs) {
+    if (qb == 64) YSIM_Q(true, 64); else if (qb == 128) YSIM_Q(true, 128); else YSIM_Q(true, 256);
+  } else {
+    if (qb == 64) YSIM_Q(false, 64); else if (qb == 128) YSIM_Q(false, 128); else YSIM_Q(false, 256);
+  }
+#undef YSIM_Q
   YSIM_CHECK_LAUNCH();
   return YSIM_OK;
 }

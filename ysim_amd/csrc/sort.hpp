@@ -6,25 +6,34 @@
 
 namespace ysim {
 
-constexpr uint32_t SORT_ITEMS = 8;        // keys per thread per tile (tile = 2048)
-constexpr uint32_t SORT_MAX_PASSES = 8;   // 8-bit digits, keys up to 64 bits
+constexpr uint32_t SORT_MAX_PASSES = 8;       // 8-bit digits cover 64-bit keys in 8 passes
+constexpr uint32_t SORT_HIST_WORDS = 12288;   // max over digit widths of passes * 2^digit_bits
 
+// A sort configuration: digit width (8 or 11 bits) and keys per thread (8 or
+// 16; tile = 256 * items).  Every configuration yields the same permutation.
 struct SortPlan {
   uint32_t n = 0, key_bits = 0, passes = 0, tile = 0, num_blocks = 0;
+  uint32_t digit_bits = 8, digits = 256, items = 8;
   bool wide = false;  // uint64 keys
   size_t hist_words = 0, status_words = 0, counter_words = 0;
 };
 
-SortPlan sort_plan(uint32_t n, uint32_t key_bits);
-// Zero the look-back state (and optionally the histogram) with kernels (graph-safe).
-int launch_sort_reset(hipStream_t s, const SortPlan& p, uint32_t* hist, uint32_t* status,
-                      uint32_t* counters, bool zero_hist);
+bool sort_config_ok(uint32_t digit_bits, uint32_t items);
+SortPlan sort_plan(uint32_t n, uint32_t key_bits, uint32_t digit_bits = 8, uint32_t items = 8);
+// Largest status footprint (64-bit words) over all configurations for n keys.
+size_t sort_status_words_max(uint32_t n, uint32_t key_bits);
+// Look-back status words are 64-bit: (epoch << 32) | flag << 30 | count.  A word
+// belongs to the sort whose epoch equals the device word *epoch, so the status
+// array is never cleared between sorts; launch_sort_reset clears it and sets
+// the epoch to 1 (for scratch memory of unknown contents).
+int launch_sort_reset(hipStream_t s, const SortPlan& p, uint32_t* hist, uint64_t* status,
+                      uint32_t* counters, uint32_t* epoch, bool zero_hist);
 int launch_sort_histogram(hipStream_t s, const SortPlan& p, const void* keys, uint32_t* hist);
 // Runs p.passes passes ping-ponging a <-> b; the result is in a when passes is
 // even, else in b.
 int launch_sort_passes(hipStream_t s, const SortPlan& p, void* keys_a, uint32_t* vals_a,
-                       void* keys_b, uint32_t* vals_b, const uint32_t* hist, uint32_t* status,
-                       uint32_t* counters, uint32_t* err);
+                       void* keys_b, uint32_t* vals_b, const uint32_t* hist, uint64_t* status,
+                       uint32_t* counters, const uint32_t* epoch, uint32_t* err);
 
 __global__ void k_zero_u32(uint32_t* __restrict__ p, uint32_t n);
 

@@ -267,6 +267,28 @@ class HipScene:
         call("ysim_hip_scene_flags", self.h, C.byref(f))
         return f.value
 
+    # ------------------------------------------------------------ tuning
+    def tuning(self) -> dict:
+        t = abi.Tuning()
+        call("ysim_hip_scene_get_tuning", self.h, C.byref(t))
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        """Launch parameters (sort_digit_bits, sort_items, query_block,
+        cloth_block); unspecified ones keep their current value."""
+        t = abi.Tuning()
+        call("ysim_hip_scene_get_tuning", self.h, C.byref(t))
+        for k, v in kw.items():
+            if k not in dict(abi.Tuning._fields_):
+                raise KeyError(k)
+            setattr(t, k, v)
+        call("ysim_hip_scene_set_tuning", self.h, C.byref(t))
+
+    def autotune(self, trial_substeps: int = 10) -> dict:
+        t = abi.Tuning()
+        call("ysim_hip_scene_autotune", self.h, int(trial_substeps), C.byref(t))
+        return t.as_dict()
+
     # ------------------------------------------------------------ differentiation
     # Adjoint of the substep for a one-cloth scene with static obstacles
     # (include/ysim_hip.h, "differentiation").  The *_ptr forms take device

@@ -126,6 +126,79 @@ def batched_vs_sequential(ctx, substeps: int, log):
     }
 
 
+def lbvh_vs_hash(ctx, substeps: int, log):
+    """1024 x 1024 self-colliding cloth over 4 level-6 icospheres: LBVH traversal
+    vs spatial hash (2^22 slots, cell 2 * thickness), same candidate set; both
+    runs must end in bit-identical states."""
+    from ysim_amd import scenes
+    from ysim_amd.hip import HipScene
+    out = {}
+    states = {}
+    for mode, label in ((0, "lbvh"), (1, "hash")):
+        sc = scenes.config_c5(broad_mode=mode)
+        hs = HipScene(ctx, sc)
+        hs.snapshot()
+        hs.step(2, graph=True)
+        hs.restore()
+        ctx.sync()
+        t0 = time.perf_counter()
+        hs.step(substeps, graph=True)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        states[label] = hs.state()[0]
+        hs.restore()
+        st = hs.stage_timings(min(substeps, 10))
+        out[label] = {"substeps_per_s": round(substeps / dt, 2), "flags": hs.flags(),
+                      "stage_ms": {k: round(v, 5) for k, v in st.items()}}
+        log(f"C5 {label}: {substeps / dt:.1f} substeps/s")
+        V, T = sc.num_vertices, sc.num_facets
+        del hs
+    out["identical_states"] = bool(np.array_equal(states["lbvh"], states["hash"]))
+    out["workload"] = (f"1024x1024 self-colliding cloth + 4 level-6 icospheres ({V} vertices, {T} triangles), "
+                       f"63-bit LBVH rebuilt every substep vs spatial hash, dt=1/2400, {substeps} substeps")
+    return out
+
+
+def adjoint_timing(ctx, log, reps: int = 3):
+    """C4: 128 x 128 drape, 60 substeps at 1/600, checkpoints every 10: the
+    forward (with checkpoints) and the backward (segment recompute + adjoint)."""
+    from ysim_amd import scenes
+    from ysim_amd.hip import HipScene
+    sc = scenes.config_c4()
+    steps, K = sc.info["substeps"], sc.info["checkpoint_every"]
+    tt, t0 = scenes.c4_theta(sc)
+    kd = float(np.float32(sc.params["kd"]))
+    V = sc.meshes[0].x.shape[0]
+    ht = HipScene(ctx, sc)
+    ht.adj_forward(tt, kd, steps, K)
+    xT = ht.state()[0][:V]
+    hs = HipScene(ctx, sc)
+    hs.snapshot()
+    fw, bw = [], []
+    for _ in range(reps + 1):
+        hs.restore()
+        ctx.sync()
+        a = time.perf_counter()
+        hs.adj_forward(t0, kd, steps, K)
+        ctx.sync()
+        b = time.perf_counter()
+        x = hs.state()[0][:V]
+        c = time.perf_counter()
+        g, gkd, _, _ = hs.adj_backward(x - xT)
+        d = time.perf_counter()
+        fw.append(b - a)
+        bw.append(d - c)
+    fwd = float(np.median(fw[1:]))
+    bwd = float(np.median(bw[1:]))
+    log(f"adjoint C4: forward {1e3 * fwd:.2f} ms, backward {1e3 * bwd:.2f} ms")
+    return {"workload": "C4: 128x128 cloth dropped onto a UV sphere, 60 substeps at dt=1/600, checkpoints every 10, "
+                        "97,026 per-spring log-stiffness + damping gradients",
+            "forward_ms": round(1e3 * fwd, 3), "backward_ms": round(1e3 * bwd, 3),
+            "backward_includes": "segment recompute from checkpoints, contact re-detection, 2 adjoint kernels per "
+                                 "substep, host-visible gradient download",
+            "grad_norm": float(np.linalg.norm(g))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +211,9 @@ def main():
     ap.add_argument("--episode", type=int, default=100, help="substeps per restored episode (config: 100)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-sequential comparison")
     ap.add_argument("--batched-substeps", type=int, default=50)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch tuning")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the C5 LBVH-vs-hash, adjoint and copy-ceiling measurements")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on exactly one GPU (single-GPU scope); "
@@ -171,6 +247,11 @@ def main():
             hs.step(e, graph=use_graph)
             left -= e
 
+    tuning = hs.tuning()
+    if not args.no_tune:
+        log("autotune (digit width, sort tile, query/cloth block sizes)")
+        tuning = hs.autotune(10)
+        log(f"autotune: {tuning}")
     log(f"warmup {args.warmup} substeps (graph capture: {use_graph})")
     run(args.warmup)
     ctx.sync()
@@ -197,10 +278,12 @@ def main():
     log("stage timings: " + ", ".join(f"{k}={v:.4f}ms" for k, v in stages_ms.items()))
     contacts = cnt2["contacts"]
     pairs_per_step = cnt["pairs"] / max(1, cnt["steps"])
-    passes = 8  # 63-bit keys, 8-bit digits
+    digit_bits = int(tuning["sort_digit_bits"])
+    passes = -(-63 // digit_bits)  # 63-bit keys
     sb = stage_bytes(V, T, 8, passes, contacts)
     launches = {"box": 1, "morton": 1, "sort": passes, "hierarchy": 1, "refit": 1, "query": 1, "integrate": 1}
-    kernels = {"box": "k_box", "morton": "k_morton<uint64_t>", "sort": "k_onesweep<uint64_t, 8>",
+    kernels = {"box": "k_box", "morton": f"k_morton<uint64_t, {digit_bits}>",
+               "sort": f"k_onesweep<uint64_t, {digit_bits}, {int(tuning['sort_items'])}>",
                "hierarchy": "k_hierarchy<uint64_t>", "refit": "k_refit<false>", "query": "k_query<false>",
                "integrate": "k_cloth"}
     stages = {}
@@ -212,6 +295,9 @@ def main():
     per_launch_bytes = sb[dom] / launches[dom]
     per_launch_ms = stages_ms[dom] / launches[dom]
     achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
+    copy = None if args.no_extras else ctx.copy_bandwidth(1 << 30, 20)
+    if copy:
+        log(f"copy ceiling: {copy['GBps']:.0f} GB/s")
 
     out = {
         "metric": METRIC,
@@ -230,7 +316,8 @@ def main():
             "workload": f"{scene.name}: {args.n}x{args.n} cloth folded 0.02 apart, self-collision, "
                         "63-bit Morton LBVH rebuilt every substep, dt=1/1200",
             "vertices": V, "triangles": T, "morton_bits": 63, "rebuild_every": 1,
-            "sort": {"digit_bits": 8, "passes": passes},
+            "sort": {"digit_bits": digit_bits, "passes": passes, "items_per_thread": int(tuning["sort_items"])},
+            "tuning": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in tuning.items()},
             "hipgraph": use_graph, "vertex_mass": 1.0, "episode_substeps": args.episode,
             "candidate_pairs_per_substep": round(pairs_per_step, 1), "contacts_last_substep": contacts,
             "flags": flags,
@@ -240,11 +327,15 @@ def main():
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "bytes_per_launch": int(per_launch_bytes), "ms_per_launch": round(per_launch_ms, 5),
+            "copy_ceiling_GBps": None if copy is None else round(copy["GBps"], 1),
         },
         "stages": stages,
     }
     if not args.no_batched:
         out["batched"] = batched_vs_sequential(ctx, args.batched_substeps, log)
+    if not args.no_extras:
+        out["c5_lbvh_vs_hash"] = lbvh_vs_hash(ctx, 20, log)
+        out["adjoint"] = adjoint_timing(ctx, log)
     if not args.no_cpu:
         log(f"cpu baseline: {args.cpu_steps} oracle substeps, 1 thread and all cores")
         cb = cpu_baseline(scene, args.cpu_steps)

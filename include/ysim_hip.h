@@ -280,6 +280,14 @@ int ysim_hip_scene_restore(ysim_scene* scene);
  * query, out[1] substeps run, out[2] contacts of the last substep. */
 int ysim_hip_scene_counters(ysim_scene* scene, uint64_t* h_out, uint32_t reset);
 
+/* ---- measurement ------------------------------------------------------------
+ * No counterpart in the reference.  Achievable HBM bandwidth: a plain
+ * 16-byte-vector copy kernel ("k_copy_u4") of `bytes` from one device buffer to
+ * another, `reps` times after one warm-up; *h_ms = mean duration per copy
+ * (HIP events).  Also the calibration kernel of known traffic (bytes read and
+ * bytes written) for hardware counter runs. */
+int ysim_hip_copy_bandwidth(ysim_ctx* ctx, size_t bytes, uint32_t reps, float* h_ms);
+
 /* ---- tuning -----------------------------------------------------------------
  * No counterpart in the reference (its kernels use fixed threadgroup sizes and
  * 8-bit digits, radixsort.metal:21-112).  Launch parameters of the scene's
@@ -291,6 +299,7 @@ typedef struct ysim_tuning {
   uint32_t cloth_block;      /* springs + integrate threads per block: 128, 256 or 512 */
   float ms_per_substep;      /* autotune: the chosen setting's measured time; else 0 */
   float ms_per_substep_default; /* autotune: the default setting's measured time */
+  uint32_t refit_block;      /* bottom-up refit leaves per block (LDS levels): 256, 512 or 1024 */
 } ysim_tuning;
 int ysim_hip_scene_get_tuning(ysim_scene* scene, ysim_tuning* h_out);
 int ysim_hip_scene_set_tuning(ysim_scene* scene, const ysim_tuning* h_in);

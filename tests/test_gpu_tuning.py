@@ -19,6 +19,23 @@ def _same_state(a, b):
         assert np.array_equal(u, w)
 
 
+@pytest.mark.parametrize("rb", [256, 1024])
+def test_refit_blocks_bit_exact_c2(ctx, oracle_threads, rb):
+    """Refit with 256- and 1024-leaf LDS blocks on the 256x256 fold: every node box
+    equals the oracle's."""
+    sc = scenes.config_c2(n=256)
+    hs, os_ = HipScene(ctx, sc), OracleScene(sc)
+    hs.set_tuning(refit_block=rb)
+    hs.step(3, graph=True)
+    os_.step(3)
+    _same_state(hs.state(), os_.state())
+    ng, pg, kg, ig = hs.tree(0, keys=True)
+    no, po, ko, io = os_.tree(0)
+    for f in ("min", "max", "child_a", "child_b"):
+        assert np.array_equal(ng[f], no[f]), f
+    assert hs.flags() == 0
+
+
 @pytest.mark.parametrize("db,items", SORT_CONFIGS)
 def test_sort_configs_bit_exact_63bit(ctx, oracle_threads, db, items):
     print(f"sort config digit_bits={db} items={items} on the 96x96 fold (63-bit)", flush=True)
@@ -52,12 +69,12 @@ def test_sort_configs_bit_exact_30bit_rebuild(ctx, oracle_threads, db, items):
     assert np.array_equal(kg, ko) and np.array_equal(ig, io)
 
 
-@pytest.mark.parametrize("qb", [64, 128, 256])
+@pytest.mark.parametrize("qb,rb", [(64, 256), (128, 512), (256, 1024)])
 @pytest.mark.parametrize("cb", [128, 256, 512])
-def test_block_sizes_bit_exact(ctx, oracle_threads, qb, cb):
+def test_block_sizes_bit_exact(ctx, oracle_threads, qb, rb, cb):
     sc = scenes.config_drape(n=32, height=0.8)
     hs, os_ = HipScene(ctx, sc), OracleScene(sc)
-    hs.set_tuning(query_block=qb, cloth_block=cb)
+    hs.set_tuning(query_block=qb, cloth_block=cb, refit_block=rb)
     hs.step(20, graph=True)
     os_.step(20)
     _same_state(hs.state(), os_.state())
@@ -100,7 +117,8 @@ def test_autotune_batched_keeps_results(ctx):
 
 def test_invalid_tuning_rejected(ctx):
     hs = HipScene(ctx, scenes.config_drape(n=16))
-    for kw in ({"sort_digit_bits": 9}, {"sort_items": 4}, {"query_block": 96}, {"cloth_block": 1024}):
+    for kw in ({"sort_digit_bits": 9}, {"sort_items": 4}, {"query_block": 96}, {"cloth_block": 1024},
+               {"refit_block": 128}):
         with pytest.raises(YsimError):
             hs.set_tuning(**kw)
     hs.step(2)

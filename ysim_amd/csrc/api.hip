@@ -97,6 +97,7 @@ struct ysim_scene {
   ysim_bvh_node* tree = nullptr;
   int32_t* parent = nullptr;
   uint32_t* visit = nullptr;
+  uint2* range = nullptr;      // per internal node: sorted leaf range (LDS refit)
   std::vector<BuildGroup> dyn_groups, stat_groups;
   SegTable dyn_tsegs, dyn_vsegs;
   uint32_t *d_tgt_off = nullptr, *d_tgt = nullptr;
@@ -147,7 +148,7 @@ struct ysim_scene {
   } adj;
   float graph_kd = 0.f;
   const float* graph_ks = nullptr;
-  ysim_tuning tune{8, 8, 128, 256, 0.f, 0.f};
+  ysim_tuning tune{8, 8, 128, 256, 0.f, 0.f, 256};
   float* tune_buf = nullptr;   // autotune state save: x, v, xPrev of the whole scene
 
   int grow(float** p, size_t& have, size_t need) {
@@ -188,11 +189,13 @@ struct ysim_scene {
       return fail(YSIM_ERR_INVALID, "tuning: sort_digit_bits must be 8 or 11 and sort_items 8 or 16");
     if (!query_block_ok(t.query_block)) return fail(YSIM_ERR_INVALID, "tuning: query_block must be 64, 128 or 256");
     if (!cloth_block_ok(t.cloth_block)) return fail(YSIM_ERR_INVALID, "tuning: cloth_block must be 128, 256 or 512");
+    if (!refit_block_ok(t.refit_block)) return fail(YSIM_ERR_INVALID, "tuning: refit_block must be 256, 512 or 1024");
     const bool replan = t.sort_digit_bits != tune.sort_digit_bits || t.sort_items != tune.sort_items;
     tune.sort_digit_bits = t.sort_digit_bits;
     tune.sort_items = t.sort_items;
     tune.query_block = t.query_block;
     tune.cloth_block = t.cloth_block;
+    tune.refit_block = t.refit_block;
     if (replan) {
       for (auto& g : dyn_groups) g.plan = sort_plan(g.T, g.plan.key_bits, t.sort_digit_bits, t.sort_items);
       YSIM_TRY(zero(hist, SORT_HIST_WORDS));
@@ -275,6 +278,15 @@ struct ysim_scene {
         if (qb == best.query_block) continue;
         ysim_tuning t = best;
         t.query_block = qb;
+        float ms = 0.f;
+        if ((rc = measure(t, ms)) != YSIM_OK) break;
+        if (ms < best_ms) { best_ms = ms; best = t; }
+      }
+    if (rc == YSIM_OK && !hash_mode)
+      for (uint32_t rb : {256u, 512u, 1024u}) {
+        if (rb == best.refit_block) continue;
+        ysim_tuning t = best;
+        t.refit_block = rb;
         float ms = 0.f;
         if ((rc = measure(t, ms)) != YSIM_OK) break;
         if (ms < best_ms) { best_ms = ms; best = t; }
@@ -382,7 +394,7 @@ struct ysim_scene {
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[3], st));
     const uint32_t sorted = (g.plan.passes % 2 == 0) ? 0 : 1;
     YSIM_TRY(launch_hierarchy(st, g.tsegs, mt, keys[sorted], g.key64, vals[sorted], g.code_bits, tree, parent,
-                              visit, g.T, box_acc, g.tsegs.nseg, hist, (uint32_t)g.plan.hist_words));
+                              visit, g.T, box_acc, g.tsegs.nseg, hist, (uint32_t)g.plan.hist_words, range));
     last_sorted = sorted;
     return YSIM_OK;
   }
@@ -434,7 +446,7 @@ struct ysim_scene {
     }
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[4], st));
     YSIM_TRY(launch_refit(st, dyn_tsegs, mt, xc, v[par], P.subh, P.swept != 0, facets, tree, parent, visit, Tdyn,
-                          flags, formal_refit));
+                          flags, formal_refit, range, tune.refit_block));
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[5], st));
     return YSIM_OK;
   }
@@ -983,6 +995,7 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
   YSIM_TRY(s.alloc(&s.tree, s.Ntot));
   YSIM_TRY(s.alloc(&s.parent, s.Ntot));
   YSIM_TRY(s.alloc(&s.visit, s.Ntot));
+  YSIM_TRY(s.alloc(&s.range, s.Ntot));
   YSIM_TRY(s.zero(s.tree, 8 * (size_t)s.Ntot));
   YSIM_TRY(s.zero(s.visit, s.Ntot));
   // segment tables
@@ -1112,7 +1125,7 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
     for (auto& g : *gs) {
       YSIM_TRY(s.enqueue_build(g, s.x[0]));
       YSIM_TRY(launch_refit(s.st, g.tsegs, s.mt, s.x[0], s.v[0], 0.f, false, s.facets, s.tree, s.parent, s.visit,
-                            g.T, s.flags, s.formal_refit));
+                            g.T, s.flags, s.formal_refit, s.range, s.tune.refit_block));
     }
   YSIM_HIP_TRY(hipStreamSynchronize(s.st));
   uint32_t fl = 0;
@@ -1558,6 +1571,51 @@ int ysim_hip_scene_set_tuning(ysim_scene* s, const ysim_tuning* h_in) {
 int ysim_hip_scene_autotune(ysim_scene* s, uint32_t trial_substeps, ysim_tuning* h_out) {
   YSIM_ARG(s, "scene_autotune: null scene");
   return s->autotune(trial_substeps, h_out);
+}
+
+}  // extern "C"
+
+// ====================================================================== measurement
+namespace ysim {
+__global__ __launch_bounds__(256) void k_copy_u4(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+}  // namespace ysim
+
+extern "C" {
+
+int ysim_hip_copy_bandwidth(ysim_ctx* ctx, size_t bytes, uint32_t reps, float* h_ms) {
+  YSIM_ARG(ctx && h_ms && bytes >= 16 && reps > 0, "copy_bandwidth: bad argument");
+  const size_t n = bytes / 16;
+  void *a = nullptr, *b = nullptr;
+  YSIM_HIP_TRY(hipMalloc(&a, n * 16));
+  if (hipMalloc(&b, n * 16) != hipSuccess) {
+    (void)hipFree(a);
+    return fail(YSIM_ERR_HIP, "copy_bandwidth: hipMalloc failed");
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = YSIM_OK;
+  float ms = 0.f;
+  const uint32_t grid = (uint32_t)std::min<size_t>((n + 255) / 256, (size_t)ctx->num_cus * 8);
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = fail(YSIM_ERR_HIP, "event");
+  if (rc == YSIM_OK) {
+    (void)hipMemsetAsync(a, 0, n * 16, ctx->stream);
+    hipLaunchKernelGGL(k_copy_u4, dim3(grid), dim3(256), 0, ctx->stream, (const uint4*)a, (uint4*)b, n);
+    (void)hipEventRecord(e0, ctx->stream);
+    for (uint32_t r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(k_copy_u4, dim3(grid), dim3(256), 0, ctx->stream, (const uint4*)a, (uint4*)b, n);
+    (void)hipEventRecord(e1, ctx->stream);
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      rc = fail(YSIM_ERR_HIP, "copy_bandwidth: timing failed");
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  YSIM_TRY(rc);
+  *h_ms = ms / (float)reps;
+  return YSIM_OK;
 }
 
 }  // extern "C"

@@ -47,14 +47,19 @@ int launch_morton(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x,
                   uint32_t* hist, uint32_t* counters, uint32_t* epoch);
 // S3: Karras hierarchy per segment; leaf child links; zeroes the refit visit
 // counters; resets box_acc (nbox segments) and the sort histogram for the next build.
+// range (optional): per internal node its sorted leaf range (lo, hi), node-window indexed.
 int launch_hierarchy(hipStream_t s, SegTable tsegs, MeshTables mt, const void* keys, bool key64,
                      const uint32_t* vals, uint32_t code_bits, ysim_bvh_node* tree, int32_t* parent,
                      uint32_t* visit, uint32_t nitems, uint32_t* box_acc, uint32_t nbox,
-                     uint32_t* hist, uint32_t hist_words);
+                     uint32_t* hist, uint32_t hist_words, uint2* range = nullptr);
 // S4+S5: leaf boxes (swept optional) + atomic bottom-up union per segment.
+// With range (from launch_hierarchy) the levels inside blocks of `block`
+// (256, 512 or 1024) sorted leaves are combined in LDS.
+bool refit_block_ok(uint32_t b);
 int launch_refit(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x, const float* v,
                  float dt, bool swept, const uint32_t* facets, ysim_bvh_node* tree,
-                 const int32_t* parent, uint32_t* visit, uint32_t nitems, uint32_t* err, bool formal);
+                 const int32_t* parent, uint32_t* visit, uint32_t nitems, uint32_t* err, bool formal,
+                 const uint2* range = nullptr, uint32_t block = 256);
 
 // ---------------------------------------------------------------- query.hip
 struct QueryArgs {

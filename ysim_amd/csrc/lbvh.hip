@@ -201,7 +201,8 @@ __global__ __launch_bounds__(256) void k_hierarchy(SegTable ts, MeshTables mt, c
                                                    int32_t* __restrict__ parent,
                                                    uint32_t* __restrict__ visit, uint32_t n,
                                                    uint32_t* __restrict__ box_acc, uint32_t nbox,
-                                                   uint32_t* __restrict__ hist, uint32_t hist_words) {
+                                                   uint32_t* __restrict__ hist, uint32_t hist_words,
+                                                   uint2* __restrict__ range) {
   const uint32_t j = blockIdx.x * 256 + threadIdx.x;
   // reset accumulators for the next build (their readers ran in earlier kernels)
   if (j < nbox * 6) box_acc[j] = ((j % 6) < 3) ? 0xffffffffu : 0u;
@@ -252,23 +253,24 @@ __global__ __launch_bounds__(256) void k_hierarchy(SegTable ts, MeshTables mt, c
   T[i].child_b = cb;
   P[ca] = i;
   P[cb] = i;
+  if (range) range[base + i] = make_uint2((uint32_t)lo, (uint32_t)hi);
 }
 
 int launch_hierarchy(hipStream_t s, SegTable tsegs, MeshTables mt, const void* keys, bool key64,
                      const uint32_t* vals, uint32_t code_bits, ysim_bvh_node* tree, int32_t* parent,
                      uint32_t* visit, uint32_t nitems, uint32_t* box_acc, uint32_t nbox, uint32_t* hist,
-                     uint32_t hist_words) {
+                     uint32_t hist_words, uint2* range) {
   uint32_t work = std::max<uint32_t>(nitems, std::max<uint32_t>(nbox * 6, hist_words));
   if (!work) return YSIM_OK;
   uint32_t blocks = ceil_div(work, 256);
   if (key64)
     hipLaunchKernelGGL(k_hierarchy<uint64_t>, dim3(blocks), dim3(256), 0, s, tsegs, mt,
                        (const uint64_t*)keys, vals, code_bits, tree, parent, visit, nitems, box_acc, nbox,
-                       hist, hist_words);
+                       hist, hist_words, range);
   else
     hipLaunchKernelGGL(k_hierarchy<uint32_t>, dim3(blocks), dim3(256), 0, s, tsegs, mt,
                        (const uint32_t*)keys, vals, code_bits, tree, parent, visit, nitems, box_acc, nbox,
-                       hist, hist_words);
+                       hist, hist_words, range);
   YSIM_CHECK_LAUNCH();
   return YSIM_OK;
 }
@@ -379,10 +381,160 @@ __global__ __launch_bounds__(256) void k_refit(SegTable ts, MeshTables mt, const
   atomicOr(err, 4u);
 }
 
+// Same refit with the lower levels resolved inside the workgroup.  A block
+// owns RB (256, 512 or 1024; chosen by the autotuner) consecutive sorted leaves; an internal node whose leaf range
+// (written by the hierarchy kernel) lies inside the block is combined through
+// LDS (the children's boxes and an arrival counter per node; a Karras internal
+// node is an endpoint of its own range, so its index falls inside the block and
+// addresses the LDS slot).  Only nodes spanning blocks use the device-wide
+// second-arrival protocol above, so a leaf's chain of device round trips starts
+// at the block's top instead of at the leaf.  Unions keep (child_a, child_b)
+// operand order; the boxes are identical to k_refit's.
+template <bool FORMAL, int RB>
+__global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, const float* __restrict__ x,
+                                                   const float* __restrict__ v, float dt, int swept,
+                                                   const uint32_t* __restrict__ facets, ysim_bvh_node* tree,
+                                                   const int32_t* __restrict__ parent,
+                                                   const uint2* __restrict__ range, uint32_t* visit, uint32_t n,
+                                                   uint32_t* err) {
+  __shared__ uint32_t s_cnt[RB];
+  __shared__ float s_box[RB][2][6];
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t b0 = blk * RB;
+  s_cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t j = b0 + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t sg = seg_of(ts, j);
+  const uint32_t m = ts.mesh[sg];
+  const uint32_t s = ts.off[sg];
+  const int nn = (int)(ts.off[sg + 1] - s);
+  const int i = (int)(j - s);
+  const uint32_t base = mt.noff[m];
+  ysim_bvh_node* T = tree + base;
+  const int32_t* P = parent + base;
+  const uint2* R = range + base;
+  uint32_t* C = visit + base;
+
+  int child = nn - 1 + i;
+  const uint32_t fid = (uint32_t)T[child].child_b;
+  const uint32_t* f = facets + 3 * (size_t)(mt.foff[m] + fid);
+  const uint32_t vb = mt.voff[m];
+  f3 lo = ld3(x, vb + f[0]), hi = lo;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    f3 p = ld3(x, vb + f[c]);
+    lo = vmin(lo, p);
+    hi = vmax(hi, p);
+    if (swept) {
+      f3 vv = ld3(v, vb + f[c]);
+      f3 q = p + vv * dt;
+      lo = vmin(lo, q);
+      hi = vmax(hi, q);
+    }
+  }
+  auto store_box = [&](int node, f3 a, f3 b) {
+    float* q = reinterpret_cast<float*>(T + node);
+    if (FORMAL) {
+      q[0] = a.x; q[1] = a.y; q[2] = a.z; q[4] = b.x; q[5] = b.y; q[6] = b.z;
+    } else {
+      st_agent_u64(reinterpret_cast<uint64_t*>(q + 0), pack2(a.x, a.y));
+      st_agent(q + 2, a.z);
+      st_agent_u64(reinterpret_cast<uint64_t*>(q + 4), pack2(b.x, b.y));
+      st_agent(q + 6, b.z);
+    }
+  };
+  store_box(child, lo, hi);
+  if (nn <= 1) return;
+  const int nnodes = 2 * nn - 1;
+  uint32_t clo = (uint32_t)i;  // leaf range of the current node
+  for (int step = 0; step < 2 * nn; ++step) {
+    const int p = P[child];
+    if (p < 0 || p >= nn - 1) {
+      atomicOr(err, 4u);
+      return;
+    }
+    const uint2 pr = R[p];
+    const bool local = s + pr.x >= b0 && s + pr.y < b0 + RB;
+    if (local) {
+      const uint32_t slot = s + (uint32_t)p - b0;
+      const int side = (clo == pr.x) ? 0 : 1;  // child_a covers the left part of p's range
+      float* mine = s_box[slot][side];
+      mine[0] = lo.x; mine[1] = lo.y; mine[2] = lo.z; mine[3] = hi.x; mine[4] = hi.y; mine[5] = hi.z;
+      const uint32_t old = __hip_atomic_fetch_add(&s_cnt[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (old == 0u) return;  // first arrival
+      const float* o = s_box[slot][side ^ 1];
+      const f3 omin = {o[0], o[1], o[2]}, omax = {o[3], o[4], o[5]};
+      if (side == 0) {
+        lo = vmin(lo, omin);
+        hi = vmax(hi, omax);
+      } else {
+        lo = vmin(omin, lo);
+        hi = vmax(omax, hi);
+      }
+    } else {
+      uint32_t old;
+      if (FORMAL) {
+        old = __hip_atomic_fetch_add(C + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        wait_all_mem();
+        old = __hip_atomic_fetch_add(C + p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      if ((old & 1u) == 0u) return;
+      const int ca = T[p].child_a, cb = T[p].child_b;
+      if (ca < 0 || ca >= nnodes || cb < 0 || cb >= nnodes) {
+        atomicOr(err, 4u);
+        return;
+      }
+      const int sib = (ca == child) ? cb : ca;
+      const float* q = reinterpret_cast<const float*>(T + sib);
+      f3 smin, smax;
+      if (FORMAL) {
+        smin = f3{q[0], q[1], q[2]};
+        smax = f3{q[4], q[5], q[6]};
+      } else {
+        const uint64_t m01 = ld_agent_u64(reinterpret_cast<const uint64_t*>(q + 0));
+        const uint64_t x01 = ld_agent_u64(reinterpret_cast<const uint64_t*>(q + 4));
+        smin = f3{lo_f(m01), hi_f(m01), ld_agent(q + 2)};
+        smax = f3{lo_f(x01), hi_f(x01), ld_agent(q + 6)};
+      }
+      if (sib == cb) {
+        lo = vmin(lo, smin);
+        hi = vmax(hi, smax);
+      } else {
+        lo = vmin(smin, lo);
+        hi = vmax(smax, hi);
+      }
+    }
+    store_box(p, lo, hi);
+    child = p;
+    clo = pr.x;
+    if (p == 0) return;
+  }
+  atomicOr(err, 4u);
+}
+
+bool refit_block_ok(uint32_t b) { return b == 256 || b == 512 || b == 1024; }
+
 int launch_refit(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x, const float* v, float dt,
                  bool swept, const uint32_t* facets, ysim_bvh_node* tree, const int32_t* parent,
-                 uint32_t* visit, uint32_t nitems, uint32_t* err, bool formal) {
+                 uint32_t* visit, uint32_t nitems, uint32_t* err, bool formal, const uint2* range,
+                 uint32_t block) {
   if (!nitems) return YSIM_OK;
+  if (range) {
+    if (!refit_block_ok(block)) return YSIM_ERR_INVALID;
+#define YSIM_R(F, B)                                                                                     \
+  hipLaunchKernelGGL((k_refit_lds<F, B>), dim3(ceil_div(nitems, B)), dim3(B), 0, s, tsegs, mt, x, v, dt, \
+                     (int)swept, facets, tree, parent, range, visit, nitems, err)
+    if (formal) {
+      if (block == 256) YSIM_R(true, 256); else if (block == 512) YSIM_R(true, 512); else YSIM_R(true, 1024);
+    } else {
+      if (block == 256) YSIM_R(false, 256); else if (block == 512) YSIM_R(false, 512); else YSIM_R(false, 1024);
+    }
+#undef YSIM_R
+    YSIM_CHECK_LAUNCH();
+    return YSIM_OK;
+  }
   if (formal)
     hipLaunchKernelGGL(k_refit<true>, dim3(ceil_div(nitems, 256)), dim3(256), 0, s, tsegs, mt, x, v, dt,
                        (int)swept, facets, tree, parent, visit, nitems, err);

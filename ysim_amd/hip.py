@@ -43,6 +43,12 @@ class HipContext:
     def alloc(self, nbytes: int) -> "DeviceArray":
         return DeviceArray(self, nbytes)
 
+    def copy_bandwidth(self, nbytes: int = 1 << 30, reps: int = 20) -> dict:
+        """Achievable HBM bandwidth of a plain device-to-device vector copy."""
+        ms = C.c_float(0)
+        call("ysim_hip_copy_bandwidth", self.h, int(nbytes), int(reps), C.byref(ms))
+        return {"bytes": int(nbytes), "ms": ms.value, "GBps": 2 * nbytes / (ms.value * 1e-3) / 1e9}
+
     def to_device(self, a: np.ndarray) -> "DeviceArray":
         a = np.ascontiguousarray(a)
         d = DeviceArray(self, a.nbytes, dtype=a.dtype, shape=a.shape)

@@ -212,6 +212,10 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-sequential comparison")
     ap.add_argument("--batched-substeps", type=int, default=50)
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch tuning")
+    ap.add_argument("--tuning", default="",
+                    help="fixed tuning instead of autotune: sort_digit_bits,sort_items,query_block,cloth_block,"
+                         "refit_block (e.g. 8,16,128,512,1024)")
+    ap.add_argument("--no-copy", action="store_true", help="skip the copy-bandwidth ceiling")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the C5 LBVH-vs-hash, adjoint and copy-ceiling measurements")
     args = ap.parse_args()
@@ -248,7 +252,12 @@ def main():
             left -= e
 
     tuning = hs.tuning()
-    if not args.no_tune:
+    if args.tuning:
+        names = ("sort_digit_bits", "sort_items", "query_block", "cloth_block", "refit_block")
+        hs.set_tuning(**dict(zip(names, (int(t) for t in args.tuning.split(",")))))
+        tuning = hs.tuning()
+        log(f"fixed tuning: {tuning}")
+    elif not args.no_tune:
         log("autotune (digit width, sort tile, query/cloth block sizes)")
         tuning = hs.autotune(10)
         log(f"autotune: {tuning}")
@@ -295,7 +304,7 @@ def main():
     per_launch_bytes = sb[dom] / launches[dom]
     per_launch_ms = stages_ms[dom] / launches[dom]
     achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
-    copy = None if args.no_extras else ctx.copy_bandwidth(1 << 30, 20)
+    copy = None if args.no_copy else ctx.copy_bandwidth(1 << 30, 20)
     if copy:
         log(f"copy ceiling: {copy['GBps']:.0f} GB/s")
 

@@ -392,11 +392,11 @@ __global__ __launch_bounds__(256) void k_refit(SegTable ts, MeshTables mt, const
 // operand order; the boxes are identical to k_refit's.
 template <bool FORMAL, int RB>
 __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, const float* __restrict__ x,
-                                                   const float* __restrict__ v, float dt, int swept,
-                                                   const uint32_t* __restrict__ facets, ysim_bvh_node* tree,
-                                                   const int32_t* __restrict__ parent,
-                                                   const uint2* __restrict__ range, uint32_t* visit, uint32_t n,
-                                                   uint32_t* err) {
+                                                  const float* __restrict__ v, float dt, int swept,
+                                                  const uint32_t* __restrict__ facets, ysim_bvh_node* tree,
+                                                  const int32_t* __restrict__ parent,
+                                                  const uint2* __restrict__ range, uint32_t* visit, uint32_t n,
+                                                  uint32_t* err) {
   __shared__ uint32_t s_cnt[RB];
   __shared__ float s_box[RB][2][6];
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
@@ -433,9 +433,13 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
       hi = vmax(hi, q);
     }
   }
-  auto store_box = [&](int node, f3 a, f3 b) {
+  // A box read by another workgroup during this kernel (its parent is combined
+  // through the device-wide protocol) is written with device-coherent stores
+  // in the fast variant; every other box with plain stores, which L2 combines
+  // into whole lines (the traversal reads them after the kernel ends).
+  auto store_box = [&](int node, f3 a, f3 b, bool coherent) {
     float* q = reinterpret_cast<float*>(T + node);
-    if (FORMAL) {
+    if (FORMAL || !coherent) {
       q[0] = a.x; q[1] = a.y; q[2] = a.z; q[4] = b.x; q[5] = b.y; q[6] = b.z;
     } else {
       st_agent_u64(reinterpret_cast<uint64_t*>(q + 0), pack2(a.x, a.y));
@@ -444,8 +448,10 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
       st_agent(q + 6, b.z);
     }
   };
-  store_box(child, lo, hi);
-  if (nn <= 1) return;
+  if (nn <= 1) {
+    store_box(child, lo, hi, false);
+    return;
+  }
   const int nnodes = 2 * nn - 1;
   uint32_t clo = (uint32_t)i;  // leaf range of the current node
   for (int step = 0; step < 2 * nn; ++step) {
@@ -456,6 +462,7 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
     }
     const uint2 pr = R[p];
     const bool local = s + pr.x >= b0 && s + pr.y < b0 + RB;
+    store_box(child, lo, hi, !local);
     if (local) {
       const uint32_t slot = s + (uint32_t)p - b0;
       const int side = (clo == pr.x) ? 0 : 1;  // child_a covers the left part of p's range
@@ -506,10 +513,12 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
         hi = vmax(smax, hi);
       }
     }
-    store_box(p, lo, hi);
     child = p;
     clo = pr.x;
-    if (p == 0) return;
+    if (p == 0) {
+      store_box(0, lo, hi, false);
+      return;
+    }
   }
   atomicOr(err, 4u);
 }

@@ -35,6 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PMC_TRAFFIC = os.path.join(ROOT, "profiles", "c2_512_fold_pmc_traffic.json")
 METRIC = "cloth substeps/s (LBVH+narrow+springs) + HBM GB/s; plain vs batched, 1 GPU"
 
 
@@ -60,6 +61,21 @@ def stage_bytes(V: int, T: int, k: int, passes: int, C: int) -> dict:
         # x, v, ext, mass, pin mask, contact count read; x, v written; contacts read
         "integrate": 24 * V + 12 * V + 4 * V + 4 * V + 4 * V + 24 * V + 24 * C,
     }
+
+
+def pmc_traffic(kernel: str):
+    """HBM bytes per dispatch of `kernel` from the committed hardware-counter
+    summary (FETCH_SIZE and WRITE_SIZE runs of this workload, each scaled by the
+    copy-kernel calibration recorded with them), or None."""
+    try:
+        with open(PMC_TRAFFIC) as f:
+            d = json.load(f)
+    except OSError:
+        return None, None
+    k = d.get("kernels", {}).get(kernel)
+    if not k:
+        return None, None
+    return int(k["hbm_bytes_calibrated"]), os.path.relpath(PMC_TRAFFIC, ROOT)
 
 
 def cpu_baseline(scene, sample_steps: int, warm: int = 1):
@@ -293,17 +309,21 @@ def main():
     launches = {"box": 1, "morton": 1, "sort": passes, "hierarchy": 1, "refit": 1, "query": 1, "integrate": 1}
     kernels = {"box": "k_box", "morton": f"k_morton<uint64_t, {digit_bits}>",
                "sort": f"k_onesweep<uint64_t, {digit_bits}, {int(tuning['sort_items'])}>",
-               "hierarchy": "k_hierarchy<uint64_t>", "refit": "k_refit<false>", "query": "k_query<false>",
-               "integrate": "k_cloth"}
+               "hierarchy": "k_hierarchy<uint64_t>", "refit": f"k_refit_lds<false, {int(tuning['refit_block'])}>",
+               "query": f"k_query<false, {int(tuning['query_block'])}>",
+               "integrate": f"k_cloth<{int(tuning['cloth_block'])}>"}
     stages = {}
     for s, ms in stages_ms.items():
         gbs = sb[s] / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        tb, _ = pmc_traffic(kernels[s])
         stages[s] = {"ms": round(ms, 5), "algorithmic_bytes": int(sb[s]), "GBps": round(gbs, 1),
-                     "frac_hbm": round(gbs / HBM_PEAK_GBS, 4), "launches": launches[s]}
+                     "frac_hbm": round(gbs / HBM_PEAK_GBS, 4), "launches": launches[s], "kernel": kernels[s],
+                     "pmc_hbm_bytes": None if tb is None else tb * launches[s]}
     dom = max(stages_ms, key=lambda s: stages_ms[s])
     per_launch_bytes = sb[dom] / launches[dom]
     per_launch_ms = stages_ms[dom] / launches[dom]
     achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
+    traffic, traffic_src = pmc_traffic(kernels[dom])
     copy = None if args.no_copy else ctx.copy_bandwidth(1 << 30, 20)
     if copy:
         log(f"copy ceiling: {copy['GBps']:.0f} GB/s")
@@ -334,7 +354,8 @@ def main():
         "roofline": {
             "bound": "hbm", "kernel": kernels[dom], "stage": dom,
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_source": traffic_src,
             "bytes_per_launch": int(per_launch_bytes), "ms_per_launch": round(per_launch_ms, 5),
             "copy_ceiling_GBps": None if copy is None else round(copy["GBps"], 1),
         },

@@ -117,6 +117,61 @@ int main() {
   std::vector<uint32_t> sv = dv.download();
   report(sv == std::vector<uint32_t>({5, 3, 1, 4, 0, 2}), "RadixSorter<HIP> / stable order");
 
+  // ByteMemoryPool<HIP>: 256-byte bump allocation, fill, overflow reported
+  {
+    ByteMemoryPool<HIP> pool(4096);
+    auto a = pool.zeros<uint32_t>(10);
+    auto b = pool.allocFill<float>(5, 2.5f);
+    std::vector<float> hb(5);
+    ysim_hip_detail::check(ysim_hip_download(ysim_hip_detail::global_ctx(), hb.data(), b.ptr, 20), "download");
+    std::vector<uint32_t> ha(10, 7u);
+    ysim_hip_detail::check(ysim_hip_download(ysim_hip_detail::global_ctx(), ha.data(), a.ptr, 40), "download");
+    bool threw = false;
+    try {
+      pool.alloc<char>(4096);
+    } catch (const std::length_error&) {
+      threw = true;
+    }
+    report(a.offset == 0 && b.offset == 256 && hb == std::vector<float>(5, 2.5f) &&
+               ha == std::vector<uint32_t>(10, 0u) && threw,
+           "ByteMemoryPool<HIP> / alignment, zeros, fill, overflow");
+  }
+
+  // FastGridClothBehavior<HIP>: clothStiffnessScale scales k*, not rest lengths,
+  // and a scaled sheet moves differently from an unscaled one
+  {
+    SymplecticSystem<HIP, float> sysS(1.0f / 60.f, 10);
+    Simulator<HIP, float> S1(sysS), S2(sysS);
+    build(S1, 0.8f, n);
+    S2.scene.meshes.clear();
+    build(S2, 0.8f, n);
+    S2.scene.meshes[0].clothStiffnessScale = 4.0f;
+    ysim_fastgrid_params p1 = FastGridClothBehavior<HIP, float>::params(S1.scene.meshes[0]);
+    ysim_fastgrid_params p2 = FastGridClothBehavior<HIP, float>::params(S2.scene.meshes[0]);
+    S2.initialize();
+    S1.update();
+    S2.update();
+    std::vector<float> y1 = S1.scene.positions(), y2 = S2.scene.positions();
+    report(p2.kstretch == 4.0f * p1.kstretch && p2.kbend == 4.0f * p1.kbend &&
+               p2.stretch_rest_x == p1.stretch_rest_x && std::memcmp(y1.data(), y2.data(), 4 * y1.size()) != 0,
+           "FastGridClothBehavior<HIP> / stiffness scale");
+  }
+
+  // Simulator<HIP>::autotune keeps the state and the results
+  {
+    SymplecticSystem<HIP, float> sysT(1.0f / 60.f, 10);
+    Simulator<HIP, float> T1(sysT), T2(sysT);
+    build(T1, 0.8f, n);
+    build(T2, 0.8f, n);
+    std::vector<float> before = T1.scene.positions();
+    ysim_tuning t = T1.autotune(4);
+    std::vector<float> after = T1.scene.positions();
+    T1.update();
+    T2.update();
+    std::vector<float> z1 = T1.scene.positions(), z2 = T2.scene.positions();
+    report(before == after && z1 == z2 && t.ms_per_substep > 0.f, "Simulator<HIP>::autotune / results unchanged");
+  }
+
   std::fprintf(stderr, "%s: %d failure(s)\n", failures ? "FAILED" : "ALL PASSED", failures);
   return failures ? 1 : 0;
 }

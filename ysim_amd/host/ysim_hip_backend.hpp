@@ -33,6 +33,8 @@ struct CPU : Backend {};
 struct CUDA : Backend {};
 struct METAL : Backend {};
 template <typename BE, typename T> struct VectorBase;
+template <typename BE> struct ByteMemoryPool;
+template <typename BE, typename PR> struct FastGridClothBehavior;
 template <typename BE, typename E> struct RadixSorter;
 template <typename BE, typename PR> struct Scene;
 struct SCENE {};
@@ -88,6 +90,84 @@ struct VectorBase<HIP, T> {
   }
 };
 
+// ---------------------------------------------------------------- ByteMemoryPool
+// ByteMemoryPool<METAL> (include/sim/core_types.hpp:268-330): one device arena,
+// bump allocation with 256-byte alignment.  alloc() returns a non-owning block;
+// overflow throws with the required size instead of printing and returning an
+// empty block.
+template <typename PR>
+struct MemoryBlockHIP {
+  PR* ptr = nullptr;   // device pointer
+  size_t size = 0;
+  size_t offset = 0;   // bytes from the arena start
+};
+
+template <>
+struct ByteMemoryPool<HIP> {
+  char* pool = nullptr;  // device
+  size_t marker = 0;
+  size_t capacity = 0;
+  ByteMemoryPool() = default;
+  explicit ByteMemoryPool(size_t n) : capacity(n) {
+    void* p = nullptr;
+    ysim_hip_detail::check(ysim_hip_alloc(ysim_hip_detail::global_ctx(), n, &p), "ByteMemoryPool<HIP> alloc");
+    pool = static_cast<char*>(p);
+  }
+  ByteMemoryPool(const ByteMemoryPool&) = delete;
+  ByteMemoryPool& operator=(const ByteMemoryPool&) = delete;
+  ByteMemoryPool(ByteMemoryPool&& o) noexcept : pool(o.pool), marker(o.marker), capacity(o.capacity) {
+    o.pool = nullptr;
+  }
+  ByteMemoryPool& operator=(ByteMemoryPool&& o) noexcept {
+    if (this != &o) {
+      release();
+      pool = o.pool;
+      marker = o.marker;
+      capacity = o.capacity;
+      o.pool = nullptr;
+    }
+    return *this;
+  }
+  ~ByteMemoryPool() { release(); }
+  void release() {
+    if (pool) ysim_hip_free(ysim_hip_detail::global_ctx(), pool);
+    pool = nullptr;
+  }
+  template <typename PR>
+  MemoryBlockHIP<PR> alloc(size_t count) {
+    MemoryBlockHIP<PR> b;
+    if (count == 0) return b;
+    const size_t bytes = (sizeof(PR) * count + 255) & ~size_t(255);
+    if (marker + bytes > capacity)
+      throw std::length_error("ByteMemoryPool<HIP>: needs " + std::to_string(marker + bytes) + " bytes, capacity " +
+                              std::to_string(capacity));
+    b.ptr = reinterpret_cast<PR*>(pool + marker);
+    b.size = count;
+    b.offset = marker;
+    marker += bytes;
+    return b;
+  }
+  template <typename PR>
+  MemoryBlockHIP<PR> zeros(size_t count) {
+    auto b = alloc<PR>(count);
+    if (count)
+      ysim_hip_detail::check(ysim_hip_memset(ysim_hip_detail::global_ctx(), b.ptr, 0, count * sizeof(PR)), "memset");
+    return b;
+  }
+  template <typename PR>
+  MemoryBlockHIP<PR> allocFill(size_t count, PR fill) {
+    auto b = alloc<PR>(count);
+    if (count) {
+      std::vector<PR> h(count, fill);
+      ysim_hip_detail::check(ysim_hip_upload(ysim_hip_detail::global_ctx(), b.ptr, h.data(), count * sizeof(PR)),
+                             "upload");
+    }
+    return b;
+  }
+  void reset() { marker = 0; }   // the pool rewind of Simulator::initialize (core_types.hpp:462-473)
+  char* bytePtr() { return pool; }
+};
+
 // ---------------------------------------------------------------- RadixSorter
 // Keys and values as separate arrays (the reference sorts MortonNode pairs).
 template <typename E>
@@ -111,6 +191,7 @@ struct Scene<HIP, PR> {
     std::vector<uint32_t> facets;  // packed uint3, mesh-local ids
     BehaviorType behaviorType = BehaviorType::Float;
     ysim_fastgrid_params cloth{};
+    float clothStiffnessScale = 1.0f;  // GeneralMesh::clothStiffnessScale: scales k*, not rest lengths
     float mass = 1.0f;             // per vertex
     bool collidable = true;
     uint32_t group = 0;
@@ -183,7 +264,7 @@ struct Scene<HIP, PR> {
       d[i].behavior = m.behaviorType == BehaviorType::FastGridCloth ? YSIM_BEHAVIOR_FASTGRID_CLOTH : YSIM_BEHAVIOR_FLOAT;
       d[i].collidable = m.collidable ? 1u : 0u;
       d[i].group = m.group;
-      d[i].cloth = m.cloth;
+      d[i].cloth = FastGridClothBehavior<HIP, PR>::params(m);
       x.insert(x.end(), m.x.begin(), m.x.end());
       v.insert(v.end(), m.x.size(), 0.f);
       mass.insert(mass.end(), nv, m.mass);
@@ -208,6 +289,29 @@ struct Scene<HIP, PR> {
     std::vector<float> x(n);
     ysim_hip_detail::check(ysim_hip_scene_get_state(handle, x.data(), nullptr, nullptr), "scene_get_state");
     return x;
+  }
+};
+
+// ---------------------------------------------------------------- FastGridClothBehavior
+// FastGridClothBehavior<METAL> (include/sim/collision_pod.hpp:516-555) binds a
+// mesh's buffers and the stiffness-scaled FastGridClothBehaviorParams, then
+// dispatches compute_cloth_grid_forces_fast; integrate_cloth_grid follows in a
+// second pass (symplectic_system.hpp:355-409).  On this backend forces and
+// integration are one fused kernel over every cloth of the resident scene, so
+// the behaviour packs the parameters once (at Scene::pack) and update() is the
+// fused pass.
+template <typename PR>
+struct FastGridClothBehavior<HIP, PR> {
+  // kstretch/kshear/kbend scaled by the mesh's clothStiffnessScale, rest lengths untouched
+  static ysim_fastgrid_params params(const typename Scene<HIP, PR>::Mesh& m) {
+    ysim_fastgrid_params fp = m.cloth;
+    fp.kstretch *= m.clothStiffnessScale;
+    fp.kshear *= m.clothStiffnessScale;
+    fp.kbend *= m.clothStiffnessScale;
+    return fp;
+  }
+  static void update(Scene<HIP, PR>& s) {
+    ysim_hip_detail::check(ysim_hip_scene_integrate(s.handle), "scene_integrate");
   }
 };
 
@@ -286,7 +390,7 @@ struct SymplecticSystem<HIP, PR> {
   SymplecticSystem(PR h_ = PR(1) / PR(60), size_t subSteps_ = 50) : h(h_), subSteps(subSteps_), subh(h_ / subSteps_) {}
   void snapshotXPrev(Scene<HIP, PR>&) {}  // implicit: x is double-buffered on the device
   void update(Scene<HIP, PR>& s) {
-    ysim_hip_detail::check(ysim_hip_scene_integrate(s.handle), "scene_integrate");
+    FastGridClothBehavior<HIP, PR>::update(s);
     acctime += subh;
   }
   void pollAnomaly(Scene<HIP, PR>& s) {
@@ -326,6 +430,14 @@ struct Simulator<HIP, PR> {
     scene.pack(p);
     broadPhase.build(scene);
     narrowPhase.broad = &broadPhase;
+  }
+
+  // Launch tuning for this scene (no counterpart in the reference): times the
+  // substep under each setting, keeps the fastest; state and results unchanged.
+  ysim_tuning autotune(uint32_t trialSubsteps = 10) {
+    ysim_tuning t{};
+    ysim_hip_detail::check(ysim_hip_scene_autotune(scene.handle, trialSubsteps, &t), "scene_autotune");
+    return t;
   }
 
   // One frame: the substep loop of Simulator::update (simulator.hpp:3512-3815).

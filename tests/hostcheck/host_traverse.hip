@@ -9,6 +9,10 @@
 
 using namespace ysim;
 
+extern "C" void yt_tile_order(uint32_t N, uint32_t* out) {
+  for (uint32_t i = 0; i < N * N; ++i) out[i] = tile_order(i, N);
+}
+
 extern "C" int yt_query(const float* x, uint32_t nq, uint32_t qobj, const ysim_bvh_node* tree, const int32_t* parent,
                         uint32_t nT, const uint32_t* tf, uint32_t tobj, float margin, int sdepth_small,
                         uint32_t* out_pairs, uint64_t cap, uint64_t* out_count, uint64_t* out_iters,

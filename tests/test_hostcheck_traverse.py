@@ -34,6 +34,17 @@ def host_pairs(L, xq, tree, parent, facets, same, margin, small):
     return out.reshape(-1, 2), it.value / len(xq)
 
 
+def test_tile_order_is_a_bijection(L):
+    """The fused query visits grid vertices in 8x8 tiles; every vertex exactly once."""
+    for N in (3, 7, 8, 9, 17, 31, 33, 100, 128, 512, 1024):
+        out = np.zeros(N * N, np.uint32)
+        L.yt_tile_order(C.c_uint32(N), out.ctypes.data_as(C.c_void_p))
+        assert np.array_equal(np.sort(out), np.arange(N * N, dtype=np.uint32)), N
+    out = np.zeros(256, np.uint32)
+    L.yt_tile_order(C.c_uint32(16), out.ctypes.data_as(C.c_void_p))
+    assert out[:8].tolist() == list(range(8)) and out[8] == 16  # first tile, second row
+
+
 @pytest.mark.parametrize("small", [0, 1])
 def test_fold_self_query(L, small):
     sc = scenes.config_c2(n=128)

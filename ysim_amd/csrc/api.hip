@@ -92,7 +92,7 @@ struct ysim_scene {
   uint32_t* facets = nullptr;
   MeshTables mt;
   ysim_fastgrid_params* d_cloth = nullptr;
-  uint32_t *d_nfacets = nullptr, *d_behavior = nullptr, *d_shape = nullptr;
+  uint32_t *d_nfacets = nullptr, *d_behavior = nullptr, *d_shape = nullptr, *d_grid_n = nullptr;
   float *d_margin = nullptr, *d_band = nullptr;
   ysim_bvh_node* tree = nullptr;
   int32_t* parent = nullptr;
@@ -410,6 +410,7 @@ struct ysim_scene {
     a.mesh_band = d_band;
     a.mesh_behavior = d_behavior;
     a.mesh_shape = d_shape;
+    a.mesh_grid_n = d_grid_n;
     a.x = x[par];
     a.xprev = x[par ^ 1];
     a.facets = facets;
@@ -989,6 +990,16 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
   YSIM_TRY(s.put(&s.d_cloth, cloth));
   YSIM_TRY(s.put(&s.d_nfacets, nf));
   YSIM_TRY(s.put(&s.d_behavior, beh));
+  {
+    std::vector<uint32_t> gn(num_meshes, 0u);
+    for (uint32_t m = 0; m < num_meshes; ++m) {
+      const uint32_t N = s.meshes[m].cloth.particle_num_1d;
+      if (s.meshes[m].behavior == YSIM_BEHAVIOR_FASTGRID_CLOTH && N >= 8 &&
+          (uint64_t)N * N == s.meshes[m].num_vertices)
+        gn[m] = N;
+    }
+    YSIM_TRY(s.put(&s.d_grid_n, gn));
+  }
   YSIM_TRY(s.put(&s.d_shape, shp));
   YSIM_TRY(s.put(&s.d_margin, margin));
   YSIM_TRY(s.put(&s.d_band, band));

@@ -72,6 +72,7 @@ struct QueryArgs {
   const float* mesh_band = nullptr;   // per query mesh narrow band (radius + thickness)
   const uint32_t* mesh_behavior = nullptr;
   const uint32_t* mesh_shape = nullptr;
+  const uint32_t* mesh_grid_n = nullptr; // grid side of FastGrid cloths (queries in tile order), else 0
   const float* x = nullptr;
   const float* xprev = nullptr;
   const uint32_t* facets = nullptr;

@@ -24,12 +24,16 @@ constexpr int SDEPTH = 16;   // LDS stack entries per thread
 template <bool EMIT_PAIRS, int QB>
 __global__ __launch_bounds__(QB) void k_query(QueryArgs a) {
   __shared__ int s_stack[SDEPTH * QB];
-  const uint32_t j = xcd_block(blockIdx.x, gridDim.x) * QB + threadIdx.x;
+  const uint32_t tq = xcd_block(blockIdx.x, gridDim.x) * QB + threadIdx.x;
   uint32_t npair = 0;
-  if (j < a.nitems) {
-    const uint32_t sg = seg_of(a.qsegs, j);
+  if (tq < a.nitems) {
+    const uint32_t sg = seg_of(a.qsegs, tq);
     const uint32_t m = a.qsegs.mesh[sg];
-    const uint32_t p = j - a.qsegs.off[sg];
+    // grid cloths are queried in 8 x 8 vertex tiles (one tile per wave); the
+    // contact slots stay indexed by vertex
+    const uint32_t gn = a.mesh_grid_n ? a.mesh_grid_n[m] : 0u;
+    const uint32_t p = gn ? tile_order(tq - a.qsegs.off[sg], gn) : tq - a.qsegs.off[sg];
+    const uint32_t j = a.qsegs.off[sg] + p;
     const uint32_t g = a.mt.voff[m] + p;
     const f3 qcur = ld3(a.x, g);
     const f3 qprev = ld3(a.xprev, g);

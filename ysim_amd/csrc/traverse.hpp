@@ -13,6 +13,25 @@
 
 namespace ysim {
 
+// Query order over an N x N grid cloth: the i-th query of the mesh handles
+// vertex tile_order(i, N).  Full 8 x 8 tiles first (tile-major, a wave of 64
+// lanes covers one tile, so its lanes walk nearby subtrees), then the columns
+// right of the last full tile (row-major), then the rows below it (row-major).
+// A bijection on [0, N*N).
+YSIM_HD uint32_t tile_order(uint32_t i, uint32_t N) {
+  const uint32_t nt = N / 8, Nf = nt * 8;
+  if (i < Nf * Nf) {
+    const uint32_t tile = i >> 6, w = i & 63u;
+    const uint32_t r = (tile / nt) * 8 + (w >> 3), c = (tile % nt) * 8 + (w & 7u);
+    return r * N + c;
+  }
+  uint32_t k = i - Nf * Nf;
+  const uint32_t wb = N - Nf;
+  if (k < Nf * wb) return (k / wb) * N + Nf + k % wb;
+  k -= Nf * wb;
+  return (Nf + k / N) * N + k % N;
+}
+
 struct NodeV {
   f3 lo, hi;
   int a, b;

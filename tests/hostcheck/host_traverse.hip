@@ -19,7 +19,7 @@ extern "C" int yt_query(const float* x, uint32_t nq, uint32_t qobj, const ysim_b
                         uint32_t* out_max_iters) {
   uint64_t count = 0, iters = 0;
   uint32_t maxit = 0;
-  std::vector<int> stack(64);
+  std::vector<int2> stack(64);
   for (uint32_t p = 0; p < nq; ++p) {
     f3 q = {x[3 * p], x[3 * p + 1], x[3 * p + 2]};
     f3 qmin = {q.x - margin, q.y - margin, q.z - margin}, qmax = {q.x + margin, q.y + margin, q.z + margin};

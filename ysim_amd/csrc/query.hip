@@ -23,7 +23,7 @@ constexpr int SDEPTH = 16;   // LDS stack entries per thread
 // QB threads per block (64, 128 or 256; chosen by the autotuner).
 template <bool EMIT_PAIRS, int QB>
 __global__ __launch_bounds__(QB) void k_query(QueryArgs a) {
-  __shared__ int s_stack[SDEPTH * QB];
+  __shared__ int2 s_stack[SDEPTH * QB];
   const uint32_t tq = xcd_block(blockIdx.x, gridDim.x) * QB + threadIdx.x;
   uint32_t npair = 0;
   if (tq < a.nitems) {

@@ -42,10 +42,11 @@ YSIM_HD NodeV load_node(const ysim_bvh_node* T, int i) {
   return NodeV{{n.min[0], n.min[1], n.min[2]}, {n.max[0], n.max[1], n.max[2]}, n.child_a, n.child_b};
 }
 
-// stk[k * stride] is stack slot k.  on_leaf(facet) per overlapping leaf.
+// stk[k * stride] is stack slot k: the child links (a, b) of a deferred node,
+// so a pop needs no reload of the node.  on_leaf(facet) per overlapping leaf.
 // Returns the number of loop iterations, or -1 if the guard (corrupt tree) fired.
 template <int SDEPTH, class F>
-YSIM_HD int traverse(const ysim_bvh_node* T, const int32_t* P, uint32_t nT, f3 qmin, f3 qmax, int* stk,
+YSIM_HD int traverse(const ysim_bvh_node* T, const int32_t* P, uint32_t nT, f3 qmin, f3 qmax, int2* stk,
                      int stride, F&& on_leaf) {
   NodeV r = load_node(T, 0);
   if (!overlap(qmin, qmax, r.lo, r.hi)) return 0;
@@ -66,7 +67,7 @@ YSIM_HD int traverse(const ysim_bvh_node* T, const int32_t* P, uint32_t nT, f3 q
     const bool goA = hA && A.a >= 0, goB = hB && B.a >= 0;
     if (goA) {
       if (goB && !stackless) {
-        if (sp < SDEPTH) stk[(sp++) * stride] = cb;
+        if (sp < SDEPTH) stk[(sp++) * stride] = make_int2(B.a, B.b);
         else stackless = true;
       }
       c = ca; ca = A.a; cb = A.b;
@@ -77,10 +78,11 @@ YSIM_HD int traverse(const ysim_bvh_node* T, const int32_t* P, uint32_t nT, f3 q
       continue;
     }
     if (!stackless) {
+      // The current node c is not needed after a pop: the walk only turns
+      // stackless at a push, which is followed by a descent that sets c.
       if (sp == 0) return (int)guard;
-      c = stk[(--sp) * stride];
-      NodeV N = load_node(T, c);
-      ca = N.a; cb = N.b;
+      const int2 e = stk[(--sp) * stride];
+      ca = e.x; cb = e.y;
       continue;
     }
     bool found = false;

@@ -168,6 +168,11 @@ int ysim_hip_morton(ysim_ctx* ctx, const float* d_x, const uint32_t* d_facets,
  * the input order. */
 int ysim_hip_sort_pairs(ysim_ctx* ctx, void* d_keys, uint32_t* d_vals, uint32_t n,
                         uint32_t key_bits);
+/* The same sort under an explicit configuration (results identical for all):
+ * digit_bits 8 or 11 and items (keys per thread per tile) 8 or 16 for the LSD
+ * onesweep; hybrid = 1 selects one 11-bit MSD pass + per-bucket LSD in LDS. */
+int ysim_hip_sort_pairs_ex(ysim_ctx* ctx, void* d_keys, uint32_t* d_vals, uint32_t n, uint32_t key_bits,
+                           uint32_t digit_bits, uint32_t items, uint32_t hybrid);
 
 /* Karras hierarchy over sorted codes: buildTree_Tri (src/metal/bvh.metal:197-258,
  * 352-393).  Writes child links of the 2n-1 node slots and d_parent
@@ -300,6 +305,9 @@ typedef struct ysim_tuning {
   float ms_per_substep;      /* autotune: the chosen setting's measured time; else 0 */
   float ms_per_substep_default; /* autotune: the default setting's measured time */
   uint32_t refit_block;      /* bottom-up refit leaves per block (LDS levels): 256, 512 or 1024 */
+  uint32_t sort_hybrid;      /* 1: one 11-bit MSD pass + per-bucket LSD in LDS (sort_digit_bits unused) */
+  uint32_t sort_incremental; /* 1: keys in the last build's sorted order, LDS window sorts and an exact
+                              * check; the hybrid sort runs only when the check fails */
 } ysim_tuning;
 int ysim_hip_scene_get_tuning(ysim_scene* scene, ysim_tuning* h_out);
 int ysim_hip_scene_set_tuning(ysim_scene* scene, const ysim_tuning* h_in);

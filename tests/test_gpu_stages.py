@@ -49,6 +49,32 @@ def test_sort_matches_oracle(ctx, n, bits):
     assert np.array_equal(kg, ko)
 
 
+SORT_CFGS = [(8, 16, False), (11, 8, False), (11, 16, False), (8, 8, True), (8, 16, True)]
+
+
+@pytest.mark.parametrize("cfg", SORT_CFGS)
+@pytest.mark.parametrize("n,bits", [(1, 63), (100, 7), (5000, 11), (5000, 12), (70000, 19), (70000, 30),
+                                    (300000, 63), (522242, 63)])
+def test_sort_configs_match_argsort(ctx, cfg, n, bits):
+    """Every sort configuration, including the hybrid MSD + per-bucket sort with
+    buckets far above LDS capacity (a third of the keys share one key), gives
+    numpy's stable permutation."""
+    db, items, hybrid = cfg
+    print(f"sort cfg={cfg} n={n} bits={bits}", flush=True)
+    rng = np.random.default_rng(7 * n + bits)
+    hi = np.uint64((1 << bits) - 1)
+    keys = rng.integers(0, 1 << min(bits, 62), n, dtype=np.uint64) & hi
+    if bits == 63:
+        keys |= rng.integers(0, 2, n, dtype=np.uint64) << np.uint64(62)
+    keys[: n // 3] = keys[0]
+    keys[n // 3: n // 2] = rng.integers(0, 64, n // 2 - n // 3, dtype=np.uint64) & hi  # one crowded low bucket
+    vals = np.arange(n, dtype=np.uint32)
+    kg, vg = hip.RadixSorter(ctx).sort(keys, vals, bits, digit_bits=db, items=items, hybrid=hybrid)
+    perm = np.argsort(keys, kind="stable")
+    assert np.array_equal(vg, perm)
+    assert np.array_equal(kg, keys[perm])
+
+
 @pytest.mark.parametrize("bits", [30, 63])
 def test_hierarchy_matches_oracle(ctx, bits):
     cl, _ = cloth_and_sphere(1)

@@ -11,12 +11,62 @@ from ysim_amd.hip import HipScene
 
 pytestmark = pytest.mark.gpu
 
-SORT_CONFIGS = [(8, 8), (8, 16), (11, 8), (11, 16)]
+SORT_CONFIGS = [(8, 8, 0, 0), (8, 16, 0, 0), (11, 8, 0, 0), (11, 16, 0, 0), (8, 8, 1, 0), (8, 16, 1, 0),
+                (8, 8, 0, 1), (8, 16, 0, 1)]
 
 
 def _same_state(a, b):
     for u, w in zip(a, b):
         assert np.array_equal(u, w)
+
+
+@pytest.mark.parametrize("mode", ["hybrid", "incremental"])
+def test_sort_modes_full_fold_bit_exact(ctx, oracle_threads, mode):
+    """The hybrid and incremental sorts on the full 512x512 fold (63-bit, rebuild
+    every substep): state, keys and trees equal the oracle's."""
+    sc = scenes.config_c2()
+    hs, os_ = HipScene(ctx, sc), OracleScene(sc)
+    hs.set_tuning(sort_hybrid=int(mode == "hybrid"), sort_incremental=int(mode == "incremental"), sort_items=16)
+    hs.step(3, graph=True)
+    os_.step(3)
+    _same_state(hs.state(), os_.state())
+    ng, pg, kg, ig = hs.tree(0, keys=True)
+    no, po, ko, io = os_.tree(0)
+    assert np.array_equal(kg, ko) and np.array_equal(ig, io)
+    for f in ("min", "max", "child_a", "child_b"):
+        assert np.array_equal(ng[f], no[f]), f
+    assert hs.flags() == 0
+
+
+def test_incremental_sort_after_state_jumps(ctx, oracle_threads):
+    """The incremental sort starts from the last build's order; after a restore
+    (or a new state) that order is far from the new one, the window passes fail
+    the check and the full sort runs.  Every state stays equal to the oracle's."""
+    sc = scenes.config_c2(n=128)
+    hs, os_ = HipScene(ctx, sc), OracleScene(sc)
+    hs.set_tuning(sort_incremental=1)
+    hs.snapshot()
+    x0, v0, p0 = os_.state()
+    for graph in (True, False):
+        hs.step(6, graph=graph)
+        os_.step(6)
+        _same_state(hs.state(), os_.state())
+        hs.restore()
+        os_.set_state(x0, v0, p0)
+        _same_state(hs.state(), os_.state())
+    # the cloth mirrored in x (an isometry, so the physics stays tame): every
+    # key changes and the sorted order with it
+    x, v, xp = hs.state()
+    V0 = sc.meshes[0].x.shape[0]
+    x[:V0, 0] = np.float32(1.0) - x[:V0, 0]
+    hs.set_state(x, v, x)
+    os_.set_state(x, v, x)
+    hs.step(3, graph=True)
+    os_.step(3)
+    _same_state(hs.state(), os_.state())
+    ng, pg, kg, ig = hs.tree(0, keys=True)
+    no, po, ko, io = os_.tree(0)
+    assert np.array_equal(kg, ko) and np.array_equal(ig, io)
 
 
 @pytest.mark.parametrize("rb", [256, 1024])
@@ -36,12 +86,12 @@ def test_refit_blocks_bit_exact_c2(ctx, oracle_threads, rb):
     assert hs.flags() == 0
 
 
-@pytest.mark.parametrize("db,items", SORT_CONFIGS)
-def test_sort_configs_bit_exact_63bit(ctx, oracle_threads, db, items):
-    print(f"sort config digit_bits={db} items={items} on the 96x96 fold (63-bit)", flush=True)
+@pytest.mark.parametrize("db,items,hyb,inc", SORT_CONFIGS)
+def test_sort_configs_bit_exact_63bit(ctx, oracle_threads, db, items, hyb, inc):
+    print(f"sort config digit_bits={db} items={items} hybrid={hyb} incremental={inc} on the 96x96 fold", flush=True)
     sc = scenes.config_c2(n=96)
     hs, os_ = HipScene(ctx, sc), OracleScene(sc)
-    hs.set_tuning(sort_digit_bits=db, sort_items=items)
+    hs.set_tuning(sort_digit_bits=db, sort_items=items, sort_hybrid=hyb, sort_incremental=inc)
     assert hs.tuning()["sort_digit_bits"] == db
     for graph in (False, True):
         hs.step(2, graph=graph)
@@ -56,11 +106,11 @@ def test_sort_configs_bit_exact_63bit(ctx, oracle_threads, db, items):
     assert hs.flags() == 0
 
 
-@pytest.mark.parametrize("db,items", SORT_CONFIGS)
-def test_sort_configs_bit_exact_30bit_rebuild(ctx, oracle_threads, db, items):
+@pytest.mark.parametrize("db,items,hyb,inc", SORT_CONFIGS)
+def test_sort_configs_bit_exact_30bit_rebuild(ctx, oracle_threads, db, items, hyb, inc):
     sc = scenes.config_drape(n=40, height=0.8, rebuild_every=1)
     hs, os_ = HipScene(ctx, sc), OracleScene(sc)
-    hs.set_tuning(sort_digit_bits=db, sort_items=items)
+    hs.set_tuning(sort_digit_bits=db, sort_items=items, sort_hybrid=hyb, sort_incremental=inc)
     hs.step(12, graph=True)
     os_.step(12)
     _same_state(hs.state(), os_.state())
@@ -118,7 +168,7 @@ def test_autotune_batched_keeps_results(ctx):
 def test_invalid_tuning_rejected(ctx):
     hs = HipScene(ctx, scenes.config_drape(n=16))
     for kw in ({"sort_digit_bits": 9}, {"sort_items": 4}, {"query_block": 96}, {"cloth_block": 1024},
-               {"refit_block": 128}):
+               {"refit_block": 128}, {"sort_hybrid": 2}, {"sort_incremental": 3}):
         with pytest.raises(YsimError):
             hs.set_tuning(**kw)
     hs.step(2)

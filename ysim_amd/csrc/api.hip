@@ -70,6 +70,7 @@ struct BuildGroup {
   uint32_t code_bits = 30;
   bool key64 = false;
   SortPlan plan;
+  uint32_t* perm = nullptr;  // dynamic groups: sorted facet order of the last build (incremental sort)
 };
 
 constexpr int NEV = YSIM_STAGE_COUNT + 1;
@@ -106,6 +107,7 @@ struct ysim_scene {
   uint32_t *hist = nullptr, *counters = nullptr, *box_acc = nullptr;
   uint64_t* status = nullptr;   // sort look-back words (epoch-tagged, never cleared)
   uint32_t* epoch = nullptr;    // [0] look-back epoch of the current sort
+  uint32_t* need_full = nullptr;  // [0] incremental sort: the window passes left the keys unsorted
   uint32_t cap = 16;
   uint32_t* ccount = nullptr;
   float4* cnd = nullptr;
@@ -148,7 +150,7 @@ struct ysim_scene {
   } adj;
   float graph_kd = 0.f;
   const float* graph_ks = nullptr;
-  ysim_tuning tune{8, 8, 128, 256, 0.f, 0.f, 256};
+  ysim_tuning tune{8, 8, 128, 256, 0.f, 0.f, 256, 0, 0};
   float* tune_buf = nullptr;   // autotune state save: x, v, xPrev of the whole scene
 
   int grow(float** p, size_t& have, size_t need) {
@@ -190,14 +192,22 @@ struct ysim_scene {
     if (!query_block_ok(t.query_block)) return fail(YSIM_ERR_INVALID, "tuning: query_block must be 64, 128 or 256");
     if (!cloth_block_ok(t.cloth_block)) return fail(YSIM_ERR_INVALID, "tuning: cloth_block must be 128, 256 or 512");
     if (!refit_block_ok(t.refit_block)) return fail(YSIM_ERR_INVALID, "tuning: refit_block must be 256, 512 or 1024");
-    const bool replan = t.sort_digit_bits != tune.sort_digit_bits || t.sort_items != tune.sort_items;
+    if (t.sort_hybrid > 1) return fail(YSIM_ERR_INVALID, "tuning: sort_hybrid must be 0 or 1");
+    if (t.sort_incremental > 1) return fail(YSIM_ERR_INVALID, "tuning: sort_incremental must be 0 or 1");
+    const bool replan = t.sort_digit_bits != tune.sort_digit_bits || t.sort_items != tune.sort_items ||
+                        t.sort_hybrid != tune.sort_hybrid || t.sort_incremental != tune.sort_incremental;
     tune.sort_digit_bits = t.sort_digit_bits;
     tune.sort_items = t.sort_items;
     tune.query_block = t.query_block;
     tune.cloth_block = t.cloth_block;
     tune.refit_block = t.refit_block;
+    tune.sort_hybrid = t.sort_hybrid;
+    tune.sort_incremental = t.sort_incremental;
     if (replan) {
-      for (auto& g : dyn_groups) g.plan = sort_plan(g.T, g.plan.key_bits, t.sort_digit_bits, t.sort_items);
+      for (auto& g : dyn_groups)
+        g.plan = (t.sort_hybrid || t.sort_incremental)
+                     ? sort_plan_hybrid(g.T, g.plan.key_bits, t.sort_items)
+                     : sort_plan(g.T, g.plan.key_bits, t.sort_digit_bits, t.sort_items);
       YSIM_TRY(zero(hist, SORT_HIST_WORDS));
     }
     drop_graph();
@@ -260,14 +270,19 @@ struct ysim_scene {
     int rc = measure(best, best_ms);
     const float default_ms = best_ms;
     const bool rebuilds = P.rebuild_every > 0 && !hash_mode;
-    struct Cand { uint32_t a, b; };
     // coordinate descent: sort (only when the substep rebuilds), query block, cloth block
     if (rc == YSIM_OK && rebuilds) {
-      for (Cand c : {Cand{8, 8}, Cand{8, 16}, Cand{11, 8}, Cand{11, 16}}) {
+      struct SortCand { uint32_t db, items, hybrid, inc; };
+      for (SortCand c : {SortCand{8, 8, 0, 0}, SortCand{8, 16, 0, 0}, SortCand{11, 8, 0, 0}, SortCand{11, 16, 0, 0},
+                         SortCand{8, 8, 1, 0}, SortCand{8, 16, 1, 0}, SortCand{8, 16, 0, 1}}) {
         ysim_tuning t = best;
-        t.sort_digit_bits = c.a;
-        t.sort_items = c.b;
-        if (t.sort_digit_bits == best.sort_digit_bits && t.sort_items == best.sort_items) continue;
+        t.sort_digit_bits = c.db;
+        t.sort_items = c.items;
+        t.sort_hybrid = c.hybrid;
+        t.sort_incremental = c.inc;
+        if (t.sort_digit_bits == best.sort_digit_bits && t.sort_items == best.sort_items &&
+            t.sort_hybrid == best.sort_hybrid && t.sort_incremental == best.sort_incremental)
+          continue;
         float ms = 0.f;
         if ((rc = measure(t, ms)) != YSIM_OK) break;
         if (ms < best_ms) { best_ms = ms; best = t; }
@@ -376,7 +391,8 @@ struct ysim_scene {
       g.code_bits = cb;
       uint32_t key_bits = cb + (l.size() > 1 ? bits_for((uint32_t)l.size()) : 0);
       g.key64 = key_bits > 32;
-      g.plan = sort_plan(g.T, key_bits, tune.sort_digit_bits, tune.sort_items);
+      g.plan = tune.sort_hybrid ? sort_plan_hybrid(g.T, key_bits, tune.sort_items)
+                                : sort_plan(g.T, key_bits, tune.sort_digit_bits, tune.sort_items);
       out.push_back(g);
     }
     return YSIM_OK;
@@ -386,15 +402,37 @@ struct ysim_scene {
     if (!g.T) return YSIM_OK;
     YSIM_TRY(launch_box(st, g.vsegs, mt, xc, g.V, box_acc));
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[1], st));
+    if (g.perm && tune.sort_incremental) {
+      // keys in the last build's sorted order + LDS window sorts + exact check;
+      // the full build (Morton in facet order + hybrid sort) runs only if the
+      // check failed (its kernels return at once otherwise)
+      YSIM_TRY(launch_morton_inc(st, g.tsegs, mt, xc, facets, box_acc, g.code_bits, g.key64, g.perm, keys[0],
+                                 vals[0], g.T, need_full));
+      if (timing) YSIM_HIP_TRY(hipEventRecord(ev[2], st));
+      YSIM_TRY(launch_incremental_fix(st, g.key64, keys[0], vals[0], g.T, need_full));
+      YSIM_TRY(launch_morton(st, g.tsegs, mt, xc, facets, box_acc, g.code_bits, g.key64, keys[0], vals[0], g.T,
+                             g.plan, hist, counters, epoch, need_full));
+      YSIM_TRY(launch_sort_hybrid(st, g.plan, keys[0], vals[0], keys[1], vals[1], hist, status, counters, epoch,
+                                  flags, need_full));
+      if (timing) YSIM_HIP_TRY(hipEventRecord(ev[3], st));
+      YSIM_TRY(launch_hierarchy(st, g.tsegs, mt, keys[0], g.key64, vals[0], g.code_bits, tree, parent, visit, g.T,
+                                box_acc, g.tsegs.nseg, hist, (uint32_t)g.plan.hist_words, range, g.perm));
+      last_sorted = 0;
+      return YSIM_OK;
+    }
     YSIM_TRY(launch_morton(st, g.tsegs, mt, xc, facets, box_acc, g.code_bits, g.key64, keys[0], vals[0], g.T,
                            g.plan, hist, counters, epoch));
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[2], st));
-    YSIM_TRY(launch_sort_passes(st, g.plan, keys[0], vals[0], keys[1], vals[1], hist, status, counters, epoch,
-                                flags));
+    if (g.plan.hybrid)
+      YSIM_TRY(launch_sort_hybrid(st, g.plan, keys[0], vals[0], keys[1], vals[1], hist, status, counters, epoch,
+                                  flags));
+    else
+      YSIM_TRY(launch_sort_passes(st, g.plan, keys[0], vals[0], keys[1], vals[1], hist, status, counters, epoch,
+                                  flags));
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[3], st));
-    const uint32_t sorted = (g.plan.passes % 2 == 0) ? 0 : 1;
+    const uint32_t sorted = (g.plan.hybrid || g.plan.passes % 2 == 0) ? 0 : 1;
     YSIM_TRY(launch_hierarchy(st, g.tsegs, mt, keys[sorted], g.key64, vals[sorted], g.code_bits, tree, parent,
-                              visit, g.T, box_acc, g.tsegs.nseg, hist, (uint32_t)g.plan.hist_words, range));
+                              visit, g.T, box_acc, g.tsegs.nseg, hist, (uint32_t)g.plan.hist_words, range, g.perm));
     last_sorted = sorted;
     return YSIM_OK;
   }
@@ -724,11 +762,18 @@ int ysim_hip_morton(ysim_ctx* ctx, const float* d_x, const uint32_t* d_facets, u
 }
 
 int ysim_hip_sort_pairs(ysim_ctx* ctx, void* d_keys, uint32_t* d_vals, uint32_t n, uint32_t key_bits) {
+  return ysim_hip_sort_pairs_ex(ctx, d_keys, d_vals, n, key_bits, 8, 8, 0);
+}
+
+int ysim_hip_sort_pairs_ex(ysim_ctx* ctx, void* d_keys, uint32_t* d_vals, uint32_t n, uint32_t key_bits,
+                           uint32_t digit_bits, uint32_t items, uint32_t hybrid) {
   YSIM_ARG(ctx && (n == 0 || (d_keys && d_vals)), "sort_pairs: null argument");
   YSIM_ARG(key_bits >= 1 && key_bits <= 64, "sort_pairs: key_bits in [1, 64]");
+  YSIM_ARG(hybrid ? (items == 8 || items == 16) : sort_config_ok(digit_bits, items),
+           "sort_pairs: digit_bits 8 or 11, items 8 or 16");
   if (n == 0) return YSIM_OK;
   YSIM_ARG(n < (1u << 30), "sort_pairs: n must be < 2^30");
-  SortPlan p = sort_plan(n, key_bits);
+  SortPlan p = hybrid ? sort_plan_hybrid(n, key_bits, items) : sort_plan(n, key_bits, digit_bits, items);
   const size_t kb = (p.wide ? 8 : 4);
   // layout: [4096 words misc][hist][counters][err 16][epoch 16][status (64-bit words)][pad to 256 B]
   //         [keys_b, 256-B rounded][vals_b]
@@ -748,8 +793,11 @@ int ysim_hip_sort_pairs(ysim_ctx* ctx, void* d_keys, uint32_t* d_vals, uint32_t 
   YSIM_TRY(launch_sort_reset(ctx->stream, p, hist, status, counters, epoch, true));
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, err, 16u);
   YSIM_TRY(launch_sort_histogram(ctx->stream, p, d_keys, hist));
-  YSIM_TRY(launch_sort_passes(ctx->stream, p, d_keys, d_vals, keys_b, vals_b, hist, status, counters, epoch, err));
-  if (p.passes % 2 == 1) {
+  if (p.hybrid)
+    YSIM_TRY(launch_sort_hybrid(ctx->stream, p, d_keys, d_vals, keys_b, vals_b, hist, status, counters, epoch, err));
+  else
+    YSIM_TRY(launch_sort_passes(ctx->stream, p, d_keys, d_vals, keys_b, vals_b, hist, status, counters, epoch, err));
+  if (!p.hybrid && p.passes % 2 == 1) {
     YSIM_HIP_TRY(hipMemcpyAsync(d_keys, keys_b, n * kb, hipMemcpyDeviceToDevice, ctx->stream));
     YSIM_HIP_TRY(hipMemcpyAsync(d_vals, vals_b, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
   }
@@ -1054,6 +1102,16 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
   YSIM_TRY(s.alloc(&s.status, maxStatus));
   YSIM_TRY(s.alloc(&s.counters, SORT_MAX_PASSES));
   YSIM_TRY(s.alloc(&s.epoch, 16));
+  YSIM_TRY(s.alloc(&s.need_full, 16));
+  YSIM_TRY(s.zero(s.need_full, 16));
+  // incremental sort: each dynamic group starts from the facet order
+  for (auto& g : s.dyn_groups) {
+    std::vector<uint32_t> id;
+    id.reserve(g.T);
+    for (uint32_t m : g.meshes)
+      for (uint32_t f = 0; f < s.meshes[m].num_facets; ++f) id.push_back(f);
+    YSIM_TRY(s.put(&g.perm, id));
+  }
   YSIM_TRY(s.alloc(&s.box_acc, 6 * maxSeg));
   YSIM_TRY(s.zero(s.hist, SORT_HIST_WORDS));
   YSIM_TRY(s.zero(s.status, 2 * maxStatus));

@@ -41,17 +41,23 @@ int launch_box(hipStream_t s, SegTable vsegs, MeshTables mt, const float* x, uin
 // bits when nseg > 1), values = mesh-local facet id, plus the digit histograms
 // of all sort passes of `plan`; zeroes the sort's block counters and advances
 // its look-back epoch (either may be null when no sort follows).
+// run_flag (optional): the kernel does nothing while *run_flag == 0.
 int launch_morton(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x,
                   const uint32_t* facets, const uint32_t* box_acc, uint32_t code_bits,
                   bool key64, void* keys, uint32_t* vals, uint32_t nitems, const SortPlan& plan,
-                  uint32_t* hist, uint32_t* counters, uint32_t* epoch);
+                  uint32_t* hist, uint32_t* counters, uint32_t* epoch, const uint32_t* run_flag = nullptr);
+// Incremental S1: keys in the order perm (previous sorted facets), each
+// 2048-key window sorted by (key, facet); zeroes *need_full.
+int launch_morton_inc(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x, const uint32_t* facets,
+                      const uint32_t* box_acc, uint32_t code_bits, bool key64, const uint32_t* perm, void* keys,
+                      uint32_t* vals, uint32_t nitems, uint32_t* need_full);
 // S3: Karras hierarchy per segment; leaf child links; zeroes the refit visit
 // counters; resets box_acc (nbox segments) and the sort histogram for the next build.
 // range (optional): per internal node its sorted leaf range (lo, hi), node-window indexed.
 int launch_hierarchy(hipStream_t s, SegTable tsegs, MeshTables mt, const void* keys, bool key64,
                      const uint32_t* vals, uint32_t code_bits, ysim_bvh_node* tree, int32_t* parent,
                      uint32_t* visit, uint32_t nitems, uint32_t* box_acc, uint32_t nbox,
-                     uint32_t* hist, uint32_t hist_words, uint2* range = nullptr);
+                     uint32_t* hist, uint32_t hist_words, uint2* range = nullptr, uint32_t* perm_out = nullptr);
 // S4+S5: leaf boxes (swept optional) + atomic bottom-up union per segment.
 // With range (from launch_hierarchy) the levels inside blocks of `block`
 // (256, 512 or 1024) sorted leaves are combined in LDS.

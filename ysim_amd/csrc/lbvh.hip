@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "incsort.hpp"
 #include "kernels.hpp"
 
 namespace ysim {
@@ -110,16 +111,20 @@ int launch_box(hipStream_t s, SegTable vsegs, MeshTables mt, const float* x, uin
 }
 
 // ------------------------------------------------------------------ S1 Morton
-template <typename K, int DB>
+// NH histograms of DB-bit digits at shifts hist_shift + DB * p (p < passes <= NH).
+template <typename K, int DB, int NH>
 __global__ __launch_bounds__(256) void k_morton(SegTable ts, MeshTables mt, const float* __restrict__ x,
                                                 const uint32_t* __restrict__ facets,
                                                 const uint32_t* __restrict__ acc, uint32_t code_bits,
                                                 K* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                uint32_t n, uint32_t passes, uint32_t* __restrict__ hist,
-                                                uint32_t* __restrict__ counters, uint32_t* __restrict__ epoch) {
+                                                uint32_t n, uint32_t passes, uint32_t hist_shift,
+                                                uint32_t* __restrict__ hist,
+                                                uint32_t* __restrict__ counters, uint32_t* __restrict__ epoch,
+                                                const uint32_t* __restrict__ run_flag) {
   constexpr uint32_t DIGITS = 1u << DB;
-  constexpr uint32_t HW = (DB == 8) ? SORT_MAX_PASSES * 256 : SORT_HIST_WORDS;
+  constexpr uint32_t HW = NH * DIGITS;
   __shared__ uint32_t s_h[HW];
+  if (run_flag && *run_flag == 0u) return;  // the incremental sort succeeded: nothing to do
   for (uint32_t i = threadIdx.x; i < passes * DIGITS; i += 256) s_h[i] = 0;
   // the next sort's block counters start at zero and its look-back words carry
   // a fresh epoch (the sort passes run after this kernel completes)
@@ -145,7 +150,7 @@ __global__ __launch_bounds__(256) void k_morton(SegTable ts, MeshTables mt, cons
     keys[j] = key;
     vals[j] = fid;
     for (uint32_t p = 0; p < passes; ++p)
-      atomicAdd(&s_h[p * DIGITS + (uint32_t)((key >> (DB * p)) & (DIGITS - 1))], 1u);
+      atomicAdd(&s_h[p * DIGITS + (uint32_t)((key >> (hist_shift + DB * p)) & (DIGITS - 1))], 1u);
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < passes * DIGITS; i += 256)
@@ -155,18 +160,86 @@ __global__ __launch_bounds__(256) void k_morton(SegTable ts, MeshTables mt, cons
 int launch_morton(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x,
                   const uint32_t* facets, const uint32_t* box_acc, uint32_t code_bits, bool key64,
                   void* keys, uint32_t* vals, uint32_t nitems, const SortPlan& plan, uint32_t* hist,
-                  uint32_t* counters, uint32_t* epoch) {
+                  uint32_t* counters, uint32_t* epoch, const uint32_t* run_flag) {
   if (!nitems) return YSIM_OK;
   uint32_t blocks = ceil_div(nitems, 256 * MORTON_ITEMS);
-#define YSIM_MORTON(KT, DBV)                                                                                  \
-  hipLaunchKernelGGL((k_morton<KT, DBV>), dim3(blocks), dim3(256), 0, s, tsegs, mt, x, facets, box_acc,      \
-                     code_bits, (KT*)keys, vals, nitems, plan.passes, hist, counters, epoch)
-  if (plan.digit_bits == 11) {
-    if (key64) YSIM_MORTON(uint64_t, 11); else YSIM_MORTON(uint32_t, 11);
+#define YSIM_MORTON(KT, DBV, NHV)                                                                             \
+  hipLaunchKernelGGL((k_morton<KT, DBV, NHV>), dim3(blocks), dim3(256), 0, s, tsegs, mt, x, facets, box_acc, \
+                     code_bits, (KT*)keys, vals, nitems, plan.passes, plan.hist_shift, hist, counters, epoch, \
+                     run_flag)
+  if (plan.hybrid) {
+    if (key64) YSIM_MORTON(uint64_t, 11, 1); else YSIM_MORTON(uint32_t, 11, 1);
+  } else if (plan.digit_bits == 11) {
+    if (key64) YSIM_MORTON(uint64_t, 11, 6); else YSIM_MORTON(uint32_t, 11, 6);
   } else {
-    if (key64) YSIM_MORTON(uint64_t, 8); else YSIM_MORTON(uint32_t, 8);
+    if (key64) YSIM_MORTON(uint64_t, 8, 8); else YSIM_MORTON(uint32_t, 8, 8);
   }
 #undef YSIM_MORTON
+  YSIM_CHECK_LAUNCH();
+  return YSIM_OK;
+}
+
+// S1 of the incremental build: Morton keys in the previous substep's sorted
+// order (perm[j] = facet at sorted position j; the positions of a mesh stay in
+// its segment because keys carry the segment above the code bits), each block's
+// 2048 positions then sorted in LDS by (key, facet).  Clears need_full for the
+// check that follows.
+// 1024 threads per 2048-key window: the per-key chain of dependent loads
+// (order -> facet -> positions) needs many waves in flight.
+constexpr uint32_t INC_THREADS = 1024;
+
+template <typename K>
+__global__ __launch_bounds__(INC_THREADS) void k_morton_inc(SegTable ts, MeshTables mt, const float* __restrict__ x,
+                                                    const uint32_t* __restrict__ facets,
+                                                    const uint32_t* __restrict__ acc, uint32_t code_bits,
+                                                    const uint32_t* __restrict__ perm, K* __restrict__ keys,
+                                                    uint32_t* __restrict__ vals, uint32_t n,
+                                                    uint32_t* __restrict__ need_full) {
+  __shared__ K sk[INC_WINDOW];
+  __shared__ uint32_t sv[INC_WINDOW];
+  if (blockIdx.x == 0 && threadIdx.x == 0) *need_full = 0u;
+  const bool wide = code_bits > 32;
+  const uint32_t start = blockIdx.x * INC_WINDOW;
+  const uint32_t cnt = min(INC_WINDOW, n - start);
+  for (uint32_t i = threadIdx.x; i < INC_WINDOW; i += INC_THREADS) {
+    if (i >= cnt) {
+      sk[i] = (K)~(K)0;
+      sv[i] = ~0u;
+      continue;
+    }
+    const uint32_t j = start + i;
+    const uint32_t sg = seg_of(ts, j);
+    const uint32_t m = ts.mesh[sg];
+    const uint32_t fid = perm[j];
+    const uint32_t* f = facets + 3 * (size_t)(mt.foff[m] + fid);
+    const uint32_t vb = mt.voff[m];
+    const f3 x0 = ld3(x, vb + f[0]), x1 = ld3(x, vb + f[1]), x2 = ld3(x, vb + f[2]);
+    const uint32_t* a = acc + sg * 6;
+    const f3 bmin = {funkey(a[0]), funkey(a[1]), funkey(a[2])};
+    const f3 bmax = {funkey(a[3]), funkey(a[4]), funkey(a[5])};
+    const uint64_t code = morton_of(x0, x1, x2, bmin, bmax, wide);
+    sk[i] = (ts.nseg > 1) ? (K)(((uint64_t)sg << code_bits) | code) : (K)code;
+    sv[i] = fid;
+  }
+  __syncthreads();
+  window_sort<K, INC_WINDOW, INC_THREADS>(sk, sv, cnt);
+  for (uint32_t i = threadIdx.x; i < cnt; i += INC_THREADS) {
+    keys[start + i] = sk[i];
+    vals[start + i] = sv[i];
+  }
+}
+
+int launch_morton_inc(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x, const uint32_t* facets,
+                      const uint32_t* box_acc, uint32_t code_bits, bool key64, const uint32_t* perm, void* keys,
+                      uint32_t* vals, uint32_t nitems, uint32_t* need_full) {
+  if (!nitems) return YSIM_OK;
+  const uint32_t blocks = ceil_div(nitems, INC_WINDOW);
+  if (key64)
+    hipLaunchKernelGGL(k_morton_inc<uint64_t>, dim3(blocks), dim3(INC_THREADS), 0, s, tsegs, mt, x, facets, box_acc,
+                       code_bits, perm, (uint64_t*)keys, vals, nitems, need_full);
+  else
+    hipLaunchKernelGGL(k_morton_inc<uint32_t>, dim3(blocks), dim3(INC_THREADS), 0, s, tsegs, mt, x, facets, box_acc,
+                       code_bits, perm, (uint32_t*)keys, vals, nitems, need_full);
   YSIM_CHECK_LAUNCH();
   return YSIM_OK;
 }
@@ -202,7 +275,7 @@ __global__ __launch_bounds__(256) void k_hierarchy(SegTable ts, MeshTables mt, c
                                                    uint32_t* __restrict__ visit, uint32_t n,
                                                    uint32_t* __restrict__ box_acc, uint32_t nbox,
                                                    uint32_t* __restrict__ hist, uint32_t hist_words,
-                                                   uint2* __restrict__ range) {
+                                                   uint2* __restrict__ range, uint32_t* __restrict__ perm_out) {
   const uint32_t j = blockIdx.x * 256 + threadIdx.x;
   // reset accumulators for the next build (their readers ran in earlier kernels)
   if (j < nbox * 6) box_acc[j] = ((j % 6) < 3) ? 0xffffffffu : 0u;
@@ -219,6 +292,8 @@ __global__ __launch_bounds__(256) void k_hierarchy(SegTable ts, MeshTables mt, c
   const bool wide = code_bits > 32;
   const uint64_t mask = (code_bits >= 64) ? ~0ull : ((1ull << code_bits) - 1);
   KarrasSeg<K> K_{keys + s, vals + s, nn, mask, wide};
+  // the sorted facet order seeds the next incremental build
+  if (perm_out) perm_out[j] = K_.idx[i];
   // leaf slot nn-1+i: child_a = -1, child_b = facet id
   T[nn - 1 + i].child_a = -1;
   T[nn - 1 + i].child_b = (int32_t)K_.idx[i];
@@ -259,18 +334,18 @@ __global__ __launch_bounds__(256) void k_hierarchy(SegTable ts, MeshTables mt, c
 int launch_hierarchy(hipStream_t s, SegTable tsegs, MeshTables mt, const void* keys, bool key64,
                      const uint32_t* vals, uint32_t code_bits, ysim_bvh_node* tree, int32_t* parent,
                      uint32_t* visit, uint32_t nitems, uint32_t* box_acc, uint32_t nbox, uint32_t* hist,
-                     uint32_t hist_words, uint2* range) {
+                     uint32_t hist_words, uint2* range, uint32_t* perm_out) {
   uint32_t work = std::max<uint32_t>(nitems, std::max<uint32_t>(nbox * 6, hist_words));
   if (!work) return YSIM_OK;
   uint32_t blocks = ceil_div(work, 256);
   if (key64)
     hipLaunchKernelGGL(k_hierarchy<uint64_t>, dim3(blocks), dim3(256), 0, s, tsegs, mt,
                        (const uint64_t*)keys, vals, code_bits, tree, parent, visit, nitems, box_acc, nbox,
-                       hist, hist_words, range);
+                       hist, hist_words, range, perm_out);
   else
     hipLaunchKernelGGL(k_hierarchy<uint32_t>, dim3(blocks), dim3(256), 0, s, tsegs, mt,
                        (const uint32_t*)keys, vals, code_bits, tree, parent, visit, nitems, box_acc, nbox,
-                       hist, hist_words, range);
+                       hist, hist_words, range, perm_out);
   YSIM_CHECK_LAUNCH();
   return YSIM_OK;
 }

@@ -11,6 +11,7 @@
 // the global scatter writes contiguous runs.  Digit width (8 or 11 bits) and
 // tile size (256 x 8 or 256 x 16 keys) are template parameters chosen per
 // scene by the autotuner; the permutation does not depend on them.
+#include "incsort.hpp"
 #include "sort.hpp"
 
 namespace ysim {
@@ -49,15 +50,15 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* s_w
 // Digit histograms of every pass from one read of the keys (used by the stage
 // entry point; the scene path fuses the same accumulation into Morton).
 template <typename K, int DB>
-__global__ __launch_bounds__(256) void k_sort_histogram(const K* __restrict__ keys, uint32_t n,
-                                                        uint32_t passes, uint32_t* __restrict__ hist) {
+__global__ __launch_bounds__(256) void k_sort_histogram(const K* __restrict__ keys, uint32_t n, uint32_t passes,
+                                                        uint32_t shift0, uint32_t* __restrict__ hist) {
   constexpr uint32_t DIGITS = 1u << DB;
   __shared__ uint32_t s_h[SORT_HIST_WORDS];
   for (uint32_t i = threadIdx.x; i < passes * DIGITS; i += 256) s_h[i] = 0;
   __syncthreads();
   for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     K k = keys[i];
-    for (uint32_t p = 0; p < passes; ++p) atomicAdd(&s_h[p * DIGITS + digit_of(k, DB * p, DIGITS - 1)], 1u);
+    for (uint32_t p = 0; p < passes; ++p) atomicAdd(&s_h[p * DIGITS + digit_of(k, shift0 + DB * p, DIGITS - 1)], 1u);
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < passes * DIGITS; i += 256)
@@ -81,7 +82,8 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
                                                   uint64_t* __restrict__ status,
                                                   uint32_t* __restrict__ counter,
                                                   const uint32_t* __restrict__ epoch_p,
-                                                  uint32_t* __restrict__ err) {
+                                                  uint32_t* __restrict__ err,
+                                                  const uint32_t* __restrict__ run_flag) {
   constexpr int WAVES = 4, BLOCK = 256, TILE = BLOCK * ITEMS;
   constexpr uint32_t DIGITS = 1u << DB, DMASK = DIGITS - 1;
   constexpr int DPT = DIGITS / BLOCK;
@@ -93,6 +95,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
   __shared__ uint32_t s_vals[TILE];
   __shared__ uint32_t s_bid;
 
+  if (run_flag && *run_flag == 0u) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   if (tid == 0) s_bid = atomicAdd(counter, 1u);
   for (int i = tid; i < WAVES * (int)DIGITS; i += BLOCK) (&s_wc[0][0])[i] = 0;
@@ -220,6 +223,191 @@ __global__ __launch_bounds__(256) void k_onesweep(const K* __restrict__ kin, con
   }
 }
 
+// ------------------------------------------------------------------ hybrid
+// MSD-then-local sort: one onesweep pass on the top 11 key bits partitions the
+// keys into 2048 buckets (stably), then k_bucket_sort gives each bucket one
+// workgroup that sorts it by the remaining bits with stable 8-bit LSD passes.
+// A bucket of up to 256 * LITEMS keys stays in LDS for all of its passes; a
+// larger one runs the same passes over global memory in chunks of that size,
+// ping-ponging between the two buffers inside its own range.  Sorting each
+// bucket stably by the low bits after a stable partition by the high bits is
+// the same permutation as the full LSD sort.
+
+namespace {
+// Stable within-chunk ranks of one wave's rows: rank[k] = number of earlier
+// keys of the wave with the same 8-bit digit; wc[] counts per digit.
+template <typename K, int ITEMS>
+__device__ __forceinline__ void rank_rows8(const K (&key)[ITEMS], uint32_t base_idx, uint32_t limit, uint32_t shift,
+                                           uint32_t* wc, uint32_t (&rank)[ITEMS]) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const uint32_t idx = base_idx + (uint32_t)k * 64u + lane;
+    const bool ok = idx < limit;
+    const uint32_t d = digit_of(key[k], shift, 255u);
+    uint64_t match = __ballot(ok);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bal = __ballot(bit);
+      match &= bit ? bal : ~bal;
+    }
+    const uint32_t lt = (uint32_t)__popcll(match & lt_mask);
+    const uint32_t base = ok ? wc[d] : 0u;
+    rank[k] = base + lt;
+    if (ok && lt == 0) wc[d] = base + (uint32_t)__popcll(match);
+  }
+}
+}  // namespace
+
+template <typename K, int LITEMS>
+__global__ __launch_bounds__(256) void k_bucket_sort(K* __restrict__ ka, uint32_t* __restrict__ va,
+                                                     K* __restrict__ kb, uint32_t* __restrict__ vb,
+                                                     const uint32_t* __restrict__ msd_hist, uint32_t local_passes,
+                                                     const uint32_t* __restrict__ run_flag) {
+  constexpr int WAVES = 4, BLOCK = 256, CAP = BLOCK * LITEMS;
+  __shared__ uint32_t s_wc[WAVES][256];
+  __shared__ uint32_t s_lbase[256];
+  __shared__ uint32_t s_gbase[256];
+  __shared__ uint32_t s_tot[256];
+  __shared__ uint32_t s_wsum[WAVES];
+  __shared__ K s_keys[CAP];
+  __shared__ uint32_t s_vals[CAP];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  if (run_flag && *run_flag == 0u) return;
+  const uint32_t bucket = blockIdx.x;
+  const uint32_t S = msd_hist[bucket];
+  if (S == 0) return;
+  // bucket start: sum of the preceding bucket sizes
+  uint32_t part = 0;
+  for (uint32_t i = tid; i < bucket; i += BLOCK) part += msd_hist[i];
+  const uint32_t start = block_excl_scan256(part, s_wsum) + part;  // inclusive for the last thread
+  __shared__ uint32_t s_start;
+  if (tid == BLOCK - 1) s_start = start;
+  __syncthreads();
+  const uint32_t st = s_start;
+  const uint32_t wbase = (uint32_t)w * 64u * LITEMS;
+
+  // per-digit totals of the staged rows, exclusive wave offsets in s_wc,
+  // chunk-local digit bases in s_lbase, totals in s_tot
+  auto digit_bases = [&]() {
+    const uint32_t d = (uint32_t)tid;
+    uint32_t total = 0;
+#pragma unroll
+    for (int ww = 0; ww < WAVES; ++ww) {
+      const uint32_t c = s_wc[ww][d];
+      s_wc[ww][d] = total;
+      total += c;
+    }
+    s_tot[d] = total;
+    s_lbase[d] = block_excl_scan256(total, s_wsum);
+  };
+
+  if (S <= (uint32_t)CAP) {
+    // the bucket (the MSD pass left it in b) lives in LDS for every pass
+    for (uint32_t i = tid; i < S; i += BLOCK) {
+      s_keys[i] = kb[st + i];
+      s_vals[i] = vb[st + i];
+    }
+    __syncthreads();
+    for (uint32_t pass = 0; pass < local_passes; ++pass) {
+      K key[LITEMS];
+      uint32_t val[LITEMS], rank[LITEMS];
+#pragma unroll
+      for (int k = 0; k < LITEMS; ++k) {
+        const uint32_t idx = wbase + (uint32_t)k * 64u + lane;
+        key[k] = idx < S ? s_keys[idx] : (K)0;
+        val[k] = idx < S ? s_vals[idx] : 0u;
+      }
+      for (int i = tid; i < WAVES * 256; i += BLOCK) (&s_wc[0][0])[i] = 0;
+      __syncthreads();
+      rank_rows8<K, LITEMS>(key, wbase, S, 8u * pass, s_wc[w], rank);
+      __syncthreads();
+      digit_bases();
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < LITEMS; ++k) {
+        const uint32_t idx = wbase + (uint32_t)k * 64u + lane;
+        if (idx < S) {
+          const uint32_t dg = digit_of(key[k], 8u * pass, 255u);
+          const uint32_t lp = s_lbase[dg] + s_wc[w][dg] + rank[k];
+          s_keys[lp] = key[k];
+          s_vals[lp] = val[k];
+        }
+      }
+      __syncthreads();
+    }
+    for (uint32_t i = tid; i < S; i += BLOCK) {
+      ka[st + i] = s_keys[i];
+      va[st + i] = s_vals[i];
+    }
+    return;
+  }
+
+  // large bucket: chunked passes over global memory, b -> a -> b ...
+  K* ksrc = kb;
+  uint32_t* vsrc = vb;
+  K* kdst = ka;
+  uint32_t* vdst = va;
+  for (uint32_t pass = 0; pass < local_passes; ++pass) {
+    const uint32_t shift = 8u * pass;
+    s_tot[tid] = 0;
+    __syncthreads();
+    for (uint32_t i = tid; i < S; i += BLOCK) atomicAdd(&s_tot[digit_of(ksrc[st + i], shift, 255u)], 1u);
+    __syncthreads();
+    const uint32_t cnt = s_tot[tid];
+    s_gbase[tid] = st + block_excl_scan256(cnt, s_wsum);
+    __syncthreads();
+    for (uint32_t c0 = 0; c0 < S; c0 += CAP) {
+      const uint32_t cs = min((uint32_t)CAP, S - c0);
+      K key[LITEMS];
+      uint32_t val[LITEMS], rank[LITEMS];
+#pragma unroll
+      for (int k = 0; k < LITEMS; ++k) {
+        const uint32_t idx = wbase + (uint32_t)k * 64u + lane;
+        key[k] = idx < cs ? ksrc[st + c0 + idx] : (K)0;
+        val[k] = idx < cs ? vsrc[st + c0 + idx] : 0u;
+      }
+      for (int i = tid; i < WAVES * 256; i += BLOCK) (&s_wc[0][0])[i] = 0;
+      __syncthreads();
+      rank_rows8<K, LITEMS>(key, wbase, cs, shift, s_wc[w], rank);
+      __syncthreads();
+      digit_bases();
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < LITEMS; ++k) {
+        const uint32_t idx = wbase + (uint32_t)k * 64u + lane;
+        if (idx < cs) {
+          const uint32_t dg = digit_of(key[k], shift, 255u);
+          const uint32_t lp = s_lbase[dg] + s_wc[w][dg] + rank[k];
+          s_keys[lp] = key[k];
+          s_vals[lp] = val[k];
+        }
+      }
+      __syncthreads();
+      for (uint32_t i = tid; i < cs; i += BLOCK) {
+        const K kk = s_keys[i];
+        const uint32_t dg = digit_of(kk, shift, 255u);
+        const uint32_t dst = s_gbase[dg] + (i - s_lbase[dg]);
+        kdst[dst] = kk;
+        vdst[dst] = s_vals[i];
+      }
+      __syncthreads();
+      s_gbase[tid] += s_tot[tid];
+      __syncthreads();
+    }
+    K* kt = ksrc; ksrc = kdst; kdst = kt;
+    uint32_t* vt = vsrc; vsrc = vdst; vdst = vt;
+  }
+  if (ksrc != ka) {  // result must end in a
+    for (uint32_t i = tid; i < S; i += BLOCK) {
+      ka[st + i] = ksrc[st + i];
+      va[st + i] = vsrc[st + i];
+    }
+  }
+}
+
 bool sort_config_ok(uint32_t digit_bits, uint32_t items) {
   return (digit_bits == 8 || digit_bits == 11) && (items == 8 || items == 16);
 }
@@ -241,12 +429,24 @@ SortPlan sort_plan(uint32_t n, uint32_t key_bits, uint32_t digit_bits, uint32_t 
   return p;
 }
 
+SortPlan sort_plan_hybrid(uint32_t n, uint32_t key_bits, uint32_t items) {
+  SortPlan p = sort_plan(n, key_bits, 11, items);
+  p.hybrid = true;
+  p.passes = 1;
+  p.hist_shift = key_bits > 11 ? key_bits - 11 : 0;
+  p.local_passes = key_bits > 11 ? (key_bits - 11 + 7) / 8 : 0;
+  p.hist_words = p.digits;
+  p.status_words = (size_t)p.num_blocks * p.digits;
+  return p;
+}
+
 size_t sort_status_words_max(uint32_t n, uint32_t key_bits) {
   size_t m = 0;
   for (uint32_t db : {8u, 11u})
     for (uint32_t it : {8u, 16u}) m = std::max(m, sort_plan(n, key_bits, db, it).status_words);
   return m;
 }
+
 
 int launch_sort_reset(hipStream_t s, const SortPlan& p, uint32_t* hist, uint64_t* status,
                       uint32_t* counters, uint32_t* epoch, bool zero_hist) {
@@ -273,17 +473,17 @@ int launch_sort_histogram(hipStream_t s, const SortPlan& p, const void* keys, ui
   if (p.digit_bits == 8) {
     if (p.wide)
       hipLaunchKernelGGL((k_sort_histogram<uint64_t, 8>), dim3(blocks), dim3(256), 0, s, (const uint64_t*)keys,
-                         p.n, p.passes, hist);
+                         p.n, p.passes, p.hist_shift, hist);
     else
       hipLaunchKernelGGL((k_sort_histogram<uint32_t, 8>), dim3(blocks), dim3(256), 0, s, (const uint32_t*)keys,
-                         p.n, p.passes, hist);
+                         p.n, p.passes, p.hist_shift, hist);
   } else {
     if (p.wide)
       hipLaunchKernelGGL((k_sort_histogram<uint64_t, 11>), dim3(blocks), dim3(256), 0, s, (const uint64_t*)keys,
-                         p.n, p.passes, hist);
+                         p.n, p.passes, p.hist_shift, hist);
     else
       hipLaunchKernelGGL((k_sort_histogram<uint32_t, 11>), dim3(blocks), dim3(256), 0, s, (const uint32_t*)keys,
-                         p.n, p.passes, hist);
+                         p.n, p.passes, p.hist_shift, hist);
   }
   YSIM_CHECK_LAUNCH();
   return YSIM_OK;
@@ -293,9 +493,9 @@ namespace {
 template <typename K, int DB, int ITEMS>
 void launch_pass(hipStream_t s, const SortPlan& p, const void* ki, const uint32_t* vi, void* ko, uint32_t* vo,
                  uint32_t shift, const uint32_t* hist, uint64_t* st, uint32_t* counter, const uint32_t* epoch,
-                 uint32_t* err) {
+                 uint32_t* err, const uint32_t* run_flag = nullptr) {
   hipLaunchKernelGGL((k_onesweep<K, DB, ITEMS>), dim3(p.num_blocks), dim3(256), 0, s, (const K*)ki, vi, (K*)ko,
-                     vo, p.n, shift, hist, st, counter, epoch, err);
+                     vo, p.n, shift, hist, st, counter, epoch, err, run_flag);
 }
 }  // namespace
 
@@ -325,6 +525,90 @@ int launch_sort_passes(hipStream_t s, const SortPlan& p, void* keys_a, uint32_t*
 #undef YSIM_SORT_PASS
     YSIM_CHECK_LAUNCH();
   }
+  return YSIM_OK;
+}
+
+int launch_sort_hybrid(hipStream_t s, const SortPlan& p, void* keys_a, uint32_t* vals_a, void* keys_b,
+                       uint32_t* vals_b, const uint32_t* hist, uint64_t* status, uint32_t* counters,
+                       const uint32_t* epoch, uint32_t* err, const uint32_t* run_flag) {
+  if (!p.n) return YSIM_OK;
+  if (!p.hybrid || !sort_config_ok(11, p.items)) return YSIM_ERR_INVALID;
+  const uint32_t sh = p.hist_shift;
+  if (p.wide) {
+    if (p.items == 8)
+      launch_pass<uint64_t, 11, 8>(s, p, keys_a, vals_a, keys_b, vals_b, sh, hist, status, counters, epoch, err, run_flag);
+    else
+      launch_pass<uint64_t, 11, 16>(s, p, keys_a, vals_a, keys_b, vals_b, sh, hist, status, counters, epoch, err, run_flag);
+  } else {
+    if (p.items == 8)
+      launch_pass<uint32_t, 11, 8>(s, p, keys_a, vals_a, keys_b, vals_b, sh, hist, status, counters, epoch, err, run_flag);
+    else
+      launch_pass<uint32_t, 11, 16>(s, p, keys_a, vals_a, keys_b, vals_b, sh, hist, status, counters, epoch, err, run_flag);
+  }
+  YSIM_CHECK_LAUNCH();
+  if (p.wide)
+    hipLaunchKernelGGL((k_bucket_sort<uint64_t, 16>), dim3(2048), dim3(256), 0, s, (uint64_t*)keys_a, vals_a,
+                       (uint64_t*)keys_b, vals_b, hist, p.local_passes, run_flag);
+  else
+    hipLaunchKernelGGL((k_bucket_sort<uint32_t, 16>), dim3(2048), dim3(256), 0, s, (uint32_t*)keys_a, vals_a,
+                       (uint32_t*)keys_b, vals_b, hist, p.local_passes, run_flag);
+  YSIM_CHECK_LAUNCH();
+  return YSIM_OK;
+}
+
+
+// ------------------------------------------------------------------ incremental
+// Second window pass of the incremental sort: windows straddling the aligned
+// window boundaries (block b covers [b*W - W/2, b*W + W/2)), sorted in place.
+template <typename K>
+__global__ __launch_bounds__(256) void k_window_sort_shifted(K* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                             uint32_t n) {
+  __shared__ K sk[INC_WINDOW];
+  __shared__ uint32_t sv[INC_WINDOW];
+  const int64_t s0 = (int64_t)blockIdx.x * INC_WINDOW - INC_WINDOW / 2;
+  const uint32_t lo = (uint32_t)(s0 < 0 ? 0 : s0);
+  const uint32_t hi = (uint32_t)min<int64_t>(s0 + INC_WINDOW, (int64_t)n);
+  if (lo >= hi) return;
+  const uint32_t cnt = hi - lo;
+  for (uint32_t i = threadIdx.x; i < INC_WINDOW; i += 256) {
+    sk[i] = i < cnt ? keys[lo + i] : (K)~(K)0;
+    sv[i] = i < cnt ? vals[lo + i] : ~0u;
+  }
+  __syncthreads();
+  if (!window_sort<K, INC_WINDOW>(sk, sv, cnt)) return;
+  for (uint32_t i = threadIdx.x; i < cnt; i += 256) {
+    keys[lo + i] = sk[i];
+    vals[lo + i] = sv[i];
+  }
+}
+
+// Every adjacent pair strictly increasing by (key, facet)?  Otherwise request
+// the full sort.
+template <typename K>
+__global__ __launch_bounds__(256) void k_sorted_check(const K* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                                      uint32_t n, uint32_t* __restrict__ need_full) {
+  int bad = 0;
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i + 1 < n; i += gridDim.x * 256)
+    if (!kv_greater(keys[i + 1], vals[i + 1], keys[i], vals[i])) bad = 1;
+  if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(need_full, 1u);
+}
+
+int launch_incremental_fix(hipStream_t s, bool wide, void* keys, uint32_t* vals, uint32_t n, uint32_t* need_full) {
+  if (!n) return YSIM_OK;
+  const uint32_t wblocks = ceil_div(n, INC_WINDOW) + 1;
+  const uint32_t cblocks = std::min<uint32_t>(1024, ceil_div(n, 256 * 4));
+  if (wide) {
+    hipLaunchKernelGGL(k_window_sort_shifted<uint64_t>, dim3(wblocks), dim3(256), 0, s, (uint64_t*)keys, vals, n);
+    YSIM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_sorted_check<uint64_t>, dim3(cblocks), dim3(256), 0, s, (const uint64_t*)keys, vals, n,
+                       need_full);
+  } else {
+    hipLaunchKernelGGL(k_window_sort_shifted<uint32_t>, dim3(wblocks), dim3(256), 0, s, (uint32_t*)keys, vals, n);
+    YSIM_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_sorted_check<uint32_t>, dim3(cblocks), dim3(256), 0, s, (const uint32_t*)keys, vals, n,
+                       need_full);
+  }
+  YSIM_CHECK_LAUNCH();
   return YSIM_OK;
 }
 

@@ -16,10 +16,25 @@ struct SortPlan {
   uint32_t digit_bits = 8, digits = 256, items = 8;
   bool wide = false;  // uint64 keys
   size_t hist_words = 0, status_words = 0, counter_words = 0;
+  // hybrid: one onesweep pass on the 11-bit digit at hist_shift (the top key
+  // bits), then per-bucket LSD passes (8-bit digits) in one workgroup each
+  bool hybrid = false;
+  uint32_t hist_shift = 0;     // shift of the first histogrammed digit
+  uint32_t local_passes = 0;   // hybrid: 8-bit passes per bucket
 };
 
 bool sort_config_ok(uint32_t digit_bits, uint32_t items);
 SortPlan sort_plan(uint32_t n, uint32_t key_bits, uint32_t digit_bits = 8, uint32_t items = 8);
+SortPlan sort_plan_hybrid(uint32_t n, uint32_t key_bits, uint32_t items = 8);
+// Runs the hybrid sort; the result is in a.
+// run_flag (optional): every kernel does nothing while *run_flag == 0.
+int launch_sort_hybrid(hipStream_t s, const SortPlan& p, void* keys_a, uint32_t* vals_a, void* keys_b,
+                       uint32_t* vals_b, const uint32_t* hist, uint64_t* status, uint32_t* counters,
+                       const uint32_t* epoch, uint32_t* err, const uint32_t* run_flag = nullptr);
+// Incremental sort, after launch_morton_inc: the half-shifted window pass and
+// the check of every adjacent pair, which sets *need_full if the keys are not
+// yet in (key, value) order.
+int launch_incremental_fix(hipStream_t s, bool wide, void* keys, uint32_t* vals, uint32_t n, uint32_t* need_full);
 // Largest status footprint (64-bit words) over all configurations for n keys.
 size_t sort_status_words_max(uint32_t n, uint32_t key_bits);
 // Look-back status words are 64-bit: (epoch << 32) | flag << 30 | count.  A word

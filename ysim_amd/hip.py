@@ -125,11 +125,17 @@ class RadixSorter:
     def __init__(self, ctx: HipContext):
         self.ctx = ctx
 
-    def sort(self, keys: np.ndarray, vals: np.ndarray, key_bits: int):
+    def sort(self, keys: np.ndarray, vals: np.ndarray, key_bits: int, digit_bits: int = 8, items: int = 8,
+             hybrid: bool = False):
+        """Configuration (digit_bits, items, hybrid) changes speed only, never the result."""
         kd = np.uint64 if key_bits > 32 else np.uint32
         dk = self.ctx.to_device(np.asarray(keys, kd))
         dv = self.ctx.to_device(np.asarray(vals, np.uint32))
-        call("ysim_hip_sort_pairs", self.ctx.h, dk.ptr, dv.ptr, len(keys), key_bits)
+        if (digit_bits, items, hybrid) == (8, 8, False):
+            call("ysim_hip_sort_pairs", self.ctx.h, dk.ptr, dv.ptr, len(keys), key_bits)
+        else:
+            call("ysim_hip_sort_pairs_ex", self.ctx.h, dk.ptr, dv.ptr, len(keys), key_bits, digit_bits, items,
+                 int(hybrid))
         return dk.get().astype(np.uint64), dv.get()
 
 

@@ -230,7 +230,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch tuning")
     ap.add_argument("--tuning", default="",
                     help="fixed tuning instead of autotune: sort_digit_bits,sort_items,query_block,cloth_block,"
-                         "refit_block (e.g. 8,16,128,512,1024)")
+                         "refit_block[,sort_incremental,query_waves] (e.g. 8,16,128,512,1024,1,6)")
     ap.add_argument("--no-copy", action="store_true", help="skip the copy-bandwidth ceiling")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the C5 LBVH-vs-hash, adjoint and copy-ceiling measurements")
@@ -269,7 +269,8 @@ def main():
 
     tuning = hs.tuning()
     if args.tuning:
-        names = ("sort_digit_bits", "sort_items", "query_block", "cloth_block", "refit_block")
+        names = ("sort_digit_bits", "sort_items", "query_block", "cloth_block", "refit_block", "sort_incremental",
+                 "query_waves")
         hs.set_tuning(**dict(zip(names, (int(t) for t in args.tuning.split(",")))))
         tuning = hs.tuning()
         log(f"fixed tuning: {tuning}")
@@ -310,7 +311,7 @@ def main():
     kernels = {"box": "k_box", "morton": f"k_morton<uint64_t, {digit_bits}>",
                "sort": f"k_onesweep<uint64_t, {digit_bits}, {int(tuning['sort_items'])}>",
                "hierarchy": "k_hierarchy<uint64_t>", "refit": f"k_refit_lds<false, {int(tuning['refit_block'])}>",
-               "query": f"k_query<false, {int(tuning['query_block'])}>",
+               "query": f"k_query<false, {int(tuning['query_block'])}, {int(tuning['query_waves'])}>",
                "integrate": f"k_cloth<{int(tuning['cloth_block'])}>"}
     stages = {}
     for s, ms in stages_ms.items():

@@ -308,6 +308,7 @@ typedef struct ysim_tuning {
   uint32_t sort_hybrid;      /* 1: one 11-bit MSD pass + per-bucket LSD in LDS (sort_digit_bits unused) */
   uint32_t sort_incremental; /* 1: keys in the last build's sorted order, LDS window sorts and an exact
                               * check; the hybrid sort runs only when the check fails */
+  uint32_t query_waves;      /* traversal occupancy target, waves per SIMD: 5, 6 or 8 */
 } ysim_tuning;
 int ysim_hip_scene_get_tuning(ysim_scene* scene, ysim_tuning* h_out);
 int ysim_hip_scene_set_tuning(ysim_scene* scene, const ysim_tuning* h_in);

@@ -32,8 +32,10 @@ extern "C" int yt_query(const float* x, uint32_t nq, uint32_t qobj, const ysim_b
       }
       ++count;
     };
-    int it = sdepth_small ? traverse<2>(tree, parent, nT, qmin, qmax, stack.data(), 1, leaf)
-                          : traverse<16>(tree, parent, nT, qmin, qmax, stack.data(), 1, leaf);
+    int it = sdepth_small == 1 ? traverse<2>(tree, parent, nT, qmin, qmax, stack.data(), 1, leaf)
+             : sdepth_small == 8 ? traverse<8>(tree, parent, nT, qmin, qmax, stack.data(), 1, leaf)
+             : sdepth_small == 12 ? traverse<12>(tree, parent, nT, qmin, qmax, stack.data(), 1, leaf)
+                                 : traverse<16>(tree, parent, nT, qmin, qmax, stack.data(), 1, leaf);
     if (it < 0) return -1;
     iters += (uint64_t)it;
     if ((uint32_t)it > maxit) maxit = (uint32_t)it;

@@ -119,12 +119,12 @@ def test_sort_configs_bit_exact_30bit_rebuild(ctx, oracle_threads, db, items, hy
     assert np.array_equal(kg, ko) and np.array_equal(ig, io)
 
 
-@pytest.mark.parametrize("qb,rb", [(64, 256), (128, 512), (256, 1024)])
+@pytest.mark.parametrize("qb,rb,qw", [(64, 256, 8), (128, 512, 5), (256, 1024, 6)])
 @pytest.mark.parametrize("cb", [128, 256, 512])
-def test_block_sizes_bit_exact(ctx, oracle_threads, qb, rb, cb):
+def test_block_sizes_bit_exact(ctx, oracle_threads, qb, rb, qw, cb):
     sc = scenes.config_drape(n=32, height=0.8)
     hs, os_ = HipScene(ctx, sc), OracleScene(sc)
-    hs.set_tuning(query_block=qb, cloth_block=cb, refit_block=rb)
+    hs.set_tuning(query_block=qb, cloth_block=cb, refit_block=rb, query_waves=qw)
     hs.step(20, graph=True)
     os_.step(20)
     _same_state(hs.state(), os_.state())
@@ -168,7 +168,7 @@ def test_autotune_batched_keeps_results(ctx):
 def test_invalid_tuning_rejected(ctx):
     hs = HipScene(ctx, scenes.config_drape(n=16))
     for kw in ({"sort_digit_bits": 9}, {"sort_items": 4}, {"query_block": 96}, {"cloth_block": 1024},
-               {"refit_block": 128}, {"sort_hybrid": 2}, {"sort_incremental": 3}):
+               {"refit_block": 128}, {"sort_hybrid": 2}, {"sort_incremental": 3}, {"query_waves": 7}):
         with pytest.raises(YsimError):
             hs.set_tuning(**kw)
     hs.step(2)

@@ -150,7 +150,7 @@ struct ysim_scene {
   } adj;
   float graph_kd = 0.f;
   const float* graph_ks = nullptr;
-  ysim_tuning tune{8, 8, 128, 256, 0.f, 0.f, 256, 0, 0};
+  ysim_tuning tune{8, 8, 128, 256, 0.f, 0.f, 256, 0, 0, 5};
   float* tune_buf = nullptr;   // autotune state save: x, v, xPrev of the whole scene
 
   int grow(float** p, size_t& have, size_t need) {
@@ -194,6 +194,7 @@ struct ysim_scene {
     if (!refit_block_ok(t.refit_block)) return fail(YSIM_ERR_INVALID, "tuning: refit_block must be 256, 512 or 1024");
     if (t.sort_hybrid > 1) return fail(YSIM_ERR_INVALID, "tuning: sort_hybrid must be 0 or 1");
     if (t.sort_incremental > 1) return fail(YSIM_ERR_INVALID, "tuning: sort_incremental must be 0 or 1");
+    if (!query_waves_ok(t.query_waves)) return fail(YSIM_ERR_INVALID, "tuning: query_waves must be 5, 6 or 8");
     const bool replan = t.sort_digit_bits != tune.sort_digit_bits || t.sort_items != tune.sort_items ||
                         t.sort_hybrid != tune.sort_hybrid || t.sort_incremental != tune.sort_incremental;
     tune.sort_digit_bits = t.sort_digit_bits;
@@ -203,6 +204,7 @@ struct ysim_scene {
     tune.refit_block = t.refit_block;
     tune.sort_hybrid = t.sort_hybrid;
     tune.sort_incremental = t.sort_incremental;
+    tune.query_waves = t.query_waves;
     if (replan) {
       for (auto& g : dyn_groups)
         g.plan = (t.sort_hybrid || t.sort_incremental)
@@ -293,6 +295,15 @@ struct ysim_scene {
         if (qb == best.query_block) continue;
         ysim_tuning t = best;
         t.query_block = qb;
+        float ms = 0.f;
+        if ((rc = measure(t, ms)) != YSIM_OK) break;
+        if (ms < best_ms) { best_ms = ms; best = t; }
+      }
+    if (rc == YSIM_OK && !hash_mode)
+      for (uint32_t qw : {5u, 6u, 8u}) {
+        if (qw == best.query_waves) continue;
+        ysim_tuning t = best;
+        t.query_waves = qw;
         float ms = 0.f;
         if ((rc = measure(t, ms)) != YSIM_OK) break;
         if (ms < best_ms) { best_ms = ms; best = t; }
@@ -462,6 +473,7 @@ struct ysim_scene {
     a.npairs = npairs;
     a.flags = flags;
     a.block = tune.query_block;
+    a.waves = tune.query_waves;
     return a;
   }
 

@@ -95,8 +95,10 @@ struct QueryArgs {
   uint32_t* pairs = nullptr;          // 4 words per pair (qobj, point, tobj, tri)
   uint32_t pair_cap = 0;
   uint32_t block = 128;               // threads per block: 64, 128 or 256
+  uint32_t waves = 5;                 // occupancy target (waves per SIMD): 5, 6 or 8
 };
 bool query_block_ok(uint32_t b);
+bool query_waves_ok(uint32_t w);
 int launch_query(hipStream_t s, const QueryArgs& a, bool emit_pairs);
 // Drop-in broad query of one (query mesh, target tree): BroadCollision records.
 int launch_broad_pairs(hipStream_t s, const float* xq, uint32_t nq, uint32_t qobj, uint32_t qbeh,

@@ -17,12 +17,15 @@
 namespace ysim {
 
 namespace {
-constexpr int SDEPTH = 16;   // LDS stack entries per thread
+constexpr int SDEPTH = 8;    // LDS stack entries per thread (deeper stacks overflow no less on cloth)
 }  // namespace
 
-// QB threads per block (64, 128 or 256; chosen by the autotuner).
-template <bool EMIT_PAIRS, int QB>
-__global__ __launch_bounds__(QB) void k_query(QueryArgs a) {
+// QB threads per block (64, 128 or 256) and an occupancy target of WPE waves
+// per SIMD (5: registers as needed; 6-8: fewer registers, some spilled), both
+// chosen by the autotuner: the traversal is bound by dependent node loads, so
+// more waves in flight can pay for a few spills.
+template <bool EMIT_PAIRS, int QB, int WPE>
+__attribute__((amdgpu_waves_per_eu(WPE, WPE))) __global__ __launch_bounds__(QB) void k_query(QueryArgs a) {
   __shared__ int2 s_stack[SDEPTH * QB];
   const uint32_t tq = xcd_block(blockIdx.x, gridDim.x) * QB + threadIdx.x;
   uint32_t npair = 0;
@@ -94,17 +97,25 @@ __global__ __launch_bounds__(QB) void k_query(QueryArgs a) {
 }
 
 bool query_block_ok(uint32_t b) { return b == 64 || b == 128 || b == 256; }
+bool query_waves_ok(uint32_t w) { return w == 5 || w == 6 || w == 8; }
 
 int launch_query(hipStream_t s, const QueryArgs& a, bool emit_pairs) {
   if (!a.nitems) return YSIM_OK;
   const uint32_t qb = a.block;
-  if (!query_block_ok(qb)) return YSIM_ERR_INVALID;
-#define YSIM_Q(E, B) hipLaunchKernelGGL((k_query<E, B>), dim3(ceil_div(a.nitems, B)), dim3(B), 0, s, a)
+  if (!query_block_ok(qb) || !query_waves_ok(a.waves)) return YSIM_ERR_INVALID;
+#define YSIM_Q(E, B, W) hipLaunchKernelGGL((k_query<E, B, W>), dim3(ceil_div(a.nitems, B)), dim3(B), 0, s, a)
+#define YSIM_QB(E, W) \
+  if (qb == 64) YSIM_Q(E, 64, W); else if (qb == 128) YSIM_Q(E, 128, W); else YSIM_Q(E, 256, W)
   if (emit_pairs) {
-    if (qb == 64) YSIM_Q(true, 64); else if (qb == 128) YSIM_Q(true, 128); else YSIM_Q(true, 256);
+    YSIM_QB(true, 5);
+  } else if (a.waves == 5) {
+    YSIM_QB(false, 5);
+  } else if (a.waves == 6) {
+    YSIM_QB(false, 6);
   } else {
-    if (qb == 64) YSIM_Q(false, 64); else if (qb == 128) YSIM_Q(false, 128); else YSIM_Q(false, 256);
+    YSIM_QB(false, 8);
   }
+#undef YSIM_QB
 #undef YSIM_Q
   YSIM_CHECK_LAUNCH();
   return YSIM_OK;

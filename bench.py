@@ -39,9 +39,17 @@ PMC_TRAFFIC = os.path.join(ROOT, "profiles", "c2_512_fold_pmc_traffic.json")
 METRIC = "cloth substeps/s (LBVH+narrow+springs) + HBM GB/s; plain vs batched, 1 GPU"
 
 
-def stage_bytes(V: int, T: int, k: int, passes: int, C: int) -> dict:
+def stage_bytes(V: int, T: int, k: int, passes: int, C: int, incremental: bool = False) -> dict:
     """Compulsory (algorithmic) bytes per substep of each stage: every input read
-    once and every output written once.  k = key bytes, C = contacts."""
+    once and every output written once.  k = key bytes, C = contacts.  The
+    incremental sort reads the previous order in Morton and only checks (and
+    rarely reorders) the keys in the sort stage."""
+    if incremental:
+        return {**stage_bytes(V, T, k, passes, C),
+                # previous order, facets, positions read; keys + values written
+                "morton": 4 * T + 12 * T + 12 * V + (k + 4) * T,
+                # shifted windows and the adjacency check each read keys + values
+                "sort": 2 * (k + 4) * T}
     return {
         # positions read
         "box": 12 * V,
@@ -306,20 +314,31 @@ def main():
     pairs_per_step = cnt["pairs"] / max(1, cnt["steps"])
     digit_bits = int(tuning["sort_digit_bits"])
     passes = -(-63 // digit_bits)  # 63-bit keys
-    sb = stage_bytes(V, T, 8, passes, contacts)
-    launches = {"box": 1, "morton": 1, "sort": passes, "hierarchy": 1, "refit": 1, "query": 1, "integrate": 1}
-    kernels = {"box": "k_box", "morton": f"k_morton<uint64_t, {digit_bits}>",
-               "sort": f"k_onesweep<uint64_t, {digit_bits}, {int(tuning['sort_items'])}>",
+    incremental = bool(tuning.get("sort_incremental", 0))
+    sb = stage_bytes(V, T, 8, passes, contacts, incremental)
+    # incremental: shifted windows, check, then the fallback (Morton, MSD pass,
+    # buckets) whose kernels return at once unless the check failed
+    launches = {"box": 1, "morton": 1, "sort": 5 if incremental else passes, "hierarchy": 1, "refit": 1,
+                "query": 1, "integrate": 1}
+    kernels = {"box": "k_box",
+               "morton": "k_morton_inc<uint64_t>" if incremental else f"k_morton<uint64_t, {digit_bits}, 8>",
+               "sort": "k_window_sort_shifted<uint64_t>" if incremental
+               else f"k_onesweep<uint64_t, {digit_bits}, {int(tuning['sort_items'])}>",
                "hierarchy": "k_hierarchy<uint64_t>", "refit": f"k_refit_lds<false, {int(tuning['refit_block'])}>",
                "query": f"k_query<false, {int(tuning['query_block'])}, {int(tuning['query_waves'])}>",
                "integrate": f"k_cloth<{int(tuning['cloth_block'])}>"}
     stages = {}
     for s, ms in stages_ms.items():
         gbs = sb[s] / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-        tb, _ = pmc_traffic(kernels[s])
+        if s == "sort" and incremental:
+            ks = ["k_window_sort_shifted<uint64_t>", "k_sorted_check<uint64_t>", "k_morton<uint64_t, 11, 1>",
+                  f"k_onesweep<uint64_t, 11, {int(tuning['sort_items'])}>", "k_bucket_sort<uint64_t, 16>"]
+        else:
+            ks = [kernels[s]] * launches[s]
+        tbs = [pmc_traffic(k)[0] for k in ks]
         stages[s] = {"ms": round(ms, 5), "algorithmic_bytes": int(sb[s]), "GBps": round(gbs, 1),
                      "frac_hbm": round(gbs / HBM_PEAK_GBS, 4), "launches": launches[s], "kernel": kernels[s],
-                     "pmc_hbm_bytes": None if tb is None else tb * launches[s]}
+                     "pmc_hbm_bytes": None if any(t is None for t in tbs) else int(sum(tbs))}
     dom = max(stages_ms, key=lambda s: stages_ms[s])
     per_launch_bytes = sb[dom] / launches[dom]
     per_launch_ms = stages_ms[dom] / launches[dom]
@@ -346,7 +365,11 @@ def main():
             "workload": f"{scene.name}: {args.n}x{args.n} cloth folded 0.02 apart, self-collision, "
                         "63-bit Morton LBVH rebuilt every substep, dt=1/1200",
             "vertices": V, "triangles": T, "morton_bits": 63, "rebuild_every": 1,
-            "sort": {"digit_bits": digit_bits, "passes": passes, "items_per_thread": int(tuning["sort_items"])},
+            "sort": ({"mode": "incremental", "window_keys": 2048,
+                      "fallback": "11-bit MSD onesweep + per-bucket LDS LSD, run only when the adjacency check fails"}
+                     if incremental else
+                     {"mode": "lsd", "digit_bits": digit_bits, "passes": passes,
+                      "items_per_thread": int(tuning["sort_items"])}),
             "tuning": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in tuning.items()},
             "hipgraph": use_graph, "vertex_mass": 1.0, "episode_substeps": args.episode,
             "candidate_pairs_per_substep": round(pairs_per_step, 1), "contacts_last_substep": contacts,

@@ -238,7 +238,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch tuning")
     ap.add_argument("--tuning", default="",
                     help="fixed tuning instead of autotune: sort_digit_bits,sort_items,query_block,cloth_block,"
-                         "refit_block[,sort_incremental,query_waves] (e.g. 8,16,128,512,1024,1,6)")
+                         "refit_block[,sort_incremental,query_waves,query_tile] (e.g. 8,16,128,256,1024,1,5,1)")
     ap.add_argument("--no-copy", action="store_true", help="skip the copy-bandwidth ceiling")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the C5 LBVH-vs-hash, adjoint and copy-ceiling measurements")
@@ -278,7 +278,7 @@ def main():
     tuning = hs.tuning()
     if args.tuning:
         names = ("sort_digit_bits", "sort_items", "query_block", "cloth_block", "refit_block", "sort_incremental",
-                 "query_waves")
+                 "query_waves", "query_tile")
         hs.set_tuning(**dict(zip(names, (int(t) for t in args.tuning.split(",")))))
         tuning = hs.tuning()
         log(f"fixed tuning: {tuning}")
@@ -325,7 +325,8 @@ def main():
                "sort": "k_window_sort_shifted<uint64_t>" if incremental
                else f"k_onesweep<uint64_t, {digit_bits}, {int(tuning['sort_items'])}>",
                "hierarchy": "k_hierarchy<uint64_t>", "refit": f"k_refit_lds<false, {int(tuning['refit_block'])}>",
-               "query": f"k_query<false, {int(tuning['query_block'])}, {int(tuning['query_waves'])}>",
+               "query": "k_query_tile" if int(tuning.get("query_tile", 0))
+               else f"k_query<false, {int(tuning['query_block'])}, {int(tuning['query_waves'])}>",
                "integrate": f"k_cloth<{int(tuning['cloth_block'])}>"}
     stages = {}
     for s, ms in stages_ms.items():

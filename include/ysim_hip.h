@@ -309,6 +309,10 @@ typedef struct ysim_tuning {
   uint32_t sort_incremental; /* 1: keys in the last build's sorted order, LDS window sorts and an exact
                               * check; the hybrid sort runs only when the check fails */
   uint32_t query_waves;      /* traversal occupancy target, waves per SIMD: 5, 6 or 8 */
+  uint32_t query_tile;       /* 1: one wave per 64-vertex tile walks the tree breadth-first with the
+                              * tile's union box (default); 0: one thread per vertex (query_block and
+                              * query_waves apply); 2: as 1 with 128-leaf and 32-node tile buffers, so
+                              * that the tiles' buffer flushes and per-vertex redos run (tests those) */
 } ysim_tuning;
 int ysim_hip_scene_get_tuning(ysim_scene* scene, ysim_tuning* h_out);
 int ysim_hip_scene_set_tuning(ysim_scene* scene, const ysim_tuning* h_in);

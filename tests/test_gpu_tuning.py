@@ -122,15 +122,65 @@ def test_sort_configs_bit_exact_30bit_rebuild(ctx, oracle_threads, db, items, hy
 @pytest.mark.parametrize("qb,rb,qw", [(64, 256, 8), (128, 512, 5), (256, 1024, 6)])
 @pytest.mark.parametrize("cb", [128, 256, 512])
 def test_block_sizes_bit_exact(ctx, oracle_threads, qb, rb, qw, cb):
+    """The per-vertex query kernel at every block size and occupancy target."""
     sc = scenes.config_drape(n=32, height=0.8)
     hs, os_ = HipScene(ctx, sc), OracleScene(sc)
-    hs.set_tuning(query_block=qb, cloth_block=cb, refit_block=rb, query_waves=qw)
+    hs.set_tuning(query_block=qb, cloth_block=cb, refit_block=rb, query_waves=qw, query_tile=0)
     hs.step(20, graph=True)
     os_.step(20)
     _same_state(hs.state(), os_.state())
     og, cg = hs.contacts()
     oo, co = os_.contacts()
     assert np.array_equal(og, oo) and len(co) > 0
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_query_modes_drape_bit_exact(ctx, oracle_threads, mode):
+    """Per-vertex walk (0), tile walk (1) and the tile walk pushed onto its
+    per-vertex fallback (2) on a 30x30 cloth (its edge rows and columns are not
+    whole 8x8 tiles) over a UV sphere: two targets per query vertex."""
+    print(f"query_tile={mode} on the 30x30 drape", flush=True)
+    sc = scenes.config_drape(n=30, height=0.8)
+    hs, os_ = HipScene(ctx, sc), OracleScene(sc)
+    hs.set_tuning(query_tile=mode)
+    for graph in (False, True):
+        hs.step(12, graph=graph)
+        os_.step(12)
+        _same_state(hs.state(), os_.state())
+        og, cg = hs.contacts()
+        oo, co = os_.contacts()
+        assert np.array_equal(og, oo) and np.array_equal(cg, co)
+    assert len(co) > 0 and hs.flags() == 0
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_query_modes_fold_bit_exact(ctx, oracle_threads, mode):
+    """The three query modes on the 96x96 fold (self queries, 63-bit tree)."""
+    print(f"query_tile={mode} on the 96x96 fold", flush=True)
+    sc = scenes.config_c2(n=96)
+    hs, os_ = HipScene(ctx, sc), OracleScene(sc)
+    hs.set_tuning(query_tile=mode)
+    hs.step(30, graph=True)
+    os_.step(30)
+    _same_state(hs.state(), os_.state())
+    og, cg = hs.contacts()
+    oo, co = os_.contacts()
+    assert np.array_equal(og, oo) and np.array_equal(cg, co)
+    assert hs.flags() == 0
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_query_modes_batched_identical(ctx, mode):
+    """Ragged batch (tiles straddle instance boundaries): every query mode gives
+    the same states."""
+    sc = scenes.config_c3(count=12, hi=48)
+    ref, hs = HipScene(ctx, sc), HipScene(ctx, sc)
+    ref.set_tuning(query_tile=0)
+    hs.set_tuning(query_tile=mode)
+    ref.step(15, graph=True)
+    hs.step(15, graph=True)
+    _same_state(hs.state(), ref.state())
+    assert hs.counters()["pairs"] == ref.counters()["pairs"]
 
 
 def test_autotune_keeps_results(ctx, oracle_threads):
@@ -168,7 +218,8 @@ def test_autotune_batched_keeps_results(ctx):
 def test_invalid_tuning_rejected(ctx):
     hs = HipScene(ctx, scenes.config_drape(n=16))
     for kw in ({"sort_digit_bits": 9}, {"sort_items": 4}, {"query_block": 96}, {"cloth_block": 1024},
-               {"refit_block": 128}, {"sort_hybrid": 2}, {"sort_incremental": 3}, {"query_waves": 7}):
+               {"refit_block": 128}, {"sort_hybrid": 2}, {"sort_incremental": 3}, {"query_waves": 7},
+               {"query_tile": 3}):
         with pytest.raises(YsimError):
             hs.set_tuning(**kw)
     hs.step(2)

@@ -112,7 +112,10 @@ struct ysim_scene {
   uint32_t* ccount = nullptr;
   float4* cnd = nullptr;
   uint2* cid = nullptr;
-  uint32_t* npairs = nullptr;  // [0] candidate statistics, [1] pair-emission counter
+  uint32_t* npairs = nullptr;  // [1] pair-emission counter
+  uint32_t* wstats = nullptr;  // per query wave: running candidate-pair count
+  uint32_t wstats_words = 0;
+  std::vector<uint32_t> tune_wstats;
   uint32_t* flags = nullptr;   // [0] sticky flags
   uint32_t* pairs = nullptr;
   uint32_t pair_cap = 0;
@@ -150,7 +153,7 @@ struct ysim_scene {
   } adj;
   float graph_kd = 0.f;
   const float* graph_ks = nullptr;
-  ysim_tuning tune{8, 8, 128, 256, 0.f, 0.f, 256, 0, 0, 5};
+  ysim_tuning tune{8, 8, 128, 256, 0.f, 0.f, 256, 0, 0, 5, 1};
   float* tune_buf = nullptr;   // autotune state save: x, v, xPrev of the whole scene
 
   int grow(float** p, size_t& have, size_t need) {
@@ -195,6 +198,7 @@ struct ysim_scene {
     if (t.sort_hybrid > 1) return fail(YSIM_ERR_INVALID, "tuning: sort_hybrid must be 0 or 1");
     if (t.sort_incremental > 1) return fail(YSIM_ERR_INVALID, "tuning: sort_incremental must be 0 or 1");
     if (!query_waves_ok(t.query_waves)) return fail(YSIM_ERR_INVALID, "tuning: query_waves must be 5, 6 or 8");
+    if (t.query_tile > 2) return fail(YSIM_ERR_INVALID, "tuning: query_tile must be 0, 1 or 2");
     const bool replan = t.sort_digit_bits != tune.sort_digit_bits || t.sort_items != tune.sort_items ||
                         t.sort_hybrid != tune.sort_hybrid || t.sort_incremental != tune.sort_incremental;
     tune.sort_digit_bits = t.sort_digit_bits;
@@ -205,6 +209,7 @@ struct ysim_scene {
     tune.sort_hybrid = t.sort_hybrid;
     tune.sort_incremental = t.sort_incremental;
     tune.query_waves = t.query_waves;
+    tune.query_tile = t.query_tile;
     if (replan) {
       for (auto& g : dyn_groups)
         g.plan = (t.sort_hybrid || t.sort_incremental)
@@ -225,6 +230,9 @@ struct ysim_scene {
     YSIM_HIP_TRY(hipMemcpyAsync(tune_buf + 2 * n, x[parity ^ 1], 4 * n, hipMemcpyDeviceToDevice, st));
     YSIM_HIP_TRY(hipMemcpyAsync(fl_np, flags, 4, hipMemcpyDeviceToHost, st));
     YSIM_HIP_TRY(hipMemcpyAsync(fl_np + 1, npairs, 4, hipMemcpyDeviceToHost, st));
+    tune_wstats.resize(wstats_words);
+    if (wstats_words)
+      YSIM_HIP_TRY(hipMemcpyAsync(tune_wstats.data(), wstats, 4 * (size_t)wstats_words, hipMemcpyDeviceToHost, st));
     YSIM_HIP_TRY(hipStreamSynchronize(st));
     return YSIM_OK;
   }
@@ -235,6 +243,8 @@ struct ysim_scene {
     YSIM_HIP_TRY(hipMemcpyAsync(x[par ^ 1], tune_buf + 2 * n, 4 * n, hipMemcpyDeviceToDevice, st));
     YSIM_HIP_TRY(hipMemcpyAsync(flags, fl_np, 4, hipMemcpyHostToDevice, st));
     YSIM_HIP_TRY(hipMemcpyAsync(npairs, fl_np + 1, 4, hipMemcpyHostToDevice, st));
+    if (wstats_words && tune_wstats.size() == wstats_words)
+      YSIM_HIP_TRY(hipMemcpyAsync(wstats, tune_wstats.data(), 4 * (size_t)wstats_words, hipMemcpyHostToDevice, st));
     YSIM_HIP_TRY(hipStreamSynchronize(st));
     parity = par;
     step = stp;
@@ -293,7 +303,17 @@ struct ysim_scene {
         if (ms < best_ms * kGain) { best_ms = ms; best = t; }
       }
     }
-    if (rc == YSIM_OK)
+    if (rc == YSIM_OK && !hash_mode)
+      for (uint32_t qt : {1u, 0u}) {
+        if (qt == best.query_tile) continue;
+        ysim_tuning t = best;
+        t.query_tile = qt;
+        float ms = 0.f;
+        if ((rc = measure(t, ms)) != YSIM_OK) break;
+        if (ms < best_ms * kGain) { best_ms = ms; best = t; }
+      }
+    // block size and occupancy apply to the per-vertex kernel only
+    if (rc == YSIM_OK && !hash_mode && !best.query_tile)
       for (uint32_t qb : {64u, 128u, 256u}) {
         if (qb == best.query_block) continue;
         ysim_tuning t = best;
@@ -302,7 +322,7 @@ struct ysim_scene {
         if ((rc = measure(t, ms)) != YSIM_OK) break;
         if (ms < best_ms * kGain) { best_ms = ms; best = t; }
       }
-    if (rc == YSIM_OK && !hash_mode)
+    if (rc == YSIM_OK && !hash_mode && !best.query_tile)
       for (uint32_t qw : {5u, 6u, 8u}) {
         if (qw == best.query_waves) continue;
         ysim_tuning t = best;
@@ -474,9 +494,11 @@ struct ysim_scene {
     a.cnd = cnd;
     a.cid = cid;
     a.npairs = npairs;
+    a.wstats = wstats;
     a.flags = flags;
     a.block = tune.query_block;
     a.waves = tune.query_waves;
+    a.tile = tune.query_tile;
     return a;
   }
 
@@ -1143,6 +1165,10 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
   YSIM_TRY(s.alloc(&s.npairs, 4));
   YSIM_TRY(s.alloc(&s.flags, 4));
   YSIM_TRY(s.zero(s.npairs, 4));
+  // one statistics word per wave of any query launch (blocks of up to 256 threads)
+  s.wstats_words = 4 * ceil_div(s.Vdyn, 256) + 4;
+  YSIM_TRY(s.alloc(&s.wstats, s.wstats_words));
+  YSIM_TRY(s.zero(s.wstats, s.wstats_words));
   YSIM_TRY(s.zero(s.flags, 4));
   for (auto& e : s.ev) YSIM_HIP_TRY(hipEventCreate(&e));
   if (P.broad_mode == 1) {
@@ -1471,19 +1497,21 @@ int ysim_hip_scene_restore(ysim_scene* s) {
 
 int ysim_hip_scene_counters(ysim_scene* s, uint64_t* h_out, uint32_t reset) {
   YSIM_ARG(s && h_out, "scene_counters: null argument");
-  uint32_t np = 0;
-  YSIM_HIP_TRY(hipMemcpyAsync(&np, s->npairs, 4, hipMemcpyDeviceToHost, s->st));
+  std::vector<uint32_t> ws(s->wstats_words);
+  if (s->wstats_words)
+    YSIM_HIP_TRY(hipMemcpyAsync(ws.data(), s->wstats, 4 * (size_t)s->wstats_words, hipMemcpyDeviceToHost, s->st));
   std::vector<uint32_t> cnt(s->Vdyn);
   if (s->Vdyn) YSIM_HIP_TRY(hipMemcpyAsync(cnt.data(), s->ccount, 4 * (size_t)s->Vdyn, hipMemcpyDeviceToHost, s->st));
   YSIM_HIP_TRY(hipStreamSynchronize(s->st));
   uint64_t c = 0;
   for (uint32_t k : cnt) c += std::min(k, s->cap);
+  uint64_t np = 0;
+  for (uint32_t k : ws) np += k;
   h_out[0] = s->pairs_hi + np;
   h_out[1] = s->steps_run - s->counter_step0;
   h_out[2] = c;
   if (reset) {
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s->st, s->npairs, 1u);
-    YSIM_CHECK_LAUNCH();
+    YSIM_TRY(s->zero(s->wstats, s->wstats_words));
     s->pairs_hi = 0;
     s->counter_step0 = s->steps_run;
   }

@@ -89,13 +89,15 @@ struct QueryArgs {
   uint32_t* ccount = nullptr;         // per query vertex contact count
   float4* cnd = nullptr;              // [nitems * cap] normal + distance
   uint2* cid = nullptr;               // [nitems * cap] (target mesh, facet)
-  uint32_t* npairs = nullptr;         // running candidate-pair counter (stats)
+  uint32_t* npairs = nullptr;         // pair-emission counter
+  uint32_t* wstats = nullptr;         // per-wave running candidate-pair counts (one word per wave)
   uint32_t* flags = nullptr;          // bit1 contact overflow, bit2 traversal guard
   // pair-emission mode (parity / drop-in broad phase)
   uint32_t* pairs = nullptr;          // 4 words per pair (qobj, point, tobj, tri)
   uint32_t pair_cap = 0;
   uint32_t block = 128;               // threads per block: 64, 128 or 256
   uint32_t waves = 5;                 // occupancy target (waves per SIMD): 5, 6 or 8
+  uint32_t tile = 1;                  // 1: one wave per 64-vertex tile (k_query_tile); 0: one thread per vertex
 };
 bool query_block_ok(uint32_t b);
 bool query_waves_ok(uint32_t w);

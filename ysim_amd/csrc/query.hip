@@ -1,29 +1,104 @@
 // query.hip — BVH traversal fused with the point-triangle narrow phase.
 //
-// One thread per query vertex (queryPoints, src/metal/bvh.metal:1393-1412).  For
-// every target tree of its collision group the thread runs a short-stack DFS
-// that tests both children at once and keeps only "both children hit" siblings
-// on a per-thread LDS stack; if that stack fills, the walk continues
-// stacklessly through parent links (exactly the same visit set).  Every
-// candidate (leaf box overlapping [x - m, x + m], closed; facets containing the
-// vertex skipped on self queries, bvh.metal:1366-1371) is either emitted as a
-// pair record (broad-phase mode) or immediately tested with the swept
-// point-triangle narrow phase (narrow_pt_tri, src/metal/bruteforce.metal:41-178),
+// queryPoints (src/metal/bvh.metal:1393-1412) reports, for every query vertex
+// and every target tree of its collision group, each leaf whose box overlaps
+// [x - m, x + m] (closed; facets containing the vertex skipped on self queries,
+// bvh.metal:1366-1371); each candidate is then tested with the swept
+// point-triangle narrow phase (narrow_pt_tri, src/metal/bruteforce.metal:41-178)
 // and contacts are kept in per-vertex slots.  Contacts are applied later in
-// (target mesh, facet) order, so the result does not depend on traversal order.
+// (target mesh, facet) order, so the result does not depend on the order in
+// which candidates are found.  Two kernels compute the same slots:
+//
+//   k_query_tile  one wave per 64 query vertices (an 8 x 8 vertex tile of a
+//                 grid cloth).  The wave walks the target tree breadth-first
+//                 with the union of its 64 boxes, one tree level per round
+//                 with a whole wave of lanes, collects the overlapping leaves,
+//                 distributes them to the vertices through 16 sub-boxes of
+//                 2 x 2 vertices, and runs the narrow phase on the resulting
+//                 (vertex, facet) pairs with every lane busy.  A tree node is
+//                 loaded once per tile instead of once per vertex.
+//   k_query       one thread per query vertex: a short-stack walk with the
+//                 narrow phase inline (also the pair-emission broad phase).
+//
+// Candidate-pair statistics are per-wave running counts (each wave owns one
+// word; no atomics): same-address atomics from thousands of waves serialize
+// and cost tens of microseconds on this GPU.
 #include "kernels.hpp"
 #include "traverse.hpp"
 
 namespace ysim {
 
 namespace {
-constexpr int SDEPTH = 8;    // LDS stack entries per thread (deeper stacks overflow no less on cloth)
+constexpr int SDEPTH = 8;          // LDS stack entries per thread (deeper stacks overflow no less on cloth)
+constexpr uint32_t TQ_STACK = 256; // tile walk: pending child pairs
+constexpr uint32_t TQ_LEAVES = 256;
+constexpr uint32_t TQ_PAIRS = 256;
+
+__device__ __forceinline__ uint64_t lanes_below() { return (1ull << __lane_id()) - 1ull; }
+__device__ __forceinline__ uint32_t popc64(uint64_t v) { return (uint32_t)__popcll(v); }
+
+// LDS handoff between the lanes of one wave (the tile kernel runs one wave per
+// workgroup): all earlier LDS writes of the wave are visible to later reads.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Adds a wave's candidate count to its own statistics word.
+__device__ __forceinline__ void wave_stats(const QueryArgs& a, uint32_t wave, uint32_t npair) {
+  uint32_t s = npair;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (a.wstats && __lane_id() == 0 && s) a.wstats[wave] += s;
+}
+
+// One query vertex against one target tree, narrow phase inline; cnt is the
+// vertex's running slot count.  Returns false if the walk's guard fired.
+template <bool EMIT_PAIRS, int SD>
+__device__ __forceinline__ bool lane_target(const QueryArgs& a, uint32_t m, uint32_t p, uint32_t j, uint32_t t,
+                                            f3 qcur, f3 qprev, f3 qmin, f3 qmax, float band, int2* stk,
+                                            int stride, uint32_t& cnt, uint32_t& npair) {
+  const uint32_t nT = a.mesh_nfacets[t];
+  if (nT == 0) return true;
+  const ysim_bvh_node* T = a.tree + a.mt.noff[t];
+  const int32_t* P = a.parent + a.mt.noff[t];
+  const uint32_t* F = a.facets + 3 * (size_t)a.mt.foff[t];
+  const uint32_t tvb = a.mt.voff[t];
+  const bool self = (t == m);
+  const uint32_t cap = a.cap;
+  const int it = traverse<SD>(T, P, nT, qmin, qmax, stk, stride, [&](int fid) {
+    const uint32_t* f = F + 3 * (size_t)fid;
+    const uint32_t f0 = f[0], f1 = f[1], f2 = f[2];
+    if (self && (p == f0 || p == f1 || p == f2)) return;
+    ++npair;
+    if (EMIT_PAIRS) {
+      uint32_t slot = atomicAdd(a.npairs, 1u);
+      if (slot < a.pair_cap) reinterpret_cast<uint4*>(a.pairs)[slot] = make_uint4(m, p, t, (uint32_t)fid);
+      return;
+    }
+    f3 nrm;
+    float d;
+    if (narrow_test(qcur, qprev, ld3(a.x, tvb + f0), ld3(a.x, tvb + f1), ld3(a.x, tvb + f2), band, nrm, d)) {
+      if (cnt < cap) {
+        a.cnd[(size_t)j * cap + cnt] = make_float4(nrm.x, nrm.y, nrm.z, d);
+        a.cid[(size_t)j * cap + cnt] = make_uint2(t, (uint32_t)fid);
+      }
+      ++cnt;
+    }
+  });
+  return it >= 0;
+}
+
+// Query vertex of item tq: mesh-local vertex p (grid cloths in tile order).
+__device__ __forceinline__ uint32_t query_vertex(const QueryArgs& a, uint32_t sg, uint32_t m, uint32_t tq) {
+  const uint32_t gn = a.mesh_grid_n ? a.mesh_grid_n[m] : 0u;
+  return gn ? tile_order(tq - a.qsegs.off[sg], gn) : tq - a.qsegs.off[sg];
+}
 }  // namespace
 
 // QB threads per block (64, 128 or 256) and an occupancy target of WPE waves
-// per SIMD (5: registers as needed; 6-8: fewer registers, some spilled), both
-// chosen by the autotuner: the traversal is bound by dependent node loads, so
-// more waves in flight can pay for a few spills.
+// per SIMD (5: registers as needed; 6-8: fewer registers, some spilled).
 template <bool EMIT_PAIRS, int QB, int WPE>
 __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __global__ __launch_bounds__(QB) void k_query(QueryArgs a) {
   __shared__ int2 s_stack[SDEPTH * QB];
@@ -32,11 +107,8 @@ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __global__ __launch_bounds__(QB) 
   if (tq < a.nitems) {
     const uint32_t sg = seg_of(a.qsegs, tq);
     const uint32_t m = a.qsegs.mesh[sg];
-    // grid cloths are queried in 8 x 8 vertex tiles (one tile per wave); the
-    // contact slots stay indexed by vertex
-    const uint32_t gn = a.mesh_grid_n ? a.mesh_grid_n[m] : 0u;
-    const uint32_t p = gn ? tile_order(tq - a.qsegs.off[sg], gn) : tq - a.qsegs.off[sg];
-    const uint32_t j = a.qsegs.off[sg] + p;
+    const uint32_t p = query_vertex(a, sg, m, tq);
+    const uint32_t j = a.qsegs.off[sg] + p;  // contact slots stay indexed by vertex
     const uint32_t g = a.mt.voff[m] + p;
     const f3 qcur = ld3(a.x, g);
     const f3 qprev = ld3(a.xprev, g);
@@ -45,55 +117,299 @@ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __global__ __launch_bounds__(QB) 
     const f3 qmin = {qcur.x - margin, qcur.y - margin, qcur.z - margin};
     const f3 qmax = {qcur.x + margin, qcur.y + margin, qcur.z + margin};
     uint32_t cnt = 0;
-    const uint32_t cap = a.cap;
-    for (uint32_t ti = a.tgt_off[sg]; ti < a.tgt_off[sg + 1]; ++ti) {
-      const uint32_t t = a.tgt[ti];
-      const uint32_t nT = a.mesh_nfacets[t];
-      if (nT == 0) continue;
-      const ysim_bvh_node* T = a.tree + a.mt.noff[t];
-      const int32_t* P = a.parent + a.mt.noff[t];
-      const uint32_t* F = a.facets + 3 * (size_t)a.mt.foff[t];
-      const uint32_t tvb = a.mt.voff[t];
-      const bool self = (t == m);
-      const int it = traverse<SDEPTH>(T, P, nT, qmin, qmax, s_stack + threadIdx.x, QB,
-               [&](int fid) {
-                 const uint32_t* f = F + 3 * (size_t)fid;
-                 const uint32_t f0 = f[0], f1 = f[1], f2 = f[2];
-                 if (self && (p == f0 || p == f1 || p == f2)) return;
-                 ++npair;
-                 if (EMIT_PAIRS) {
-                   uint32_t slot = atomicAdd(a.npairs, 1u);
-                   if (slot < a.pair_cap) {
-                     uint4* o = reinterpret_cast<uint4*>(a.pairs) + slot;
-                     *o = make_uint4(m, p, t, (uint32_t)fid);
-                   }
-                   return;
-                 }
-                 f3 nrm;
-                 float d;
-                 if (narrow_test(qcur, qprev, ld3(a.x, tvb + f0), ld3(a.x, tvb + f1), ld3(a.x, tvb + f2),
-                                 band, nrm, d)) {
-                   if (cnt < cap) {
-                     a.cnd[(size_t)j * cap + cnt] = make_float4(nrm.x, nrm.y, nrm.z, d);
-                     a.cid[(size_t)j * cap + cnt] = make_uint2(t, (uint32_t)fid);
-                   }
-                   ++cnt;
-                 }
-               });
-      if (it < 0) atomicOr(a.flags, 4u);
-    }
+    for (uint32_t ti = a.tgt_off[sg]; ti < a.tgt_off[sg + 1]; ++ti)
+      if (!lane_target<EMIT_PAIRS, SDEPTH>(a, m, p, j, a.tgt[ti], qcur, qprev, qmin, qmax, band,
+                                           s_stack + threadIdx.x, QB, cnt, npair))
+        atomicOr(a.flags, 4u);
     if (!EMIT_PAIRS) {
       a.ccount[j] = cnt;
-      if (cnt > cap) atomicOr(a.flags, 2u);
+      if (cnt > a.cap) atomicOr(a.flags, 2u);
     }
   }
-  if (!EMIT_PAIRS && a.npairs) {
-    // candidate-pair statistics: one atomic per wave
-    uint32_t s = npair;
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
-    if ((threadIdx.x & 63) == 0 && s) atomicAdd(a.npairs, s);
+  if (!EMIT_PAIRS) wave_stats(a, (xcd_block(blockIdx.x, gridDim.x) * QB + threadIdx.x) >> 6, npair);
+}
+
+// One wave per 64 consecutive query items.  Waves whose items span two query
+// segments take the per-lane walk.  When the leaf list fills up, the leaves
+// found so far go through phases 2-3 and the walk goes on.  When the walk's
+// pending list outgrows its buffer, the (wave, target) is redone per lane from
+// the slot counts it started with (the slots it filled are overwritten).
+// a.tile == 2 shrinks both buffers (128 leaves, 32 pending pairs) so that the
+// leaf flushes and the per-lane redo both run (a test mode for those paths).
+__attribute__((amdgpu_waves_per_eu(4, 4))) __global__ __launch_bounds__(64) void k_query_tile(QueryArgs a) {
+  // s_buf: [0, TQ_STACK) child-link pairs of overlapping internal nodes,
+  // [TQ_STACK, TQ_STACK + TQ_PAIRS) (lane, facet) candidates; the whole buffer
+  // is the per-lane stack (depth 8) of the fallback walks
+  __shared__ int2 s_buf[TQ_STACK + TQ_PAIRS];
+  __shared__ uint2 s_leaf[TQ_LEAVES];  // (leaf node, facet) overlapping the union box
+  __shared__ float s_q[64 * 6];        // qcur, qprev per lane
+  __shared__ float s_sub[16 * 6];      // 2 x 2-vertex sub-boxes
+  __shared__ uint32_t s_cnt[64], s_p[64], s_j[64];
+  static_assert(TQ_STACK + TQ_PAIRS == 64 * SDEPTH, "fallback stack: SDEPTH entries per lane");
+  int2* s_stk = s_buf;
+  uint2* s_pair = reinterpret_cast<uint2*>(s_buf + TQ_STACK);
+  const uint32_t lane = threadIdx.x;
+  const uint32_t w = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t base = w * 64;
+  const uint32_t tq = base + lane;
+  const bool active = tq < a.nitems;
+  const uint32_t last = min(base + 63u, a.nitems - 1u);
+  const uint32_t sg = seg_of(a.qsegs, last);  // wave-uniform from here on
+  uint32_t npair = 0;
+  int2* fstk = s_buf + lane;
+  if (seg_of(a.qsegs, base) != sg) {
+    // mixed segments: per-lane walk over this lane's own targets
+    if (active) {
+      const uint32_t sl = seg_of(a.qsegs, tq);
+      const uint32_t ml = a.qsegs.mesh[sl];
+      const uint32_t pl = query_vertex(a, sl, ml, tq);
+      const uint32_t jl = a.qsegs.off[sl] + pl;
+      const f3 qc = ld3(a.x, a.mt.voff[ml] + pl), qp = ld3(a.xprev, a.mt.voff[ml] + pl);
+      const float mg = a.mesh_margin[ml];
+      const f3 qlo = {qc.x - mg, qc.y - mg, qc.z - mg}, qhi = {qc.x + mg, qc.y + mg, qc.z + mg};
+      uint32_t cnt = 0;
+      for (uint32_t ti = a.tgt_off[sl]; ti < a.tgt_off[sl + 1]; ++ti)
+        if (!lane_target<false, SDEPTH>(a, ml, pl, jl, a.tgt[ti], qc, qp, qlo, qhi, a.mesh_band[ml], fstk, 64,
+                                        cnt, npair))
+          atomicOr(a.flags, 4u);
+      a.ccount[jl] = cnt;
+      if (cnt > a.cap) atomicOr(a.flags, 2u);
+    }
+    wave_stats(a, w, npair);
+    return;
   }
+  const uint32_t m = a.qsegs.mesh[sg];
+  const uint32_t p = query_vertex(a, sg, m, active ? tq : last);
+  const uint32_t j = a.qsegs.off[sg] + p;
+  const uint32_t g = a.mt.voff[m] + p;
+  const f3 qcur = ld3(a.x, g);
+  const f3 qprev = ld3(a.xprev, g);
+  const float margin = a.mesh_margin[m];
+  const float band = a.mesh_band[m];
+  const f3 qmin = {qcur.x - margin, qcur.y - margin, qcur.z - margin};
+  const f3 qmax = {qcur.x + margin, qcur.y + margin, qcur.z + margin};
+  // per-lane query data for the cooperative phases; idle lanes carry an empty box
+  const float INF = __uint_as_float(0x7f800000u);
+  f3 lo = active ? qmin : f3{INF, INF, INF}, hi = active ? qmax : f3{-INF, -INF, -INF};
+  s_q[lane * 6 + 0] = qcur.x;
+  s_q[lane * 6 + 1] = qcur.y;
+  s_q[lane * 6 + 2] = qcur.z;
+  s_q[lane * 6 + 3] = qprev.x;
+  s_q[lane * 6 + 4] = qprev.y;
+  s_q[lane * 6 + 5] = qprev.z;
+  s_cnt[lane] = 0;
+  s_p[lane] = p;
+  s_j[lane] = j;
+  auto reduce = [&](int o) {
+    lo = {fminf(lo.x, __shfl_xor(lo.x, o, 64)), fminf(lo.y, __shfl_xor(lo.y, o, 64)),
+          fminf(lo.z, __shfl_xor(lo.z, o, 64))};
+    hi = {fmaxf(hi.x, __shfl_xor(hi.x, o, 64)), fmaxf(hi.y, __shfl_xor(hi.y, o, 64)),
+          fmaxf(hi.z, __shfl_xor(hi.z, o, 64))};
+  };
+  // sub-box s holds lanes {b, b+1, b+8, b+9}, b = 16 (s / 4) + 2 (s % 4): a
+  // 2 x 2 block of an 8 x 8 tile
+  reduce(1);
+  reduce(8);
+  if ((lane & 9u) == 0) {
+    const uint32_t s = ((lane >> 4) << 2) | ((lane & 7u) >> 1);
+    s_sub[s * 6 + 0] = lo.x;
+    s_sub[s * 6 + 1] = lo.y;
+    s_sub[s * 6 + 2] = lo.z;
+    s_sub[s * 6 + 3] = hi.x;
+    s_sub[s * 6 + 4] = hi.y;
+    s_sub[s * 6 + 5] = hi.z;
+  }
+  reduce(2);
+  reduce(4);
+  reduce(16);
+  reduce(32);
+  const f3 Ulo = lo, Uhi = hi;  // union of the wave's boxes
+  const uint32_t leaf_cap = a.tile == 2 ? 128u : TQ_LEAVES;  // >= 2 x 64
+  const uint32_t stack_cap = a.tile == 2 ? 32u : TQ_STACK;
+  wave_sync();
+
+  for (uint32_t ti = a.tgt_off[sg]; ti < a.tgt_off[sg + 1]; ++ti) {
+    const uint32_t t = a.tgt[ti];
+    const uint32_t nT = a.mesh_nfacets[t];
+    if (nT == 0) continue;
+    const ysim_bvh_node* T = a.tree + a.mt.noff[t];
+    const uint32_t* F = a.facets + 3 * (size_t)a.mt.foff[t];
+    const uint32_t tvb = a.mt.voff[t];
+    const bool self = (t == m);
+    const uint32_t cnt0 = s_cnt[lane], npair0 = npair;  // for a per-lane redo of this target
+    uint32_t np = 0;
+    // ---- phase 3 (whenever the pair buffer fills): narrow phase per pair
+    auto flush = [&]() {
+      for (uint32_t c = 0; c < np; c += 64) {
+        if (c + lane < np) {
+          const uint2 pr = s_pair[c + lane];
+          const uint32_t v = pr.x, fid = pr.y;
+          const uint32_t* f = F + 3 * (size_t)fid;
+          const uint32_t f0 = f[0], f1 = f[1], f2 = f[2];
+          const f3 vc = {s_q[v * 6 + 0], s_q[v * 6 + 1], s_q[v * 6 + 2]};
+          const f3 vp = {s_q[v * 6 + 3], s_q[v * 6 + 4], s_q[v * 6 + 5]};
+          f3 nrm;
+          float d;
+          if (narrow_test(vc, vp, ld3(a.x, tvb + f0), ld3(a.x, tvb + f1), ld3(a.x, tvb + f2), band, nrm, d)) {
+            const uint32_t slot = atomicAdd(&s_cnt[v], 1u);
+            if (slot < a.cap) {
+              const uint32_t jv = s_j[v];
+              a.cnd[(size_t)jv * a.cap + slot] = make_float4(nrm.x, nrm.y, nrm.z, d);
+              a.cid[(size_t)jv * a.cap + slot] = make_uint2(t, fid);
+            }
+          }
+        }
+      }
+      np = 0;
+      wave_sync();
+    };
+    // ---- phase 2 (whenever the leaf list fills): leaves -> (vertex, facet)
+    // candidates through the sub-boxes
+    auto leaves = [&](uint32_t nl) {
+      for (uint32_t c0 = 0; c0 < nl; c0 += 64) {
+        const bool have = c0 + lane < nl;
+        uint32_t mask = 0, fid = 0, f0 = 0, f1 = 0, f2 = 0;
+        NodeV L;
+        if (have) {
+          const uint2 le = s_leaf[c0 + lane];
+          L = load_node(T, (int)le.x);
+          fid = le.y;
+          const uint32_t* f = F + 3 * (size_t)fid;
+          f0 = f[0];
+          f1 = f[1];
+          f2 = f[2];
+#pragma unroll 4
+          for (int s = 0; s < 16; ++s) {
+            const f3 sl = {s_sub[s * 6 + 0], s_sub[s * 6 + 1], s_sub[s * 6 + 2]};
+            const f3 sh = {s_sub[s * 6 + 3], s_sub[s * 6 + 4], s_sub[s * 6 + 5]};
+            if (overlap(sl, sh, L.lo, L.hi)) mask |= 1u << s;
+          }
+        }
+        while (__ballot(mask != 0)) {
+          uint32_t nc = 0, cand = 0;  // up to 4 candidate lanes, 6 bits each
+          if (mask) {
+            const uint32_t s = __builtin_ctz(mask);
+            mask &= mask - 1;
+            const uint32_t b0 = ((s >> 2) * 2) * 8 + (s & 3u) * 2;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const uint32_t v = b0 + (q & 1) + (q >> 1) * 8;
+              if (base + v >= a.nitems) continue;
+              const f3 vc = {s_q[v * 6 + 0], s_q[v * 6 + 1], s_q[v * 6 + 2]};
+              const f3 vl = {vc.x - margin, vc.y - margin, vc.z - margin};
+              const f3 vh = {vc.x + margin, vc.y + margin, vc.z + margin};
+              if (!overlap(vl, vh, L.lo, L.hi)) continue;
+              const uint32_t pv = s_p[v];
+              if (self && (pv == f0 || pv == f1 || pv == f2)) continue;
+              cand |= v << (6 * nc);
+              ++nc;
+            }
+          }
+          uint32_t incl = nc;  // inclusive scan of the candidate counts
+#pragma unroll
+          for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(incl, o, 64);
+            if ((int)lane >= o) incl += u;
+          }
+          const uint32_t tot = __shfl(incl, 63, 64);
+          if (tot) {
+            if (np + tot > TQ_PAIRS) flush();
+            const uint32_t o = np + incl - nc;
+            for (uint32_t q = 0; q < nc; ++q) s_pair[o + q] = make_uint2((cand >> (6 * q)) & 63u, fid);
+            np += tot;
+            npair += nc;
+            wave_sync();
+          }
+        }
+      }
+    };
+    // ---- phase 1: breadth-first walk with the union box, up to 64 pending
+    // nodes per round
+    uint32_t sp = 0, nl = 0;
+    bool overflow = false;
+    {
+      const NodeV r = load_node(T, 0);
+      if (overlap(Ulo, Uhi, r.lo, r.hi)) {
+        if (r.a < 0) {
+          if (lane == 0) s_leaf[0] = make_uint2(0u, (uint32_t)r.b);
+          nl = 1;
+        } else {
+          if (lane == 0) s_stk[0] = make_int2(r.a, r.b);
+          sp = 1;
+        }
+      }
+    }
+    wave_sync();
+    uint32_t rounds = 0;
+    while (sp > 0) {
+      if (++rounds > 2 * nT + 2) {  // a valid tree is walked in fewer rounds
+        overflow = true;
+        break;
+      }
+      const uint32_t k = sp < 64u ? sp : 64u, b0 = sp - k;
+      if (nl + 2 * k > leaf_cap) {  // this round may add 2 leaves per node
+        leaves(nl);
+        nl = 0;
+      }
+      const bool have = lane < k;
+      int2 e = make_int2(0, 0);
+      if (have) e = s_stk[b0 + lane];
+      wave_sync();
+      sp = b0;
+      bool pA = false, pB = false, lA = false, lB = false;
+      NodeV A, B;
+      if (have) {
+        A = load_node(T, e.x);
+        B = load_node(T, e.y);
+        const bool hA = overlap(Ulo, Uhi, A.lo, A.hi), hB = overlap(Ulo, Uhi, B.lo, B.hi);
+        pA = hA && A.a >= 0;
+        pB = hB && B.a >= 0;
+        lA = hA && A.a < 0;
+        lB = hB && B.a < 0;
+      }
+      const uint64_t bp1 = __ballot(pA || pB), bp2 = __ballot(pA && pB);
+      const uint64_t bl1 = __ballot(lA || lB), bl2 = __ballot(lA && lB);
+      const uint32_t tp = popc64(bp1) + popc64(bp2), tl = popc64(bl1) + popc64(bl2);
+      if (sp + tp > stack_cap) {
+        overflow = true;
+        break;
+      }
+      uint32_t op = sp + popc64(bp1 & lanes_below()) + popc64(bp2 & lanes_below());
+      uint32_t ol = nl + popc64(bl1 & lanes_below()) + popc64(bl2 & lanes_below());
+      if (pA) s_stk[op++] = make_int2(A.a, A.b);
+      if (pB) s_stk[op++] = make_int2(B.a, B.b);
+      if (lA) s_leaf[ol++] = make_uint2((uint32_t)e.x, (uint32_t)A.b);
+      if (lB) s_leaf[ol++] = make_uint2((uint32_t)e.y, (uint32_t)B.b);
+      sp += tp;
+      nl += tl;
+      wave_sync();
+    }
+    if (overflow) {
+      // too wide: this target again per lane, from the slot counts it started with
+      wave_sync();
+      uint32_t c = cnt0;
+      npair = npair0;
+      const f3 qc = {s_q[lane * 6 + 0], s_q[lane * 6 + 1], s_q[lane * 6 + 2]};
+      const f3 qp = {s_q[lane * 6 + 3], s_q[lane * 6 + 4], s_q[lane * 6 + 5]};
+      const f3 qlo = {qc.x - margin, qc.y - margin, qc.z - margin}, qhi = {qc.x + margin, qc.y + margin, qc.z + margin};
+      if (active && !lane_target<false, SDEPTH>(a, m, s_p[lane], s_j[lane], t, qc, qp, qlo, qhi, band, fstk, 64, c,
+                                                npair))
+        atomicOr(a.flags, 4u);
+      wave_sync();
+      s_cnt[lane] = c;
+      wave_sync();
+      continue;
+    }
+    leaves(nl);
+    flush();
+  }
+  wave_sync();
+  if (active) {
+    const uint32_t c = s_cnt[lane];
+    a.ccount[j] = c;
+    if (c > a.cap) atomicOr(a.flags, 2u);
+  }
+  wave_stats(a, w, npair);
 }
 
 bool query_block_ok(uint32_t b) { return b == 64 || b == 128 || b == 256; }
@@ -101,6 +417,11 @@ bool query_waves_ok(uint32_t w) { return w == 5 || w == 6 || w == 8; }
 
 int launch_query(hipStream_t s, const QueryArgs& a, bool emit_pairs) {
   if (!a.nitems) return YSIM_OK;
+  if (!emit_pairs && a.tile) {
+    hipLaunchKernelGGL(k_query_tile, dim3(ceil_div(a.nitems, 64)), dim3(64), 0, s, a);
+    YSIM_CHECK_LAUNCH();
+    return YSIM_OK;
+  }
   const uint32_t qb = a.block;
   if (!query_block_ok(qb) || !query_waves_ok(a.waves)) return YSIM_ERR_INVALID;
 #define YSIM_Q(E, B, W) hipLaunchKernelGGL((k_query<E, B, W>), dim3(ceil_div(a.nitems, B)), dim3(B), 0, s, a)

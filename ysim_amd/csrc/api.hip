@@ -98,7 +98,7 @@ struct ysim_scene {
   ysim_bvh_node* tree = nullptr;
   int32_t* parent = nullptr;
   uint32_t* visit = nullptr;
-  uint2* range = nullptr;      // per internal node: sorted leaf range (LDS refit)
+  RefitLinks links;             // per node: parent's leaf count and side (LDS refit)
   std::vector<BuildGroup> dyn_groups, stat_groups;
   SegTable dyn_tsegs, dyn_vsegs;
   uint32_t *d_tgt_off = nullptr, *d_tgt = nullptr;
@@ -434,7 +434,7 @@ struct ysim_scene {
 
   int enqueue_build(BuildGroup& g, const float* xc) {
     if (!g.T) return YSIM_OK;
-    YSIM_TRY(launch_box(st, g.vsegs, mt, xc, g.V, box_acc));
+    YSIM_TRY(launch_box(st, g.vsegs, mt, xc, g.V, box_acc, g.meshes.size() == 1 ? g.meshes[0] : 0xffffffffu));
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[1], st));
     if (g.perm && tune.sort_incremental) {
       // keys in the last build's sorted order + LDS window sorts + exact check;
@@ -450,7 +450,7 @@ struct ysim_scene {
                                   flags, need_full));
       if (timing) YSIM_HIP_TRY(hipEventRecord(ev[3], st));
       YSIM_TRY(launch_hierarchy(st, g.tsegs, mt, keys[0], g.key64, vals[0], g.code_bits, tree, parent, visit, g.T,
-                                box_acc, g.tsegs.nseg, hist, (uint32_t)g.plan.hist_words, range, g.perm));
+                                box_acc, g.tsegs.nseg, hist, (uint32_t)g.plan.hist_words, g.perm, links));
       last_sorted = 0;
       return YSIM_OK;
     }
@@ -466,7 +466,7 @@ struct ysim_scene {
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[3], st));
     const uint32_t sorted = (g.plan.hybrid || g.plan.passes % 2 == 0) ? 0 : 1;
     YSIM_TRY(launch_hierarchy(st, g.tsegs, mt, keys[sorted], g.key64, vals[sorted], g.code_bits, tree, parent,
-                              visit, g.T, box_acc, g.tsegs.nseg, hist, (uint32_t)g.plan.hist_words, range, g.perm));
+                              visit, g.T, box_acc, g.tsegs.nseg, hist, (uint32_t)g.plan.hist_words, g.perm, links));
     last_sorted = sorted;
     return YSIM_OK;
   }
@@ -522,7 +522,7 @@ struct ysim_scene {
     }
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[4], st));
     YSIM_TRY(launch_refit(st, dyn_tsegs, mt, xc, v[par], P.subh, P.swept != 0, facets, tree, parent, visit, Tdyn,
-                          flags, formal_refit, range, tune.refit_block));
+                          flags, formal_refit, &links, tune.refit_block));
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[5], st));
     return YSIM_OK;
   }
@@ -747,7 +747,7 @@ int ysim_hip_scene_box(ysim_ctx* ctx, const float* d_x, uint32_t num_vertices, u
   uint32_t* acc = scratch_words(ctx, 64);
   uint32_t init[6] = {~0u, ~0u, ~0u, 0u, 0u, 0u};
   YSIM_HIP_TRY(hipMemcpyAsync(acc, init, sizeof(init), hipMemcpyHostToDevice, ctx->stream));
-  YSIM_TRY(launch_box(ctx->stream, u.seg(), u.mt(), d_x, num_vertices, acc));
+  YSIM_TRY(launch_box(ctx->stream, u.seg(), u.mt(), d_x, num_vertices, acc, 0u));
   uint32_t h[6];
   YSIM_HIP_TRY(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
   YSIM_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1091,7 +1091,7 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
   YSIM_TRY(s.alloc(&s.tree, s.Ntot));
   YSIM_TRY(s.alloc(&s.parent, s.Ntot));
   YSIM_TRY(s.alloc(&s.visit, s.Ntot));
-  YSIM_TRY(s.alloc(&s.range, s.Ntot));
+  YSIM_TRY(s.alloc(&s.links.up, s.Ntot));
   YSIM_TRY(s.zero(s.tree, 8 * (size_t)s.Ntot));
   YSIM_TRY(s.zero(s.visit, s.Ntot));
   // segment tables
@@ -1235,7 +1235,7 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
     for (auto& g : *gs) {
       YSIM_TRY(s.enqueue_build(g, s.x[0]));
       YSIM_TRY(launch_refit(s.st, g.tsegs, s.mt, s.x[0], s.v[0], 0.f, false, s.facets, s.tree, s.parent, s.visit,
-                            g.T, s.flags, s.formal_refit, s.range, s.tune.refit_block));
+                            g.T, s.flags, s.formal_refit, &s.links, s.tune.refit_block));
     }
   YSIM_HIP_TRY(hipStreamSynchronize(s.st));
   uint32_t fl = 0;

@@ -35,8 +35,10 @@ struct MeshTables {
 
 // ---------------------------------------------------------------- lbvh.hip
 // S0: per-segment exact min/max of vertex positions into box_acc[seg*6] (ordered keys).
+// single_mesh: the mesh id when vsegs is that one mesh (a kernel with fewer
+// atomics), else 0xffffffff.
 int launch_box(hipStream_t s, SegTable vsegs, MeshTables mt, const float* x, uint32_t nitems,
-               uint32_t* box_acc);
+               uint32_t* box_acc, uint32_t single_mesh = 0xffffffffu);
 // S1: Morton keys of every triangle of tsegs (segment id prefixed above the code
 // bits when nseg > 1), values = mesh-local facet id, plus the digit histograms
 // of all sort passes of `plan`; zeroes the sort's block counters and advances
@@ -51,21 +53,27 @@ int launch_morton(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x,
 int launch_morton_inc(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x, const uint32_t* facets,
                       const uint32_t* box_acc, uint32_t code_bits, bool key64, const uint32_t* perm, void* keys,
                       uint32_t* vals, uint32_t nitems, uint32_t* need_full);
-// S3: Karras hierarchy per segment; leaf child links; zeroes the refit visit
-// counters; resets box_acc (nbox segments) and the sort histogram for the next build.
-// range (optional): per internal node its sorted leaf range (lo, hi), node-window indexed.
+// Extra links of a scene build for the LDS refit, written by the hierarchy
+// kernel (null: not written, and the refit takes the device-wide path only).
+struct RefitLinks {
+  uint32_t* up = nullptr;  // [node window + node] parent's leaf count << 1 | (node is child_b)
+};
+// S3: Karras hierarchy per segment; child and parent links (and rl.up when
+// set); zeroes the refit visit counters; resets box_acc (nbox segments) and the
+// sort histogram for the next build.
 int launch_hierarchy(hipStream_t s, SegTable tsegs, MeshTables mt, const void* keys, bool key64,
                      const uint32_t* vals, uint32_t code_bits, ysim_bvh_node* tree, int32_t* parent,
                      uint32_t* visit, uint32_t nitems, uint32_t* box_acc, uint32_t nbox,
-                     uint32_t* hist, uint32_t hist_words, uint2* range = nullptr, uint32_t* perm_out = nullptr);
+                     uint32_t* hist, uint32_t hist_words, uint32_t* perm_out = nullptr,
+                     RefitLinks rl = RefitLinks{});
 // S4+S5: leaf boxes (swept optional) + atomic bottom-up union per segment.
-// With range (from launch_hierarchy) the levels inside blocks of `block`
-// (256, 512 or 1024) sorted leaves are combined in LDS.
+// With rl (from launch_hierarchy) the levels inside blocks of `block` (256,
+// 512 or 1024) sorted leaves are combined in LDS.
 bool refit_block_ok(uint32_t b);
 int launch_refit(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x, const float* v,
                  float dt, bool swept, const uint32_t* facets, ysim_bvh_node* tree,
                  const int32_t* parent, uint32_t* visit, uint32_t nitems, uint32_t* err, bool formal,
-                 const uint2* range = nullptr, uint32_t block = 256);
+                 const RefitLinks* rl = nullptr, uint32_t block = 256);
 
 // ---------------------------------------------------------------- query.hip
 struct QueryArgs {

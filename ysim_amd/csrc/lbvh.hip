@@ -101,11 +101,50 @@ __global__ __launch_bounds__(256) void k_box(SegTable vs, MeshTables mt, const f
   if (cur != 0xffffffffu) flush_box(acc, cur, lo, hi);
 }
 
+// One mesh: 16 K vertices per 1024-thread block, so a 262 K-vertex mesh makes
+// 16 blocks and 96 atomics on the six box words (same-address atomics from
+// many workgroups serialize at the memory side).
+constexpr uint32_t BOX1_THREADS = 1024, BOX1_ITEMS = 16;
+__global__ __launch_bounds__(BOX1_THREADS) void k_box1(MeshTables mt, uint32_t mesh, const float* __restrict__ x,
+                                                      uint32_t n, uint32_t* __restrict__ acc) {
+  __shared__ float s_red[6][BOX1_THREADS / 64];
+  const uint32_t start = blockIdx.x * (BOX1_THREADS * BOX1_ITEMS);
+  const uint32_t end = min(start + BOX1_THREADS * BOX1_ITEMS, n);
+  const uint32_t gbase = mt.voff[mesh];
+  const float INF = __uint_as_float(0x7f800000u);
+  f3 lo = {INF, INF, INF}, hi = {-INF, -INF, -INF};
+#pragma unroll
+  for (uint32_t i = 0; i < BOX1_ITEMS; ++i) {
+    const uint32_t j = start + threadIdx.x + BOX1_THREADS * i;
+    if (j < end) {
+      const f3 p = ld3(x, gbase + j);
+      lo = vmin(lo, p);
+      hi = vmax(hi, p);
+    }
+  }
+  const float r[6] = {wave_min(lo.x), wave_min(lo.y), wave_min(lo.z), wave_max(hi.x), wave_max(hi.y), wave_max(hi.z)};
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int k = 0; k < 6; ++k) s_red[k][w] = r[k];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const uint32_t k = threadIdx.x;
+    float v = s_red[k][0];
+    for (uint32_t ww = 1; ww < BOX1_THREADS / 64; ++ww) v = k < 3 ? fminf(v, s_red[k][ww]) : fmaxf(v, s_red[k][ww]);
+    if (k < 3) atomicMin(acc + k, fkey(v));
+    else atomicMax(acc + k, fkey(v));
+  }
+}
+
 int launch_box(hipStream_t s, SegTable vsegs, MeshTables mt, const float* x, uint32_t nitems,
-               uint32_t* box_acc) {
+               uint32_t* box_acc, uint32_t single_mesh) {
   if (!nitems) return YSIM_OK;
-  hipLaunchKernelGGL(k_box, dim3(ceil_div(nitems, 256 * BOX_ITEMS)), dim3(256), 0, s, vsegs, mt, x,
-                     nitems, box_acc);
+  if (single_mesh != 0xffffffffu)
+    hipLaunchKernelGGL(k_box1, dim3(ceil_div(nitems, BOX1_THREADS * BOX1_ITEMS)), dim3(BOX1_THREADS), 0, s, mt,
+                       single_mesh, x, nitems, box_acc);
+  else
+    hipLaunchKernelGGL(k_box, dim3(ceil_div(nitems, 256 * BOX_ITEMS)), dim3(256), 0, s, vsegs, mt, x, nitems,
+                       box_acc);
   YSIM_CHECK_LAUNCH();
   return YSIM_OK;
 }
@@ -275,7 +314,7 @@ __global__ __launch_bounds__(256) void k_hierarchy(SegTable ts, MeshTables mt, c
                                                    uint32_t* __restrict__ visit, uint32_t n,
                                                    uint32_t* __restrict__ box_acc, uint32_t nbox,
                                                    uint32_t* __restrict__ hist, uint32_t hist_words,
-                                                   uint2* __restrict__ range, uint32_t* __restrict__ perm_out) {
+                                                   uint32_t* __restrict__ perm_out, RefitLinks rl) {
   const uint32_t j = blockIdx.x * 256 + threadIdx.x;
   // reset accumulators for the next build (their readers ran in earlier kernels)
   if (j < nbox * 6) box_acc[j] = ((j % 6) < 3) ? 0xffffffffu : 0u;
@@ -328,24 +367,28 @@ __global__ __launch_bounds__(256) void k_hierarchy(SegTable ts, MeshTables mt, c
   T[i].child_b = cb;
   P[ca] = i;
   P[cb] = i;
-  if (range) range[base + i] = make_uint2((uint32_t)lo, (uint32_t)hi);
+  if (rl.up) {
+    const uint32_t size2 = (uint32_t)(hi - lo + 1) << 1;
+    rl.up[base + ca] = size2;       // left child
+    rl.up[base + cb] = size2 | 1u;  // right child
+  }
 }
 
 int launch_hierarchy(hipStream_t s, SegTable tsegs, MeshTables mt, const void* keys, bool key64,
                      const uint32_t* vals, uint32_t code_bits, ysim_bvh_node* tree, int32_t* parent,
                      uint32_t* visit, uint32_t nitems, uint32_t* box_acc, uint32_t nbox, uint32_t* hist,
-                     uint32_t hist_words, uint2* range, uint32_t* perm_out) {
+                     uint32_t hist_words, uint32_t* perm_out, RefitLinks rl) {
   uint32_t work = std::max<uint32_t>(nitems, std::max<uint32_t>(nbox * 6, hist_words));
   if (!work) return YSIM_OK;
   uint32_t blocks = ceil_div(work, 256);
   if (key64)
     hipLaunchKernelGGL(k_hierarchy<uint64_t>, dim3(blocks), dim3(256), 0, s, tsegs, mt,
                        (const uint64_t*)keys, vals, code_bits, tree, parent, visit, nitems, box_acc, nbox,
-                       hist, hist_words, range, perm_out);
+                       hist, hist_words, perm_out, rl);
   else
     hipLaunchKernelGGL(k_hierarchy<uint32_t>, dim3(blocks), dim3(256), 0, s, tsegs, mt,
                        (const uint32_t*)keys, vals, code_bits, tree, parent, visit, nitems, box_acc, nbox,
-                       hist, hist_words, range, perm_out);
+                       hist, hist_words, perm_out, rl);
   YSIM_CHECK_LAUNCH();
   return YSIM_OK;
 }
@@ -457,42 +500,52 @@ __global__ __launch_bounds__(256) void k_refit(SegTable ts, MeshTables mt, const
 }
 
 // Same refit with the lower levels resolved inside the workgroup.  A block
-// owns RB (256, 512 or 1024; chosen by the autotuner) consecutive sorted leaves; an internal node whose leaf range
-// (written by the hierarchy kernel) lies inside the block is combined through
-// LDS (the children's boxes and an arrival counter per node; a Karras internal
-// node is an endpoint of its own range, so its index falls inside the block and
-// addresses the LDS slot).  Only nodes spanning blocks use the device-wide
-// second-arrival protocol above, so a leaf's chain of device round trips starts
-// at the block's top instead of at the leaf.  Unions keep (child_a, child_b)
-// operand order; the boxes are identical to k_refit's.
+// owns RB (256, 512 or 1024; chosen by the autotuner) consecutive sorted leaves.
+// Each thread tracks the leaf range of its current node; the parent's range
+// follows from the parent's leaf count and the side, both in `up` (written by
+// the hierarchy kernel), and so does the sibling: a Karras node's children are
+// the nodes at its split and split + 1.  The block first copies the parent
+// links of its leaves and of the internal nodes indexed inside it to LDS.  An
+// internal node whose range lies inside the block is then combined through LDS
+// with no device memory access (the children's boxes and an arrival counter
+// per node; a Karras internal node is an endpoint of its own range, so its
+// index falls inside the block and addresses the LDS slot).  Only nodes
+// spanning blocks use the device-wide second-arrival protocol above, with the
+// next level's links loaded while the arrival resolves.  Unions keep
+// (child_a, child_b) operand order; the boxes are identical to k_refit's.
 template <bool FORMAL, int RB>
 __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, const float* __restrict__ x,
                                                   const float* __restrict__ v, float dt, int swept,
                                                   const uint32_t* __restrict__ facets, ysim_bvh_node* tree,
                                                   const int32_t* __restrict__ parent,
-                                                  const uint2* __restrict__ range, uint32_t* visit, uint32_t n,
-                                                  uint32_t* err) {
+                                                  const uint32_t* __restrict__ up_links, uint32_t* visit,
+                                                  uint32_t n, uint32_t* err) {
   __shared__ uint32_t s_cnt[RB];
   __shared__ float s_box[RB][2][6];
+  __shared__ int2 s_link[RB];  // internal node (slot) of the thread's mesh: (parent, up)
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t b0 = blk * RB;
   s_cnt[threadIdx.x] = 0;
-  __syncthreads();
   const uint32_t j = b0 + threadIdx.x;
-  if (j >= n) return;
-  const uint32_t sg = seg_of(ts, j);
+  const bool valid = j < n;
+  const uint32_t sg = seg_of(ts, valid ? j : n - 1);
   const uint32_t m = ts.mesh[sg];
   const uint32_t s = ts.off[sg];
   const int nn = (int)(ts.off[sg + 1] - s);
-  const int i = (int)(j - s);
+  const int i = (int)((valid ? j : n - 1) - s);
   const uint32_t base = mt.noff[m];
   ysim_bvh_node* T = tree + base;
   const int32_t* P = parent + base;
-  const uint2* R = range + base;
+  const uint32_t* UP = up_links + base;
   uint32_t* C = visit + base;
+  if (valid && i < nn - 1) s_link[threadIdx.x] = make_int2(P[i], (int)UP[i]);
+  __syncthreads();
+  if (!valid) return;
 
   int child = nn - 1 + i;
   const uint32_t fid = (uint32_t)T[child].child_b;
+  int pnext = P[child];
+  uint32_t upnext = UP[child];
   const uint32_t* f = facets + 3 * (size_t)(mt.foff[m] + fid);
   const uint32_t vb = mt.voff[m];
   f3 lo = ld3(x, vb + f[0]), hi = lo;
@@ -527,20 +580,25 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
     store_box(child, lo, hi, false);
     return;
   }
-  const int nnodes = 2 * nn - 1;
-  uint32_t clo = (uint32_t)i;  // leaf range of the current node
+  uint32_t clo = (uint32_t)i, chi = (uint32_t)i;  // leaf range of the current node
   for (int step = 0; step < 2 * nn; ++step) {
-    const int p = P[child];
+    const int p = pnext;
+    const uint32_t up = upnext;
     if (p < 0 || p >= nn - 1) {
       atomicOr(err, 4u);
       return;
     }
-    const uint2 pr = R[p];
-    const bool local = s + pr.x >= b0 && s + pr.y < b0 + RB;
+    const int side = (int)(up & 1u);  // 0: child_a (left part of p's range)
+    const uint32_t psize = up >> 1;
+    const uint32_t plo = side ? chi + 1 - psize : clo, phi = side ? chi : clo + psize - 1;
+    const bool local = s + plo >= b0 && s + phi < b0 + RB;
     store_box(child, lo, hi, !local);
     if (local) {
       const uint32_t slot = s + (uint32_t)p - b0;
-      const int side = (clo == pr.x) ? 0 : 1;  // child_a covers the left part of p's range
+      if (slot >= (uint32_t)RB) {  // p is an endpoint of its range in a valid tree
+        atomicOr(err, 4u);
+        return;
+      }
       float* mine = s_box[slot][side];
       mine[0] = lo.x; mine[1] = lo.y; mine[2] = lo.z; mine[3] = hi.x; mine[4] = hi.y; mine[5] = hi.z;
       const uint32_t old = __hip_atomic_fetch_add(&s_cnt[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -554,7 +612,13 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
         lo = vmin(omin, lo);
         hi = vmax(omax, hi);
       }
+      const int2 l = s_link[slot];
+      pnext = l.x;
+      upnext = (uint32_t)l.y;
     } else {
+      // the next level's links load while this arrival resolves
+      pnext = P[p];
+      upnext = UP[p];
       uint32_t old;
       if (FORMAL) {
         old = __hip_atomic_fetch_add(C + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
@@ -563,12 +627,14 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
         old = __hip_atomic_fetch_add(C + p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
       if ((old & 1u) == 0u) return;
-      const int ca = T[p].child_a, cb = T[p].child_b;
-      if (ca < 0 || ca >= nnodes || cb < 0 || cb >= nnodes) {
+      // sibling: child_b (split + 1) for a child_a, child_a (split) for a child_b
+      const uint32_t sp = side ? clo - 1 : chi;
+      const int sib = side ? (sp == plo ? nn - 1 + (int)sp : (int)sp)
+                           : (sp + 1 == phi ? nn + (int)sp : (int)sp + 1);
+      if (sib < 0 || sib >= 2 * nn - 1) {
         atomicOr(err, 4u);
         return;
       }
-      const int sib = (ca == child) ? cb : ca;
       const float* q = reinterpret_cast<const float*>(T + sib);
       f3 smin, smax;
       if (FORMAL) {
@@ -580,7 +646,7 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
         smin = f3{lo_f(m01), hi_f(m01), ld_agent(q + 2)};
         smax = f3{lo_f(x01), hi_f(x01), ld_agent(q + 6)};
       }
-      if (sib == cb) {
+      if (side == 0) {
         lo = vmin(lo, smin);
         hi = vmax(hi, smax);
       } else {
@@ -589,7 +655,8 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
       }
     }
     child = p;
-    clo = pr.x;
+    clo = plo;
+    chi = phi;
     if (p == 0) {
       store_box(0, lo, hi, false);
       return;
@@ -602,14 +669,14 @@ bool refit_block_ok(uint32_t b) { return b == 256 || b == 512 || b == 1024; }
 
 int launch_refit(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x, const float* v, float dt,
                  bool swept, const uint32_t* facets, ysim_bvh_node* tree, const int32_t* parent,
-                 uint32_t* visit, uint32_t nitems, uint32_t* err, bool formal, const uint2* range,
+                 uint32_t* visit, uint32_t nitems, uint32_t* err, bool formal, const RefitLinks* rl,
                  uint32_t block) {
   if (!nitems) return YSIM_OK;
-  if (range) {
+  if (rl && rl->up) {
     if (!refit_block_ok(block)) return YSIM_ERR_INVALID;
 #define YSIM_R(F, B)                                                                                     \
   hipLaunchKernelGGL((k_refit_lds<F, B>), dim3(ceil_div(nitems, B)), dim3(B), 0, s, tsegs, mt, x, v, dt, \
-                     (int)swept, facets, tree, parent, range, visit, nitems, err)
+                     (int)swept, facets, tree, parent, rl->up, visit, nitems, err)
     if (formal) {
       if (block == 256) YSIM_R(true, 256); else if (block == 512) YSIM_R(true, 512); else YSIM_R(true, 1024);
     } else {

@@ -80,7 +80,11 @@ typedef struct ysim_fastgrid_params {  /* FastGridClothBehaviorParams<float>, 44
 /* BehaviorType subset on this path (include/sim/mesh_state.hpp:108-121). */
 enum {
   YSIM_BEHAVIOR_FLOAT = 0,            /* static obstacle: target only, never moves */
-  YSIM_BEHAVIOR_FASTGRID_CLOTH = 1    /* grid cloth: query + target, integrated */
+  YSIM_BEHAVIOR_FASTGRID_CLOTH = 1,   /* grid cloth: query + target, integrated */
+  YSIM_BEHAVIOR_TRIANGULAR_CLOTH = 2  /* any manifold triangle mesh: springs along its edges (k_stretch)
+                                       * and between the opposite vertices of every interior edge
+                                       * (k_bend), rest lengths from the creation positions
+                                       * (compute_tri_spring_forces, physics.metal:319-378) */
 };
 
 /* One mesh of a packed scene (the Scene::pack contract, scene.hpp:783-1182). */
@@ -91,7 +95,8 @@ typedef struct ysim_mesh_desc {
   uint32_t collidable;                 /* 0: leaves the broad phase entirely */
   uint32_t group;                      /* meshes interact only within one group (batching) */
   uint32_t shape;                      /* ColliderKind lane carried into pair records; 0 = Mesh */
-  ysim_fastgrid_params cloth;          /* used when behavior == FASTGRID_CLOTH */
+  ysim_fastgrid_params cloth;          /* FASTGRID_CLOTH; TRIANGULAR_CLOTH uses kstretch, kbend and
+                                        * thickness only */
 } ysim_mesh_desc;
 
 /* Per-scene stepping parameters: SymplecticSystem fields

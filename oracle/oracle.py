@@ -60,6 +60,12 @@ def lib():
         "yo_fastgrid_forces_f64": (None, [vp, C.c_uint32, C.c_double, C.c_double, f64p, f64p, f64p,
                                           f64p, f64p]),
         "yo_spring_count": (C.c_uint32, [C.c_uint32]),
+        "yo_tri_adjacency": (C.c_int, [u32p, C.c_uint32, f32p, C.c_uint32, u32p, u32p, f32p, u32p, u32p, u32p,
+                                       u32p, f32p]),
+        "yo_tri_forces_f32": (C.c_int, [u32p, C.c_uint32, f32p, C.c_uint32, C.c_float, C.c_float, C.c_float,
+                                        C.c_float, f32p, f32p, f32p, f32p]),
+        "yo_tri_forces_f64": (C.c_int, [u32p, C.c_uint32, f32p, C.c_uint32, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, f64p, f64p, f64p, f64p]),
         "yo_integrate_f32": (C.c_uint32, [C.c_uint32, C.c_float, C.c_float, f32p, f32p, f32p, f32p,
                                           f32p, u32p, vp]),
         "yo_scene_create": (vp, [C.c_uint32, vp, f32p, f32p, f32p, f32p, f32p, u32p, vp]),
@@ -205,6 +211,42 @@ def fastgrid_forces(cloth: dict, x, v, ext, kair, kd, kspring=None, f64=False):
     return f
 
 
+def tri_adjacency(facets, x):
+    """TriangularCloth adjacency (MeshAdjacencyInitializer): dict of edges (E, 2),
+    rest_edge, adj_off/adj, opp_off/opp, rest_opp; None for a rejected mesh."""
+    F = np.ascontiguousarray(facets, np.uint32).reshape(-1)
+    X = np.ascontiguousarray(x, np.float32).reshape(-1)
+    nF, nV = F.size // 3, X.size // 3
+    cnt = np.zeros(3, np.uint32)
+    if lib().yo_tri_adjacency(P(F, u32p), nF, P(X, f32p), nV, P(cnt, u32p), None, None, None, None, None,
+                              None, None) != 0:
+        return None
+    nE, nA, nO = (int(c) for c in cnt)
+    out = dict(edges=np.zeros(2 * nE, np.uint32), rest_edge=np.zeros(nE, np.float32),
+               adj_off=np.zeros(nV + 1, np.uint32), adj=np.zeros(nA, np.uint32),
+               opp_off=np.zeros(nV + 1, np.uint32), opp=np.zeros(nO, np.uint32), rest_opp=np.zeros(nO, np.float32))
+    lib().yo_tri_adjacency(P(F, u32p), nF, P(X, f32p), nV, P(cnt, u32p), P(out["edges"], u32p),
+                           P(out["rest_edge"], f32p), P(out["adj_off"], u32p), P(out["adj"], u32p),
+                           P(out["opp_off"], u32p), P(out["opp"], u32p), P(out["rest_opp"], f32p))
+    out["edges"] = out["edges"].reshape(-1, 2)
+    return out
+
+
+def tri_forces(facets, x0, x, v, ext, kstretch, kbend, kair, kd, f64=False):
+    """compute_tri_spring_forces at (x, v), rest lengths from x0."""
+    F = np.ascontiguousarray(facets, np.uint32).reshape(-1)
+    X0 = np.ascontiguousarray(x0, np.float32).reshape(-1)
+    t, tp = (np.float64, f64p) if f64 else (np.float32, f32p)
+    xs, vs, es = (np.ascontiguousarray(a, t).reshape(-1) for a in (x, v, ext))
+    out = np.zeros_like(xs)
+    fn = lib().yo_tri_forces_f64 if f64 else lib().yo_tri_forces_f32
+    rc = fn(P(F, u32p), F.size // 3, P(X0, f32p), X0.size // 3, kstretch, kbend, kair, kd, P(xs, tp), P(vs, tp),
+            P(es, tp), P(out, tp))
+    if rc != 0:
+        raise ValueError("tri_forces: mesh rejected (repeated vertex in a facet or non-manifold edge)")
+    return out.reshape(-1, 3)
+
+
 def spring_count(n: int) -> int:
     return lib().yo_spring_count(n)
 
@@ -223,6 +265,8 @@ class OracleScene:
         self.h = L.yo_scene_create(len(scene.meshes), C.cast(self.descs, vp), P(pk["x"], f32p),
                                    P(pk["v"], f32p), P(pk["mass"], f32p), P(pk["ext"], f32p),
                                    P(pk["fixed"], f32p), P(pk["facets"], u32p), C.byref(self.params))
+        if not self.h:
+            raise ValueError("oracle scene rejected (a triangular cloth is not a manifold mesh)")
         if kspring is not None:
             ks = np.ascontiguousarray(kspring, np.float32)
             L.yo_scene_set_kspring(self.h, P(ks, f32p), ks.size)

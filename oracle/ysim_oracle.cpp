@@ -344,6 +344,118 @@ inline bool box_overlap(const float* qmin, const float* qmax, const ysim_bvh_nod
   return true;
 }
 
+// ---------------------------------------------------------------- TriangularCloth
+// MeshAdjacencyInitializer::initialize, include/sim/mesh_state.hpp:236-454.
+// Three edge records per facet ((a,b) opposite c, (b,c) opposite a, (c,a)
+// opposite b, each with v0 < v1), sorted by (v0, v1) and reduced to unique
+// edges.  A vertex lists its edges in edge order (stretch springs); an edge met
+// twice gives each of its two opposite vertices the other one (bend springs),
+// also in edge order.  Rest lengths are |x[b] - x[a]| of the given positions
+// (tinym vec3 norm: sqrt((x*x + y*y) + z*z)).  Which duplicate record sorts
+// first does not matter: the two opposite vertices list each other and the
+// norm is symmetric.  Returns false for a facet with a repeated vertex or an
+// edge shared by more than two facets (the reference leaves entries of such
+// edges unset).
+struct TriAdj {
+  std::vector<uint32_t> edges;  // (v0, v1) per edge
+  std::vector<float> rest_edge;
+  std::vector<uint32_t> adj_off, adj;  // per vertex: edge ids
+  std::vector<uint32_t> opp_off, opp;  // per vertex: opposite vertices
+  std::vector<float> rest_opp;
+};
+
+inline float edge_length(const float* x, uint32_t a, uint32_t b) {
+  const float dx = x[3 * b] - x[3 * a], dy = x[3 * b + 1] - x[3 * a + 1], dz = x[3 * b + 2] - x[3 * a + 2];
+  return std::sqrt((dx * dx + dy * dy) + dz * dz);
+}
+
+bool tri_adjacency(const uint32_t* F, uint32_t nF, const float* x, uint32_t nV, TriAdj& A) {
+  struct Rec { uint32_t v0, v1, o; };
+  std::vector<Rec> recs;
+  recs.reserve(3 * (size_t)nF);
+  for (uint32_t f = 0; f < nF; ++f) {
+    const uint32_t a = F[3 * f], b = F[3 * f + 1], c = F[3 * f + 2];
+    if (a == b || b == c || c == a || a >= nV || b >= nV || c >= nV) return false;
+    recs.push_back({std::min(a, b), std::max(a, b), c});
+    recs.push_back({std::min(b, c), std::max(b, c), a});
+    recs.push_back({std::min(c, a), std::max(c, a), b});
+  }
+  std::stable_sort(recs.begin(), recs.end(),
+                   [](const Rec& p, const Rec& q) { return p.v0 < q.v0 || (p.v0 == q.v0 && p.v1 < q.v1); });
+  struct Edge { uint32_t v0, v1, o0; int64_t o1; };
+  std::vector<Edge> E;
+  for (size_t k = 0; k < recs.size(); ++k) {
+    if (!E.empty() && E.back().v0 == recs[k].v0 && E.back().v1 == recs[k].v1) {
+      if (E.back().o1 >= 0) return false;  // third facet on one edge
+      E.back().o1 = recs[k].o;
+    } else {
+      E.push_back({recs[k].v0, recs[k].v1, recs[k].o, -1});
+    }
+  }
+  const uint32_t nE = (uint32_t)E.size();
+  A.edges.resize(2 * (size_t)nE);
+  A.rest_edge.resize(nE);
+  A.adj_off.assign(nV + 1, 0);
+  A.opp_off.assign(nV + 1, 0);
+  for (uint32_t e = 0; e < nE; ++e) {
+    A.edges[2 * e] = E[e].v0;
+    A.edges[2 * e + 1] = E[e].v1;
+    A.rest_edge[e] = edge_length(x, E[e].v0, E[e].v1);
+    A.adj_off[E[e].v0 + 1]++;
+    A.adj_off[E[e].v1 + 1]++;
+    if (E[e].o1 >= 0) {
+      A.opp_off[E[e].o0 + 1]++;
+      A.opp_off[(uint32_t)E[e].o1 + 1]++;
+    }
+  }
+  for (uint32_t i = 0; i < nV; ++i) {
+    A.adj_off[i + 1] += A.adj_off[i];
+    A.opp_off[i + 1] += A.opp_off[i];
+  }
+  A.adj.assign(A.adj_off[nV], 0);
+  A.opp.assign(A.opp_off[nV], 0);
+  A.rest_opp.assign(A.opp_off[nV], 0.f);
+  std::vector<uint32_t> fa(nV, 0), fo(nV, 0);
+  for (uint32_t e = 0; e < nE; ++e) {
+    if (E[e].o1 >= 0) {
+      const uint32_t o0 = E[e].o0, o1 = (uint32_t)E[e].o1;
+      const float r = edge_length(x, o0, o1);
+      A.opp[A.opp_off[o0] + fo[o0]] = o1;
+      A.rest_opp[A.opp_off[o0] + fo[o0]++] = r;
+      A.opp[A.opp_off[o1] + fo[o1]] = o0;
+      A.rest_opp[A.opp_off[o1] + fo[o1]++] = r;
+    }
+    A.adj[A.adj_off[E[e].v0] + fa[E[e].v0]++] = e;
+    A.adj[A.adj_off[E[e].v1] + fa[E[e].v1]++] = e;
+  }
+  return true;
+}
+
+// compute_tri_spring_forces, src/metal/physics.metal:319-378: external force +
+// air drag, then the stretch springs of the vertex's edges, then the bend
+// springs of its opposite vertices, in list order (k_shear is not used).
+template <class R>
+void tri_forces(const TriAdj& A, uint32_t nV, R kstretch, R kbend, R kair, R kd, const R* x, const R* v,
+                const R* ext, R* f) {
+#pragma omp parallel for schedule(static)
+  for (int64_t id_ = 0; id_ < (int64_t)nV; ++id_) {
+    const uint32_t id = (uint32_t)id_;
+    V3<R> pos = ld(x, id), vel = ld(v, id);
+    V3<R> force = ld(ext, id) + vel * kair;
+    for (uint32_t k = A.adj_off[id]; k < A.adj_off[id + 1]; ++k) {
+      const uint32_t e = A.adj[k];
+      const uint32_t e0 = A.edges[2 * e], e1 = A.edges[2 * e + 1];
+      const uint32_t other = (e0 == id) ? e1 : e0;
+      force = force + calc_spring(pos, vel, ld(x, other), ld(v, other), (R)A.rest_edge[e], kstretch, kd);
+    }
+    for (uint32_t k = A.opp_off[id]; k < A.opp_off[id + 1]; ++k) {
+      const uint32_t other = A.opp[k];
+      force = force + calc_spring(pos, vel, ld(x, other), ld(v, other), (R)A.rest_opp[k], kbend, kd);
+    }
+    st(f, id, force);
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -628,6 +740,44 @@ void yo_fastgrid_forces_f64(const ysim_fastgrid_params* P, uint32_t nV, double k
 }
 uint32_t yo_spring_count(uint32_t N) { return SpringIndex(N).total; }
 
+// TriangularCloth adjacency (counts[0..2] = edges, stretch entries, bend
+// entries); arrays are written when `edges` is not null.  Returns 0, or -1 for
+// a rejected mesh.
+int yo_tri_adjacency(const uint32_t* F, uint32_t nF, const float* x, uint32_t nV, uint32_t* counts,
+                     uint32_t* edges, float* rest_edge, uint32_t* adj_off, uint32_t* adj, uint32_t* opp_off,
+                     uint32_t* opp, float* rest_opp) {
+  TriAdj A;
+  if (!tri_adjacency(F, nF, x, nV, A)) return -1;
+  counts[0] = (uint32_t)A.rest_edge.size();
+  counts[1] = (uint32_t)A.adj.size();
+  counts[2] = (uint32_t)A.opp.size();
+  if (!edges) return 0;
+  std::memcpy(edges, A.edges.data(), 4 * A.edges.size());
+  std::memcpy(rest_edge, A.rest_edge.data(), 4 * A.rest_edge.size());
+  std::memcpy(adj_off, A.adj_off.data(), 4 * A.adj_off.size());
+  std::memcpy(adj, A.adj.data(), 4 * A.adj.size());
+  std::memcpy(opp_off, A.opp_off.data(), 4 * A.opp_off.size());
+  std::memcpy(opp, A.opp.data(), 4 * A.opp.size());
+  std::memcpy(rest_opp, A.rest_opp.data(), 4 * A.rest_opp.size());
+  return 0;
+}
+// TriangularCloth forces at (x, v), rest lengths from x0.
+int yo_tri_forces_f32(const uint32_t* F, uint32_t nF, const float* x0, uint32_t nV, float kstretch, float kbend,
+                      float kair, float kd, const float* x, const float* v, const float* ext, float* f) {
+  TriAdj A;
+  if (!tri_adjacency(F, nF, x0, nV, A)) return -1;
+  tri_forces<float>(A, nV, kstretch, kbend, kair, kd, x, v, ext, f);
+  return 0;
+}
+int yo_tri_forces_f64(const uint32_t* F, uint32_t nF, const float* x0, uint32_t nV, double kstretch,
+                      double kbend, double kair, double kd, const double* x, const double* v, const double* ext,
+                      double* f) {
+  TriAdj A;
+  if (!tri_adjacency(F, nF, x0, nV, A)) return -1;
+  tri_forces<double>(A, nV, kstretch, kbend, kair, kd, x, v, ext, f);
+  return 0;
+}
+
 // Integrate one FastGrid mesh in place given forces and a per-vertex contact
 // CSR (offsets nV+1, records ordered by (target_obj, triangle) per vertex).
 // Returns the number of vertices the world-bounds guard touched.
@@ -685,6 +835,7 @@ namespace {
 struct OMesh {
   ysim_mesh_desc d;
   uint32_t voff, foff;
+  TriAdj adj;  // TriangularCloth springs
   std::vector<uint64_t> keys;
   std::vector<uint32_t> idx;
   std::vector<ysim_bvh_node> tree;
@@ -712,7 +863,9 @@ struct OScene {
   double timing[YSIM_STAGE_COUNT] = {0};
 };
 
-bool is_dynamic(const ysim_mesh_desc& d) { return d.behavior == YSIM_BEHAVIOR_FASTGRID_CLOTH; }
+bool is_dynamic(const ysim_mesh_desc& d) {
+  return d.behavior == YSIM_BEHAVIOR_FASTGRID_CLOTH || d.behavior == YSIM_BEHAVIOR_TRIANGULAR_CLOTH;
+}
 
 void build_tree(OScene& S, OMesh& M) {
   uint32_t n = M.d.num_facets;
@@ -760,6 +913,13 @@ void* yo_scene_create(uint32_t nm, const ysim_mesh_desc* meshes, const float* x,
   S->ext.assign(ext, ext + 3 * vo);
   S->fixed.assign(fixed, fixed + vo);
   S->facets.assign(facets, facets + 3 * fo);
+  for (auto& M : S->meshes)
+    if (M.d.behavior == YSIM_BEHAVIOR_TRIANGULAR_CLOTH &&
+        !tri_adjacency(S->facets.data() + 3 * M.foff, M.d.num_facets, S->x.data() + 3 * M.voff, M.d.num_vertices,
+                       M.adj)) {
+      delete S;
+      return nullptr;
+    }
   for (auto& M : S->meshes) {
     build_tree(*S, M);
     yo_refit(S->x.data() + 3 * M.voff, S->v.data() + 3 * M.voff, 0.f, 0,
@@ -931,6 +1091,17 @@ static void dynamics(OScene& S) {
     const double* ksd = (first_cloth && !S.kspring_d.empty()) ? S.kspring_d.data() : nullptr;
     first_cloth = false;
     size_t o = M.voff;
+    if (M.d.behavior == YSIM_BEHAVIOR_TRIANGULAR_CLOTH) {
+      tri_forces<float>(M.adj, M.d.num_vertices, M.d.cloth.kstretch, M.d.cloth.kbend, P.kair, P.kd,
+                        S.x.data() + 3 * o, S.v.data() + 3 * o, S.ext.data() + 3 * o, S.f.data() + 3 * o);
+      if (S.twin) {
+        std::vector<double> extd(S.ext.begin() + 3 * o, S.ext.begin() + 3 * (o + M.d.num_vertices));
+        tri_forces<double>(M.adj, M.d.num_vertices, (double)M.d.cloth.kstretch, (double)M.d.cloth.kbend,
+                           (double)P.kair, (double)P.kd, S.xd.data() + 3 * o, S.vd.data() + 3 * o, extd.data(),
+                           S.fd.data() + 3 * o);
+      }
+      continue;
+    }
     fastgrid_forces<float>(M.d.cloth, M.d.num_vertices, P.kair, P.kd, S.x.data() + 3 * o,
                            S.v.data() + 3 * o, S.ext.data() + 3 * o, ks, S.f.data() + 3 * o);
     if (S.twin) {

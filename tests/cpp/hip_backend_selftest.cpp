@@ -106,6 +106,50 @@ int main() {
   Cfar.narrowPhase.contacts(off, c);
   report(c.empty(), "Block 13 / separated objects have no contacts");
 
+  // TriangularCloth: the same grid geometry as a general mesh (edge and
+  // opposite-vertex springs) drapes onto the sphere without anomalies
+  {
+    SymplecticSystem<HIP, float> sysT(1.0f / 60.f, 10);
+    Simulator<HIP, float> Tr(sysT);
+    const float l = 1.0f / (n - 1);
+    std::vector<float> tx;
+    std::vector<uint32_t> tf;
+    for (uint32_t r = 0; r < n; ++r)
+      for (uint32_t c = 0; c < n; ++c) tx.insert(tx.end(), {c * l, 0.8f, r * l});
+    for (uint32_t r = 0; r + 1 < n; ++r)
+      for (uint32_t c = 0; c + 1 < n; ++c) {
+        const uint32_t p00 = r * n + c, p10 = p00 + 1, p01 = p00 + n, p11 = p01 + 1;
+        tf.insert(tf.end(), {p00, p01, p11, p00, p11, p10});
+      }
+    Tr.scene.addTriangularCloth(tx, tf, 500.f, 50.f, 5.f, 0.5f * l, 1.0f);
+    std::vector<float> sx;
+    std::vector<uint32_t> sf;
+    const float sc[3] = {0.5f, 0.5f, 0.5f};
+    uv_sphere(24, 12, 0.3f, sc, sx, sf);
+    Tr.scene.addFloatMesh(sx, sf);
+    Tr.margin = 0.5f * l;
+    Tr.radius = 0.f;
+    Tr.system.kd = 0.01f;
+    Tr.initialize();
+    size_t tfired = 0;
+    for (int frame = 0; frame < 8; ++frame) {
+      Tr.update();
+      std::vector<uint32_t> toff;
+      std::vector<ysim_narrow_collision> tc;
+      Tr.narrowPhase.contacts(toff, tc);
+      tfired += tc.size();
+    }
+    std::vector<float> xt = Tr.scene.positions();
+    double ty = 0;
+    bool finite = true;
+    for (uint32_t i = 0; i < n * n; ++i) {
+      ty += xt[3 * i + 1];
+      finite = finite && std::isfinite(xt[3 * i]) && std::isfinite(xt[3 * i + 1]) && std::isfinite(xt[3 * i + 2]);
+    }
+    report(finite && ty < 0.8 * n * n && tfired > 0 && Tr.system.anomalyFlag == 0,
+           "TriangularCloth / drape falls, contacts fire, no anomaly");
+  }
+
   // RadixSorter<HIP> through VectorBase<HIP>
   std::vector<uint64_t> keys = {5, 3, 5, 1, 3, 0};
   std::vector<uint32_t> vals = {0, 1, 2, 3, 4, 5};

@@ -63,6 +63,92 @@ uint32_t bits_for(uint32_t n) {  // ceil(log2(n)) for n >= 1
   return b;
 }
 
+// Springs of a TriangularCloth, restating MeshAdjacencyInitializer::initialize
+// (include/sim/mesh_state.hpp:236-454).  Every facet (a, b, c) gives the edge
+// records {a,b} opposite c, {b,c} opposite a, {c,a} opposite b; the records
+// sorted by (min, max) index make the edge list.  A vertex's stretch partners
+// are the other ends of its edges in edge order; an edge of two facets makes
+// its two opposite vertices bend partners of each other, in edge order.  Rest
+// lengths are tinym's norm of x[hi] - x[lo] (sqrt((x*x + y*y) + z*z)), and the
+// bend length is symmetric in its ends.  Fails on a facet with a repeated
+// vertex and on an edge of three or more facets.
+struct TriSprings {
+  std::vector<uint32_t> first, mid;  // per local vertex: stretch [first, mid), bend [mid, first[+1])
+  std::vector<uint32_t> other;       // local partner
+  std::vector<float> rest;
+};
+
+static uint32_t float_bits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  return u;
+}
+
+static float tri_length(const float* x, uint32_t p, uint32_t q) {
+  const float dx = x[3 * q] - x[3 * p], dy = x[3 * q + 1] - x[3 * p + 1], dz = x[3 * q + 2] - x[3 * p + 2];
+  return std::sqrt((dx * dx + dy * dy) + dz * dz);
+}
+
+static bool tri_springs(const uint32_t* F, uint32_t nf, const float* x, uint32_t nv, TriSprings& ts) {
+  std::vector<std::pair<uint64_t, uint32_t>> rec;  // ((lo << 32) | hi, opposite vertex)
+  rec.reserve(3 * (size_t)nf);
+  for (uint32_t f = 0; f < nf; ++f) {
+    const uint32_t c[3] = {F[3 * f], F[3 * f + 1], F[3 * f + 2]};
+    if (c[0] == c[1] || c[1] == c[2] || c[2] == c[0]) return false;
+    for (int k = 0; k < 3; ++k) {
+      const uint32_t p = c[k], q = c[(k + 1) % 3];
+      rec.push_back({((uint64_t)std::min(p, q) << 32) | std::max(p, q), c[(k + 2) % 3]});
+    }
+  }
+  std::stable_sort(rec.begin(), rec.end(),
+                   [](const std::pair<uint64_t, uint32_t>& u, const std::pair<uint64_t, uint32_t>& w) {
+                     return u.first < w.first;
+                   });
+  struct Edge { uint32_t lo, hi, o0, o1; bool inner; };
+  std::vector<Edge> edges;
+  for (size_t k = 0; k < rec.size();) {
+    size_t e = k + 1;
+    while (e < rec.size() && rec[e].first == rec[k].first) ++e;
+    if (e - k > 2) return false;
+    edges.push_back({(uint32_t)(rec[k].first >> 32), (uint32_t)rec[k].first, rec[k].second,
+                     e - k == 2 ? rec[k + 1].second : 0u, e - k == 2});
+    k = e;
+  }
+  std::vector<uint32_t> ns(nv, 0), nb(nv, 0);
+  for (const Edge& e : edges) {
+    ns[e.lo]++;
+    ns[e.hi]++;
+    if (e.inner) {
+      nb[e.o0]++;
+      nb[e.o1]++;
+    }
+  }
+  ts.first.assign(nv + 1, 0);
+  ts.mid.assign(nv, 0);
+  for (uint32_t i = 0; i < nv; ++i) {
+    ts.mid[i] = ts.first[i] + ns[i];
+    ts.first[i + 1] = ts.mid[i] + nb[i];
+  }
+  ts.other.assign(ts.first[nv], 0);
+  ts.rest.assign(ts.first[nv], 0.f);
+  std::vector<uint32_t> ps(ts.first.begin(), ts.first.end() - 1), pb(ts.mid);
+  for (const Edge& e : edges) {
+    const float r = tri_length(x, e.lo, e.hi);
+    ts.other[ps[e.lo]] = e.hi;
+    ts.rest[ps[e.lo]++] = r;
+    ts.other[ps[e.hi]] = e.lo;
+    ts.rest[ps[e.hi]++] = r;
+    if (e.inner) {
+      const float rb = tri_length(x, e.o0, e.o1);
+      ts.other[pb[e.o0]] = e.o1;
+      ts.rest[pb[e.o0]++] = rb;
+      ts.other[pb[e.o1]] = e.o0;
+      ts.rest[pb[e.o1]++] = rb;
+    }
+  }
+  return true;
+}
+
 struct BuildGroup {
   std::vector<uint32_t> meshes;
   SegTable tsegs, vsegs;
@@ -94,6 +180,8 @@ struct ysim_scene {
   MeshTables mt;
   ysim_fastgrid_params* d_cloth = nullptr;
   uint32_t *d_nfacets = nullptr, *d_behavior = nullptr, *d_shape = nullptr, *d_grid_n = nullptr;
+  uint32_t *tri_off = nullptr, *tri_mid = nullptr;  // TriangularCloth springs (tri_force)
+  uint2* tri_ent = nullptr;
   float *d_margin = nullptr, *d_band = nullptr;
   ysim_bvh_node* tree = nullptr;
   int32_t* parent = nullptr;
@@ -386,7 +474,7 @@ struct ysim_scene {
     YSIM_CHECK_LAUNCH();
     return YSIM_OK;
   }
-  bool dynamic(uint32_t m) const { return meshes[m].behavior == YSIM_BEHAVIOR_FASTGRID_CLOTH; }
+  bool dynamic(uint32_t m) const { return meshes[m].behavior != YSIM_BEHAVIOR_FLOAT; }
 
   int make_seg(const std::vector<uint32_t>& ms, bool tris, SegTable& out) {
     std::vector<uint32_t> off(1, 0);
@@ -551,6 +639,10 @@ struct ysim_scene {
     ca.dsegs = dyn_vsegs;
     ca.mt = mt;
     ca.cloth = d_cloth;
+    ca.behavior = d_behavior;
+    ca.tri_off = tri_off;
+    ca.tri_mid = tri_mid;
+    ca.tri_ent = tri_ent;
     ca.kspring = kspring;
     ca.subh = P.subh;
     ca.kair = P.kair;
@@ -1023,12 +1115,16 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
   S->noff.assign(num_meshes + 1, 0);
   for (uint32_t m = 0; m < num_meshes; ++m) {
     const ysim_mesh_desc& d = S->meshes[m];
-    YSIM_ARG(d.behavior == YSIM_BEHAVIOR_FLOAT || d.behavior == YSIM_BEHAVIOR_FASTGRID_CLOTH,
+    YSIM_ARG(d.behavior == YSIM_BEHAVIOR_FLOAT || d.behavior == YSIM_BEHAVIOR_FASTGRID_CLOTH ||
+                 d.behavior == YSIM_BEHAVIOR_TRIANGULAR_CLOTH,
              "scene_create: unsupported behavior");
     if (d.behavior == YSIM_BEHAVIOR_FASTGRID_CLOTH) {
       YSIM_ARG(d.cloth.particle_num_1d >= 3 &&
                    (uint64_t)d.cloth.particle_num_1d * d.cloth.particle_num_1d == d.num_vertices,
                "scene_create: FastGrid cloth needs num_vertices = N^2, N >= 3");
+      S->dyn.push_back(m);
+    } else if (d.behavior == YSIM_BEHAVIOR_TRIANGULAR_CLOTH) {
+      YSIM_ARG(d.num_facets > 0, "scene_create: a triangular cloth needs facets");
       S->dyn.push_back(m);
     } else {
       S->stat.push_back(m);
@@ -1075,6 +1171,38 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
   YSIM_TRY(s.put(&s.d_cloth, cloth));
   YSIM_TRY(s.put(&s.d_nfacets, nf));
   YSIM_TRY(s.put(&s.d_behavior, beh));
+  {
+    // TriangularCloth springs: per global vertex its stretch partners, then
+    // its bend partners (global index, rest length), in the reference's order
+    std::vector<uint32_t> toff(s.Vtot + 1, 0u), tmid(s.Vtot, 0u);
+    std::vector<uint2> tent;
+    for (uint32_t m = 0; m < num_meshes; ++m) {
+      const uint32_t vb = s.voff[m], nv = s.meshes[m].num_vertices;
+      TriSprings ts;
+      if (s.meshes[m].behavior == YSIM_BEHAVIOR_TRIANGULAR_CLOTH) {
+        YSIM_ARG(tri_springs(h_facets + 3 * (size_t)s.foff[m], s.meshes[m].num_facets, h_x + 3 * (size_t)vb, nv, ts),
+                 "scene_create: a triangular cloth needs a manifold mesh without repeated vertices in a facet");
+      }
+      for (uint32_t i = 0; i < nv; ++i) {
+        toff[vb + i] = (uint32_t)tent.size();
+        if (ts.first.empty()) {
+          tmid[vb + i] = (uint32_t)tent.size();
+          continue;
+        }
+        for (uint32_t k = ts.first[i]; k < ts.mid[i]; ++k)
+          tent.push_back(make_uint2(vb + ts.other[k], float_bits(ts.rest[k])));
+        tmid[vb + i] = (uint32_t)tent.size();
+        for (uint32_t k = ts.mid[i]; k < ts.first[i + 1]; ++k)
+          tent.push_back(make_uint2(vb + ts.other[k], float_bits(ts.rest[k])));
+      }
+    }
+    toff[s.Vtot] = (uint32_t)tent.size();
+    if (!tent.empty()) {
+      YSIM_TRY(s.put(&s.tri_off, toff));
+      YSIM_TRY(s.put(&s.tri_mid, tmid));
+      YSIM_TRY(s.put(&s.tri_ent, tent));
+    }
+  }
   {
     std::vector<uint32_t> gn(num_meshes, 0u);
     for (uint32_t m = 0; m < num_meshes; ++m) {
@@ -1537,6 +1665,8 @@ int adj_check(ysim_scene* s) {
   if (s->cap > 64) return fail(YSIM_ERR_UNSUPPORTED, "adjoint: contact_capacity above 64");
   if (s->stage_phase != 0) return fail(YSIM_ERR_STATE, "adjoint: a staged substep is in progress");
   const ysim_mesh_desc& d = s->meshes[s->dyn[0]];
+  if (d.behavior != YSIM_BEHAVIOR_FASTGRID_CLOTH)
+    return fail(YSIM_ERR_UNSUPPORTED, "adjoint: the cloth must be a FastGrid cloth");
   if (d.cloth.particle_num_1d < 3 || (uint64_t)d.cloth.particle_num_1d * d.cloth.particle_num_1d != d.num_vertices)
     return fail(YSIM_ERR_INVALID, "adjoint: cloth is not an N x N grid with N >= 3");
   return YSIM_OK;

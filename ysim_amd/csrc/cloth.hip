@@ -1,5 +1,7 @@
-// cloth.hip — FastGrid springs + symplectic integrate + contact projection,
-// fused into one pass per vertex.
+// cloth.hip — FastGrid or TriangularCloth springs + symplectic integrate +
+// contact projection, fused into one pass per vertex.  integrate_cloth
+// (physics.metal:380-466, TriangularCloth) applies the same update as
+// integrate_cloth_grid.
 //
 // compute_cloth_grid_forces_fast (src/metal/physics.metal:171-224) gathers 12
 // grid springs; integrate_cloth_grid (physics.metal:226-312) then updates v,
@@ -26,7 +28,10 @@ __global__ __launch_bounds__(BS) void k_cloth(ClothArgs a) {
   const ysim_fastgrid_params P = a.cloth[m];
   const f3 pos0 = ld3(a.x, g), vel0 = ld3(a.v, g);
   const float* KS = (a.kspring && sg == 0) ? a.kspring : nullptr;
-  const f3 force = grid_force(P, i, vb, a.x, a.v, pos0, vel0, ld3(a.ext, g), a.kair, a.kd, KS);
+  const f3 force = (a.behavior && a.behavior[m] == YSIM_BEHAVIOR_TRIANGULAR_CLOTH)
+                       ? tri_force(P, g, a.tri_off, a.tri_mid, a.tri_ent, a.x, a.v, pos0, vel0, ld3(a.ext, g), a.kair,
+                                   a.kd)
+                       : grid_force(P, i, vb, a.x, a.v, pos0, vel0, ld3(a.ext, g), a.kair, a.kd, KS);
   if (a.f_out) st3(a.f_out, g, force);
 
   // integrate_cloth_grid

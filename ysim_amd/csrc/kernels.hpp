@@ -179,10 +179,34 @@ __device__ __forceinline__ f3 grid_force(const ysim_fastgrid_params& P, uint32_t
   return force;
 }
 
+// compute_tri_spring_forces (src/metal/physics.metal:319-378) on global vertex
+// g: ext + kair v, then the stretch partners [off[g], mid[g]) with k_stretch,
+// then the bend partners [mid[g], off[g+1]) with k_bend; each entry is
+// (global partner, rest length bits).
+__device__ __forceinline__ f3 tri_force(const ysim_fastgrid_params& P, uint32_t g, const uint32_t* off,
+                                        const uint32_t* mid, const uint2* ent, const float* X, const float* V,
+                                        f3 pos0, f3 vel0, f3 ext, float kair, float kd) {
+  f3 force = ext + vel0 * kair;
+  const uint32_t b = off[g], mdl = mid[g], e = off[g + 1];
+  for (uint32_t k = b; k < mdl; ++k) {
+    const uint2 q = ent[k];
+    force = force + calc_spring(pos0, vel0, ld3(X, q.x), ld3(V, q.x), __uint_as_float(q.y), P.kstretch, kd);
+  }
+  for (uint32_t k = mdl; k < e; ++k) {
+    const uint2 q = ent[k];
+    force = force + calc_spring(pos0, vel0, ld3(X, q.x), ld3(V, q.x), __uint_as_float(q.y), P.kbend, kd);
+  }
+  return force;
+}
+
 struct ClothArgs {
   SegTable dsegs;                     // dynamic vertex segments (== query segments)
   MeshTables mt;
   const ysim_fastgrid_params* cloth = nullptr;  // per mesh
+  const uint32_t* behavior = nullptr; // per mesh (null: every mesh is a FastGrid cloth)
+  const uint32_t* tri_off = nullptr;  // TriangularCloth springs per global vertex (tri_force)
+  const uint32_t* tri_mid = nullptr;
+  const uint2* tri_ent = nullptr;
   const float* kspring = nullptr;     // optional per-spring k of the first cloth
   float subh = 0, kair = 0, kd = 0;
   const float* x = nullptr;

@@ -43,7 +43,7 @@ template <typename BE, typename PR, typename LEVEL, typename PRIM> struct BVH;
 template <typename BE, typename PR> struct BruteForce;
 template <typename BE, typename PR> struct SymplecticSystem;
 template <typename BE, typename PR> struct Simulator;
-enum class BehaviorType { Float = 0, FastGridCloth = 1 };
+enum class BehaviorType { Float = 0, FastGridCloth = 1, TriangularCloth = 2 };
 #endif
 
 struct HIP : Backend {};
@@ -241,6 +241,24 @@ struct Scene<HIP, PR> {
     m.cloth.thickness = thickness;
     meshes.push_back(std::move(m));
   }
+  // A TriangularCloth from any manifold triangle mesh (MeshFile / general mesh
+  // initializers): ClothBehaviorParams {stretch, shear, bend, thickness}
+  // (collision_pod.hpp:350-352); the library builds the edge and
+  // opposite-vertex springs (MeshAdjacencyInitializer, mesh_state.hpp:236-454)
+  // with rest lengths from x.
+  void addTriangularCloth(std::vector<float> x, std::vector<uint32_t> facets, float stretch, float shear, float bend,
+                          float thickness, float mass) {
+    Mesh m;
+    m.behaviorType = BehaviorType::TriangularCloth;
+    m.x = std::move(x);
+    m.facets = std::move(facets);
+    m.mass = mass;
+    m.cloth.kstretch = stretch;
+    m.cloth.kshear = shear;
+    m.cloth.kbend = bend;
+    m.cloth.thickness = thickness;
+    meshes.push_back(std::move(m));
+  }
   // A static triangle obstacle (BehaviorType::Float, ColliderKind::Mesh).
   void addFloatMesh(std::vector<float> x, std::vector<uint32_t> facets) {
     Mesh m;
@@ -261,7 +279,9 @@ struct Scene<HIP, PR> {
       d[i] = ysim_mesh_desc{};
       d[i].num_vertices = nv;
       d[i].num_facets = (uint32_t)(m.facets.size() / 3);
-      d[i].behavior = m.behaviorType == BehaviorType::FastGridCloth ? YSIM_BEHAVIOR_FASTGRID_CLOTH : YSIM_BEHAVIOR_FLOAT;
+      d[i].behavior = m.behaviorType == BehaviorType::FastGridCloth     ? YSIM_BEHAVIOR_FASTGRID_CLOTH
+                      : m.behaviorType == BehaviorType::TriangularCloth ? YSIM_BEHAVIOR_TRIANGULAR_CLOTH
+                                                                        : YSIM_BEHAVIOR_FLOAT;
       d[i].collidable = m.collidable ? 1u : 0u;
       d[i].group = m.group;
       d[i].cloth = FastGridClothBehavior<HIP, PR>::params(m);
@@ -271,7 +291,7 @@ struct Scene<HIP, PR> {
       fixed.insert(fixed.end(), nv, 1.f);
       for (uint32_t k = 0; k < nv; ++k)  // applyEnvironmentForces: ext = g * m (simulator.hpp:3148-3164)
         for (int a = 0; a < 3; ++a)
-          ext.push_back(d[i].behavior == YSIM_BEHAVIOR_FASTGRID_CLOTH ? gravity[a] * m.mass : 0.f);
+          ext.push_back(d[i].behavior != YSIM_BEHAVIOR_FLOAT ? gravity[a] * m.mass : 0.f);
       facets.insert(facets.end(), m.facets.begin(), m.facets.end());
     }
     ysim_hip_detail::check(ysim_hip_scene_create(ysim_hip_detail::global_ctx(), (uint32_t)d.size(), d.data(), x.data(),

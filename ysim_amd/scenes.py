@@ -13,7 +13,9 @@ HIP path and the CPU oracle):
   * optional fold of the grid along x = 0.5 into two layers `gap` apart;
   * UV sphere with independent slice and stack counts (the reference's
     primitive is t x t, include/primitive_geometry.hpp:57-140);
-  * icosphere of a given subdivision level.
+  * icosphere of a given subdivision level;
+  * triangulated disk (rings of 6r vertices, zipped by angle): an irregular
+    manifold mesh for TriangularCloth.
 Rest lengths follow recomputeFastGridRest (include/sim/collision_pod.hpp:418-442).
 """
 from __future__ import annotations
@@ -29,6 +31,7 @@ GRAVITY = (0.0, -9.81, 0.0)  # SceneEnvironment.gravity, include/sim/scene.hpp:1
 
 BEHAVIOR_FLOAT = 0
 BEHAVIOR_FASTGRID_CLOTH = 1
+BEHAVIOR_TRIANGULAR_CLOTH = 2
 
 
 @dataclasses.dataclass
@@ -61,7 +64,7 @@ class Scene:
             vs.append(np.zeros((V, 3), F32))
             mass = np.full(V, m.mass, F32)
             ms.append(mass)
-            if m.behavior == BEHAVIOR_FASTGRID_CLOTH:
+            if m.behavior != BEHAVIOR_FLOAT:
                 # applyEnvironmentForces: ext = gravity * m (simulator.hpp:3148-3164)
                 es.append((g[None, :] * mass[:, None]).astype(F32))
             else:
@@ -186,6 +189,56 @@ def cloth_mesh(n: int, size: float, origin, rng, k=(500.0, 50.0, 5.0), mass=1.0,
                 mass=mass, group=group)
 
 
+def disk_mesh(rings: int, radius: float, center, rng=None, jitter_rel: float = 1e-3) -> tuple:
+    """XZ-plane disk: a centre vertex and rings r = 1..rings of 6r vertices at
+    radius r/rings * radius, consecutive rings zipped into triangles in angle
+    order (valences vary, unlike a grid).  V = 1 + 3 R (R + 1), F = 6 R^2."""
+    cx, cy, cz = (F32(c) for c in center)
+    pts = [(cx, cy, cz)]
+    rings_idx = [[0]]
+    ang = [[0.0]]
+    h = F32(radius) / F32(rings)
+    for r in range(1, rings + 1):
+        n = 6 * r
+        a = [2.0 * math.pi * k / n for k in range(n)]
+        rr = F32(r) * h
+        idx = []
+        for k in range(n):
+            idx.append(len(pts))
+            pts.append((cx + rr * F32(math.cos(a[k])), cy, cz + rr * F32(math.sin(a[k]))))
+        rings_idx.append(idx)
+        ang.append(a)
+    x = np.array(pts, dtype=F32)
+    if rng is not None and jitter_rel > 0:
+        x[1:, [0, 2]] += (rng.uniform(-1, 1, (x.shape[0] - 1, 2)) * (jitter_rel * float(h))).astype(F32)
+    fac = []
+    for k in range(6):  # centre fan
+        fac.append((0, rings_idx[1][k], rings_idx[1][(k + 1) % 6]))
+    for r in range(2, rings + 1):
+        inner, outer, ai, ao = rings_idx[r - 1], rings_idx[r], ang[r - 1], ang[r]
+        i = j = 0
+        ni, no = len(inner), len(outer)
+        while i < ni or j < no:
+            a_next_i = ai[i + 1] if i + 1 < ni else 2.0 * math.pi
+            a_next_o = ao[j + 1] if j + 1 < no else 2.0 * math.pi
+            if j < no and (i >= ni or a_next_o <= a_next_i):
+                fac.append((inner[i % ni], outer[j], outer[(j + 1) % no]))
+                j += 1
+            else:
+                fac.append((inner[i % ni], outer[j % no], inner[(i + 1) % ni]))
+                i += 1
+    return x, np.array(fac, dtype=np.uint32)
+
+
+def tri_cloth_mesh(x, facets, k=(500.0, 50.0, 5.0), mass=1.0, thickness=0.01, group=0) -> Mesh:
+    """TriangularCloth: springs along the edges (k[0]) and across interior
+    edges (k[2]); k[1] (shear) is carried but unused, as in
+    compute_tri_spring_forces."""
+    cloth = dict(kstretch=float(k[0]), kshear=float(k[1]), kbend=float(k[2]), thickness=float(thickness))
+    return Mesh(x=np.ascontiguousarray(x, dtype=F32), facets=np.ascontiguousarray(facets, dtype=np.uint32),
+                behavior=BEHAVIOR_TRIANGULAR_CLOTH, cloth=cloth, mass=mass, group=group)
+
+
 def uv_sphere(slices: int, stacks: int, radius: float, center) -> tuple:
     """UV sphere: poles + (stacks-1) rings of `slices` vertices.
 
@@ -307,6 +360,21 @@ def config_drape(seed: int = 3, n: int = 32, height=0.82, subh=1 / 600, bits=30,
     sx, sf = uv_sphere(slices, stacks, radius, (0.5, 0.5, 0.5))
     p = sim_params(subh=subh, margin=0.5 * l, morton_bits=bits, rebuild_every=rebuild_every)
     return Scene([cl, obstacle(sx, sf)], p, name=f"drape-{n}")
+
+
+def config_tri_drape(seed: int = 6, rings: int = 24, radius: float = 0.45, height: float = 0.83,
+                     subh=1 / 600, bits=30, rebuild_every=1, slices=24, stacks=12, sphere_radius=0.3):
+    """TriangularCloth disk (rings of 6r vertices) just above a UV sphere, self
+    collision on; thickness = margin = band = half the ring spacing; 32
+    contact slots per vertex."""
+    rng = np.random.default_rng(seed)
+    x, f = disk_mesh(rings, radius, (0.5, height, 0.5), rng)
+    h = radius / rings
+    cl = tri_cloth_mesh(x, f, thickness=0.5 * h)
+    sx, sf = uv_sphere(slices, stacks, sphere_radius, (0.5, 0.5, 0.5))
+    # the disk centre lands on the sphere's pole, where 24 facets meet
+    p = sim_params(subh=subh, margin=0.5 * h, morton_bits=bits, rebuild_every=rebuild_every, contact_capacity=32)
+    return Scene([cl, obstacle(sx, sf)], p, name=f"tri-drape-{rings}")
 
 
 def config_c3(seed: int = 3, count: int = 256, lo=16, hi=256, mass_mode="per_vertex", substeps=50):

@@ -290,6 +290,51 @@ int ysim_hip_scene_restore(ysim_scene* scene);
  * query, out[1] substeps run, out[2] contacts of the last substep. */
 int ysim_hip_scene_counters(ysim_scene* scene, uint64_t* h_out, uint32_t reset);
 
+/* ---- analytic colliders -----------------------------------------------------
+ * ColliderKind (include/sim/collision_pod.hpp:32, common_types.metalh:80-84):
+ * a mesh whose collider kind is not Mesh answers contacts from its shape
+ * parameters instead of its triangles. */
+enum {
+  YSIM_COLLIDER_MESH = 0,
+  YSIM_COLLIDER_SPHERE = 1,    /* ellipsoid with semi-axes half_ext_height.xyz (a sphere reduces exactly) */
+  YSIM_COLLIDER_BOX = 2,       /* half-extents half_ext_height.xyz */
+  YSIM_COLLIDER_CYLINDER = 3,  /* local +Y axis, radius center_radius.w, half-height half_ext_height.w */
+  YSIM_COLLIDER_PLANE = 4      /* infinite half-space: outside is local +Y */
+};
+/* AnalyticShape (include/sim/collision_pod.hpp:285-310, the metal mirror
+ * common_types.metalh:42-52), 96 B.  Quaternions are (w, x, y, z), local to
+ * world; prev_* hold the transform at the start of the substep. */
+typedef struct ysim_analytic_shape {
+  float center_radius[4];    /* xyz world centre, w radius (cylinder) */
+  float half_ext_height[4];  /* xyz half-extents / semi-axes, w half-height (cylinder) */
+  float rot_quat[4];
+  float prev_center_pad[4];  /* xyz previous centre, w unused */
+  float prev_rot_quat[4];
+  uint32_t kind;             /* YSIM_COLLIDER_SPHERE .. YSIM_COLLIDER_PLANE */
+  uint32_t obj_index;        /* the collider's mesh (array index); must be a FLOAT mesh whose
+                              * desc.shape equals kind; at most one shape per mesh */
+  uint32_t behavior_type;    /* must equal the mesh's behavior */
+  uint32_t flags;            /* bit0 collidable: 0 keeps the mesh out of the triangle path
+                              * without giving contacts */
+} ysim_analytic_shape;
+/* Install (or replace) the scene's analytic colliders: the BVH broad phase's
+ * analytic branch (BroadPhase::detectCollisions, include/sim/bvh.hpp:2854-2949,
+ * objectBroadAABB 2486-2518) and BruteForce::narrowAnalytic /
+ * narrow_pt_analytic (include/sim/bruteforce.hpp:192-308,
+ * src/metal/bruteforce.metal:198-499).  Each substep a cloth (query mesh of
+ * the same group, collidable) is paired with each shape whose mesh is
+ * collidable when the cloth's tree root box meets the shape's swept box padded
+ * by the cloth's narrow band plus the rotation band (a plane always pairs),
+ * or when it did at the previous substep; every vertex of a paired cloth then
+ * runs the swept motion-inverted test against the shape and keeps one contact
+ * (target_obj = obj_index, triangle = 0).  The shapes' meshes leave the
+ * triangle traversal.  A call with the same obj_index sequence updates poses
+ * (moving colliders: pass the previous pose in prev_*) and keeps the
+ * previous-substep pairing; another sequence resets it; n = 0 removes them
+ * all.  With broad_mode 1 (spatial hash) the shapes are stored but inactive
+ * and the meshes collide through their triangles, as in the reference. */
+int ysim_hip_scene_set_analytic(ysim_scene* scene, const ysim_analytic_shape* h_shapes, uint32_t n);
+
 /* ---- measurement ------------------------------------------------------------
  * No counterpart in the reference.  Achievable HBM bandwidth: a plain
  * 16-byte-vector copy kernel ("k_copy_u4") of `bytes` from one device buffer to

@@ -81,6 +81,11 @@ def lib():
         "yo_scene_get_contacts": (None, [vp, u32p, vp]),
         "yo_scene_get_tree": (None, [vp, C.c_uint32, vp, C.POINTER(C.c_int32), u64p, u32p]),
         "yo_scene_anomaly": (C.c_uint32, [vp]),
+        "yo_scene_set_analytic": (C.c_int, [vp, vp, C.c_uint32]),
+        "yo_scene_analytic_markers": (C.c_uint32, [vp, u32p, u8p]),
+        "yo_narrow_analytic_f32": (C.c_int, [vp, f32p, f32p, C.c_float, f32p]),
+        "yo_narrow_analytic_f64": (C.c_int, [vp, f64p, f64p, C.c_double, f64p]),
+        "yo_analytic_bounds": (None, [vp, C.c_float, C.c_float, f32p, f32p, f32p, f64p]),
         "yo_adj_create": (vp, [C.c_uint32, vp, vp, f32p, f32p, f32p, f32p, f32p, C.c_uint32, f32p, u32p,
                                u32p]),
         "yo_adj_destroy": (None, [C.c_uint32, vp]),
@@ -252,6 +257,31 @@ def spring_count(n: int) -> int:
 
 
 # ------------------------------------------------------------------ scene
+def narrow_analytic(shape, pcur, pprev, band, f64: bool = False):
+    """narrow_pt_analytic for one vertex: (normal, d) or None."""
+    sh = np.ascontiguousarray(shape, abi.ANALYTIC).reshape(1)
+    t, fn, ct = (np.float64, lib().yo_narrow_analytic_f64, f64p) if f64 else \
+        (np.float32, lib().yo_narrow_analytic_f32, f32p)
+    a = np.ascontiguousarray(pcur, t).reshape(3)
+    b = np.ascontiguousarray(pprev, t).reshape(3)
+    out = np.zeros(4, t)
+    if not fn(sh.ctypes.data, P(a, ct), P(b, ct), band, P(out, ct)):
+        return None
+    return out[:3], out[3]
+
+
+def analytic_bounds(shape, radius=0.0, thickness=0.0):
+    """Unpadded swept box (lo, hi), broad rotation band, narrow rotation margin,
+    narrow (theta, rmax)."""
+    sh = np.ascontiguousarray(shape, abi.ANALYTIC).reshape(1)
+    lo = np.zeros(3, np.float32)
+    hi = np.zeros(3, np.float32)
+    o2 = np.zeros(2, np.float32)
+    t2 = np.zeros(2, np.float64)
+    lib().yo_analytic_bounds(sh.ctypes.data, radius, thickness, P(lo, f32p), P(hi, f32p), P(o2, f32p), P(t2, f64p))
+    return lo, hi, float(o2[0]), float(o2[1]), (float(t2[0]), float(t2[1]))
+
+
 class OracleScene:
     """CPU twin of the HIP scene: same packed inputs, same substep."""
 
@@ -273,6 +303,22 @@ class OracleScene:
         if twin:
             L.yo_scene_enable_twin(self.h)
         self.V = scene.num_vertices
+        if scene.analytic is not None and len(scene.analytic):
+            self.set_analytic(scene.analytic)
+
+    def set_analytic(self, shapes):
+        a = np.ascontiguousarray(shapes, abi.ANALYTIC).reshape(-1)
+        if lib().yo_scene_set_analytic(self.h, a.ctypes.data if a.size else None, a.size) != 0:
+            raise ValueError("oracle: analytic shapes rejected")
+
+    def analytic_markers(self):
+        """(cloth mesh, shape mesh) pairs and whether each was paired at the last substep."""
+        L = lib()
+        n = L.yo_scene_analytic_markers(self.h, None, None)
+        pairs = np.zeros(2 * n, np.uint32)
+        marked = np.zeros(n, np.uint8)
+        L.yo_scene_analytic_markers(self.h, P(pairs, u32p), P(marked, u8p))
+        return pairs.reshape(-1, 2), marked.astype(bool)
 
     def __del__(self):
         try:

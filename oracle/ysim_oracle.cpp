@@ -456,6 +456,300 @@ void tri_forces(const TriAdj& A, uint32_t nV, R kstretch, R kbend, R kair, R kd,
   }
 }
 
+// ---------------------------------------------------------------- analytic colliders
+// quatRotate, src/metal/bruteforce.metal:198-200: v + 2 cross(qv, cross(qv, v) + qw v).
+template <class R>
+inline V3<R> quat_rotate(R qw, V3<R> qv, V3<R> v) {
+  return v + (R)2 * cross(qv, cross(qv, v) + qw * v);
+}
+template <class R>
+inline R length_(V3<R> v) { return sqrt_(dot(v, v)); }
+// normalize = v * rsqrt(dot(v, v)), restated with a correctly rounded
+// reciprocal square root (1 / sqrt).
+template <class R>
+inline V3<R> normalize_(V3<R> v) { return v * ((R)1 / sqrt_(dot(v, v))); }
+template <class R>
+inline R comp(V3<R> v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+// boxDominantAxis, src/metal/bruteforce.metal:209-217: the axis maximising
+// |q_i| - h_i (first on ties) and the side of q on it.
+template <class R>
+inline void box_dominant_axis(V3<R> q, V3<R> h, int& axis, R& sg) {
+  const R gx = std::fabs(q.x) - h.x, gy = std::fabs(q.y) - h.y, gz = std::fabs(q.z) - h.z;
+  axis = 0;
+  R best = gx;
+  if (gy > best) { best = gy; axis = 1; }
+  if (gz > best) { best = gz; axis = 2; }
+  sg = comp(q, axis) >= (R)0 ? (R)1 : (R)-1;
+}
+
+// narrow_pt_analytic, src/metal/bruteforce.metal:252-499: motion inversion of
+// the collider into the vertex segment a -> b (local frame), the contact
+// feature from the entry point (or a when a is already inside: entry-side
+// rule; else the closest feature of b), the current distance d along the
+// feature normal, gate d < band, normal rotated back with the current pose.
+template <class R>
+bool narrow_analytic(const ysim_analytic_shape& sh, V3<R> pcur, V3<R> pprev, R band, V3<R>& n_out, R& d_out) {
+  const float *C = sh.center_radius, *H = sh.half_ext_height, *Q = sh.rot_quat, *PC = sh.prev_center_pad,
+              *PQ = sh.prev_rot_quat;
+  const V3<R> a = quat_rotate<R>((R)PQ[0], V3<R>{-(R)PQ[1], -(R)PQ[2], -(R)PQ[3]},
+                                 pprev - V3<R>{(R)PC[0], (R)PC[1], (R)PC[2]});
+  const V3<R> b = quat_rotate<R>((R)Q[0], V3<R>{-(R)Q[1], -(R)Q[2], -(R)Q[3]},
+                                 pcur - V3<R>{(R)C[0], (R)C[1], (R)C[2]});
+  const V3<R> seg = b - a;
+  V3<R> nl{(R)0, (R)1, (R)0};
+  R d = (R)1e30f;
+  if (sh.kind == YSIM_COLLIDER_PLANE) {
+    d = b.y;  // outside is local +Y; s = (b.x, 0, b.z)
+  } else if (sh.kind == YSIM_COLLIDER_SPHERE) {
+    const V3<R> e{std::fmax((R)H[0], (R)1e-8f), std::fmax((R)H[1], (R)1e-8f), std::fmax((R)H[2], (R)1e-8f)};
+    const R Rr = (R)1 + band / std::fmin(e.x, std::fmin(e.y, e.z));
+    const V3<R> A{a.x / e.x, a.y / e.y, a.z / e.z}, B{b.x / e.x, b.y / e.y, b.z / e.z};
+    const R lenA = length_(A);
+    V3<R> u{(R)0, (R)1, (R)0};
+    bool got = false;
+    if (lenA < (R)1) {
+      u = (lenA > (R)1e-6f) ? A : ((length_(B) > (R)1e-6f) ? B : V3<R>{(R)0, (R)1, (R)0});
+      got = true;
+    } else {
+      const V3<R> D = B - A;
+      const R qa = dot(D, D), qb = dot(A, D), qc = dot(A, A) - Rr * Rr;
+      R t = (R)-1;
+      if (qa > (R)1e-20f) {
+        const R disc = qb * qb - qa * qc;
+        if (disc >= (R)0) {
+          const R t0 = (-qb - sqrt_(disc)) / qa;
+          if (t0 >= (R)0 && t0 <= (R)1) t = t0;
+        }
+      }
+      if (t >= (R)0) {
+        u = A + t * D;
+        got = true;
+      } else if (length_(B) > (R)1e-6f) {
+        u = B;
+        got = true;
+      }
+    }
+    if (!got) return false;
+    const V3<R> uh = normalize_(u);
+    const V3<R> s{e.x * uh.x, e.y * uh.y, e.z * uh.z};
+    nl = normalize_(V3<R>{uh.x / e.x, uh.y / e.y, uh.z / e.z});
+    d = dot(nl, b - s);
+  } else if (sh.kind == YSIM_COLLIDER_BOX) {
+    const V3<R> h{std::fmax((R)H[0], (R)1e-8f), std::fmax((R)H[1], (R)1e-8f), std::fmax((R)H[2], (R)1e-8f)};
+    const V3<R> HB{h.x + band, h.y + band, h.z + band};
+    int k = 0;
+    R sg = (R)1;
+    if (std::fabs(a.x) < h.x && std::fabs(a.y) < h.y && std::fabs(a.z) < h.z) {
+      box_dominant_axis(a, h, k, sg);
+    } else {
+      R tenter = (R)0, texit = (R)1;
+      int ek = -1;
+      R esg = (R)1;
+      bool miss = false;
+      for (int i = 0; i < 3 && !miss; ++i) {
+        const R di = comp(seg, i), ai = comp(a, i), Hi = comp(HB, i);
+        if (std::fabs(di) < (R)1e-12f) {
+          if (std::fabs(ai) > Hi) miss = true;
+        } else {
+          const R inv = (R)1 / di;
+          const R t1 = (-Hi - ai) * inv, t2 = (Hi - ai) * inv;
+          const R tn = std::fmin(t1, t2), tf = std::fmax(t1, t2);
+          if (tn > tenter) {
+            tenter = tn;
+            ek = i;
+            esg = (di > (R)0) ? (R)-1 : (R)1;
+          }
+          texit = std::fmin(texit, tf);
+        }
+      }
+      if (!miss && ek >= 0 && tenter <= texit && tenter <= (R)1) {
+        k = ek;
+        sg = esg;
+      } else {
+        box_dominant_axis(b, h, k, sg);
+      }
+    }
+    nl = V3<R>{k == 0 ? sg : (R)0, k == 1 ? sg : (R)0, k == 2 ? sg : (R)0};
+    d = sg * comp(b, k) - comp(h, k);
+  } else if (sh.kind == YSIM_COLLIDER_CYLINDER) {
+    const R r = std::fmax((R)C[3], (R)1e-8f), hh = std::fmax((R)H[3], (R)1e-8f);
+    const R Rr = r + band, HH = hh + band;
+    int feat = 0;
+    R capsg = (R)1, nx = (R)1, nz = (R)0;
+    const bool ain = (sqrt_(a.x * a.x + a.z * a.z) < r) && (std::fabs(a.y) < hh);
+    bool resolved = false;
+    if (!ain) {
+      R tenter = (R)0, texit = (R)1;
+      int ef = -1;
+      R ecap = (R)1, ex = (R)1, ez = (R)0;
+      bool miss = false;
+      const R qa = seg.x * seg.x + seg.z * seg.z, qb = a.x * seg.x + a.z * seg.z,
+              qc = (a.x * a.x + a.z * a.z) - Rr * Rr;
+      if (qa > (R)1e-20f) {
+        const R disc = qb * qb - qa * qc;
+        if (disc < (R)0) {
+          miss = true;
+        } else {
+          const R sq = sqrt_(disc);
+          const R t1 = (-qb - sq) / qa, t2 = (-qb + sq) / qa;
+          if (t1 > tenter) {
+            tenter = t1;
+            ef = 0;
+          }
+          texit = std::fmin(texit, t2);
+        }
+      } else if (qc > (R)0) {
+        miss = true;
+      }
+      if (!miss) {
+        if (std::fabs(seg.y) < (R)1e-12f) {
+          if (std::fabs(a.y) > HH) miss = true;
+        } else {
+          const R inv = (R)1 / seg.y;
+          const R t1 = (-HH - a.y) * inv, t2 = (HH - a.y) * inv;
+          const R tn = std::fmin(t1, t2), tf = std::fmax(t1, t2);
+          if (tn > tenter) {
+            tenter = tn;
+            ef = 1;
+            ecap = (seg.y > (R)0) ? (R)-1 : (R)1;
+          }
+          texit = std::fmin(texit, tf);
+        }
+      }
+      if (!miss && ef >= 0 && tenter <= texit && tenter <= (R)1) {
+        if (ef == 0) {
+          const R hx = a.x + tenter * seg.x, hz = a.z + tenter * seg.z;
+          const R l = sqrt_(hx * hx + hz * hz);
+          if (l > (R)1e-9f) {
+            ex = hx / l;
+            ez = hz / l;
+          }
+        }
+        feat = ef;
+        capsg = ecap;
+        nx = ex;
+        nz = ez;
+        resolved = true;
+      }
+    }
+    if (!resolved) {
+      const V3<R> q = ain ? a : b;
+      const R lxz = sqrt_(q.x * q.x + q.z * q.z);
+      feat = ((r - lxz) <= (hh - std::fabs(q.y))) ? 0 : 1;
+      capsg = (q.y >= (R)0) ? (R)1 : (R)-1;
+      if (lxz > (R)1e-9f) {
+        nx = q.x / lxz;
+        nz = q.z / lxz;
+      } else {
+        nx = (R)1;
+        nz = (R)0;
+      }
+    }
+    if (feat == 0) {
+      nl = V3<R>{nx, (R)0, nz};
+      d = dot(nl, b) - r;
+    } else {
+      nl = V3<R>{(R)0, capsg, (R)0};
+      d = capsg * b.y - hh;
+    }
+  } else {
+    return false;
+  }
+  if (!(d < band)) return false;
+  n_out = normalize_(quat_rotate<R>((R)Q[0], V3<R>{(R)Q[1], (R)Q[2], (R)Q[3]}, nl));
+  d_out = d;
+  return true;
+}
+
+// Geodesic angle between the current and previous orientations,
+// 2 acos(|q . q_prev|) in double (bruteforce.hpp:204-211, bvh.hpp:2466-2472).
+inline double quat_theta(const ysim_analytic_shape& s) {
+  double dq = (double)s.rot_quat[0] * (double)s.prev_rot_quat[0] + (double)s.rot_quat[1] * (double)s.prev_rot_quat[1] +
+              (double)s.rot_quat[2] * (double)s.prev_rot_quat[2] + (double)s.rot_quat[3] * (double)s.prev_rot_quat[3];
+  dq = std::fabs(dq);
+  if (dq > 1.0) dq = 1.0;
+  return 2.0 * std::acos(dq);
+}
+
+// Hamilton product and rotateVector (include/sim/material_quat.hpp:28-35, 59-63):
+// q * (0, v) * conj(q), products summed left to right.
+struct Qf { float w, x, y, z; };
+inline Qf qmul(Qf a, Qf b) {
+  return {a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z, a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y,
+          a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x, a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w};
+}
+inline V3<float> rotate_vector(Qf q, V3<float> v) {
+  const Qf r = qmul(qmul(q, Qf{0.f, v.x, v.y, v.z}), Qf{q.w, -q.x, -q.y, -q.z});
+  return {r.x, r.y, r.z};
+}
+
+// BroadPhase::analyticWorldHalf (include/sim/bvh.hpp:2431-2460): world
+// half-extents of the shape's box at orientation q (w, x, y, z).
+inline V3<float> world_half(const ysim_analytic_shape& s, const float* q) {
+  const float* H = s.half_ext_height;
+  V3<float> h{0.f, 0.f, 0.f};
+  if (s.kind == YSIM_COLLIDER_SPHERE) {
+    const float r = std::max(std::max(H[0], H[1]), H[2]);
+    return {r, r, r};
+  } else if (s.kind == YSIM_COLLIDER_BOX) {
+    h = {H[0], H[1], H[2]};
+  } else if (s.kind == YSIM_COLLIDER_CYLINDER) {
+    h = {s.center_radius[3], H[3], s.center_radius[3]};
+  } else {
+    return h;
+  }
+  const Qf Q{q[0], q[1], q[2], q[3]};
+  const V3<float> cx = rotate_vector(Q, {1.f, 0.f, 0.f}), cy = rotate_vector(Q, {0.f, 1.f, 0.f}),
+                  cz = rotate_vector(Q, {0.f, 0.f, 1.f});
+  return {std::fabs(cx.x) * h.x + std::fabs(cy.x) * h.y + std::fabs(cz.x) * h.z,
+          std::fabs(cx.y) * h.x + std::fabs(cy.y) * h.y + std::fabs(cz.y) * h.z,
+          std::fabs(cx.z) * h.x + std::fabs(cy.z) * h.y + std::fabs(cz.z) * h.z};
+}
+
+// Per-shape constants: the swept box before padding (objectBroadAABB,
+// bvh.hpp:2505-2517: min(c - hc, p - hp), max(c + hc, p + hp)), the broad
+// rotation band (analyticRotBand, bvh.hpp:2465-2480) and the narrow rotation
+// margin's angle and radius (BruteForce::analyticRotMargin,
+// bruteforce.hpp:201-242: theta = 0 below 1e-9, plane 0).
+struct ABounds {
+  float lo[3], hi[3];
+  float rot_band;
+  double theta, rmax;
+};
+inline ABounds analytic_bounds(const ysim_analytic_shape& s) {
+  ABounds B{};
+  const float* H = s.half_ext_height;
+  const float* C = s.center_radius;
+  const float* P = s.prev_center_pad;
+  const V3<float> hc = world_half(s, s.rot_quat), hp = world_half(s, s.prev_rot_quat);
+  const float c[3] = {C[0], C[1], C[2]}, p[3] = {P[0], P[1], P[2]};
+  const float hcv[3] = {hc.x, hc.y, hc.z}, hpv[3] = {hp.x, hp.y, hp.z};
+  for (int k = 0; k < 3; ++k) {
+    B.lo[k] = std::min(c[k] - hcv[k], p[k] - hpv[k]);
+    B.hi[k] = std::max(c[k] + hcv[k], p[k] + hpv[k]);
+  }
+  const double theta = quat_theta(s);
+  if (theta <= 0.0) {
+    B.rot_band = 0.f;
+  } else {
+    const double rb = std::sqrt((double)H[0] * H[0] + (double)H[1] * H[1] + (double)H[2] * H[2]) + (double)C[3];
+    B.rot_band = (float)(theta * rb);
+  }
+  B.theta = (theta > 1e-9 && s.kind != YSIM_COLLIDER_PLANE) ? theta : 0.0;
+  switch (s.kind) {
+    case YSIM_COLLIDER_SPHERE: B.rmax = std::max({(double)H[0], (double)H[1], (double)H[2]}); break;
+    case YSIM_COLLIDER_BOX: B.rmax = std::sqrt((double)H[0] * H[0] + (double)H[1] * H[1] + (double)H[2] * H[2]); break;
+    case YSIM_COLLIDER_CYLINDER: B.rmax = std::sqrt((double)C[3] * C[3] + (double)H[3] * H[3]); break;
+    default: B.rmax = 0.0; break;
+  }
+  return B;
+}
+// rotMargin of one (shape, cloth) pair: theta (rMax + radius + thickness).
+inline float rot_margin(const ABounds& B, float radius, float thickness) {
+  return B.theta > 0.0 ? (float)(B.theta * (B.rmax + (double)radius + (double)thickness)) : 0.f;
+}
+
 }  // namespace
 
 extern "C" {
@@ -861,6 +1155,16 @@ struct OScene {
   std::vector<double> kspring_d;
   std::vector<float> kspring;    // optional per-spring k for mesh 0 (adjoint configs)
   double timing[YSIM_STAGE_COUNT] = {0};
+  // analytic colliders: shapes, their per-shape constants, the shape of each
+  // mesh (-1: none), the (cloth, shape) pairs and the previous substep's
+  // pairing of each (PackedCollisionData::endAnalyticPairs: held = last fresh)
+  std::vector<ysim_analytic_shape> shapes;
+  std::vector<ABounds> abounds;
+  std::vector<int> shape_of;
+  std::vector<uint32_t> apair_q, apair_k;
+  std::vector<uint8_t> held;
+  bool analytic_active() const { return !shapes.empty() && P.broad_mode == 0; }
+  float thick(const OMesh& Q) const { return P.margin > 0.f ? P.narrow_thickness : Q.d.cloth.thickness; }
 };
 
 bool is_dynamic(const ysim_mesh_desc& d) {
@@ -926,6 +1230,7 @@ void* yo_scene_create(uint32_t nm, const ysim_mesh_desc* meshes, const float* x,
              S->facets.data() + 3 * M.foff, M.d.num_facets, M.tree.data());
   }
   S->csr_off.assign(vo + 1, 0);
+  S->shape_of.assign(nm, -1);
   return S;
 }
 
@@ -947,6 +1252,51 @@ void yo_scene_set_kspring(void* s, const float* k, uint32_t n) {
   if (!k) { S->kspring.clear(); S->kspring_d.clear(); return; }
   S->kspring.assign(k, k + n);
   S->kspring_d.assign(k, k + n);
+}
+
+// Analytic branch of one broad detection + narrowAnalytic
+// (bvh.hpp:2897-2949, scene.hpp:511-529, bruteforce.hpp:256-308): a pair
+// (cloth q, shape k) is marked when q's tree root box meets the shape's swept
+// box padded by q's band plus the rotation band (closed intervals,
+// AABB4::intersect, bvh.hpp:66-71; a plane always), and it runs when marked
+// now or at the previous detection.  Each vertex of a running pair tests the
+// shape with band + rotMargin and keeps one contact (shape mesh, facet 0).
+static float analytic_band(const OScene& S, const OMesh& Q, uint32_t k) {
+  const float th = S.thick(Q);
+  return (S.P.radius + th) + rot_margin(S.abounds[k], S.P.radius, th);
+}
+static void analytic_contacts(OScene& S, std::vector<Contact>& contacts) {
+  std::vector<uint8_t> fresh(S.apair_q.size(), 0);
+  for (size_t i = 0; i < S.apair_q.size(); ++i) {
+    const uint32_t q = S.apair_q[i], k = S.apair_k[i];
+    const OMesh& Q = S.meshes[q];
+    const ysim_analytic_shape& sh = S.shapes[k];
+    const ABounds& B = S.abounds[k];
+    bool hit = true;
+    if (sh.kind != YSIM_COLLIDER_PLANE) {
+      const float pad = (S.P.radius + S.thick(Q)) + B.rot_band;
+      const ysim_bvh_node& R = Q.tree[0];
+      hit = true;
+      for (int c = 0; c < 3; ++c)
+        if (B.hi[c] + pad < R.min[c] || B.lo[c] - pad > R.max[c]) hit = false;
+    }
+    fresh[i] = hit ? 1 : 0;
+    if (!(hit || S.held[i]) || !(sh.flags & 1u)) continue;
+    const float band = analytic_band(S, Q, k);
+    const float* xq = S.x.data() + 3 * Q.voff;
+    const float* xqp = S.xprev.data() + 3 * Q.voff;
+    for (uint32_t p = 0; p < Q.d.num_vertices; ++p) {
+      V3<float> n;
+      float d;
+      if (!narrow_analytic<float>(sh, ld(xq, p), ld(xqp, p), band, n, d)) continue;
+      Contact c;
+      c.gv = Q.voff + p; c.tobj = sh.obj_index; c.tri = 0; c.qobj = q; c.point = p;
+      c.n[0] = n.x; c.n[1] = n.y; c.n[2] = n.z; c.d = d;
+      c.qbeh = Q.d.behavior; c.tbeh = sh.behavior_type; c.qshape = Q.d.shape; c.tshape = sh.kind;
+      contacts.push_back(c);
+    }
+  }
+  S.held.swap(fresh);
 }
 
 // Collision pipeline at the current state: returns the per-vertex contact CSR
@@ -973,6 +1323,7 @@ static void collide(OScene& S, uint64_t* npairs_out, std::vector<uint32_t>* pair
       OMesh& T = S.meshes[t];
       if (!T.d.collidable || T.d.group != Q.d.group || T.d.num_facets == 0) continue;
       if (q == t && !P.self_collision) continue;
+      if (S.analytic_active() && S.shape_of[t] >= 0) continue;  // analyticSkip, bvh.hpp:2520-2523
       const float* xq = S.x.data() + 3 * Q.voff;
       const float* xqp = S.xprev.data() + 3 * Q.voff;
       const float* xt = S.x.data() + 3 * T.voff;
@@ -1027,6 +1378,7 @@ static void collide(OScene& S, uint64_t* npairs_out, std::vector<uint32_t>* pair
       }
     }
   }
+  if (S.analytic_active()) analytic_contacts(S, contacts);
   // CSR ordered by (vertex, target_obj, triangle): bruteforce.hpp:358-383 with
   // the deterministic in-bucket order of this build.
   std::sort(contacts.begin(), contacts.end(), [](const Contact& a, const Contact& b) {
@@ -1057,6 +1409,20 @@ static void twin_contacts(OScene& S) {
     const ysim_narrow_collision& r = S.csr[i];
     const double band = 1.0;  // the FP32 decision stands; FP64 only re-evaluates the geometry
     const OMesh& Q = S.meshes[r.query_obj];
+    if (S.analytic_active() && S.shape_of[r.target_obj] >= 0) {
+      const uint32_t k = (uint32_t)S.shape_of[r.target_obj];
+      V3<double> n;
+      double d;
+      const size_t gv = Q.voff + r.point;
+      if (narrow_analytic<double>(S.shapes[k], ld(S.xd.data(), gv), ld(S.xprevd.data(), gv),
+                                  (double)analytic_band(S, Q, k), n, d)) {
+        S.twin_nd4[4 * i] = n.x; S.twin_nd4[4 * i + 1] = n.y; S.twin_nd4[4 * i + 2] = n.z;
+        S.twin_nd4[4 * i + 3] = d;
+      } else {
+        for (int c = 0; c < 4; ++c) S.twin_nd4[4 * i + c] = r.normal_distance[c];
+      }
+      continue;
+    }
     const OMesh& T = S.meshes[r.target_obj];
     const uint32_t* f = S.facets.data() + 3 * (T.foff + r.triangle);
     V3<double> n;
@@ -1202,5 +1568,80 @@ void yo_scene_get_tree(void* s, uint32_t mesh, ysim_bvh_node* nodes, int32_t* pa
   if (idx) std::memcpy(idx, M.idx.data(), 4 * M.idx.size());
 }
 uint32_t yo_scene_anomaly(void* s) { return ((OScene*)s)->anomaly; }
+
+// Install the analytic colliders (same contract as the library's
+// scene_set_analytic).  Returns 0, or -1 with the previous set unchanged.
+int yo_scene_set_analytic(void* s, const ysim_analytic_shape* shapes, uint32_t n) {
+  OScene* S = (OScene*)s;
+  const uint32_t nm = (uint32_t)S->meshes.size();
+  std::vector<int> shape_of(nm, -1);
+  for (uint32_t k = 0; k < n; ++k) {
+    const ysim_analytic_shape& sh = shapes[k];
+    if (sh.kind < YSIM_COLLIDER_SPHERE || sh.kind > YSIM_COLLIDER_PLANE || sh.obj_index >= nm) return -1;
+    const ysim_mesh_desc& d = S->meshes[sh.obj_index].d;
+    if (d.behavior != YSIM_BEHAVIOR_FLOAT || d.shape != sh.kind || sh.behavior_type != d.behavior) return -1;
+    if (shape_of[sh.obj_index] >= 0) return -1;
+    shape_of[sh.obj_index] = (int)k;
+  }
+  bool same = n == S->shapes.size();
+  for (uint32_t k = 0; same && k < n; ++k) same = shapes[k].obj_index == S->shapes[k].obj_index;
+  S->shapes.assign(shapes, shapes + n);
+  S->shape_of = shape_of;
+  S->abounds.clear();
+  for (uint32_t k = 0; k < n; ++k) S->abounds.push_back(analytic_bounds(shapes[k]));
+  if (!same) {
+    S->apair_q.clear();
+    S->apair_k.clear();
+    for (uint32_t q = 0; q < nm; ++q) {
+      const ysim_mesh_desc& Q = S->meshes[q].d;
+      if (!is_dynamic(Q) || !Q.collidable) continue;
+      for (uint32_t k = 0; k < n; ++k) {
+        const ysim_mesh_desc& T = S->meshes[shapes[k].obj_index].d;
+        if (!T.collidable || T.group != Q.group) continue;
+        S->apair_q.push_back(q);
+        S->apair_k.push_back(k);
+      }
+    }
+    S->held.assign(S->apair_q.size(), 0);
+  }
+  return 0;
+}
+// The (cloth, shape mesh) pairs and their pairing at the last detection.
+uint32_t yo_scene_analytic_markers(void* s, uint32_t* pairs2, uint8_t* marked) {
+  OScene* S = (OScene*)s;
+  const uint32_t n = (uint32_t)S->apair_q.size();
+  for (uint32_t i = 0; i < n; ++i) {
+    if (pairs2) { pairs2[2 * i] = S->apair_q[i]; pairs2[2 * i + 1] = S->shapes[S->apair_k[i]].obj_index; }
+    if (marked) marked[i] = S->held[i];
+  }
+  return n;
+}
+int yo_narrow_analytic_f32(const ysim_analytic_shape* sh, const float* pcur, const float* pprev, float band,
+                           float* out4) {
+  V3<float> n;
+  float d;
+  if (!narrow_analytic<float>(*sh, ld(pcur, 0), ld(pprev, 0), band, n, d)) return 0;
+  out4[0] = n.x; out4[1] = n.y; out4[2] = n.z; out4[3] = d;
+  return 1;
+}
+int yo_narrow_analytic_f64(const ysim_analytic_shape* sh, const double* pcur, const double* pprev, double band,
+                           double* out4) {
+  V3<double> n;
+  double d;
+  if (!narrow_analytic<double>(*sh, ld(pcur, 0), ld(pprev, 0), band, n, d)) return 0;
+  out4[0] = n.x; out4[1] = n.y; out4[2] = n.z; out4[3] = d;
+  return 1;
+}
+// Per-shape constants: lo3, hi3 (unpadded swept box), out3 = (rot_band,
+// rot_margin(radius, thickness)) as floats, theta2 = (narrow theta, rmax).
+void yo_analytic_bounds(const ysim_analytic_shape* sh, float radius, float thickness, float* lo3, float* hi3,
+                        float* out2, double* theta2) {
+  const ABounds B = analytic_bounds(*sh);
+  for (int k = 0; k < 3; ++k) { lo3[k] = B.lo[k]; hi3[k] = B.hi[k]; }
+  out2[0] = B.rot_band;
+  out2[1] = rot_margin(B, radius, thickness);
+  theta2[0] = B.theta;
+  theta2[1] = B.rmax;
+}
 
 }  // extern "C"

@@ -1,0 +1,163 @@
+"""Analytic colliders on the GPU (k_analytic after the fused query): states and
+contacts equal the oracle's bit for bit for an ellipsoid, a rotated box, a
+tilted cylinder and a tilted half-space next to a triangle-mesh obstacle,
+for moving and rotating colliders, for the one-substep pairing carry-over,
+and through snapshot / restore, autotune and shape replacement."""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleScene
+from ysim_amd import scenes
+from ysim_amd._lib import YsimError
+from ysim_amd.hip import HipScene
+from ysim_amd.scenes import COLLIDER_BOX, COLLIDER_SPHERE, analytic_shape, quat_axis_angle
+
+pytestmark = pytest.mark.gpu
+
+
+def _same_state(a, b):
+    for u, w in zip(a, b):
+        assert np.array_equal(u, w)
+
+
+def _same_contacts(hs, os_):
+    og, cg = hs.contacts()
+    oo, co = os_.contacts()
+    assert np.array_equal(og, oo) and np.array_equal(cg, co)
+    return co
+
+
+@pytest.mark.parametrize("rebuild_every,mode", [(1, 1), (0, 1), (1, 0)])
+def test_analytic_drop_bit_exact(ctx, oracle_threads, rebuild_every, mode):
+    """A 48x48 cloth falls onto the four shapes and the icosphere; compared
+    every 10 substeps for 300 substeps (first contacts near substep 90, the
+    half-space from substep 170)."""
+    print(f"analytic drop rebuild_every={rebuild_every} query_tile={mode}", flush=True)
+    sc = scenes.config_analytic(rebuild_every=rebuild_every)
+    hs, os_ = HipScene(ctx, sc), OracleScene(sc)
+    hs.set_tuning(query_tile=mode)
+    for k in range(30):
+        hs.step(10, graph=(k % 2 == 0))
+        os_.step(10)
+        _same_state(hs.state(), os_.state())
+    co = _same_contacts(hs, os_)
+    shapes = set(sc.analytic["obj_index"].tolist())
+    assert len(co) > 500 and shapes <= set(co["target_obj"].tolist()) and hs.flags() == 0
+
+
+def test_moving_colliders(ctx, oracle_threads):
+    """An ellipsoid sweeping sideways through the cloth and a box spinning
+    under it, poses updated every 10 substeps (previous pose in prev_*): the
+    swept motion inversion and both rotation bands are exercised."""
+    sc = scenes.config_analytic(n=40, kinds=("sphere", "box"), mesh_obstacle=False, height=0.72)
+    hs, os_ = HipScene(ctx, sc), OracleScene(sc)
+    sh = sc.analytic.copy()
+    seen = set()
+    for k in range(25):
+        prev = sh.copy()
+        sh["prev_center_pad"][:, :3] = prev["center_radius"][:, :3]
+        sh["prev_rot_quat"] = prev["rot_quat"]
+        sh["center_radius"][0, 0] += 0.012
+        sh["center_radius"][0, 1] += 0.004
+        sh["rot_quat"][1] = quat_axis_angle((0.3, 1.0, 0.2), 0.6 + 0.08 * (k + 1))
+        hs.set_analytic(sh)
+        os_.set_analytic(sh)
+        hs.step(10, graph=(k % 2 == 1))
+        os_.step(10)
+        _same_state(hs.state(), os_.state())
+        seen |= set(_same_contacts(hs, os_)["target_obj"].tolist())
+    assert {1, 2} <= seen and hs.flags() == 0
+
+
+def _box_scene():
+    rng = np.random.default_rng(3)
+    cl = scenes.cloth_mesh(8, 0.1, (-0.05, 0.3, -0.05), rng)
+    h = (0.1, 0.1, 0.1)
+    bx = scenes.collider_mesh(*scenes.box_mesh(h, (0.0, 0.0, 0.0)), COLLIDER_BOX)
+    p = scenes.sim_params(subh=1 / 600, margin=0.01, rebuild_every=1)
+    return scenes.Scene([cl, bx], p, analytic=analytic_shape(COLLIDER_BOX, 1, (0.0, 0.0, 0.0), half=h))
+
+
+def test_pairing_carry_over(ctx, oracle_threads):
+    """A cloth moved far from the box after a substep inside it still gets the
+    start points' contacts for one substep (the previous pairing), then none."""
+    sc = _box_scene()
+    hs, os_ = HipScene(ctx, sc), OracleScene(sc)
+    for s in (hs, os_):
+        s.step(1)
+    x, _, _ = os_.state()
+    inside = x.copy()
+    inside[:64, 1] = 0.07
+    far = inside.copy()
+    far[:64, 0] += 5.0
+    for s in (hs, os_):
+        s.set_state(x=inside, xprev=inside)
+        s.step(1)
+        s.set_state(x=far, v=np.zeros_like(far), xprev=inside)
+        s.step(1)
+    co = _same_contacts(hs, os_)
+    assert len(co) == 64 and set(co["target_obj"].tolist()) == {1}
+    _same_state(hs.state(), os_.state())
+    for s in (hs, os_):
+        s.set_state(x=far, v=np.zeros_like(far), xprev=far)
+        s.step(1)
+    assert len(_same_contacts(hs, os_)) == 0
+
+
+def test_snapshot_autotune_and_replacement(ctx, oracle_threads):
+    """Snapshot / restore and autotune keep the pairing flags with the state;
+    removing the shapes puts their meshes back into the triangle path."""
+    sc = scenes.config_analytic(n=32, height=0.7)
+    hs, os_ = HipScene(ctx, sc), OracleScene(sc)
+    hs.step(60)
+    os_.step(60)
+    hs.snapshot()
+    hs.step(40, graph=True)
+    a = hs.state()
+    hs.restore()
+    hs.step(40, graph=True)
+    _same_state(hs.state(), a)
+    os_.step(40)
+    _same_state(hs.state(), os_.state())
+    hs.autotune(4)
+    hs.step(30, graph=True)
+    os_.step(30)
+    _same_state(hs.state(), os_.state())
+    _same_contacts(hs, os_)
+    empty = sc.analytic[:0]
+    hs.set_analytic(empty)
+    os_.set_analytic(empty)
+    hs.step(30, graph=True)
+    os_.step(30)
+    _same_state(hs.state(), os_.state())
+    co = _same_contacts(hs, os_)
+    assert np.any(np.isin(co["target_obj"], sc.analytic["obj_index"]) & (co["triangle"] > 0))
+
+
+def test_hash_mode_keeps_triangles(ctx, oracle_threads):
+    """Under the spatial-hash broad phase the shapes are inactive and the
+    collider meshes collide through their triangles, as in the oracle."""
+    sc = scenes.config_analytic(n=32, height=0.7)
+    sc.params["broad_mode"] = 1
+    hs, os_ = HipScene(ctx, sc), OracleScene(sc)
+    for k in range(8):
+        hs.step(20, graph=(k % 2 == 0))
+        os_.step(20)
+        _same_state(hs.state(), os_.state())
+    co = _same_contacts(hs, os_)
+    assert len(co) and np.any(co["triangle"] > 0)
+
+
+def test_set_analytic_rejections(ctx):
+    sc = _box_scene()
+    hs = HipScene(ctx, sc)
+    bad = [analytic_shape(COLLIDER_BOX, 0, (0, 0, 0), half=(0.1, 0.1, 0.1)),
+           analytic_shape(COLLIDER_SPHERE, 1, (0, 0, 0), half=(0.1, 0.1, 0.1)),
+           analytic_shape(0, 1, (0, 0, 0)),
+           analytic_shape(COLLIDER_BOX, 2, (0, 0, 0), half=(0.1, 0.1, 0.1)),
+           np.concatenate([sc.analytic, sc.analytic])]
+    for b in bad:
+        with pytest.raises(YsimError):
+            hs.set_analytic(b)
+    with pytest.raises(YsimError):
+        hs.num_springs()

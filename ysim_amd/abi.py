@@ -60,6 +60,13 @@ NARROW = np.dtype([("point", np.uint32), ("triangle", np.uint32), ("query_obj", 
                    ("query_behavior", np.uint32), ("target_behavior", np.uint32),
                    ("query_shape", np.uint32), ("target_shape", np.uint32)])
 assert BVH_NODE.itemsize == 32 and BROAD.itemsize == 32 and NARROW.itemsize == 48
+# ysim_analytic_shape (AnalyticShape, include/sim/collision_pod.hpp:299-310)
+ANALYTIC = np.dtype([("center_radius", np.float32, 4), ("half_ext_height", np.float32, 4),
+                     ("rot_quat", np.float32, 4), ("prev_center_pad", np.float32, 4),
+                     ("prev_rot_quat", np.float32, 4), ("kind", np.uint32), ("obj_index", np.uint32),
+                     ("behavior_type", np.uint32), ("flags", np.uint32)])
+assert ANALYTIC.itemsize == 96
+COLLIDER_MESH, COLLIDER_SPHERE, COLLIDER_BOX, COLLIDER_CYLINDER, COLLIDER_PLANE = range(5)
 
 STAGES = ("box", "morton", "sort", "hierarchy", "refit", "query", "integrate")
 

@@ -243,6 +243,21 @@ struct ysim_scene {
   const float* graph_ks = nullptr;
   ysim_tuning tune{8, 8, 128, 256, 0.f, 0.f, 256, 0, 0, 5, 1};
   float* tune_buf = nullptr;   // autotune state save: x, v, xPrev of the whole scene
+  // analytic colliders: the installed shapes, their device copies, the
+  // (cloth, shape) pairs per dynamic segment and each pair's pairing flag of
+  // the last two substeps (amark[par * napairs + pair] is written by the
+  // substep of parity par)
+  std::vector<ysim_analytic_shape> ashapes;
+  std::vector<int> shape_of;    // per mesh: its shape index, -1 none
+  AShape* d_ashapes = nullptr;
+  uint32_t ashape_cap = 0;
+  uint32_t *d_apair_off = nullptr, *d_apair_shape = nullptr;
+  uint32_t napairs = 0;
+  uint32_t* amark = nullptr;
+  std::vector<uint32_t> snap_amark;
+  std::vector<uint32_t> tune_amark;
+  float* d_thick = nullptr;
+  bool analytic_on() const { return !ashapes.empty() && !hash_mode; }
 
   int grow(float** p, size_t& have, size_t need) {
     if (need <= have && *p) return YSIM_OK;
@@ -322,7 +337,7 @@ struct ysim_scene {
     if (wstats_words)
       YSIM_HIP_TRY(hipMemcpyAsync(tune_wstats.data(), wstats, 4 * (size_t)wstats_words, hipMemcpyDeviceToHost, st));
     YSIM_HIP_TRY(hipStreamSynchronize(st));
-    return YSIM_OK;
+    return save_amark(tune_amark);
   }
   int tune_restore(uint32_t par, uint64_t stp, const uint32_t* fl_np) {
     const size_t n = 3 * (size_t)Vtot;
@@ -334,6 +349,7 @@ struct ysim_scene {
     if (wstats_words && tune_wstats.size() == wstats_words)
       YSIM_HIP_TRY(hipMemcpyAsync(wstats, tune_wstats.data(), 4 * (size_t)wstats_words, hipMemcpyHostToDevice, st));
     YSIM_HIP_TRY(hipStreamSynchronize(st));
+    YSIM_TRY(load_amark(tune_amark, par));
     parity = par;
     step = stp;
     return YSIM_OK;
@@ -457,6 +473,14 @@ struct ysim_scene {
     allocs.push_back((void*)*p);
     return YSIM_OK;
   }
+  // free one allocation of this scene (stream drained first)
+  int release(void* p) {
+    if (!p) return YSIM_OK;
+    YSIM_HIP_TRY(hipStreamSynchronize(st));
+    allocs.erase(std::remove(allocs.begin(), allocs.end(), p), allocs.end());
+    YSIM_HIP_TRY(hipFree(p));
+    return YSIM_OK;
+  }
   template <typename T>
   int put(T** p, const T* h, size_t n) {
     YSIM_TRY(alloc(p, n));
@@ -475,6 +499,75 @@ struct ysim_scene {
     return YSIM_OK;
   }
   bool dynamic(uint32_t m) const { return meshes[m].behavior != YSIM_BEHAVIOR_FLOAT; }
+
+  // Triangle targets per query (dynamic) mesh: the collidable meshes of its
+  // group, itself only with self collision, analytic colliders' meshes never
+  // while the analytic path is on (BroadPhase::analyticSkip, bvh.hpp:2520-2523).
+  int build_targets() {
+    std::vector<uint32_t> toff(1, 0), tg;
+    for (uint32_t q : dyn) {
+      const ysim_mesh_desc& Q = meshes[q];
+      if (Q.collidable) {
+        for (uint32_t t = 0; t < nm; ++t) {
+          const ysim_mesh_desc& T = meshes[t];
+          if (!T.collidable || T.group != Q.group || T.num_facets == 0) continue;
+          if (t == q && !P.self_collision) continue;
+          if (analytic_on() && shape_of[t] >= 0) continue;
+          tg.push_back(t);
+        }
+      }
+      toff.push_back((uint32_t)tg.size());
+    }
+    YSIM_TRY(release(d_tgt_off));
+    YSIM_TRY(release(d_tgt));
+    d_tgt_off = d_tgt = nullptr;
+    YSIM_TRY(put(&d_tgt_off, toff));
+    YSIM_TRY(put(&d_tgt, tg));
+    return YSIM_OK;
+  }
+
+  int set_analytic(const ysim_analytic_shape* h, uint32_t n);
+
+  int enqueue_analytic(uint32_t par) {
+    if (!analytic_on() || !napairs) return YSIM_OK;
+    AnalyticArgs a;
+    a.dsegs = dyn_vsegs;
+    a.mt = mt;
+    a.shapes = d_ashapes;
+    a.pair_off = d_apair_off;
+    a.pair_shape = d_apair_shape;
+    a.x = x[par];
+    a.xprev = x[par ^ 1];
+    a.tree = tree;
+    a.mesh_band = d_band;
+    a.mesh_thick = d_thick;
+    a.radius = P.radius;
+    a.mark_prev = amark + (size_t)(par ^ 1) * napairs;
+    a.mark_cur = amark + (size_t)par * napairs;
+    a.ccount = ccount;
+    a.cnd = cnd;
+    a.cid = cid;
+    a.cap = cap;
+    a.flags = flags;
+    a.nitems = Vdyn;
+    return launch_analytic(st, a);
+  }
+  // previous-substep pairing flags (those the next substep reads) to / from host
+  int save_amark(std::vector<uint32_t>& out) {
+    out.assign(napairs, 0u);
+    if (!napairs) return YSIM_OK;
+    YSIM_HIP_TRY(hipMemcpyAsync(out.data(), amark + (size_t)(parity ^ 1) * napairs, 4 * (size_t)napairs,
+                                hipMemcpyDeviceToHost, st));
+    YSIM_HIP_TRY(hipStreamSynchronize(st));
+    return YSIM_OK;
+  }
+  int load_amark(const std::vector<uint32_t>& in, uint32_t par) {
+    if (!napairs || in.size() != napairs) return YSIM_OK;
+    YSIM_HIP_TRY(hipMemcpyAsync(amark + (size_t)(par ^ 1) * napairs, in.data(), 4 * (size_t)napairs,
+                                hipMemcpyHostToDevice, st));
+    YSIM_HIP_TRY(hipStreamSynchronize(st));
+    return YSIM_OK;
+  }
 
   int make_seg(const std::vector<uint32_t>& ms, bool tris, SegTable& out) {
     std::vector<uint32_t> off(1, 0);
@@ -629,6 +722,7 @@ struct ysim_scene {
       YSIM_TRY(launch_hash_query(st, h, qa, false));
     } else {
       YSIM_TRY(launch_query(st, qa, false));
+      YSIM_TRY(enqueue_analytic(par));
     }
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[6], st));
     return YSIM_OK;
@@ -730,6 +824,145 @@ struct ysim_scene {
     for (void* p : allocs) hipFree(p);
   }
 };
+
+// ====================================================================== analytic colliders
+namespace {
+
+// Hamilton product and rotateVector (include/sim/material_quat.hpp:28-35,
+// 59-63): q * (0, v) * conj(q), each component summed left to right.
+struct HQuat {
+  float w, x, y, z;
+};
+HQuat hmul(HQuat a, HQuat b) {
+  return {a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z, a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y,
+          a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x, a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w};
+}
+f3 rotate_vec(HQuat q, f3 v) {
+  const HQuat r = hmul(hmul(q, HQuat{0.f, v.x, v.y, v.z}), HQuat{q.w, -q.x, -q.y, -q.z});
+  return {r.x, r.y, r.z};
+}
+
+// World half-extents of a shape's bounding box at orientation q (w, x, y, z):
+// BroadPhase::analyticWorldHalf (include/sim/bvh.hpp:2431-2460).  A sphere
+// takes its largest semi-axis on every axis; box and cylinder (r, hh, r) go
+// through the |R| h sum.
+f3 world_half(const ysim_analytic_shape& s, const float* q) {
+  const float* H = s.half_ext_height;
+  f3 h{0.f, 0.f, 0.f};
+  if (s.kind == YSIM_COLLIDER_SPHERE) {
+    const float r = std::max(std::max(H[0], H[1]), H[2]);
+    return {r, r, r};
+  }
+  if (s.kind == YSIM_COLLIDER_BOX)
+    h = {H[0], H[1], H[2]};
+  else if (s.kind == YSIM_COLLIDER_CYLINDER)
+    h = {s.center_radius[3], H[3], s.center_radius[3]};
+  else
+    return h;
+  const HQuat Q{q[0], q[1], q[2], q[3]};
+  const f3 cx = rotate_vec(Q, {1.f, 0.f, 0.f}), cy = rotate_vec(Q, {0.f, 1.f, 0.f}),
+           cz = rotate_vec(Q, {0.f, 0.f, 1.f});
+  return {std::fabs(cx.x) * h.x + std::fabs(cy.x) * h.y + std::fabs(cz.x) * h.z,
+          std::fabs(cx.y) * h.x + std::fabs(cy.y) * h.y + std::fabs(cz.y) * h.z,
+          std::fabs(cx.z) * h.x + std::fabs(cy.z) * h.y + std::fabs(cz.z) * h.z};
+}
+
+// The per-shape constants of the device record: swept box before padding
+// (objectBroadAABB, bvh.hpp:2505-2517), broad rotation band (analyticRotBand,
+// bvh.hpp:2465-2480) and the narrow rotation margin's angle and radius
+// (BruteForce::analyticRotMargin, bruteforce.hpp:201-242), all from the
+// geodesic angle 2 acos(|q . q_prev|) taken in double.
+AShape analytic_consts(const ysim_analytic_shape& sh) {
+  AShape a{};
+  a.s = sh;
+  const float* H = sh.half_ext_height;
+  const float* C = sh.center_radius;
+  const float* Pc = sh.prev_center_pad;
+  const f3 hc = world_half(sh, sh.rot_quat), hp = world_half(sh, sh.prev_rot_quat);
+  const float hcv[3] = {hc.x, hc.y, hc.z}, hpv[3] = {hp.x, hp.y, hp.z};
+  for (int k = 0; k < 3; ++k) {
+    a.lo[k] = std::min(C[k] - hcv[k], Pc[k] - hpv[k]);
+    a.hi[k] = std::max(C[k] + hcv[k], Pc[k] + hpv[k]);
+  }
+  double dq = (double)sh.rot_quat[0] * (double)sh.prev_rot_quat[0] +
+              (double)sh.rot_quat[1] * (double)sh.prev_rot_quat[1] +
+              (double)sh.rot_quat[2] * (double)sh.prev_rot_quat[2] +
+              (double)sh.rot_quat[3] * (double)sh.prev_rot_quat[3];
+  dq = std::fabs(dq);
+  if (dq > 1.0) dq = 1.0;
+  const double theta = 2.0 * std::acos(dq);
+  const double hn = std::sqrt((double)H[0] * H[0] + (double)H[1] * H[1] + (double)H[2] * H[2]);
+  a.rot_band = theta <= 0.0 ? 0.f : (float)(theta * (hn + (double)C[3]));
+  a.theta = (theta > 1e-9 && sh.kind != YSIM_COLLIDER_PLANE) ? theta : 0.0;
+  if (sh.kind == YSIM_COLLIDER_SPHERE)
+    a.rmax = std::max(std::max((double)H[0], (double)H[1]), (double)H[2]);
+  else if (sh.kind == YSIM_COLLIDER_BOX)
+    a.rmax = hn;
+  else if (sh.kind == YSIM_COLLIDER_CYLINDER)
+    a.rmax = std::sqrt((double)C[3] * C[3] + (double)H[3] * H[3]);
+  return a;
+}
+
+}  // namespace
+
+int ysim_scene::set_analytic(const ysim_analytic_shape* h, uint32_t n) {
+  if (stage_phase != 0) return fail(YSIM_ERR_STATE, "scene_set_analytic: a staged substep is in progress");
+  YSIM_ARG(n == 0 || h, "scene_set_analytic: null shapes");
+  std::vector<int> so(nm, -1);
+  for (uint32_t k = 0; k < n; ++k) {
+    const ysim_analytic_shape& sh = h[k];
+    YSIM_ARG(sh.kind >= YSIM_COLLIDER_SPHERE && sh.kind <= YSIM_COLLIDER_PLANE,
+             "scene_set_analytic: kind must be sphere, box, cylinder or plane");
+    YSIM_ARG(sh.obj_index < nm, "scene_set_analytic: obj_index out of range");
+    const ysim_mesh_desc& d = meshes[sh.obj_index];
+    YSIM_ARG(d.behavior == YSIM_BEHAVIOR_FLOAT, "scene_set_analytic: a collider's mesh must be a static (FLOAT) mesh");
+    YSIM_ARG(d.shape == sh.kind, "scene_set_analytic: the mesh's shape lane must equal the collider kind");
+    YSIM_ARG(sh.behavior_type == d.behavior, "scene_set_analytic: behavior_type must equal the mesh's behavior");
+    YSIM_ARG(so[sh.obj_index] < 0, "scene_set_analytic: two shapes for one mesh");
+    so[sh.obj_index] = (int)k;
+  }
+  bool same = n == ashapes.size();
+  for (uint32_t k = 0; same && k < n; ++k) same = h[k].obj_index == ashapes[k].obj_index;
+  std::vector<AShape> hs(n);
+  for (uint32_t k = 0; k < n; ++k) hs[k] = analytic_consts(h[k]);
+  ashapes.assign(h, h + n);
+  shape_of = so;
+  if (!same) {
+    std::vector<uint32_t> off(1, 0), ps;
+    for (uint32_t q : dyn) {
+      const ysim_mesh_desc& Q = meshes[q];
+      if (Q.collidable)
+        for (uint32_t k = 0; k < n; ++k) {
+          const ysim_mesh_desc& T = meshes[h[k].obj_index];
+          if (T.collidable && T.group == Q.group) ps.push_back(k);
+        }
+      off.push_back((uint32_t)ps.size());
+    }
+    YSIM_TRY(release(d_apair_off));
+    YSIM_TRY(release(d_apair_shape));
+    YSIM_TRY(release(amark));
+    d_apair_off = d_apair_shape = amark = nullptr;
+    YSIM_TRY(put(&d_apair_off, off));
+    YSIM_TRY(put(&d_apair_shape, ps));
+    napairs = (uint32_t)ps.size();
+    YSIM_TRY(alloc(&amark, 2 * (size_t)napairs));
+    YSIM_TRY(zero(amark, 2 * (size_t)napairs));
+    snap_amark.clear();
+    tune_amark.clear();
+    YSIM_TRY(build_targets());
+    drop_graph();
+  }
+  if (n > ashape_cap) {
+    YSIM_TRY(release(d_ashapes));
+    d_ashapes = nullptr;
+    YSIM_TRY(alloc(&d_ashapes, n));
+    ashape_cap = n;
+    drop_graph();
+  }
+  if (n) YSIM_HIP_TRY(hipMemcpyAsync(d_ashapes, hs.data(), n * sizeof(AShape), hipMemcpyHostToDevice, st));
+  YSIM_HIP_TRY(hipStreamSynchronize(st));
+  return YSIM_OK;
+}
 
 // ====================================================================== C ABI
 extern "C" {
@@ -1216,6 +1449,12 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
   YSIM_TRY(s.put(&s.d_shape, shp));
   YSIM_TRY(s.put(&s.d_margin, margin));
   YSIM_TRY(s.put(&s.d_band, band));
+  {
+    std::vector<float> thick(num_meshes);
+    for (uint32_t m = 0; m < num_meshes; ++m)
+      thick[m] = P.margin > 0.f ? P.narrow_thickness : s.meshes[m].cloth.thickness;
+    YSIM_TRY(s.put(&s.d_thick, thick));
+  }
   YSIM_TRY(s.alloc(&s.tree, s.Ntot));
   YSIM_TRY(s.alloc(&s.parent, s.Ntot));
   YSIM_TRY(s.alloc(&s.visit, s.Ntot));
@@ -1233,22 +1472,8 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
     s.Tdyn += s.meshes[m].num_facets;
     s.dyn_voff.push_back(s.Vdyn);
   }
-  // targets per query (dynamic) mesh: collidable meshes of the same group
-  std::vector<uint32_t> toff(1, 0), tg;
-  for (uint32_t q : s.dyn) {
-    const ysim_mesh_desc& Q = s.meshes[q];
-    if (Q.collidable) {
-      for (uint32_t t = 0; t < num_meshes; ++t) {
-        const ysim_mesh_desc& T = s.meshes[t];
-        if (!T.collidable || T.group != Q.group || T.num_facets == 0) continue;
-        if (t == q && !P.self_collision) continue;
-        tg.push_back(t);
-      }
-    }
-    toff.push_back((uint32_t)tg.size());
-  }
-  YSIM_TRY(s.put(&s.d_tgt_off, toff));
-  YSIM_TRY(s.put(&s.d_tgt, tg));
+  s.shape_of.assign(num_meshes, -1);
+  YSIM_TRY(s.build_targets());
   // sort buffers sized for the largest group
   size_t maxT = 0, maxStatus = 0, maxSeg = 1;
   for (auto* gs : {&s.dyn_groups, &s.stat_groups})
@@ -1605,6 +1830,7 @@ int ysim_hip_scene_snapshot(ysim_scene* s) {
   YSIM_HIP_TRY(hipMemcpyAsync(s->snap_v, s->v[s->parity], b, hipMemcpyDeviceToDevice, s->st));
   YSIM_HIP_TRY(hipMemcpyAsync(s->snap_xp, s->x[s->parity ^ 1], b, hipMemcpyDeviceToDevice, s->st));
   s->snap_step = s->step;
+  YSIM_TRY(s->save_amark(s->snap_amark));
   s->snap_valid = true;
   return YSIM_OK;
 }
@@ -1618,6 +1844,7 @@ int ysim_hip_scene_restore(ysim_scene* s) {
   YSIM_HIP_TRY(hipMemcpyAsync(s->x[par], s->snap_x, b, hipMemcpyDeviceToDevice, s->st));
   YSIM_HIP_TRY(hipMemcpyAsync(s->v[par], s->snap_v, b, hipMemcpyDeviceToDevice, s->st));
   YSIM_HIP_TRY(hipMemcpyAsync(s->x[par ^ 1], s->snap_xp, b, hipMemcpyDeviceToDevice, s->st));
+  YSIM_TRY(s->load_amark(s->snap_amark, par));
   s->parity = par;
   s->step = s->snap_step;
   return YSIM_OK;
@@ -1646,6 +1873,11 @@ int ysim_hip_scene_counters(ysim_scene* s, uint64_t* h_out, uint32_t reset) {
   return YSIM_OK;
 }
 
+int ysim_hip_scene_set_analytic(ysim_scene* s, const ysim_analytic_shape* h_shapes, uint32_t n) {
+  YSIM_ARG(s, "null scene");
+  return s->set_analytic(h_shapes, n);
+}
+
 int ysim_hip_scene_flags(ysim_scene* s, uint32_t* h_flags) {
   YSIM_ARG(s && h_flags, "scene_flags: null argument");
   YSIM_HIP_TRY(hipMemcpyAsync(h_flags, s->flags, 4, hipMemcpyDeviceToHost, s->st));
@@ -1658,6 +1890,8 @@ int ysim_hip_scene_flags(ysim_scene* s, uint32_t* h_flags) {
 // ====================================================================== adjoint
 namespace {
 int adj_check(ysim_scene* s) {
+  if (!s->ashapes.empty())
+    return fail(YSIM_ERR_UNSUPPORTED, "adjoint: analytic colliders are not differentiated");
   if (s->dyn.size() != 1)
     return fail(YSIM_ERR_UNSUPPORTED, "adjoint: the scene must hold exactly one FastGrid cloth");
   if (s->P.self_collision)

@@ -232,6 +232,43 @@ struct ClothArgs {
 bool cloth_block_ok(uint32_t b);
 int launch_cloth(hipStream_t s, const ClothArgs& a);
 
+// ---------------------------------------------------------------- analytic.hip
+// One analytic collider on the device: the caller's record plus constants the
+// host computes once per install (the swept box before padding, the broad
+// rotation band, the narrow rotation angle (0 below 1e-9 and for planes) and
+// radius).
+struct AShape {
+  ysim_analytic_shape s;
+  float lo[3], rot_band;
+  float hi[3], pad0;
+  double theta, rmax;
+};
+static_assert(sizeof(AShape) == 144, "AShape layout");
+struct AnalyticArgs {
+  SegTable dsegs;                  // dynamic vertex segments (one per cloth)
+  MeshTables mt;
+  const AShape* shapes = nullptr;
+  const uint32_t* pair_off = nullptr;    // per cloth segment: its (cloth, shape) pairs, nseg + 1
+  const uint32_t* pair_shape = nullptr;  // shape index of each pair
+  const float* x = nullptr;
+  const float* xprev = nullptr;
+  const ysim_bvh_node* tree = nullptr;
+  const float* mesh_band = nullptr;      // radius + narrow thickness per mesh
+  const float* mesh_thick = nullptr;     // narrow thickness per mesh
+  float radius = 0.f;
+  const uint32_t* mark_prev = nullptr;   // pairing at the previous substep, per pair
+  uint32_t* mark_cur = nullptr;          // pairing at this substep
+  uint32_t* ccount = nullptr;
+  float4* cnd = nullptr;
+  uint2* cid = nullptr;
+  uint32_t cap = 16;
+  uint32_t* flags = nullptr;
+  uint32_t nitems = 0;
+};
+// After the query: per dynamic vertex, the contacts with the analytic
+// colliders its cloth is paired with, appended to the vertex's slots.
+int launch_analytic(hipStream_t s, const AnalyticArgs& a);
+
 // ---------------------------------------------------------------- adjoint.hip
 // Adjoint of springs + integrate + contact projection for one FastGrid cloth
 // whose contacts are against static meshes (contact slots indexed by the

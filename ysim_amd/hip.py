@@ -203,6 +203,13 @@ class HipScene:
         self.V = scene.num_vertices
         if kspring is not None:
             raise NotImplementedError("per-spring stiffness is served by the adjoint API")
+        if scene.analytic is not None and len(scene.analytic):
+            self.set_analytic(scene.analytic)
+
+    def set_analytic(self, shapes):
+        """Install analytic colliders (a numpy array of dtype abi.ANALYTIC)."""
+        a = np.ascontiguousarray(shapes, abi.ANALYTIC).reshape(-1)
+        call("ysim_hip_scene_set_analytic", self.h, a.ctypes.data if a.size else None, a.size)
 
     def close(self):
         if getattr(self, "h", None):

@@ -53,6 +53,7 @@ class Scene:
     params: dict
     name: str = ""
     info: dict = dataclasses.field(default_factory=dict)
+    analytic: Optional[np.ndarray] = None   # abi.ANALYTIC records (analytic colliders)
 
     # -------------------------------------------------------------- packing
     def packed(self):
@@ -307,6 +308,87 @@ def obstacle(x, f, group=0) -> Mesh:
     return Mesh(x=x, facets=f, behavior=BEHAVIOR_FLOAT, mass=1.0, group=group)
 
 
+# ------------------------------------------------------------------ analytic colliders
+COLLIDER_SPHERE, COLLIDER_BOX, COLLIDER_CYLINDER, COLLIDER_PLANE = 1, 2, 3, 4
+
+
+def quat_axis_angle(axis, angle: float) -> tuple:
+    """Unit quaternion (w, x, y, z) of a rotation by `angle` about `axis`."""
+    a = np.asarray(axis, np.float64)
+    a = a / np.linalg.norm(a)
+    s = math.sin(0.5 * angle)
+    return (math.cos(0.5 * angle), a[0] * s, a[1] * s, a[2] * s)
+
+
+def quat_apply(q, v: np.ndarray) -> np.ndarray:
+    """Rotate points (n, 3) by the unit quaternion q = (w, x, y, z), float64."""
+    w, qv = float(q[0]), np.asarray(q[1:], np.float64)
+    v = np.asarray(v, np.float64)
+    t = 2.0 * np.cross(qv[None, :], v)
+    return v + w * t + np.cross(qv[None, :], t)
+
+
+def analytic_shape(kind: int, obj_index: int, center, half=(0.0, 0.0, 0.0), radius=0.0, half_height=0.0,
+                   rot=(1.0, 0.0, 0.0, 0.0), prev_center=None, prev_rot=None, flags=1,
+                   behavior=BEHAVIOR_FLOAT) -> np.ndarray:
+    """One ysim_analytic_shape record (AnalyticShape field meaning,
+    include/sim/scene.hpp:717-737): sphere semi-axes in `half`, box
+    half-extents in `half`, cylinder `radius` / `half_height` about local +Y,
+    plane: centre and rotation only (outside = local +Y)."""
+    from . import abi
+    r = np.zeros(1, abi.ANALYTIC)
+    c = np.asarray(center, np.float32)
+    r["center_radius"][0] = (c[0], c[1], c[2], radius)
+    r["half_ext_height"][0] = (half[0], half[1], half[2], half_height)
+    r["rot_quat"][0] = rot
+    pc = c if prev_center is None else np.asarray(prev_center, np.float32)
+    r["prev_center_pad"][0] = (pc[0], pc[1], pc[2], 0.0)
+    r["prev_rot_quat"][0] = rot if prev_rot is None else prev_rot
+    r["kind"] = kind
+    r["obj_index"] = obj_index
+    r["behavior_type"] = behavior
+    r["flags"] = flags
+    return r
+
+
+def box_mesh(half, center, rot=(1.0, 0.0, 0.0, 0.0)) -> tuple:
+    """Closed box surface: 8 corners, 12 facets."""
+    h = np.asarray(half, np.float64)
+    c = np.array([[sx, sy, sz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)], np.float64) * h
+    x = quat_apply(rot, c) + np.asarray(center, np.float64)[None, :]
+    f = np.array([[0, 1, 3], [0, 3, 2], [4, 6, 7], [4, 7, 5], [0, 4, 5], [0, 5, 1],
+                  [2, 3, 7], [2, 7, 6], [0, 2, 6], [0, 6, 4], [1, 5, 7], [1, 7, 3]], np.uint32)
+    return x.astype(F32), f
+
+
+def cylinder_mesh(radius: float, half_height: float, center, rot=(1.0, 0.0, 0.0, 0.0), slices: int = 24) -> tuple:
+    """Capped cylinder about local +Y: two rings of `slices` vertices and two cap centres."""
+    th = 2 * np.pi * np.arange(slices) / slices
+    ring = np.stack([radius * np.cos(th), np.zeros(slices), radius * np.sin(th)], 1)
+    top = ring + [0.0, half_height, 0.0]
+    bot = ring - [0.0, half_height, 0.0]
+    v = np.concatenate([top, bot, [[0.0, half_height, 0.0], [0.0, -half_height, 0.0]]])
+    ct, cb = 2 * slices, 2 * slices + 1
+    f = []
+    for j in range(slices):
+        k = (j + 1) % slices
+        f += [(j, k, slices + k), (j, slices + k, slices + j), (ct, k, j), (cb, slices + j, slices + k)]
+    x = quat_apply(rot, v) + np.asarray(center, np.float64)[None, :]
+    return x.astype(F32), np.array(f, np.uint32)
+
+
+def plane_mesh(half_size: float, center, rot=(1.0, 0.0, 0.0, 0.0)) -> tuple:
+    """A square of the plane (local y = 0), 4 vertices, 2 facets."""
+    v = np.array([[-1, 0, -1], [1, 0, -1], [1, 0, 1], [-1, 0, 1]], np.float64) * half_size
+    x = quat_apply(rot, v) + np.asarray(center, np.float64)[None, :]
+    return x.astype(F32), np.array([[0, 2, 1], [0, 3, 2]], np.uint32)
+
+
+def collider_mesh(x, f, kind: int, group: int = 0) -> Mesh:
+    """A static mesh whose collider kind (shape lane) is `kind`."""
+    return Mesh(x=x, facets=f, behavior=BEHAVIOR_FLOAT, mass=1.0, group=group, shape=kind)
+
+
 # ------------------------------------------------------------------ params
 def sim_params(subh, margin, morton_bits=30, rebuild_every=0, self_collision=1, swept=1,
                broad_mode=0, hash_slots_log2=22, hash_cell=0.0, radius=0.0, kd=0.01,
@@ -375,6 +457,41 @@ def config_tri_drape(seed: int = 6, rings: int = 24, radius: float = 0.45, heigh
     # the disk centre lands on the sphere's pole, where 24 facets meet
     p = sim_params(subh=subh, margin=0.5 * h, morton_bits=bits, rebuild_every=rebuild_every, contact_capacity=32)
     return Scene([cl, obstacle(sx, sf)], p, name=f"tri-drape-{rings}")
+
+
+def config_analytic(seed: int = 7, n: int = 48, height: float = 0.8, subh=1 / 600, rebuild_every=1,
+                    kinds=("sphere", "box", "cylinder", "plane"), mesh_obstacle: bool = True,
+                    self_collision: int = 1) -> Scene:
+    """An n x n FastGrid cloth falling onto analytic colliders: an ellipsoid,
+    a rotated box, a tilted capped cylinder and a tilted half-space below them,
+    next to a triangle-mesh icosphere that keeps the triangle path in play.
+    Each collider also has a triangle mesh of its shape (shape lane = kind)."""
+    rng = np.random.default_rng(seed)
+    l = 1.0 / (n - 1)
+    meshes = [cloth_mesh(n, 1.0, (0.0, height, 0.0), rng, mass=1.0)]
+    shapes = []
+    if "sphere" in kinds:
+        c, e = (0.28, 0.55, 0.28), (0.15, 0.1, 0.13)
+        sx, sf = uv_sphere(24, 12, 1.0, (0.0, 0.0, 0.0))
+        meshes.append(collider_mesh((sx * np.asarray(e, F32) + np.asarray(c, F32)).astype(F32), sf, COLLIDER_SPHERE))
+        shapes.append(analytic_shape(COLLIDER_SPHERE, len(meshes) - 1, c, half=e))
+    if "box" in kinds:
+        c, h, q = (0.72, 0.56, 0.28), (0.12, 0.08, 0.1), quat_axis_angle((0.3, 1.0, 0.2), 0.6)
+        meshes.append(collider_mesh(*box_mesh(h, c, q), COLLIDER_BOX))
+        shapes.append(analytic_shape(COLLIDER_BOX, len(meshes) - 1, c, half=h, rot=q))
+    if "cylinder" in kinds:
+        c, q = (0.28, 0.55, 0.72), quat_axis_angle((1.0, 0.0, 0.3), 0.45)
+        meshes.append(collider_mesh(*cylinder_mesh(0.11, 0.09, c, q), COLLIDER_CYLINDER))
+        shapes.append(analytic_shape(COLLIDER_CYLINDER, len(meshes) - 1, c, radius=0.11, half_height=0.09, rot=q))
+    if "plane" in kinds:
+        c, q = (0.5, 0.38, 0.5), quat_axis_angle((0.2, 0.0, 1.0), 0.12)
+        meshes.append(collider_mesh(*plane_mesh(1.5, c, q), COLLIDER_PLANE))
+        shapes.append(analytic_shape(COLLIDER_PLANE, len(meshes) - 1, c, rot=q))
+    if mesh_obstacle:
+        meshes.append(obstacle(*icosphere(2, 0.12, (0.72, 0.5, 0.72))))
+    p = sim_params(subh=subh, margin=0.5 * l, morton_bits=30, rebuild_every=rebuild_every,
+                   self_collision=self_collision)
+    return Scene(meshes, p, name=f"analytic-{n}", analytic=np.concatenate(shapes) if shapes else None)
 
 
 def config_c3(seed: int = 3, count: int = 256, lo=16, hi=256, mass_mode="per_vertex", substeps=50):

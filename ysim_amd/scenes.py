@@ -45,6 +45,8 @@ class Mesh:
     group: int = 0
     shape: int = 0
     fixed: Optional[np.ndarray] = None   # (V,) float32 pin mask, 1 = free
+    apply_gravity: bool = True           # per-object environment-force gates
+    apply_wind: bool = True
 
 
 @dataclasses.dataclass
@@ -54,11 +56,14 @@ class Scene:
     name: str = ""
     info: dict = dataclasses.field(default_factory=dict)
     analytic: Optional[np.ndarray] = None   # abi.ANALYTIC records (analytic colliders)
+    gravity: tuple = GRAVITY
+    wind: tuple = (0.0, 0.0, 0.0)
 
     # -------------------------------------------------------------- packing
     def packed(self):
         xs, vs, ms, es, fx, fs = [], [], [], [], [], []
-        g = np.array(GRAVITY, dtype=F32)
+        g = np.array(self.gravity, dtype=F32)
+        w = np.array(self.wind, dtype=F32)
         for m in self.meshes:
             V = m.x.shape[0]
             xs.append(np.ascontiguousarray(m.x, dtype=F32))
@@ -66,8 +71,13 @@ class Scene:
             mass = np.full(V, m.mass, F32)
             ms.append(mass)
             if m.behavior != BEHAVIOR_FLOAT:
-                # applyEnvironmentForces: ext = gravity * m (simulator.hpp:3148-3164)
-                es.append((g[None, :] * mass[:, None]).astype(F32))
+                # applyEnvironmentForces: ext = gravity * m when the object
+                # takes gravity, + wind for cloths that take wind, in float
+                # (simulator.hpp:3122-3166)
+                e = (g[None, :] * mass[:, None]).astype(F32) if m.apply_gravity else np.zeros((V, 3), F32)
+                if m.apply_wind:
+                    e = (e + w[None, :]).astype(F32)
+                es.append(e)
             else:
                 es.append(np.zeros((V, 3), F32))
             fx.append(np.ones(V, F32) if m.fixed is None else m.fixed.astype(F32))

@@ -320,7 +320,7 @@ def main():
     # buckets) whose kernels return at once unless the check failed
     launches = {"box": 1, "morton": 1, "sort": 5 if incremental else passes, "hierarchy": 1, "refit": 1,
                 "query": 1, "integrate": 1}
-    kernels = {"box": "k_box",
+    kernels = {"box": "k_box1",
                "morton": "k_morton_inc<uint64_t>" if incremental else f"k_morton<uint64_t, {digit_bits}, 8>",
                "sort": "k_window_sort_shifted<uint64_t>" if incremental
                else f"k_onesweep<uint64_t, {digit_bits}, {int(tuning['sort_items'])}>",

@@ -238,7 +238,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch tuning")
     ap.add_argument("--tuning", default="",
                     help="fixed tuning instead of autotune: sort_digit_bits,sort_items,query_block,cloth_block,"
-                         "refit_block[,sort_incremental,query_waves,query_tile] (e.g. 8,16,128,256,1024,1,5,1)")
+                         "refit_block[,sort_incremental,query_waves,query_tile,build_mode] "
+                         "(e.g. 8,16,128,256,1024,1,5,1,0)")
     ap.add_argument("--no-copy", action="store_true", help="skip the copy-bandwidth ceiling")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the C5 LBVH-vs-hash, adjoint and copy-ceiling measurements")
@@ -278,7 +279,7 @@ def main():
     tuning = hs.tuning()
     if args.tuning:
         names = ("sort_digit_bits", "sort_items", "query_block", "cloth_block", "refit_block", "sort_incremental",
-                 "query_waves", "query_tile")
+                 "query_waves", "query_tile", "build_mode")
         hs.set_tuning(**dict(zip(names, (int(t) for t in args.tuning.split(",")))))
         tuning = hs.tuning()
         log(f"fixed tuning: {tuning}")
@@ -328,6 +329,14 @@ def main():
                "query": "k_query_tile" if int(tuning.get("query_tile", 0))
                else f"k_query<false, {int(tuning['query_block'])}, {int(tuning['query_waves'])}>",
                "integrate": f"k_cloth<{int(tuning['cloth_block'])}>"}
+    if int(tuning.get("build_mode", 0)):
+        # the agglomerative build writes the swept boxes itself: the refit
+        # stage has no launch on a rebuilding substep
+        kernels["hierarchy"] = "k_agglo<uint64_t, false>"
+        kernels["refit"] = "(fused into k_agglo)"
+        sb["hierarchy"] += sb["refit"]
+        sb["refit"] = 0
+        launches["refit"] = 0
     stages = {}
     for s, ms in stages_ms.items():
         gbs = sb[s] / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
@@ -335,12 +344,12 @@ def main():
             ks = ["k_window_sort_shifted<uint64_t>", "k_sorted_check<uint64_t>", "k_morton<uint64_t, 11, 1>",
                   f"k_onesweep<uint64_t, 11, {int(tuning['sort_items'])}>", "k_bucket_sort<uint64_t, 16>"]
         else:
-            ks = [kernels[s]] * launches[s]
+            ks = [kernels[s]] * launches[s] if launches[s] else []
         tbs = [pmc_traffic(k)[0] for k in ks]
         stages[s] = {"ms": round(ms, 5), "algorithmic_bytes": int(sb[s]), "GBps": round(gbs, 1),
                      "frac_hbm": round(gbs / HBM_PEAK_GBS, 4), "launches": launches[s], "kernel": kernels[s],
                      "pmc_hbm_bytes": None if any(t is None for t in tbs) else int(sum(tbs))}
-    dom = max(stages_ms, key=lambda s: stages_ms[s])
+    dom = max((s for s in stages_ms if launches[s]), key=lambda s: stages_ms[s])
     per_launch_bytes = sb[dom] / launches[dom]
     per_launch_ms = stages_ms[dom] / launches[dom]
     achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9

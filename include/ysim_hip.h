@@ -187,6 +187,17 @@ int ysim_hip_lbvh_hierarchy(ysim_ctx* ctx, const void* d_sorted_keys,
                             const uint32_t* d_sorted_idx, uint32_t n, uint32_t bits,
                             ysim_bvh_node* d_tree, int32_t* d_parent);
 
+/* Agglomerative (Apetrei 2014) build over sorted codes: agglomerativeBuild_Tri +
+ * agglomerativeSwapRoot (src/metal/bvh.metal:884-1011, 1135-1199), host
+ * BVH::agglomerativeBuildGPU / agglomerativeSwapRootGPU (include/sim/bvh.hpp:
+ * 649-694).  Child links, d_parent and the boxes of the 2n-1 slots in one pass
+ * (leaf box = facet hull of d_x); the same tree as lbvh_hierarchy + lbvh_refit
+ * with internal nodes numbered by their split and the root moved to slot 0. */
+int ysim_hip_lbvh_agglomerative(ysim_ctx* ctx, const void* d_sorted_keys,
+                                const uint32_t* d_sorted_idx, uint32_t n, uint32_t bits,
+                                const float* d_x, const uint32_t* d_facets,
+                                ysim_bvh_node* d_tree, int32_t* d_parent);
+
 /* Leaf boxes + atomic bottom-up union: buildLeaf_Tri / buildSweptLeaf_Tri +
  * bottomUpBoxes (src/metal/bvh.metal:260-351, 464-586).  swept != 0 unions each
  * vertex with x + v*dt (enlargeTrajectory, bvh.hpp:1608-1682). */
@@ -363,6 +374,9 @@ typedef struct ysim_tuning {
                               * tile's union box (default); 0: one thread per vertex (query_block and
                               * query_waves apply); 2: as 1 with 128-leaf and 32-node tile buffers, so
                               * that the tiles' buffer flushes and per-vertex redos run (tests those) */
+  uint32_t build_mode;       /* rebuilds: 0 Karras hierarchy + refit (default); 1 agglomerative (Apetrei)
+                              * build with the leaf boxes fused in (one kernel), refits between rebuilds
+                              * through parent links.  Same tree, nodes numbered by their split */
 } ysim_tuning;
 int ysim_hip_scene_get_tuning(ysim_scene* scene, ysim_tuning* h_out);
 int ysim_hip_scene_set_tuning(ysim_scene* scene, const ysim_tuning* h_in);

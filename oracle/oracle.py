@@ -82,6 +82,8 @@ def lib():
         "yo_scene_get_tree": (None, [vp, C.c_uint32, vp, C.POINTER(C.c_int32), u64p, u32p]),
         "yo_scene_anomaly": (C.c_uint32, [vp]),
         "yo_scene_set_analytic": (C.c_int, [vp, vp, C.c_uint32]),
+        "yo_scene_set_build_mode": (None, [vp, C.c_uint32]),
+        "yo_agglomerative": (None, [u64p, u32p, C.c_uint32, C.c_uint32, vp, C.POINTER(C.c_int32)]),
         "yo_scene_analytic_markers": (C.c_uint32, [vp, u32p, u8p]),
         "yo_narrow_analytic_f32": (C.c_int, [vp, f32p, f32p, C.c_float, f32p]),
         "yo_narrow_analytic_f64": (C.c_int, [vp, f64p, f64p, C.c_double, f64p]),
@@ -148,6 +150,17 @@ def karras(keys, idx, bits):
     k = np.ascontiguousarray(keys, np.uint64)
     i = np.ascontiguousarray(idx, np.uint32)
     lib().yo_karras(P(k, u64p), P(i, u32p), n, bits, tree.ctypes.data, P(parent, C.POINTER(C.c_int32)))
+    return tree, parent
+
+
+def agglomerative(keys, idx, bits):
+    """Apetrei hierarchy (topology, root in slot 0) over sorted keys."""
+    n = keys.size
+    tree = np.zeros(max(2 * n - 1, 0), abi.BVH_NODE)
+    parent = np.full(max(2 * n - 1, 0), -1, np.int32)
+    k = np.ascontiguousarray(keys, np.uint64)
+    i = np.ascontiguousarray(idx, np.uint32)
+    lib().yo_agglomerative(P(k, u64p), P(i, u32p), n, bits, tree.ctypes.data, P(parent, C.POINTER(C.c_int32)))
     return tree, parent
 
 
@@ -310,6 +323,10 @@ class OracleScene:
         a = np.ascontiguousarray(shapes, abi.ANALYTIC).reshape(-1)
         if lib().yo_scene_set_analytic(self.h, a.ctypes.data if a.size else None, a.size) != 0:
             raise ValueError("oracle: analytic shapes rejected")
+
+    def set_build_mode(self, mode: int):
+        """Builder of later dynamic rebuilds: 0 Karras, 1 agglomerative."""
+        lib().yo_scene_set_build_mode(self.h, mode)
 
     def analytic_markers(self):
         """(cloth mesh, shape mesh) pairs and whether each was paired at the last substep."""

@@ -849,6 +849,89 @@ void yo_karras(const uint64_t* keys, const uint32_t* idx, uint32_t n, uint32_t b
   }
 }
 
+// Agglomerative (Apetrei 2014) hierarchy over the same sorted keys:
+// agglomerativeBuild_Tri + agglomerativeSwapRoot (src/metal/bvh.metal:884-1011,
+// 1135-1199).  Each leaf climbs: a node covering [L, R] merges toward the side
+// whose neighbouring key pair differs less (code XOR, then facet-index XOR on
+// equal codes), its parent is the internal node numbered by that boundary (R
+// when merging right, L - 1 when merging left), and the second arrival at a
+// parent continues.  The parent numbering makes every internal node the split
+// of its range; the root, found at some slot k, is then swapped into slot 0
+// with every link that named either slot retargeted.  Topology only (the same
+// binary radix tree as yo_karras, numbered differently); boxes come from
+// yo_refit.  Sequential: the result does not depend on arrival order.
+void yo_agglomerative(const uint64_t* keys, const uint32_t* idx, uint32_t n, uint32_t bits, ysim_bvh_node* tree,
+                      int32_t* parent) {
+  if (n == 0) return;
+  for (uint32_t i = 0; i < n; ++i) {
+    ysim_bvh_node& lf = tree[n - 1 + i];
+    lf.child_a = -1;
+    lf.child_b = (int32_t)idx[i];
+  }
+  if (n == 1) {
+    parent[0] = -1;
+    return;
+  }
+  const uint64_t mask = (bits > 32) ? ~0ull : 0xffffffffull;
+  std::vector<uint8_t> visit(n - 1, 0);
+  std::vector<int> rl(n - 1, -1), rr(n - 1, -1);
+  int root = -1;
+  for (uint32_t i = 0; i < n; ++i) {
+    int cur = (int)(n - 1 + i), L = (int)i, R = (int)i;
+    while (true) {
+      if (L == 0 && R == (int)n - 1) {
+        root = cur;
+        break;
+      }
+      bool right;
+      if (L == 0) {
+        right = true;
+      } else if (R == (int)n - 1) {
+        right = false;
+      } else {
+        const uint64_t dr = (keys[R] & mask) ^ (keys[R + 1] & mask), dl = (keys[L - 1] & mask) ^ (keys[L] & mask);
+        right = (dr != dl) ? (dr < dl) : ((idx[R] ^ idx[R + 1]) < (idx[L - 1] ^ idx[L]));
+      }
+      int p;
+      if (right) {
+        p = R;
+        tree[p].child_a = cur;
+        rl[p] = L;
+      } else {
+        p = L - 1;
+        tree[p].child_b = cur;
+        rr[p] = R;
+      }
+      parent[cur] = p;
+      if (visit[p]++ == 0) break;  // first arrival
+      cur = p;
+      L = rl[p];
+      R = rr[p];
+    }
+  }
+  // agglomerativeSwapRoot
+  const int k = root;
+  if (k > 0) {
+    const ysim_bvh_node Rn = tree[k], On = tree[0];
+    const int p0 = parent[0];
+    const int ra = (Rn.child_a == 0) ? k : Rn.child_a, rb = (Rn.child_b == 0) ? k : Rn.child_b;
+    tree[0] = Rn;
+    tree[0].child_a = ra;
+    tree[0].child_b = rb;
+    tree[k] = On;
+    parent[On.child_a] = k;
+    parent[On.child_b] = k;
+    parent[ra] = 0;
+    parent[rb] = 0;
+    if (p0 != k) {
+      if (tree[p0].child_a == 0) tree[p0].child_a = k;
+      else tree[p0].child_b = k;
+      parent[k] = p0;
+    }
+  }
+  parent[0] = -1;
+}
+
 // ------------------------------------------------------------------ S4/S5 refit
 // Leaf boxes (buildLeaf_Tri / buildSweptLeaf_Tri, src/metal/bvh.metal:260-351;
 // CPU enlarge bvh.hpp:1669-1680: s = x + v*dt with separate rounding) then exact
@@ -1164,6 +1247,7 @@ struct OScene {
   std::vector<uint32_t> apair_q, apair_k;
   std::vector<uint8_t> held;
   bool analytic_active() const { return !shapes.empty() && P.broad_mode == 0; }
+  uint32_t build_mode = 0;  // dynamic rebuilds: 0 Karras, 1 agglomerative
   float thick(const OMesh& Q) const { return P.margin > 0.f ? P.narrow_thickness : Q.d.cloth.thickness; }
 };
 
@@ -1184,7 +1268,10 @@ void build_tree(OScene& S, OMesh& M) {
   yo_scene_box(x, M.d.num_vertices, box);
   yo_morton(x, f, n, box, S.P.morton_bits, M.keys.data(), M.idx.data());
   yo_sort_pairs(M.keys.data(), M.idx.data(), n, S.P.morton_bits == 30 ? 30 : 63);
-  yo_karras(M.keys.data(), M.idx.data(), n, S.P.morton_bits, M.tree.data(), M.parent.data());
+  if (S.build_mode == 1 && is_dynamic(M.d) && M.built)
+    yo_agglomerative(M.keys.data(), M.idx.data(), n, S.P.morton_bits, M.tree.data(), M.parent.data());
+  else
+    yo_karras(M.keys.data(), M.idx.data(), n, S.P.morton_bits, M.tree.data(), M.parent.data());
   M.built = true;
 }
 
@@ -1568,6 +1655,10 @@ void yo_scene_get_tree(void* s, uint32_t mesh, ysim_bvh_node* nodes, int32_t* pa
   if (idx) std::memcpy(idx, M.idx.data(), 4 * M.idx.size());
 }
 uint32_t yo_scene_anomaly(void* s) { return ((OScene*)s)->anomaly; }
+
+// Builder of later dynamic rebuilds (the library's tuning build_mode): 0
+// Karras, 1 agglomerative.  The trees built at creation are Karras.
+void yo_scene_set_build_mode(void* s, uint32_t mode) { ((OScene*)s)->build_mode = mode; }
 
 // Install the analytic colliders (same contract as the library's
 // scene_set_analytic).  Returns 0, or -1 with the previous set unchanged.

@@ -45,6 +45,7 @@ _SIG = {
     "ysim_hip_morton": (C.c_int, [vp, vp, vp, u32, vp, u32, vp, vp]),
     "ysim_hip_sort_pairs": (C.c_int, [vp, vp, vp, u32, u32]),
     "ysim_hip_lbvh_hierarchy": (C.c_int, [vp, vp, vp, u32, u32, vp, vp]),
+    "ysim_hip_lbvh_agglomerative": (C.c_int, [vp, vp, vp, u32, u32, vp, vp, vp, vp]),
     "ysim_hip_lbvh_refit": (C.c_int, [vp, vp, vp, C.c_float, u32, vp, u32, vp, vp]),
     "ysim_hip_broad_query": (C.c_int, [vp, vp, u32, u32, u32, u32, vp, u32, vp, u32, u32, u32,
                                        C.c_float, vp, u32, vp, u32p]),

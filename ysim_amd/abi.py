@@ -39,7 +39,7 @@ class Tuning(C.Structure):
                 ("cloth_block", C.c_uint32), ("ms_per_substep", C.c_float),
                 ("ms_per_substep_default", C.c_float), ("refit_block", C.c_uint32),
                 ("sort_hybrid", C.c_uint32), ("sort_incremental", C.c_uint32), ("query_waves", C.c_uint32),
-                ("query_tile", C.c_uint32)]
+                ("query_tile", C.c_uint32), ("build_mode", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

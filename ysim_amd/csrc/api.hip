@@ -241,7 +241,7 @@ struct ysim_scene {
   } adj;
   float graph_kd = 0.f;
   const float* graph_ks = nullptr;
-  ysim_tuning tune{8, 8, 128, 256, 0.f, 0.f, 256, 0, 0, 5, 1};
+  ysim_tuning tune{8, 8, 128, 256, 0.f, 0.f, 256, 0, 0, 5, 1, 0};
   float* tune_buf = nullptr;   // autotune state save: x, v, xPrev of the whole scene
   // analytic colliders: the installed shapes, their device copies, the
   // (cloth, shape) pairs per dynamic segment and each pair's pairing flag of
@@ -258,6 +258,10 @@ struct ysim_scene {
   std::vector<uint32_t> tune_amark;
   float* d_thick = nullptr;
   bool analytic_on() const { return !ashapes.empty() && !hash_mode; }
+  // agglomerative builds: range scratch (two ints per node) and whether the
+  // dynamic trees were last built by it (their refit links are then absent)
+  int2* agg_range = nullptr;
+  bool tree_agglo = false;
 
   int grow(float** p, size_t& have, size_t need) {
     if (need <= have && *p) return YSIM_OK;
@@ -302,6 +306,8 @@ struct ysim_scene {
     if (t.sort_incremental > 1) return fail(YSIM_ERR_INVALID, "tuning: sort_incremental must be 0 or 1");
     if (!query_waves_ok(t.query_waves)) return fail(YSIM_ERR_INVALID, "tuning: query_waves must be 5, 6 or 8");
     if (t.query_tile > 2) return fail(YSIM_ERR_INVALID, "tuning: query_tile must be 0, 1 or 2");
+    if (t.build_mode > 1) return fail(YSIM_ERR_INVALID, "tuning: build_mode must be 0 or 1");
+    if (t.build_mode && !agg_range) YSIM_TRY(alloc(&agg_range, Ntot));
     const bool replan = t.sort_digit_bits != tune.sort_digit_bits || t.sort_items != tune.sort_items ||
                         t.sort_hybrid != tune.sort_hybrid || t.sort_incremental != tune.sort_incremental;
     tune.sort_digit_bits = t.sort_digit_bits;
@@ -313,6 +319,7 @@ struct ysim_scene {
     tune.sort_incremental = t.sort_incremental;
     tune.query_waves = t.query_waves;
     tune.query_tile = t.query_tile;
+    tune.build_mode = t.build_mode;
     if (replan) {
       for (auto& g : dyn_groups)
         g.plan = (t.sort_hybrid || t.sort_incremental)
@@ -405,6 +412,15 @@ struct ysim_scene {
         float ms = 0.f;
         if ((rc = measure(t, ms)) != YSIM_OK) break;
         if (ms < best_ms * kGain) { best_ms = ms; best = t; }
+      }
+    }
+    if (rc == YSIM_OK && rebuilds) {
+      ysim_tuning t = best;
+      t.build_mode = best.build_mode ? 0u : 1u;
+      float ms = 0.f;
+      if ((rc = measure(t, ms)) == YSIM_OK && ms < best_ms * kGain) {
+        best_ms = ms;
+        best = t;
       }
     }
     if (rc == YSIM_OK && !hash_mode)
@@ -613,7 +629,9 @@ struct ysim_scene {
     return YSIM_OK;
   }
 
-  int enqueue_build(BuildGroup& g, const float* xc) {
+  // agglo: the group's build is the fused agglomerative pass with leaf boxes
+  // (x + v dt when swept), so the substep's refit is not needed after it
+  int enqueue_build(BuildGroup& g, const float* xc, const float* vc = nullptr, bool agglo = false) {
     if (!g.T) return YSIM_OK;
     YSIM_TRY(launch_box(st, g.vsegs, mt, xc, g.V, box_acc, g.meshes.size() == 1 ? g.meshes[0] : 0xffffffffu));
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[1], st));
@@ -630,8 +648,7 @@ struct ysim_scene {
       YSIM_TRY(launch_sort_hybrid(st, g.plan, keys[0], vals[0], keys[1], vals[1], hist, status, counters, epoch,
                                   flags, need_full));
       if (timing) YSIM_HIP_TRY(hipEventRecord(ev[3], st));
-      YSIM_TRY(launch_hierarchy(st, g.tsegs, mt, keys[0], g.key64, vals[0], g.code_bits, tree, parent, visit, g.T,
-                                box_acc, g.tsegs.nseg, hist, (uint32_t)g.plan.hist_words, g.perm, links));
+      YSIM_TRY(enqueue_hierarchy(g, 0, xc, vc, agglo));
       last_sorted = 0;
       return YSIM_OK;
     }
@@ -646,9 +663,22 @@ struct ysim_scene {
                                   flags));
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[3], st));
     const uint32_t sorted = (g.plan.hybrid || g.plan.passes % 2 == 0) ? 0 : 1;
+    YSIM_TRY(enqueue_hierarchy(g, sorted, xc, vc, agglo));
+    last_sorted = sorted;
+    return YSIM_OK;
+  }
+
+  int enqueue_hierarchy(BuildGroup& g, uint32_t sorted, const float* xc, const float* vc, bool agglo) {
+    if (agglo) {
+      YSIM_TRY(launch_agglo(st, g.tsegs, mt, keys[sorted], g.key64, vals[sorted], g.code_bits, xc, vc, P.subh,
+                            P.swept != 0 && vc, facets, tree, parent, visit, agg_range, g.T, box_acc, g.tsegs.nseg,
+                            hist, (uint32_t)g.plan.hist_words, g.perm, flags, formal_refit));
+      tree_agglo = true;
+      return YSIM_OK;
+    }
     YSIM_TRY(launch_hierarchy(st, g.tsegs, mt, keys[sorted], g.key64, vals[sorted], g.code_bits, tree, parent,
                               visit, g.T, box_acc, g.tsegs.nseg, hist, (uint32_t)g.plan.hist_words, g.perm, links));
-    last_sorted = sorted;
+    tree_agglo = false;
     return YSIM_OK;
   }
 
@@ -696,14 +726,18 @@ struct ysim_scene {
       if (timing) YSIM_HIP_TRY(hipEventRecord(ev[5], st));
       return YSIM_OK;
     }
+    const bool agglo = rebuild && tune.build_mode == 1;
     if (rebuild) {
-      for (auto& g : dyn_groups) YSIM_TRY(enqueue_build(g, xc));
+      for (auto& g : dyn_groups) YSIM_TRY(enqueue_build(g, xc, v[par], agglo));
     } else if (timing) {
       for (int e = 1; e <= 3; ++e) YSIM_HIP_TRY(hipEventRecord(ev[e], st));
     }
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[4], st));
-    YSIM_TRY(launch_refit(st, dyn_tsegs, mt, xc, v[par], P.subh, P.swept != 0, facets, tree, parent, visit, Tdyn,
-                          flags, formal_refit, &links, tune.refit_block));
+    // the agglomerative build already wrote the swept boxes; a tree built by
+    // it has no refit links, so its refits climb through parent links
+    if (!agglo)
+      YSIM_TRY(launch_refit(st, dyn_tsegs, mt, xc, v[par], P.subh, P.swept != 0, facets, tree, parent, visit, Tdyn,
+                            flags, formal_refit, tree_agglo ? nullptr : &links, tune.refit_block));
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[5], st));
     return YSIM_OK;
   }
@@ -1182,6 +1216,31 @@ int ysim_hip_lbvh_hierarchy(ysim_ctx* ctx, const void* d_sorted_keys, const uint
   YSIM_TRY(launch_hierarchy(ctx->stream, u.seg(), u.mt(), d_sorted_keys, bits > 32, d_sorted_idx, bits, d_tree,
                             d_parent, visit, n, nullptr, 0, nullptr, 0));
   YSIM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return YSIM_OK;
+}
+
+int ysim_hip_lbvh_agglomerative(ysim_ctx* ctx, const void* d_sorted_keys, const uint32_t* d_sorted_idx, uint32_t n,
+                                uint32_t bits, const float* d_x, const uint32_t* d_facets, ysim_bvh_node* d_tree,
+                                int32_t* d_parent) {
+  YSIM_ARG(ctx && d_sorted_keys && d_sorted_idx && d_x && d_facets && d_tree && d_parent,
+           "lbvh_agglomerative: null argument");
+  YSIM_ARG(bits == 30 || bits == 63 || bits == 32 || bits == 64, "lbvh_agglomerative: bits");
+  if (!n) return YSIM_OK;
+  YSIM_TRY(ctx->ensure_scratch((size_t)(1 << 20) + (size_t)n * 24 + 4096));
+  Unit u;
+  YSIM_TRY(unit_table(ctx, n, u));
+  uint32_t* visit = (uint32_t*)((char*)ctx->scratch + (1 << 20));
+  int2* range = (int2*)((char*)ctx->scratch + (1 << 20) + (((size_t)n * 8 + 255) & ~(size_t)255));
+  uint32_t* err = scratch_words(ctx, 64);
+  hipLaunchKernelGGL(k_zero_u32, dim3(ceil_div(2 * (size_t)n, 256)), dim3(256), 0, ctx->stream, visit, 2 * n);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, ctx->stream, err, 1u);
+  YSIM_TRY(launch_agglo(ctx->stream, u.seg(), u.mt(), d_sorted_keys, bits > 32, d_sorted_idx, bits, d_x, nullptr, 0.f,
+                        false, d_facets, d_tree, d_parent, visit, range, n, nullptr, 0, nullptr, 0, nullptr, err,
+                        ctx->formal_refit));
+  uint32_t herr = 0;
+  YSIM_HIP_TRY(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, ctx->stream));
+  YSIM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (herr) return fail(YSIM_ERR_INVALID, "lbvh_agglomerative: keys not sorted (guard fired)");
   return YSIM_OK;
 }
 

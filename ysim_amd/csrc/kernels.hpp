@@ -66,6 +66,16 @@ int launch_hierarchy(hipStream_t s, SegTable tsegs, MeshTables mt, const void* k
                      uint32_t* visit, uint32_t nitems, uint32_t* box_acc, uint32_t nbox,
                      uint32_t* hist, uint32_t hist_words, uint32_t* perm_out = nullptr,
                      RefitLinks rl = RefitLinks{});
+// S3 alternative: agglomerative (Apetrei) hierarchy fused with the leaf boxes
+// (swept optional) and their bottom-up union, root swapped into slot 0;
+// writes child and parent links (no refit links), the sorted facet order and
+// the box / histogram resets of launch_hierarchy.  range: per node window, two
+// ints of scratch.  Bit 2 of *err on a corrupt key order.
+int launch_agglo(hipStream_t s, SegTable tsegs, MeshTables mt, const void* keys, bool key64, const uint32_t* vals,
+                 uint32_t code_bits, const float* x, const float* v, float dt, bool swept, const uint32_t* facets,
+                 ysim_bvh_node* tree, int32_t* parent, uint32_t* visit, int2* range, uint32_t nitems,
+                 uint32_t* box_acc, uint32_t nbox, uint32_t* hist, uint32_t hist_words, uint32_t* perm_out,
+                 uint32_t* err, bool formal);
 // S4+S5: leaf boxes (swept optional) + atomic bottom-up union per segment.
 // With rl (from launch_hierarchy) the levels inside blocks of `block` (256,
 // 512 or 1024) sorted leaves are combined in LDS.

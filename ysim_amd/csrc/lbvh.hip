@@ -393,6 +393,234 @@ int launch_hierarchy(hipStream_t s, SegTable tsegs, MeshTables mt, const void* k
   return YSIM_OK;
 }
 
+// ------------------------------------------------------------------ S3' agglomerative build
+// Apetrei's single-pass hierarchy (agglomerativeBuild_Tri + agglomerativeSwapRoot,
+// bvh.metal:884-1011, 1135-1199), fused here with the leaf boxes of the
+// substep's refit (swept when requested), so a rebuild needs no separate refit.
+// One thread per sorted leaf climbs: its node [L, R] merges toward the side
+// whose neighbouring key pair differs less (code XOR, facet-index XOR on
+// equal codes); the parent is internal node R (merging right: this node is
+// its child_a and records L) or L - 1 (merging left: child_b, records R).  The
+// second arrival at a parent (visit counters advance by two per pass, as in
+// the refit) reads the sibling's link, box and range end and continues with
+// the union in (child_a, child_b) order.  The thread that completes [0, n-1]
+// holds the root at slot k and swaps it into slot 0, retargeting every link
+// that named either slot; nothing else of its segment is still running.
+// Links, ranges and boxes that another workgroup reads are published as in
+// the refit (device-coherent stores, then the arrival RMW; FORMAL: acq_rel).
+// Also does the hierarchy kernel's duties: the sorted facet order for the
+// next incremental build and the box / histogram resets.
+template <typename K, bool FORMAL>
+__global__ __launch_bounds__(256) void k_agglo(SegTable ts, MeshTables mt, const K* __restrict__ keys,
+                                               const uint32_t* __restrict__ vals, uint32_t code_bits,
+                                               const float* __restrict__ x, const float* __restrict__ v, float dt,
+                                               int swept, const uint32_t* __restrict__ facets, ysim_bvh_node* tree,
+                                               int32_t* parent, uint32_t* visit, int2* range, uint32_t n,
+                                               uint32_t* __restrict__ box_acc, uint32_t nbox,
+                                               uint32_t* __restrict__ hist, uint32_t hist_words,
+                                               uint32_t* __restrict__ perm_out, uint32_t* err) {
+  const uint32_t j = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  if (j < nbox * 6) box_acc[j] = ((j % 6) < 3) ? 0xffffffffu : 0u;
+  if (j < hist_words) hist[j] = 0;
+  if (j >= n) return;
+  const uint32_t sg = seg_of(ts, j);
+  const uint32_t m = ts.mesh[sg];
+  const uint32_t s = ts.off[sg];
+  const int nn = (int)(ts.off[sg + 1] - s);
+  const int i = (int)(j - s);
+  const uint32_t base = mt.noff[m];
+  ysim_bvh_node* T = tree + base;
+  int32_t* P = parent + base;
+  int2* RG = range + base;
+  uint32_t* C = visit + base;
+  const K* key = keys + s;
+  const uint32_t* idx = vals + s;
+  const uint64_t mask = (code_bits >= 64) ? ~0ull : ((1ull << code_bits) - 1);
+  const uint32_t fid = idx[i];
+  if (perm_out) perm_out[j] = fid;
+
+  const uint32_t* f = facets + 3 * (size_t)(mt.foff[m] + fid);
+  const uint32_t vb = mt.voff[m];
+  f3 lo = ld3(x, vb + f[0]), hi = lo;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    f3 p = ld3(x, vb + f[c]);
+    lo = vmin(lo, p);
+    hi = vmax(hi, p);
+    if (swept) {
+      f3 vv = ld3(v, vb + f[c]);
+      f3 q = p + vv * dt;
+      lo = vmin(lo, q);
+      hi = vmax(hi, q);
+    }
+  }
+  auto store_box = [&](int node, f3 a, f3 b) {
+    float* q = reinterpret_cast<float*>(T + node);
+    if (FORMAL) {
+      q[0] = a.x; q[1] = a.y; q[2] = a.z; q[4] = b.x; q[5] = b.y; q[6] = b.z;
+    } else {
+      st_agent_u64(reinterpret_cast<uint64_t*>(q + 0), pack2(a.x, a.y));
+      st_agent(q + 2, a.z);
+      st_agent_u64(reinterpret_cast<uint64_t*>(q + 4), pack2(b.x, b.y));
+      st_agent(q + 6, b.z);
+    }
+  };
+  auto st_i = [&](int32_t* q, int32_t val) {
+    if (FORMAL) *q = val;
+    else st_agent_u32(reinterpret_cast<uint32_t*>(q), (uint32_t)val);
+  };
+  auto ld_i = [&](const int32_t* q) -> int32_t {
+    if (FORMAL) return *q;
+    return (int32_t)ld_agent_u32(reinterpret_cast<const uint32_t*>(q));
+  };
+  int cur = nn - 1 + i;
+  T[cur].child_a = -1;
+  T[cur].child_b = (int32_t)fid;
+  store_box(cur, lo, hi);
+  if (nn <= 1) {
+    P[0] = -1;
+    return;
+  }
+  int L = i, R = i;
+  for (int step = 0; step < 2 * nn; ++step) {
+    if (L == 0 && R == nn - 1) {
+      // agglomerativeSwapRoot: every other node of this segment is final
+      const int k = cur;
+      if (k == 0) {
+        float* q0 = reinterpret_cast<float*>(T);
+        q0[0] = lo.x; q0[1] = lo.y; q0[2] = lo.z; q0[4] = hi.x; q0[5] = hi.y; q0[6] = hi.z;
+      } else {
+        const int rca = ld_i(&T[k].child_a), rcb = ld_i(&T[k].child_b);
+        const int oa = ld_i(&T[0].child_a), ob = ld_i(&T[0].child_b);
+        const int p0 = ld_i(&P[0]);
+        float ob_box[6];
+        {
+          const float* q = reinterpret_cast<const float*>(T);
+          if (FORMAL) {
+            ob_box[0] = q[0]; ob_box[1] = q[1]; ob_box[2] = q[2]; ob_box[3] = q[4]; ob_box[4] = q[5]; ob_box[5] = q[6];
+          } else {
+            const uint64_t a01 = ld_agent_u64(reinterpret_cast<const uint64_t*>(q + 0));
+            const uint64_t b01 = ld_agent_u64(reinterpret_cast<const uint64_t*>(q + 4));
+            ob_box[0] = lo_f(a01); ob_box[1] = hi_f(a01); ob_box[2] = ld_agent(q + 2);
+            ob_box[3] = lo_f(b01); ob_box[4] = hi_f(b01); ob_box[5] = ld_agent(q + 6);
+          }
+        }
+        const int ra = (rca == 0) ? k : rca, rb = (rcb == 0) ? k : rcb;
+        const int nodes = 2 * nn - 1;
+        if (oa < 0 || oa >= nodes || ob < 0 || ob >= nodes || ra < 0 || ra >= nodes || rb < 0 || rb >= nodes ||
+            p0 < 0 || p0 >= nn - 1) {
+          atomicOr(err, 4u);
+          return;
+        }
+        float* q0 = reinterpret_cast<float*>(T);
+        q0[0] = lo.x; q0[1] = lo.y; q0[2] = lo.z; q0[4] = hi.x; q0[5] = hi.y; q0[6] = hi.z;
+        T[0].child_a = ra;
+        T[0].child_b = rb;
+        float* qk = reinterpret_cast<float*>(T + k);
+        qk[0] = ob_box[0]; qk[1] = ob_box[1]; qk[2] = ob_box[2]; qk[4] = ob_box[3]; qk[5] = ob_box[4]; qk[6] = ob_box[5];
+        T[k].child_a = oa;
+        T[k].child_b = ob;
+        P[oa] = k;
+        P[ob] = k;
+        P[ra] = 0;
+        P[rb] = 0;
+        if (p0 != k) {
+          if (ld_i(&T[p0].child_a) == 0) T[p0].child_a = k;
+          else T[p0].child_b = k;
+          P[k] = p0;
+        }
+      }
+      P[0] = -1;
+      return;
+    }
+    bool right;
+    if (L == 0) {
+      right = true;
+    } else if (R == nn - 1) {
+      right = false;
+    } else {
+      const uint64_t dr = ((uint64_t)key[R] & mask) ^ ((uint64_t)key[R + 1] & mask);
+      const uint64_t dl = ((uint64_t)key[L - 1] & mask) ^ ((uint64_t)key[L] & mask);
+      right = (dr != dl) ? (dr < dl) : ((idx[R] ^ idx[R + 1]) < (idx[L - 1] ^ idx[L]));
+    }
+    const int p = right ? R : L - 1;
+    if (p < 0 || p >= nn - 1) {
+      atomicOr(err, 4u);
+      return;
+    }
+    if (right) {
+      st_i(&T[p].child_a, cur);
+      st_i(&RG[p].x, L);
+    } else {
+      st_i(&T[p].child_b, cur);
+      st_i(&RG[p].y, R);
+    }
+    st_i(&P[cur], p);
+    uint32_t old;
+    if (FORMAL) {
+      old = __hip_atomic_fetch_add(C + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      wait_all_mem();
+      old = __hip_atomic_fetch_add(C + p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if ((old & 1u) == 0u) return;  // first arrival
+    const int sib = right ? ld_i(&T[p].child_b) : ld_i(&T[p].child_a);
+    if (sib < 0 || sib >= 2 * nn - 1) {
+      atomicOr(err, 4u);
+      return;
+    }
+    if (right) R = ld_i(&RG[p].y);
+    else L = ld_i(&RG[p].x);
+    const float* q = reinterpret_cast<const float*>(T + sib);
+    f3 smin, smax;
+    if (FORMAL) {
+      smin = f3{q[0], q[1], q[2]};
+      smax = f3{q[4], q[5], q[6]};
+    } else {
+      const uint64_t m01 = ld_agent_u64(reinterpret_cast<const uint64_t*>(q + 0));
+      const uint64_t x01 = ld_agent_u64(reinterpret_cast<const uint64_t*>(q + 4));
+      smin = f3{lo_f(m01), hi_f(m01), ld_agent(q + 2)};
+      smax = f3{lo_f(x01), hi_f(x01), ld_agent(q + 6)};
+    }
+    if (right) {  // this node is child_a
+      lo = vmin(lo, smin);
+      hi = vmax(hi, smax);
+    } else {
+      lo = vmin(smin, lo);
+      hi = vmax(smax, hi);
+    }
+    if (L < 0 || R >= nn || L > R) {
+      atomicOr(err, 4u);
+      return;
+    }
+    cur = p;
+    if (!(L == 0 && R == nn - 1)) store_box(cur, lo, hi);
+  }
+  atomicOr(err, 4u);
+}
+
+int launch_agglo(hipStream_t s, SegTable tsegs, MeshTables mt, const void* keys, bool key64, const uint32_t* vals,
+                 uint32_t code_bits, const float* x, const float* v, float dt, bool swept, const uint32_t* facets,
+                 ysim_bvh_node* tree, int32_t* parent, uint32_t* visit, int2* range, uint32_t nitems,
+                 uint32_t* box_acc, uint32_t nbox, uint32_t* hist, uint32_t hist_words, uint32_t* perm_out,
+                 uint32_t* err, bool formal) {
+  uint32_t work = std::max<uint32_t>(nitems, std::max<uint32_t>(nbox * 6, hist_words));
+  if (!work) return YSIM_OK;
+  const uint32_t blocks = ceil_div(work, 256);
+#define YSIM_A(KT, F)                                                                                          \
+  hipLaunchKernelGGL((k_agglo<KT, F>), dim3(blocks), dim3(256), 0, s, tsegs, mt, (const KT*)keys, vals,         \
+                     code_bits, x, v ? v : x, dt, (int)swept, facets, tree, parent, visit, range, nitems, box_acc, \
+                     nbox, hist, hist_words, perm_out, err)
+  if (key64) {
+    if (formal) YSIM_A(uint64_t, true); else YSIM_A(uint64_t, false);
+  } else {
+    if (formal) YSIM_A(uint32_t, true); else YSIM_A(uint32_t, false);
+  }
+#undef YSIM_A
+  YSIM_CHECK_LAUNCH();
+  return YSIM_OK;
+}
+
 // ------------------------------------------------------------------ S4+S5 refit
 // One thread per leaf.  The leaf box is the hull of the triangle's vertices
 // (and of x + v*dt when swept).  Each thread then climbs toward the root; at a

@@ -151,6 +151,21 @@ def lbvh_hierarchy(ctx: HipContext, sorted_keys, sorted_idx, bits: int):
     return tree.get(), parent.get()
 
 
+def lbvh_agglomerative(ctx: HipContext, sorted_keys, sorted_idx, bits: int, x, facets):
+    """agglomerativeBuild_Tri + agglomerativeSwapRoot (bvh.metal:884-1199) ->
+    (nodes with boxes, parent)."""
+    n = len(sorted_keys)
+    kd = np.uint64 if bits > 32 else np.uint32
+    dk = ctx.to_device(np.asarray(sorted_keys, kd))
+    di = ctx.to_device(np.asarray(sorted_idx, np.uint32))
+    dx = ctx.to_device(np.asarray(x, np.float32).reshape(-1))
+    df = ctx.to_device(np.asarray(facets, np.uint32).reshape(-1))
+    tree = ctx.zeros(2 * n - 1, abi.BVH_NODE)
+    parent = ctx.to_device(np.full(2 * n - 1, -1, np.int32))
+    call("ysim_hip_lbvh_agglomerative", ctx.h, dk.ptr, di.ptr, n, bits, dx.ptr, df.ptr, tree.ptr, parent.ptr)
+    return tree.get(), parent.get()
+
+
 def lbvh_refit(ctx: HipContext, tree, parent, x, v, dt, swept, facets):
     """buildLeaf/buildSweptLeaf + bottomUpBoxes (bvh.metal:260-586) -> nodes."""
     dt_ = ctx.to_device(np.ascontiguousarray(tree))

@@ -218,6 +218,19 @@ int ysim_hip_broad_query(ysim_ctx* ctx, const float* d_xq, uint32_t num_query,
                          float margin, ysim_broad_collision* d_out, uint32_t capacity,
                          uint32_t* d_count, uint32_t* h_required);
 
+/* The same query with the segmented emission of queryPointsSegmented +
+ * scanReserveSegmented + compactSegmented (src/metal/bvh.metal:1732-1827,
+ * host BVH::queryPointsSegmented): each workgroup stages its pairs in LDS and
+ * reserves its output range with one atomic.  Same pair set and count; the
+ * records are grouped by workgroup. */
+int ysim_hip_broad_query_segmented(ysim_ctx* ctx, const float* d_xq, uint32_t num_query,
+                                   uint32_t query_obj, uint32_t query_behavior, uint32_t query_shape,
+                                   const ysim_bvh_node* d_tree, uint32_t num_target_facets,
+                                   const uint32_t* d_target_facets, uint32_t target_obj,
+                                   uint32_t target_behavior, uint32_t target_shape,
+                                   float margin, ysim_broad_collision* d_out, uint32_t capacity,
+                                   uint32_t* d_count, uint32_t* h_required);
+
 /* Swept point-triangle narrow phase: narrow_pt_tri (src/metal/bruteforce.metal:9-178),
  * host BruteForce::narrow (include/sim/bruteforce.hpp:114-153).  Positions are
  * the packed scene arrays indexed through d_states_offsets / d_facets_offsets. */

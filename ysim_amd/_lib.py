@@ -49,6 +49,8 @@ _SIG = {
     "ysim_hip_lbvh_refit": (C.c_int, [vp, vp, vp, C.c_float, u32, vp, u32, vp, vp]),
     "ysim_hip_broad_query": (C.c_int, [vp, vp, u32, u32, u32, u32, vp, u32, vp, u32, u32, u32,
                                        C.c_float, vp, u32, vp, u32p]),
+    "ysim_hip_broad_query_segmented": (C.c_int, [vp, vp, u32, u32, u32, u32, vp, u32, vp, u32, u32, u32,
+                                       C.c_float, vp, u32, vp, u32p]),
     "ysim_hip_narrow": (C.c_int, [vp, vp, u32, vp, vp, vp, vp, vp, vp, vp, C.c_float, C.c_float,
                                   vp, u32, vp, u32p]),
     "ysim_hip_contact_csr": (C.c_int, [vp, vp, u32, vp, u32, vp, vp]),

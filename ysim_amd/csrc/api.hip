@@ -1267,11 +1267,12 @@ int ysim_hip_lbvh_refit(ysim_ctx* ctx, const float* d_x, const float* d_v, float
   return YSIM_OK;
 }
 
-int ysim_hip_broad_query(ysim_ctx* ctx, const float* d_xq, uint32_t num_query, uint32_t query_obj,
-                         uint32_t query_behavior, uint32_t query_shape, const ysim_bvh_node* d_tree,
-                         uint32_t num_target_facets, const uint32_t* d_target_facets, uint32_t target_obj,
-                         uint32_t target_behavior, uint32_t target_shape, float margin,
-                         ysim_broad_collision* d_out, uint32_t capacity, uint32_t* d_count, uint32_t* h_required) {
+namespace {
+int broad_query(ysim_ctx* ctx, const float* d_xq, uint32_t num_query, uint32_t query_obj, uint32_t query_behavior,
+                uint32_t query_shape, const ysim_bvh_node* d_tree, uint32_t num_target_facets,
+                const uint32_t* d_target_facets, uint32_t target_obj, uint32_t target_behavior,
+                uint32_t target_shape, float margin, ysim_broad_collision* d_out, uint32_t capacity,
+                uint32_t* d_count, uint32_t* h_required, bool segmented) {
   YSIM_ARG(ctx && d_xq && d_tree && d_target_facets && d_count, "broad_query: null argument");
   YSIM_ARG(capacity == 0 || d_out, "broad_query: null output");
   if (!num_query || !num_target_facets) return YSIM_OK;
@@ -1297,7 +1298,7 @@ int ysim_hip_broad_query(ysim_ctx* ctx, const float* d_xq, uint32_t num_query, u
   YSIM_HIP_TRY(hipStreamSynchronize(ctx->stream));
   YSIM_TRY(launch_broad_pairs(ctx->stream, d_xq, num_query, query_obj, query_behavior, query_shape, d_tree, par,
                               num_target_facets, d_target_facets, target_obj, target_behavior, target_shape, margin,
-                              d_out, capacity, d_count, flags));
+                              d_out, capacity, d_count, flags, segmented));
   uint32_t res[2] = {0, 0};
   YSIM_HIP_TRY(hipMemcpyAsync(&res[0], d_count, 4, hipMemcpyDeviceToHost, ctx->stream));
   YSIM_HIP_TRY(hipMemcpyAsync(&res[1], flags, 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -1307,6 +1308,28 @@ int ysim_hip_broad_query(ysim_ctx* ctx, const float* d_xq, uint32_t num_query, u
   if (res[0] > capacity) return fail(YSIM_ERR_CAPACITY, "broad_query: pair capacity exceeded");
   (void)start;
   return YSIM_OK;
+}
+}  // namespace
+
+int ysim_hip_broad_query(ysim_ctx* ctx, const float* d_xq, uint32_t num_query, uint32_t query_obj,
+                         uint32_t query_behavior, uint32_t query_shape, const ysim_bvh_node* d_tree,
+                         uint32_t num_target_facets, const uint32_t* d_target_facets, uint32_t target_obj,
+                         uint32_t target_behavior, uint32_t target_shape, float margin,
+                         ysim_broad_collision* d_out, uint32_t capacity, uint32_t* d_count, uint32_t* h_required) {
+  return broad_query(ctx, d_xq, num_query, query_obj, query_behavior, query_shape, d_tree, num_target_facets,
+                     d_target_facets, target_obj, target_behavior, target_shape, margin, d_out, capacity, d_count,
+                     h_required, false);
+}
+
+int ysim_hip_broad_query_segmented(ysim_ctx* ctx, const float* d_xq, uint32_t num_query, uint32_t query_obj,
+                                   uint32_t query_behavior, uint32_t query_shape, const ysim_bvh_node* d_tree,
+                                   uint32_t num_target_facets, const uint32_t* d_target_facets, uint32_t target_obj,
+                                   uint32_t target_behavior, uint32_t target_shape, float margin,
+                                   ysim_broad_collision* d_out, uint32_t capacity, uint32_t* d_count,
+                                   uint32_t* h_required) {
+  return broad_query(ctx, d_xq, num_query, query_obj, query_behavior, query_shape, d_tree, num_target_facets,
+                     d_target_facets, target_obj, target_behavior, target_shape, margin, d_out, capacity, d_count,
+                     h_required, true);
 }
 
 int ysim_hip_narrow(ysim_ctx* ctx, const ysim_broad_collision* d_pairs, uint32_t num_pairs, const float* d_x,

@@ -25,6 +25,13 @@ __device__ __forceinline__ NodeV load_node(const ysim_bvh_node* T, int i) {
 // queryPoints (src/metal/bvh.metal:1393-1412) with an explicit per-thread
 // stack in scratch-free form: plain DFS over a private stack of depth 64
 // (the reference's depth) backed by the parent-link walk when it fills.
+// SEG: queryPointsSegmented + scanReserveSegmented + compactSegmented
+// (bvh.metal:1732-1827) in one kernel: a workgroup stages its pairs in LDS,
+// reserves its output range with one atomic and copies them out coalesced;
+// pairs beyond the LDS capacity take the per-pair atomic.  Output order is
+// per workgroup instead of per pair; the set is the same.
+constexpr uint32_t SEG_CAP = 512;
+template <bool SEG>
 __global__ __launch_bounds__(QB) void k_broad_pairs(const float* __restrict__ xq, uint32_t nq, uint32_t qobj,
                                                     uint32_t qbeh, uint32_t qshape,
                                                     const ysim_bvh_node* __restrict__ T,
@@ -33,58 +40,80 @@ __global__ __launch_bounds__(QB) void k_broad_pairs(const float* __restrict__ xq
                                                     uint32_t tbeh, uint32_t tshape, float margin,
                                                     ysim_broad_collision* __restrict__ out, uint32_t cap,
                                                     uint32_t* __restrict__ count, uint32_t* __restrict__ flags) {
+  __shared__ ysim_broad_collision s_rec[SEG ? SEG_CAP : 1];
+  __shared__ uint32_t s_cnt, s_base;
+  if (SEG) {
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+  }
   const uint32_t p = blockIdx.x * QB + threadIdx.x;
-  if (p >= nq || nT == 0) return;
-  const f3 x = ld3(xq, p);
-  const f3 qmin = {x.x - margin, x.y - margin, x.z - margin};
-  const f3 qmax = {x.x + margin, x.y + margin, x.z + margin};
-  const bool self = (qobj == tobj);
-  auto emit = [&](int fid) {
-    const uint32_t* f = tf + 3 * (size_t)fid;
-    if (self && (p == f[0] || p == f[1] || p == f[2])) return;
-    uint32_t slot = atomicAdd(count, 1u);
-    if (slot < cap) {
+  if (p < nq && nT != 0) {
+    const f3 x = ld3(xq, p);
+    const f3 qmin = {x.x - margin, x.y - margin, x.z - margin};
+    const f3 qmax = {x.x + margin, x.y + margin, x.z + margin};
+    const bool self = (qobj == tobj);
+    auto emit = [&](int fid) {
+      const uint32_t* f = tf + 3 * (size_t)fid;
+      if (self && (p == f[0] || p == f[1] || p == f[2])) return;
       ysim_broad_collision r;
       r.point = p; r.triangle = (uint32_t)fid; r.query_obj = qobj; r.target_obj = tobj;
       r.query_behavior = qbeh; r.target_behavior = tbeh; r.query_shape = qshape; r.target_shape = tshape;
-      out[slot] = r;
-    }
-  };
-  // stackless DFS through parent links (node 0 = root)
-  int c = 0;
-  uint32_t guard = 0;
-  const uint32_t guard_max = 6 * nT + 16;
-  bool down = true;
-  while (true) {
-    if (++guard > guard_max) {
-      atomicOr(flags, 4u);
-      return;
-    }
-    if (down) {
-      NodeV N = load_node(T, c);
-      if (overlap(qmin, qmax, N.lo, N.hi)) {
-        if (N.a < 0) {
-          emit(N.b);
-        } else {
-          c = N.a;
-          continue;
+      if (SEG) {
+        const uint32_t k = atomicAdd(&s_cnt, 1u);
+        if (k < SEG_CAP) {
+          s_rec[k] = r;
+          return;
         }
       }
+      const uint32_t slot = atomicAdd(count, 1u);
+      if (slot < cap) out[slot] = r;
+    };
+    // stackless DFS through parent links (node 0 = root)
+    int c = 0;
+    uint32_t guard = 0;
+    const uint32_t guard_max = 6 * nT + 16;
+    bool down = true;
+    while (true) {
+      if (++guard > guard_max) {
+        atomicOr(flags, 4u);
+        break;
+      }
+      if (down) {
+        NodeV N = load_node(T, c);
+        if (overlap(qmin, qmax, N.lo, N.hi)) {
+          if (N.a < 0) {
+            emit(N.b);
+          } else {
+            c = N.a;
+            continue;
+          }
+        }
+      }
+      // climb to the next unvisited right sibling
+      if (c == 0) break;
+      int pp = P[c];
+      if (pp < 0 || pp >= (int)nT - 1) {
+        atomicOr(flags, 4u);
+        break;
+      }
+      NodeV PN = load_node(T, pp);
+      if (PN.a == c) {
+        c = PN.b;
+        down = true;
+      } else {
+        c = pp;
+        down = false;
+      }
     }
-    // climb to the next unvisited right sibling
-    if (c == 0) return;
-    int pp = P[c];
-    if (pp < 0 || pp >= (int)nT - 1) {
-      atomicOr(flags, 4u);
-      return;
-    }
-    NodeV PN = load_node(T, pp);
-    if (PN.a == c) {
-      c = PN.b;
-      down = true;
-    } else {
-      c = pp;
-      down = false;
+  }
+  if (SEG) {
+    __syncthreads();
+    const uint32_t nl = s_cnt < SEG_CAP ? s_cnt : SEG_CAP;
+    if (threadIdx.x == 0) s_base = nl ? atomicAdd(count, nl) : 0u;
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < nl; k += QB) {
+      const uint32_t slot = s_base + k;
+      if (slot < cap) out[slot] = s_rec[k];
     }
   }
 }
@@ -92,10 +121,14 @@ __global__ __launch_bounds__(QB) void k_broad_pairs(const float* __restrict__ xq
 int launch_broad_pairs(hipStream_t s, const float* xq, uint32_t nq, uint32_t qobj, uint32_t qbeh,
                        uint32_t qshape, const ysim_bvh_node* tree, const int32_t* parent, uint32_t nT,
                        const uint32_t* tfacets, uint32_t tobj, uint32_t tbeh, uint32_t tshape, float margin,
-                       ysim_broad_collision* out, uint32_t cap, uint32_t* count, uint32_t* flags) {
+                       ysim_broad_collision* out, uint32_t cap, uint32_t* count, uint32_t* flags, bool segmented) {
   if (!nq || !nT) return YSIM_OK;
-  hipLaunchKernelGGL(k_broad_pairs, dim3(ceil_div(nq, QB)), dim3(QB), 0, s, xq, nq, qobj, qbeh, qshape, tree,
-                     parent, nT, tfacets, tobj, tbeh, tshape, margin, out, cap, count, flags);
+  if (segmented)
+    hipLaunchKernelGGL(k_broad_pairs<true>, dim3(ceil_div(nq, QB)), dim3(QB), 0, s, xq, nq, qobj, qbeh, qshape, tree,
+                       parent, nT, tfacets, tobj, tbeh, tshape, margin, out, cap, count, flags);
+  else
+    hipLaunchKernelGGL(k_broad_pairs<false>, dim3(ceil_div(nq, QB)), dim3(QB), 0, s, xq, nq, qobj, qbeh, qshape,
+                       tree, parent, nT, tfacets, tobj, tbeh, tshape, margin, out, cap, count, flags);
   YSIM_CHECK_LAUNCH();
   return YSIM_OK;
 }

@@ -120,12 +120,13 @@ struct QueryArgs {
 bool query_block_ok(uint32_t b);
 bool query_waves_ok(uint32_t w);
 int launch_query(hipStream_t s, const QueryArgs& a, bool emit_pairs);
-// Drop-in broad query of one (query mesh, target tree): BroadCollision records.
+// Drop-in broad query of one (query mesh, target tree): BroadCollision records;
+// segmented: pairs staged per workgroup in LDS, one output reservation each.
 int launch_broad_pairs(hipStream_t s, const float* xq, uint32_t nq, uint32_t qobj, uint32_t qbeh,
                        uint32_t qshape, const ysim_bvh_node* tree, const int32_t* parent, uint32_t nT,
                        const uint32_t* tfacets, uint32_t tobj, uint32_t tbeh, uint32_t tshape,
                        float margin, ysim_broad_collision* out, uint32_t cap, uint32_t* count,
-                       uint32_t* flags);
+                       uint32_t* flags, bool segmented = false);
 // Drop-in narrow phase over broad pairs (NarrowCollision records, atomic append).
 int launch_narrow_pairs(hipStream_t s, const ysim_broad_collision* pairs, uint32_t np, const float* x,
                         const float* xprev, const uint32_t* states_off, const uint32_t* facets,

@@ -178,8 +178,10 @@ def lbvh_refit(ctx: HipContext, tree, parent, x, v, dt, swept, facets):
     return dt_.get()
 
 
-def broad_query(ctx: HipContext, xq, qobj, tree, tfacets, tobj, margin, capacity=None):
-    """queryPoints (bvh.metal:1393-1412) for one (query mesh, target tree)."""
+def broad_query(ctx: HipContext, xq, qobj, tree, tfacets, tobj, margin, capacity=None, segmented=False):
+    """queryPoints (bvh.metal:1393-1412) for one (query mesh, target tree);
+    segmented: queryPointsSegmented's per-workgroup emission (bvh.metal:1732-1827)."""
+    fn = load().ysim_hip_broad_query_segmented if segmented else load().ysim_hip_broad_query
     dx = ctx.to_device(np.asarray(xq, np.float32).reshape(-1))
     dtree = ctx.to_device(np.ascontiguousarray(tree))
     df = ctx.to_device(np.asarray(tfacets, np.uint32).reshape(-1))
@@ -189,8 +191,8 @@ def broad_query(ctx: HipContext, xq, qobj, tree, tfacets, tobj, margin, capacity
         out = ctx.zeros(max(cap, 1), abi.BROAD)
         cnt = ctx.zeros(1, np.uint32)
         req = C.c_uint32(0)
-        rc = load().ysim_hip_broad_query(ctx.h, dx.ptr, dx.nbytes // 12, qobj, 1, 0, dtree.ptr, nT, df.ptr, tobj,
-                                         0, 0, float(margin), out.ptr, cap, cnt.ptr, C.byref(req))
+        rc = fn(ctx.h, dx.ptr, dx.nbytes // 12, qobj, 1, 0, dtree.ptr, nT, df.ptr, tobj,
+                0, 0, float(margin), out.ptr, cap, cnt.ptr, C.byref(req))
         if rc == ERR_CAPACITY:
             cap = req.value
             continue

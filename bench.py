@@ -48,8 +48,9 @@ def stage_bytes(V: int, T: int, k: int, passes: int, C: int, incremental: bool =
         return {**stage_bytes(V, T, k, passes, C),
                 # previous order, facets, positions read; keys + values written
                 "morton": 4 * T + 12 * T + 12 * V + (k + 4) * T,
-                # shifted windows and the adjacency check each read keys + values
-                "sort": 2 * (k + 4) * T}
+                # the shifted windows read keys + values (the boundary check
+                # runs in the same kernel, on the windows' ends)
+                "sort": (k + 4) * T}
     return {
         # positions read
         "box": 12 * V,
@@ -317,9 +318,9 @@ def main():
     passes = -(-63 // digit_bits)  # 63-bit keys
     incremental = bool(tuning.get("sort_incremental", 0))
     sb = stage_bytes(V, T, 8, passes, contacts, incremental)
-    # incremental: shifted windows, check, then the fallback (Morton, MSD pass,
-    # buckets) whose kernels return at once unless the check failed
-    launches = {"box": 1, "morton": 1, "sort": 5 if incremental else passes, "hierarchy": 1, "refit": 1,
+    # incremental: shifted windows with the check, then the fallback (Morton,
+    # MSD pass, buckets) whose kernels return at once unless the check failed
+    launches = {"box": 1, "morton": 1, "sort": 4 if incremental else passes, "hierarchy": 1, "refit": 1,
                 "query": 1, "integrate": 1}
     kernels = {"box": "k_box1",
                "morton": "k_morton_inc<uint64_t>" if incremental else f"k_morton<uint64_t, {digit_bits}, 8>",
@@ -341,7 +342,7 @@ def main():
     for s, ms in stages_ms.items():
         gbs = sb[s] / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
         if s == "sort" and incremental:
-            ks = ["k_window_sort_shifted<uint64_t>", "k_sorted_check<uint64_t>", "k_morton<uint64_t, 11, 1>",
+            ks = ["k_window_sort_shifted<uint64_t>", "k_morton<uint64_t, 11, 1>",
                   f"k_onesweep<uint64_t, 11, {int(tuning['sort_items'])}>", "k_bucket_sort<uint64_t, 16>"]
         else:
             ks = [kernels[s]] * launches[s] if launches[s] else []

@@ -196,6 +196,8 @@ struct ysim_scene {
   uint64_t* status = nullptr;   // sort look-back words (epoch-tagged, never cleared)
   uint32_t* epoch = nullptr;    // [0] look-back epoch of the current sort
   uint32_t* need_full = nullptr;  // [0] incremental sort: the window passes left the keys unsorted
+  uint32_t* inc_bcnt = nullptr;   // incremental sort: window-boundary arrival counters
+  uint64_t* inc_edges = nullptr;  // incremental sort: each window's first and last (key, facet)
   uint32_t cap = 16;
   uint32_t* ccount = nullptr;
   float4* cnd = nullptr;
@@ -642,7 +644,7 @@ struct ysim_scene {
       YSIM_TRY(launch_morton_inc(st, g.tsegs, mt, xc, facets, box_acc, g.code_bits, g.key64, g.perm, keys[0],
                                  vals[0], g.T, need_full));
       if (timing) YSIM_HIP_TRY(hipEventRecord(ev[2], st));
-      YSIM_TRY(launch_incremental_fix(st, g.key64, keys[0], vals[0], g.T, need_full));
+      YSIM_TRY(launch_incremental_fix(st, g.key64, keys[0], vals[0], g.T, need_full, inc_bcnt, inc_edges));
       YSIM_TRY(launch_morton(st, g.tsegs, mt, xc, facets, box_acc, g.code_bits, g.key64, keys[0], vals[0], g.T,
                              g.plan, hist, counters, epoch, need_full));
       YSIM_TRY(launch_sort_hybrid(st, g.plan, keys[0], vals[0], keys[1], vals[1], hist, status, counters, epoch,
@@ -1576,6 +1578,9 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
   YSIM_TRY(s.alloc(&s.epoch, 16));
   YSIM_TRY(s.alloc(&s.need_full, 16));
   YSIM_TRY(s.zero(s.need_full, 16));
+  YSIM_TRY(s.alloc(&s.inc_bcnt, incremental_windows((uint32_t)maxT)));
+  YSIM_TRY(s.zero(s.inc_bcnt, incremental_windows((uint32_t)maxT)));
+  YSIM_TRY(s.alloc(&s.inc_edges, 3 * (size_t)incremental_windows((uint32_t)maxT)));
   // incremental sort: each dynamic group starts from the facet order
   for (auto& g : s.dyn_groups) {
     std::vector<uint32_t> id;

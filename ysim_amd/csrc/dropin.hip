@@ -28,8 +28,9 @@ __device__ __forceinline__ NodeV load_node(const ysim_bvh_node* T, int i) {
 // SEG: queryPointsSegmented + scanReserveSegmented + compactSegmented
 // (bvh.metal:1732-1827) in one kernel: a workgroup stages its pairs in LDS,
 // reserves its output range with one atomic and copies them out coalesced;
-// pairs beyond the LDS capacity take the per-pair atomic.  Output order is
-// per workgroup instead of per pair; the set is the same.
+// pairs beyond the LDS capacity reserve per wave (one atomic for the wave's
+// spilling lanes).  Output order is per workgroup instead of per pair; the
+// set is the same.
 constexpr uint32_t SEG_CAP = 512;
 template <bool SEG>
 __global__ __launch_bounds__(QB) void k_broad_pairs(const float* __restrict__ xq, uint32_t nq, uint32_t qobj,
@@ -59,11 +60,24 @@ __global__ __launch_bounds__(QB) void k_broad_pairs(const float* __restrict__ xq
       r.point = p; r.triangle = (uint32_t)fid; r.query_obj = qobj; r.target_obj = tobj;
       r.query_behavior = qbeh; r.target_behavior = tbeh; r.query_shape = qshape; r.target_shape = tshape;
       if (SEG) {
+        // LDS staging; past its capacity the wave's spilling lanes share one
+        // output reservation (ballot + prefix rank)
         const uint32_t k = atomicAdd(&s_cnt, 1u);
-        if (k < SEG_CAP) {
-          s_rec[k] = r;
-          return;
+        const bool spill = k >= SEG_CAP;
+        if (!spill) s_rec[k] = r;
+        const uint64_t m = __ballot(spill);
+        if (m) {
+          const int leader = __ffsll((unsigned long long)m) - 1;
+          const int lane = (int)__lane_id();
+          uint32_t base = 0;
+          if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+          base = __shfl(base, leader);
+          if (spill) {
+            const uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            if (slot < cap) out[slot] = r;
+          }
         }
+        return;
       }
       const uint32_t slot = atomicAdd(count, 1u);
       if (slot < cap) out[slot] = r;

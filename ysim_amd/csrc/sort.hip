@@ -559,13 +559,21 @@ int launch_sort_hybrid(hipStream_t s, const SortPlan& p, void* keys_a, uint32_t*
 
 // ------------------------------------------------------------------ incremental
 // Second window pass of the incremental sort: windows straddling the aligned
-// window boundaries (block b covers [b*W - W/2, b*W + W/2)), sorted in place.
+// window boundaries (block b covers [b*W - W/2, b*W + W/2)), sorted in place,
+// then the exact check.  Every adjacent pair inside a window is ordered once
+// the window is sorted, so only the pairs across the shifted windows'
+// boundaries remain: each block publishes its window's first and last (key,
+// facet) and arrives at its two boundaries (counters advance by two per pass,
+// so they need no clearing); the second arrival compares the pair and
+// requests the full sort if it is not strictly increasing.
 template <typename K>
 __global__ __launch_bounds__(256) void k_window_sort_shifted(K* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                             uint32_t n) {
+                                                             uint32_t n, uint32_t* __restrict__ need_full,
+                                                             uint32_t* __restrict__ bcnt, uint64_t* edges) {
   __shared__ K sk[INC_WINDOW];
   __shared__ uint32_t sv[INC_WINDOW];
-  const int64_t s0 = (int64_t)blockIdx.x * INC_WINDOW - INC_WINDOW / 2;
+  const uint32_t w = blockIdx.x;
+  const int64_t s0 = (int64_t)w * INC_WINDOW - INC_WINDOW / 2;
   const uint32_t lo = (uint32_t)(s0 < 0 ? 0 : s0);
   const uint32_t hi = (uint32_t)min<int64_t>(s0 + INC_WINDOW, (int64_t)n);
   if (lo >= hi) return;
@@ -575,39 +583,54 @@ __global__ __launch_bounds__(256) void k_window_sort_shifted(K* __restrict__ key
     sv[i] = i < cnt ? vals[lo + i] : ~0u;
   }
   __syncthreads();
-  if (!window_sort<K, INC_WINDOW>(sk, sv, cnt)) return;
-  for (uint32_t i = threadIdx.x; i < cnt; i += 256) {
-    keys[lo + i] = sk[i];
-    vals[lo + i] = sv[i];
+  if (window_sort<K, INC_WINDOW>(sk, sv, cnt))
+    for (uint32_t i = threadIdx.x; i < cnt; i += 256) {
+      keys[lo + i] = sk[i];
+      vals[lo + i] = sv[i];
+    }
+  if (threadIdx.x != 0) return;
+  const uint32_t nw = (n + INC_WINDOW / 2 + INC_WINDOW - 1) / INC_WINDOW;  // non-empty windows
+  uint64_t* e = edges + 3 * (size_t)w;
+  st_agent_u64(e + 0, (uint64_t)sk[0]);
+  st_agent_u64(e + 1, (uint64_t)sk[cnt - 1]);
+  st_agent_u64(e + 2, (uint64_t)sv[0] | ((uint64_t)sv[cnt - 1] << 32));
+  wait_all_mem();
+  for (int k = 0; k < 2; ++k) {
+    const int64_t b = (int64_t)w - 1 + k;  // boundary between windows b and b + 1
+    if (b < 0 || b + 1 >= (int64_t)nw) continue;
+    const uint32_t old = __hip_atomic_fetch_add(bcnt + b, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((old & 1u) == 0u) continue;  // first arrival: the neighbour checks
+    K last_k, first_k;
+    uint32_t last_v, first_v;
+    if (b == (int64_t)w) {
+      last_k = sk[cnt - 1];
+      last_v = sv[cnt - 1];
+      const uint64_t* o = edges + 3 * (size_t)(b + 1);
+      first_k = (K)ld_agent_u64(o + 0);
+      first_v = (uint32_t)ld_agent_u64(o + 2);
+    } else {
+      const uint64_t* o = edges + 3 * (size_t)b;
+      last_k = (K)ld_agent_u64(o + 1);
+      last_v = (uint32_t)(ld_agent_u64(o + 2) >> 32);
+      first_k = sk[0];
+      first_v = sv[0];
+    }
+    if (!kv_greater(first_k, first_v, last_k, last_v)) atomicOr(need_full, 1u);
   }
 }
 
-// Every adjacent pair strictly increasing by (key, facet)?  Otherwise request
-// the full sort.
-template <typename K>
-__global__ __launch_bounds__(256) void k_sorted_check(const K* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                                      uint32_t n, uint32_t* __restrict__ need_full) {
-  int bad = 0;
-  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i + 1 < n; i += gridDim.x * 256)
-    if (!kv_greater(keys[i + 1], vals[i + 1], keys[i], vals[i])) bad = 1;
-  if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(need_full, 1u);
-}
+uint32_t incremental_windows(uint32_t n) { return n / INC_WINDOW + 2; }
 
-int launch_incremental_fix(hipStream_t s, bool wide, void* keys, uint32_t* vals, uint32_t n, uint32_t* need_full) {
+int launch_incremental_fix(hipStream_t s, bool wide, void* keys, uint32_t* vals, uint32_t n, uint32_t* need_full,
+                           uint32_t* bcnt, uint64_t* edges) {
   if (!n) return YSIM_OK;
   const uint32_t wblocks = ceil_div(n, INC_WINDOW) + 1;
-  const uint32_t cblocks = std::min<uint32_t>(1024, ceil_div(n, 256 * 4));
-  if (wide) {
-    hipLaunchKernelGGL(k_window_sort_shifted<uint64_t>, dim3(wblocks), dim3(256), 0, s, (uint64_t*)keys, vals, n);
-    YSIM_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_sorted_check<uint64_t>, dim3(cblocks), dim3(256), 0, s, (const uint64_t*)keys, vals, n,
-                       need_full);
-  } else {
-    hipLaunchKernelGGL(k_window_sort_shifted<uint32_t>, dim3(wblocks), dim3(256), 0, s, (uint32_t*)keys, vals, n);
-    YSIM_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_sorted_check<uint32_t>, dim3(cblocks), dim3(256), 0, s, (const uint32_t*)keys, vals, n,
-                       need_full);
-  }
+  if (wide)
+    hipLaunchKernelGGL(k_window_sort_shifted<uint64_t>, dim3(wblocks), dim3(256), 0, s, (uint64_t*)keys, vals, n,
+                       need_full, bcnt, edges);
+  else
+    hipLaunchKernelGGL(k_window_sort_shifted<uint32_t>, dim3(wblocks), dim3(256), 0, s, (uint32_t*)keys, vals, n,
+                       need_full, bcnt, edges);
   YSIM_CHECK_LAUNCH();
   return YSIM_OK;
 }

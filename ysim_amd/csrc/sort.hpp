@@ -34,7 +34,11 @@ int launch_sort_hybrid(hipStream_t s, const SortPlan& p, void* keys_a, uint32_t*
 // Incremental sort, after launch_morton_inc: the half-shifted window pass and
 // the check of every adjacent pair, which sets *need_full if the keys are not
 // yet in (key, value) order.
-int launch_incremental_fix(hipStream_t s, bool wide, void* keys, uint32_t* vals, uint32_t n, uint32_t* need_full);
+// bcnt / edges: boundary arrival counters (zeroed once) and per-window end
+// records, incremental_windows(n) entries each (edges: 3 words per window).
+uint32_t incremental_windows(uint32_t n);
+int launch_incremental_fix(hipStream_t s, bool wide, void* keys, uint32_t* vals, uint32_t n, uint32_t* need_full,
+                           uint32_t* bcnt, uint64_t* edges);
 // Largest status footprint (64-bit words) over all configurations for n keys.
 size_t sort_status_words_max(uint32_t n, uint32_t key_bits);
 // Look-back status words are 64-bit: (epoch << 32) | flag << 30 | count.  A word

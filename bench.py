@@ -224,6 +224,28 @@ def adjoint_timing(ctx, log, reps: int = 3):
             "grad_norm": float(np.linalg.norm(g))}
 
 
+def broad_query_profile(n: int, reps: int = 5):
+    """Stage-level broad query of the n x n fold against its own tree (built
+    on the GPU with the stage entry points), per-pair and segmented emission,
+    at the bench margin and 4x it; one JSON line of pair counts.  Run under a
+    kernel tracer for the per-dispatch times."""
+    from ysim_amd import hip, scenes
+    ctx = hip.HipContext(0)
+    sc = scenes.config_c2(n=n)
+    m = sc.meshes[0]
+    k, i = hip.morton(ctx, m.x, m.facets, hip.scene_box(ctx, m.x), 30)
+    k, i = hip.RadixSorter(ctx).sort(k, i, 30)
+    tree, parent = hip.lbvh_hierarchy(ctx, k, i, 30)
+    tree = hip.lbvh_refit(ctx, tree, parent, m.x, m.x, 0.0, 0, m.facets)
+    out = {}
+    for scale in (1.0, 4.0):
+        for seg in (False, True):
+            for _ in range(reps):
+                rec = hip.broad_query(ctx, m.x, 0, tree, m.facets, 0, sc.params["margin"] * scale, segmented=seg)
+            out[f"margin_x{scale:g}_{'segmented' if seg else 'per_pair'}"] = int(len(rec))
+    print(json.dumps({"broad_query_pairs": out, "workload": f"{n}x{n} fold vs its own 30-bit tree"}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -244,7 +266,12 @@ def main():
     ap.add_argument("--no-copy", action="store_true", help="skip the copy-bandwidth ceiling")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the C5 LBVH-vs-hash, adjoint and copy-ceiling measurements")
+    ap.add_argument("--broad-query-profile", action="store_true",
+                    help="only run the stage-level broad query (per-pair vs segmented emission) for a kernel trace")
     args = ap.parse_args()
+    if args.broad_query_profile:
+        broad_query_profile(args.n)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on exactly one GPU (single-GPU scope); "
               "--gpus must be 1", file=sys.stderr)

@@ -349,7 +349,7 @@ def main():
     # MSD pass, buckets) whose kernels return at once unless the check failed
     launches = {"box": 1, "morton": 1, "sort": 4 if incremental else passes, "hierarchy": 1, "refit": 1,
                 "query": 1, "integrate": 1}
-    kernels = {"box": "k_box1",
+    kernels = {"box": "k_box_parts",
                "morton": "k_morton_inc<uint64_t>" if incremental else f"k_morton<uint64_t, {digit_bits}, 8>",
                "sort": "k_window_sort_shifted<uint64_t>" if incremental
                else f"k_onesweep<uint64_t, {digit_bits}, {int(tuning['sort_items'])}>",

@@ -313,6 +313,15 @@ int ysim_hip_scene_restore(ysim_scene* scene);
 /* Counters since the last reset: out[0] candidate pairs tested by the fused
  * query, out[1] substeps run, out[2] contacts of the last substep. */
 int ysim_hip_scene_counters(ysim_scene* scene, uint64_t* h_out, uint32_t reset);
+/* No counterpart in the reference (diagnostics).  Graph replays
+ * (scene_step with use_graph) leave out the incremental sort's fallback
+ * kernels: a rebuild whose order check fails keeps the previous topology,
+ * refit to the new positions (the candidate sets, contacts and states are
+ * unchanged; scene_get_tree shows that topology), and the next scene_step
+ * call runs its first substep eagerly, with the fallback.  A state change
+ * through this API (set_state, restore, tuning) does the same without a
+ * failed check.  *h_out: how many replays kept a previous topology. */
+int ysim_hip_scene_deferred_builds(ysim_scene* scene, uint64_t* h_out);
 
 /* ---- analytic colliders -----------------------------------------------------
  * ColliderKind (include/sim/collision_pod.hpp:32, common_types.metalh:80-84):

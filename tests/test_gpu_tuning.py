@@ -223,3 +223,35 @@ def test_invalid_tuning_rejected(ctx):
         with pytest.raises(YsimError):
             hs.set_tuning(**kw)
     hs.step(2)
+
+
+def test_graph_replays_keep_topology_when_order_check_fails(ctx, oracle_threads):
+    """A spinning fold (20 rad/s about y): the Morton order changes by many
+    places per substep, the incremental check fails in graph replays, which
+    then refit the previous topology (no fallback kernels in the graph) and
+    the next call runs an eager substep with the full sort.  States and
+    contacts stay equal to the oracle's; eager-only runs never defer."""
+    sc = scenes.config_c2(n=64)
+    hs, os_ = HipScene(ctx, sc), OracleScene(sc)
+    hs.set_tuning(sort_incremental=1)
+    x, v, xp = os_.state()
+    V0 = sc.meshes[0].x.shape[0]
+    c = x[:V0].mean(axis=0)
+    w = np.float32(20.0)
+    v[:V0, 0] = -w * (x[:V0, 2] - c[2])
+    v[:V0, 2] = w * (x[:V0, 0] - c[0])
+    for s in (hs, os_):
+        s.set_state(x, v, xp)
+    for k in range(6):
+        hs.step(8, graph=True)
+        os_.step(8)
+        _same_state(hs.state(), os_.state())
+    og, cg = hs.contacts()
+    oo, co = os_.contacts()
+    assert np.array_equal(og, oo) and np.array_equal(cg, co)
+    assert hs.deferred_builds() > 0 and hs.flags() == 0
+    n0 = hs.deferred_builds()
+    hs.step(8, graph=False)
+    os_.step(8)
+    _same_state(hs.state(), os_.state())
+    assert hs.deferred_builds() == n0

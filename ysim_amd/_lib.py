@@ -65,6 +65,7 @@ _SIG = {
     "ysim_hip_scene_get_state": (C.c_int, [vp, vp, vp, vp]),
     "ysim_hip_scene_set_state": (C.c_int, [vp, vp, vp, vp]),
     "ysim_hip_scene_set_analytic": (C.c_int, [vp, vp, C.c_uint32]),
+    "ysim_hip_scene_deferred_builds": (C.c_int, [vp, vp]),
     "ysim_hip_scene_get_contacts": (C.c_int, [vp, vp, vp, u32, u32p]),
     "ysim_hip_scene_get_pairs": (C.c_int, [vp, vp, u32, u32p]),
     "ysim_hip_scene_get_tree": (C.c_int, [vp, u32, vp, vp, vp, vp]),

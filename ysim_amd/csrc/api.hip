@@ -197,6 +197,16 @@ struct ysim_scene {
   uint32_t* epoch = nullptr;    // [0] look-back epoch of the current sort
   uint32_t* need_full = nullptr;  // [0] incremental sort: the window passes left the keys unsorted
   uint32_t* inc_bcnt = nullptr;   // incremental sort: window-boundary arrival counters
+  uint32_t* box_parts = nullptr;  // one-mesh box: per-block boxes + arrival counter
+  // graph replays leave out the incremental sort's fallback kernels: a build
+  // whose order check fails keeps the previous (valid) topology, refit to the
+  // new positions, and raises h_stale (pinned host word written by the GPU);
+  // the host then runs the next substep eagerly, where the fallback sorts
+  bool capturing = false;
+  bool full_next = true;          // the next substep runs eagerly (state jumped)
+  uint32_t* h_stale = nullptr;    // pinned host word
+  uint32_t* d_stale = nullptr;    // its device address
+  uint64_t deferred_builds = 0;
   uint64_t* inc_edges = nullptr;  // incremental sort: each window's first and last (key, facet)
   uint32_t cap = 16;
   uint32_t* ccount = nullptr;
@@ -287,6 +297,7 @@ struct ysim_scene {
     return YSIM_OK;
   }
   int load_state(const float* src, uint32_t par) {
+    full_next = true;
     const size_t n = 3 * (size_t)adj.V, o = 3 * (size_t)adj.voff;
     YSIM_HIP_TRY(hipMemcpyAsync(x[par] + o, src, 4 * n, hipMemcpyDeviceToDevice, st));
     YSIM_HIP_TRY(hipMemcpyAsync(v[par] + o, src + n, 4 * n, hipMemcpyDeviceToDevice, st));
@@ -330,6 +341,7 @@ struct ysim_scene {
       YSIM_TRY(zero(hist, SORT_HIST_WORDS));
     }
     drop_graph();
+    full_next = true;
     return YSIM_OK;
   }
 
@@ -361,6 +373,7 @@ struct ysim_scene {
     YSIM_TRY(load_amark(tune_amark, par));
     parity = par;
     step = stp;
+    full_next = true;
     return YSIM_OK;
   }
 
@@ -635,7 +648,8 @@ struct ysim_scene {
   // (x + v dt when swept), so the substep's refit is not needed after it
   int enqueue_build(BuildGroup& g, const float* xc, const float* vc = nullptr, bool agglo = false) {
     if (!g.T) return YSIM_OK;
-    YSIM_TRY(launch_box(st, g.vsegs, mt, xc, g.V, box_acc, g.meshes.size() == 1 ? g.meshes[0] : 0xffffffffu));
+    YSIM_TRY(launch_box(st, g.vsegs, mt, xc, g.V, box_acc, g.meshes.size() == 1 ? g.meshes[0] : 0xffffffffu,
+                        box_parts));
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[1], st));
     if (g.perm && tune.sort_incremental) {
       // keys in the last build's sorted order + LDS window sorts + exact check;
@@ -645,12 +659,22 @@ struct ysim_scene {
                                  vals[0], g.T, need_full));
       if (timing) YSIM_HIP_TRY(hipEventRecord(ev[2], st));
       YSIM_TRY(launch_incremental_fix(st, g.key64, keys[0], vals[0], g.T, need_full, inc_bcnt, inc_edges));
-      YSIM_TRY(launch_morton(st, g.tsegs, mt, xc, facets, box_acc, g.code_bits, g.key64, keys[0], vals[0], g.T,
-                             g.plan, hist, counters, epoch, need_full));
-      YSIM_TRY(launch_sort_hybrid(st, g.plan, keys[0], vals[0], keys[1], vals[1], hist, status, counters, epoch,
-                                  flags, need_full));
+      const bool defer = capturing && !agglo;
+      if (!defer) {
+        YSIM_TRY(launch_morton(st, g.tsegs, mt, xc, facets, box_acc, g.code_bits, g.key64, keys[0], vals[0], g.T,
+                               g.plan, hist, counters, epoch, need_full));
+        YSIM_TRY(launch_sort_hybrid(st, g.plan, keys[0], vals[0], keys[1], vals[1], hist, status, counters, epoch,
+                                    flags, need_full));
+      }
       if (timing) YSIM_HIP_TRY(hipEventRecord(ev[3], st));
-      YSIM_TRY(enqueue_hierarchy(g, 0, xc, vc, agglo));
+      if (defer) {
+        YSIM_TRY(launch_hierarchy(st, g.tsegs, mt, keys[0], g.key64, vals[0], g.code_bits, tree, parent, visit, g.T,
+                                  box_acc, g.tsegs.nseg, hist, (uint32_t)g.plan.hist_words, g.perm, links,
+                                  HierarchySkip{need_full, d_stale}));
+        tree_agglo = false;
+      } else {
+        YSIM_TRY(enqueue_hierarchy(g, 0, xc, vc, agglo));
+      }
       last_sorted = 0;
       return YSIM_OK;
     }
@@ -823,8 +847,10 @@ struct ysim_scene {
     hipGraph_t g = nullptr;
     YSIM_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     int rc = YSIM_OK;
+    capturing = true;
     for (uint32_t i = 0; i < period && rc == YSIM_OK; ++i)
       rc = enqueue_substep(i % 2, rebuild_at(i));
+    capturing = false;
     hipError_t e = hipStreamEndCapture(st, &g);
     if (rc != YSIM_OK) return rc;
     if (e != hipSuccess) return fail(YSIM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -837,24 +863,36 @@ struct ysim_scene {
   }
 
   int steps(uint32_t n, bool use_graph) {
-    if (!use_graph) return eager_steps(n);
-    YSIM_TRY(capture_graph());
-    // align to the graph phase (step % period == 0 also implies parity 0)
-    while (n && (step % graph_period) != 0) {
-      YSIM_TRY(eager_steps(1));
-      --n;
+    if (!use_graph) {
+      full_next = false;
+      return eager_steps(n);
     }
-    while (n >= graph_period) {
+    YSIM_TRY(capture_graph());
+    while (n) {
+      // a jumped state, or a replay that kept a previous topology, takes one
+      // eager substep (its build runs the fallback sort when needed)
+      if (*(volatile uint32_t*)h_stale) {
+        *(volatile uint32_t*)h_stale = 0u;
+        ++deferred_builds;
+        full_next = true;
+      }
+      if (full_next || (step % graph_period) != 0 || n < graph_period) {
+        YSIM_TRY(eager_steps(1));
+        full_next = false;
+        --n;
+        continue;
+      }
       YSIM_HIP_TRY(hipGraphLaunch(graph, st));
       step += graph_period;
       steps_run += graph_period;
       n -= graph_period;
     }
-    return eager_steps(n);
+    return YSIM_OK;
   }
 
   ~ysim_scene() {
     if (graph) hipGraphExecDestroy(graph);
+    if (h_stale) (void)hipHostFree(h_stale);
     for (auto& e : ev)
       if (e) hipEventDestroy(e);
     for (void* p : allocs) hipFree(p);
@@ -1578,6 +1616,15 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
   YSIM_TRY(s.alloc(&s.epoch, 16));
   YSIM_TRY(s.alloc(&s.need_full, 16));
   YSIM_TRY(s.zero(s.need_full, 16));
+  {
+    uint32_t maxV = 0;
+    for (uint32_t m = 0; m < num_meshes; ++m) maxV = std::max(maxV, s.meshes[m].num_vertices);
+    YSIM_TRY(s.alloc(&s.box_parts, box_parts_words(maxV)));
+    YSIM_TRY(s.zero(s.box_parts, box_parts_words(maxV)));
+  }
+  YSIM_HIP_TRY(hipHostMalloc((void**)&s.h_stale, 64, 0));
+  *s.h_stale = 0u;
+  YSIM_HIP_TRY(hipHostGetDevicePointer((void**)&s.d_stale, s.h_stale, 0));
   YSIM_TRY(s.alloc(&s.inc_bcnt, incremental_windows((uint32_t)maxT)));
   YSIM_TRY(s.zero(s.inc_bcnt, incremental_windows((uint32_t)maxT)));
   YSIM_TRY(s.alloc(&s.inc_edges, 3 * (size_t)incremental_windows((uint32_t)maxT)));
@@ -1744,6 +1791,7 @@ int ysim_hip_scene_set_state(ysim_scene* s, const float* h_x, const float* h_v, 
   if (h_v) YSIM_HIP_TRY(hipMemcpyAsync(s->v[s->parity], h_v, b, hipMemcpyHostToDevice, s->st));
   if (h_xprev) YSIM_HIP_TRY(hipMemcpyAsync(s->x[s->parity ^ 1], h_xprev, b, hipMemcpyHostToDevice, s->st));
   YSIM_HIP_TRY(hipStreamSynchronize(s->st));
+  s->full_next = true;
   return YSIM_OK;
 }
 
@@ -1934,6 +1982,7 @@ int ysim_hip_scene_restore(ysim_scene* s) {
   YSIM_TRY(s->load_amark(s->snap_amark, par));
   s->parity = par;
   s->step = s->snap_step;
+  s->full_next = true;
   return YSIM_OK;
 }
 
@@ -1957,6 +2006,18 @@ int ysim_hip_scene_counters(ysim_scene* s, uint64_t* h_out, uint32_t reset) {
     s->pairs_hi = 0;
     s->counter_step0 = s->steps_run;
   }
+  return YSIM_OK;
+}
+
+int ysim_hip_scene_deferred_builds(ysim_scene* s, uint64_t* h_out) {
+  YSIM_ARG(s && h_out, "scene_deferred_builds: null argument");
+  YSIM_HIP_TRY(hipStreamSynchronize(s->st));
+  if (*(volatile uint32_t*)s->h_stale) {
+    *(volatile uint32_t*)s->h_stale = 0u;
+    ++s->deferred_builds;
+    s->full_next = true;
+  }
+  *h_out = s->deferred_builds;
   return YSIM_OK;
 }
 

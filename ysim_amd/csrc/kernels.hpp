@@ -36,9 +36,13 @@ struct MeshTables {
 // ---------------------------------------------------------------- lbvh.hip
 // S0: per-segment exact min/max of vertex positions into box_acc[seg*6] (ordered keys).
 // single_mesh: the mesh id when vsegs is that one mesh (a kernel with fewer
-// atomics), else 0xffffffff.
+// atomics), else 0xffffffff.  parts (single mesh only): box_parts_words(n)
+// words, an arrival counter (word 0, zero before the first use) and per-block
+// boxes;
+// the last block reduces them (one same-address atomic per block).
+uint32_t box_parts_words(uint32_t nverts);
 int launch_box(hipStream_t s, SegTable vsegs, MeshTables mt, const float* x, uint32_t nitems,
-               uint32_t* box_acc, uint32_t single_mesh = 0xffffffffu);
+               uint32_t* box_acc, uint32_t single_mesh = 0xffffffffu, uint32_t* parts = nullptr);
 // S1: Morton keys of every triangle of tsegs (segment id prefixed above the code
 // bits when nseg > 1), values = mesh-local facet id, plus the digit histograms
 // of all sort passes of `plan`; zeroes the sort's block counters and advances
@@ -61,11 +65,19 @@ struct RefitLinks {
 // S3: Karras hierarchy per segment; child and parent links (and rl.up when
 // set); zeroes the refit visit counters; resets box_acc (nbox segments) and the
 // sort histogram for the next build.
+// Optional gate of the hierarchy: when *flag is set (the incremental sort's
+// order check failed and its fallback was left out) the kernel only does its
+// resets and keeps the previous topology, sorted order and links, and writes 1
+// to *note (host-visible, system scope).
+struct HierarchySkip {
+  const uint32_t* flag = nullptr;
+  uint32_t* note = nullptr;
+};
 int launch_hierarchy(hipStream_t s, SegTable tsegs, MeshTables mt, const void* keys, bool key64,
                      const uint32_t* vals, uint32_t code_bits, ysim_bvh_node* tree, int32_t* parent,
                      uint32_t* visit, uint32_t nitems, uint32_t* box_acc, uint32_t nbox,
                      uint32_t* hist, uint32_t hist_words, uint32_t* perm_out = nullptr,
-                     RefitLinks rl = RefitLinks{});
+                     RefitLinks rl = RefitLinks{}, HierarchySkip skip = HierarchySkip{});
 // S3 alternative: agglomerative (Apetrei) hierarchy fused with the leaf boxes
 // (swept optional) and their bottom-up union, root swapped into slot 0;
 // writes child and parent links (no refit links), the sorted facet order and

@@ -136,9 +136,95 @@ __global__ __launch_bounds__(BOX1_THREADS) void k_box1(MeshTables mt, uint32_t m
   }
 }
 
+// One mesh over every CU: 1024 vertices per 256-thread block.  Each block
+// writes its box (ordered keys) with device-coherent stores and arrives at a
+// counter; the last block to arrive reduces every block's box into acc and
+// resets the counter.  One same-address atomic per block instead of six.
+constexpr uint32_t BOXP_THREADS = 256, BOXP_ITEMS = 4;
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o));
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
+  return v;
+}
+__global__ __launch_bounds__(BOXP_THREADS) void k_box_parts(MeshTables mt, uint32_t mesh, const float* __restrict__ x,
+                                                            uint32_t n, uint32_t* parts, uint32_t* __restrict__ acc) {
+  constexpr uint32_t NW = BOXP_THREADS / 64;
+  __shared__ uint32_t s_red[6][NW];
+  __shared__ uint32_t s_last;
+  const uint32_t start = blockIdx.x * (BOXP_THREADS * BOXP_ITEMS);
+  const uint32_t end = min(start + BOXP_THREADS * BOXP_ITEMS, n);
+  const uint32_t gbase = mt.voff[mesh];
+  const float INF = __uint_as_float(0x7f800000u);
+  f3 lo = {INF, INF, INF}, hi = {-INF, -INF, -INF};
+#pragma unroll
+  for (uint32_t i = 0; i < BOXP_ITEMS; ++i) {
+    const uint32_t j = start + threadIdx.x + BOXP_THREADS * i;
+    if (j < end) {
+      const f3 p = ld3(x, gbase + j);
+      lo = vmin(lo, p);
+      hi = vmax(hi, p);
+    }
+  }
+  const uint32_t r[6] = {wave_min_u32(fkey(lo.x)), wave_min_u32(fkey(lo.y)), wave_min_u32(fkey(lo.z)),
+                         wave_max_u32(fkey(hi.x)), wave_max_u32(fkey(hi.y)), wave_max_u32(fkey(hi.z))};
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int k = 0; k < 6; ++k) s_red[k][w] = r[k];
+  __syncthreads();
+  uint32_t* counter = parts;  // word 0; block b's box at 1 + 6 b
+  if (threadIdx.x < 64) {  // wave 0 publishes the block's box, then arrives
+    if (threadIdx.x < 6) {
+      const uint32_t k = threadIdx.x;
+      uint32_t v = s_red[k][0];
+      for (uint32_t ww = 1; ww < NW; ++ww) v = k < 3 ? min(v, s_red[k][ww]) : max(v, s_red[k][ww]);
+      st_agent_u32(parts + 1 + 6 * (size_t)blockIdx.x + k, v);
+    }
+    wait_all_mem();
+    if (threadIdx.x == 0) {
+      const uint32_t old = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_last = (old == gridDim.x - 1) ? 1u : 0u;
+    }
+  }
+  __syncthreads();
+  if (!s_last) return;
+  uint32_t v[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+  for (uint32_t b = threadIdx.x; b < gridDim.x; b += BOXP_THREADS)
+    for (int k = 0; k < 6; ++k) {
+      const uint32_t q = ld_agent_u32(parts + 1 + 6 * (size_t)b + k);
+      v[k] = k < 3 ? min(v[k], q) : max(v[k], q);
+    }
+  for (int k = 0; k < 6; ++k) v[k] = k < 3 ? wave_min_u32(v[k]) : wave_max_u32(v[k]);
+  __syncthreads();
+  if (lane == 0)
+    for (int k = 0; k < 6; ++k) s_red[k][w] = v[k];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const uint32_t k = threadIdx.x;
+    uint32_t u = s_red[k][0];
+    for (uint32_t ww = 1; ww < NW; ++ww) u = k < 3 ? min(u, s_red[k][ww]) : max(u, s_red[k][ww]);
+    acc[k] = u;
+  }
+  if (threadIdx.x == 0) *counter = 0u;
+}
+
+uint32_t box_parts_words(uint32_t nverts) {
+  return 6 * ceil_div(nverts, BOXP_THREADS * BOXP_ITEMS) + 1;
+}
+
 int launch_box(hipStream_t s, SegTable vsegs, MeshTables mt, const float* x, uint32_t nitems,
-               uint32_t* box_acc, uint32_t single_mesh) {
+               uint32_t* box_acc, uint32_t single_mesh, uint32_t* parts) {
   if (!nitems) return YSIM_OK;
+  if (single_mesh != 0xffffffffu && parts) {
+    hipLaunchKernelGGL(k_box_parts, dim3(ceil_div(nitems, BOXP_THREADS * BOXP_ITEMS)), dim3(BOXP_THREADS), 0, s, mt,
+                       single_mesh, x, nitems, parts, box_acc);
+    YSIM_CHECK_LAUNCH();
+    return YSIM_OK;
+  }
   if (single_mesh != 0xffffffffu)
     hipLaunchKernelGGL(k_box1, dim3(ceil_div(nitems, BOX1_THREADS * BOX1_ITEMS)), dim3(BOX1_THREADS), 0, s, mt,
                        single_mesh, x, nitems, box_acc);
@@ -314,11 +400,16 @@ __global__ __launch_bounds__(256) void k_hierarchy(SegTable ts, MeshTables mt, c
                                                    uint32_t* __restrict__ visit, uint32_t n,
                                                    uint32_t* __restrict__ box_acc, uint32_t nbox,
                                                    uint32_t* __restrict__ hist, uint32_t hist_words,
-                                                   uint32_t* __restrict__ perm_out, RefitLinks rl) {
+                                                   uint32_t* __restrict__ perm_out, RefitLinks rl,
+                                                   HierarchySkip skip) {
   const uint32_t j = blockIdx.x * 256 + threadIdx.x;
   // reset accumulators for the next build (their readers ran in earlier kernels)
   if (j < nbox * 6) box_acc[j] = ((j % 6) < 3) ? 0xffffffffu : 0u;
   if (j < hist_words) hist[j] = 0;
+  if (skip.flag && *skip.flag) {
+    if (j == 0 && skip.note) __hip_atomic_store(skip.note, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    return;
+  }
   if (j >= n) return;
   const uint32_t sg = seg_of(ts, j);
   const uint32_t m = ts.mesh[sg];
@@ -377,18 +468,18 @@ __global__ __launch_bounds__(256) void k_hierarchy(SegTable ts, MeshTables mt, c
 int launch_hierarchy(hipStream_t s, SegTable tsegs, MeshTables mt, const void* keys, bool key64,
                      const uint32_t* vals, uint32_t code_bits, ysim_bvh_node* tree, int32_t* parent,
                      uint32_t* visit, uint32_t nitems, uint32_t* box_acc, uint32_t nbox, uint32_t* hist,
-                     uint32_t hist_words, uint32_t* perm_out, RefitLinks rl) {
+                     uint32_t hist_words, uint32_t* perm_out, RefitLinks rl, HierarchySkip skip) {
   uint32_t work = std::max<uint32_t>(nitems, std::max<uint32_t>(nbox * 6, hist_words));
   if (!work) return YSIM_OK;
   uint32_t blocks = ceil_div(work, 256);
   if (key64)
     hipLaunchKernelGGL(k_hierarchy<uint64_t>, dim3(blocks), dim3(256), 0, s, tsegs, mt,
                        (const uint64_t*)keys, vals, code_bits, tree, parent, visit, nitems, box_acc, nbox,
-                       hist, hist_words, perm_out, rl);
+                       hist, hist_words, perm_out, rl, skip);
   else
     hipLaunchKernelGGL(k_hierarchy<uint32_t>, dim3(blocks), dim3(256), 0, s, tsegs, mt,
                        (const uint32_t*)keys, vals, code_bits, tree, parent, visit, nitems, box_acc, nbox,
-                       hist, hist_words, perm_out, rl);
+                       hist, hist_words, perm_out, rl, skip);
   YSIM_CHECK_LAUNCH();
   return YSIM_OK;
 }

@@ -223,6 +223,12 @@ class HipScene:
         if scene.analytic is not None and len(scene.analytic):
             self.set_analytic(scene.analytic)
 
+    def deferred_builds(self) -> int:
+        """Graph replays that kept a previous topology (failed order check)."""
+        n = C.c_uint64(0)
+        call("ysim_hip_scene_deferred_builds", self.h, C.byref(n))
+        return n.value
+
     def set_analytic(self, shapes):
         """Install analytic colliders (a numpy array of dtype abi.ANALYTIC)."""
         a = np.ascontiguousarray(shapes, abi.ANALYTIC).reshape(-1)

@@ -357,6 +357,12 @@ def main():
                "query": "k_query_tile" if int(tuning.get("query_tile", 0))
                else f"k_query<false, {int(tuning['query_block'])}, {int(tuning['query_waves'])}>",
                "integrate": f"k_cloth<{int(tuning['cloth_block'])}>"}
+    if sum(1 for m in scene.meshes if m.behavior != scenes.BEHAVIOR_FLOAT) == 1:
+        # one dynamic mesh: the cloth kernel reduces the box of the positions it
+        # writes, and the next build launches no box kernel
+        kernels["box"] = "(fused into k_cloth)"
+        launches["box"] = 0
+        sb["box"] = 0
     if int(tuning.get("build_mode", 0)):
         # the agglomerative build writes the swept boxes itself: the refit
         # stage has no launch on a rebuilding substep

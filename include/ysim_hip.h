@@ -296,7 +296,10 @@ int ysim_hip_scene_get_contacts(ysim_scene* scene, uint32_t* h_offsets /* Vtot+1
  * unordered; caller sorts to compare as a set. */
 int ysim_hip_scene_get_pairs(ysim_scene* scene, ysim_broad_collision* h_pairs,
                              uint32_t capacity, uint32_t* h_count);
-/* Tree of one mesh (2T-1 nodes) and the sorted Morton keys (uint64 widened). */
+/* Tree of one mesh (2T-1 nodes) and the sorted Morton keys (uint64 widened).
+ * Keys and order are kept for the last dynamic build group only (its meshes:
+ * every dynamic mesh when one group holds them); for a mesh of another group
+ * asking for them returns YSIM_ERR_STATE. */
 int ysim_hip_scene_get_tree(ysim_scene* scene, uint32_t mesh, ysim_bvh_node* h_nodes,
                             int32_t* h_parent, uint64_t* h_sorted_keys, uint32_t* h_sorted_idx);
 /* Per-stage average device time (ms) over the last n_steps eager substeps run

@@ -255,3 +255,56 @@ def test_graph_replays_keep_topology_when_order_check_fails(ctx, oracle_threads)
     os_.step(8)
     _same_state(hs.state(), os_.state())
     assert hs.deferred_builds() == n0
+
+
+def _same_keys_and_tree(hs, os_, mesh, keys=True):
+    if keys:
+        ng, pg, kg, ig = hs.tree(mesh, keys=True)
+    else:
+        ng, pg = hs.tree(mesh)
+    no, po, ko, io = os_.tree(mesh)
+    if keys:
+        assert np.array_equal(kg, ko) and np.array_equal(ig, io)
+    for f in ("min", "max", "child_a", "child_b"):
+        assert np.array_equal(ng[f], no[f]), f
+    assert np.array_equal(pg, po)
+
+
+def test_box_from_cloth_kernel_across_block_sizes(ctx, oracle_threads):
+    """One dynamic mesh: the cloth kernel reduces the next build's Morton box.
+    Switching its block size (another grid, the same partials buffer) between
+    graph runs keeps keys and trees equal to the oracle's after each rebuild."""
+    sc = scenes.config_c2(n=96)
+    hs, os_ = HipScene(ctx, sc), OracleScene(sc)
+    hs.set_tuning(sort_incremental=1)
+    for cb in (128, 512, 256, 128, 512):
+        hs.set_tuning(cloth_block=cb)
+        hs.step(6, graph=True)
+        os_.step(6)
+        _same_state(hs.state(), os_.state())
+        # the last substep rebuilt from the box the previous cloth kernel wrote
+        _same_keys_and_tree(hs, os_, 0)
+    assert hs.flags() == 0
+
+
+def test_one_mesh_groups_of_different_sizes_share_box_partials(ctx, oracle_threads):
+    """63-bit keys leave no room for a mesh index: every cloth of a batched
+    scene is its own build group, and their one-mesh box kernels (grids of
+    different sizes) share one partials buffer.  Trees equal the oracle's for
+    every cloth (the topology follows from each cloth's box and keys), keys
+    for the last group, whose sort buffers stay resident; asking another
+    cloth for its keys is an error."""
+    sc = scenes.config_c3(count=4, lo=16, hi=48, substeps=20)
+    sc.params["morton_bits"] = 63
+    sc.params["rebuild_every"] = 1
+    hs, os_ = HipScene(ctx, sc), OracleScene(sc)
+    for k in range(3):
+        hs.step(4, graph=(k % 2 == 0))
+        os_.step(4)
+        _same_state(hs.state(), os_.state())
+    dyn = [m for m, mesh in enumerate(sc.meshes) if mesh.behavior != scenes.BEHAVIOR_FLOAT]
+    for m in dyn:
+        _same_keys_and_tree(hs, os_, m, keys=(m == dyn[-1]))
+    with pytest.raises(YsimError):
+        hs.tree(dyn[0], keys=True)
+    assert hs.flags() == 0

@@ -186,7 +186,7 @@ struct ysim_scene {
   ysim_bvh_node* tree = nullptr;
   int32_t* parent = nullptr;
   uint32_t* visit = nullptr;
-  RefitLinks links;             // per node: parent's leaf count and side (LDS refit)
+  RefitLinks links;             // per node: parent's leaf count and side, leaf facet ids (LDS refit)
   std::vector<BuildGroup> dyn_groups, stat_groups;
   SegTable dyn_tsegs, dyn_vsegs;
   uint32_t *d_tgt_off = nullptr, *d_tgt = nullptr;
@@ -204,6 +204,12 @@ struct ysim_scene {
   // the host then runs the next substep eagerly, where the fallback sorts
   bool capturing = false;
   bool full_next = true;          // the next substep runs eagerly (state jumped)
+  // One dynamic mesh: the cloth kernel also writes per block the box of its
+  // new positions (cloth_parts); while box_fresh (no host change of positions
+  // since) the next incremental build reduces them in its Morton kernel and
+  // launches no box kernel.
+  bool fuse_box = false, box_fresh = false;
+  uint32_t* cloth_parts = nullptr;
   uint32_t* h_stale = nullptr;    // pinned host word
   uint32_t* d_stale = nullptr;    // its device address
   uint64_t deferred_builds = 0;
@@ -298,6 +304,7 @@ struct ysim_scene {
   }
   int load_state(const float* src, uint32_t par) {
     full_next = true;
+    box_fresh = false;
     const size_t n = 3 * (size_t)adj.V, o = 3 * (size_t)adj.voff;
     YSIM_HIP_TRY(hipMemcpyAsync(x[par] + o, src, 4 * n, hipMemcpyDeviceToDevice, st));
     YSIM_HIP_TRY(hipMemcpyAsync(v[par] + o, src + n, 4 * n, hipMemcpyDeviceToDevice, st));
@@ -342,6 +349,7 @@ struct ysim_scene {
     }
     drop_graph();
     full_next = true;
+    box_fresh = false;
     return YSIM_OK;
   }
 
@@ -374,6 +382,7 @@ struct ysim_scene {
     parity = par;
     step = stp;
     full_next = true;
+    box_fresh = false;
     return YSIM_OK;
   }
 
@@ -648,15 +657,19 @@ struct ysim_scene {
   // (x + v dt when swept), so the substep's refit is not needed after it
   int enqueue_build(BuildGroup& g, const float* xc, const float* vc = nullptr, bool agglo = false) {
     if (!g.T) return YSIM_OK;
-    YSIM_TRY(launch_box(st, g.vsegs, mt, xc, g.V, box_acc, g.meshes.size() == 1 ? g.meshes[0] : 0xffffffffu,
-                        box_parts));
+    // the incremental Morton kernel reduces the cloth kernel's block boxes
+    const bool parts_box = fuse_box && box_fresh && &g == &dyn_groups[0] && g.perm && tune.sort_incremental;
+    if (!parts_box)
+      YSIM_TRY(launch_box(st, g.vsegs, mt, xc, g.V, box_acc, g.meshes.size() == 1 ? g.meshes[0] : 0xffffffffu,
+                          box_parts));
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[1], st));
     if (g.perm && tune.sort_incremental) {
       // keys in the last build's sorted order + LDS window sorts + exact check;
       // the full build (Morton in facet order + hybrid sort) runs only if the
       // check failed (its kernels return at once otherwise)
       YSIM_TRY(launch_morton_inc(st, g.tsegs, mt, xc, facets, box_acc, g.code_bits, g.key64, g.perm, keys[0],
-                                 vals[0], g.T, need_full));
+                                 vals[0], g.T, need_full, parts_box ? cloth_parts : nullptr,
+                                 parts_box ? cloth_box_parts(Vdyn, tune.cloth_block) : 0u));
       if (timing) YSIM_HIP_TRY(hipEventRecord(ev[2], st));
       YSIM_TRY(launch_incremental_fix(st, g.key64, keys[0], vals[0], g.T, need_full, inc_bcnt, inc_edges));
       const bool defer = capturing && !agglo;
@@ -815,7 +828,9 @@ struct ysim_scene {
     ca.nitems = Vdyn;
     ca.flags = flags;
     ca.block = tune.cloth_block;
+    if (fuse_box) ca.box_parts = cloth_parts;
     YSIM_TRY(launch_cloth(st, ca));
+    if (fuse_box) box_fresh = true;
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[7], st));
     return YSIM_OK;
   }
@@ -847,10 +862,15 @@ struct ysim_scene {
     hipGraph_t g = nullptr;
     YSIM_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     int rc = YSIM_OK;
+    // a replay always follows a substep (box_fresh), so its first build
+    // skips the box kernel too; the host flag is left as it was
+    const bool fresh = box_fresh;
+    box_fresh = true;
     capturing = true;
     for (uint32_t i = 0; i < period && rc == YSIM_OK; ++i)
       rc = enqueue_substep(i % 2, rebuild_at(i));
     capturing = false;
+    box_fresh = fresh;
     hipError_t e = hipStreamEndCapture(st, &g);
     if (rc != YSIM_OK) return rc;
     if (e != hipSuccess) return fail(YSIM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -876,7 +896,7 @@ struct ysim_scene {
         ++deferred_builds;
         full_next = true;
       }
-      if (full_next || (step % graph_period) != 0 || n < graph_period) {
+      if (full_next || (fuse_box && !box_fresh) || (step % graph_period) != 0 || n < graph_period) {
         YSIM_TRY(eager_steps(1));
         full_next = false;
         --n;
@@ -1581,6 +1601,7 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
   YSIM_TRY(s.alloc(&s.parent, s.Ntot));
   YSIM_TRY(s.alloc(&s.visit, s.Ntot));
   YSIM_TRY(s.alloc(&s.links.up, s.Ntot));
+  YSIM_TRY(s.alloc(&s.links.fid, s.Ntot));
   YSIM_TRY(s.zero(s.tree, 8 * (size_t)s.Ntot));
   YSIM_TRY(s.zero(s.visit, s.Ntot));
   // segment tables
@@ -1622,6 +1643,8 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
     YSIM_TRY(s.alloc(&s.box_parts, box_parts_words(maxV)));
     YSIM_TRY(s.zero(s.box_parts, box_parts_words(maxV)));
   }
+  s.fuse_box = s.dyn_groups.size() == 1 && s.dyn_groups[0].meshes.size() == 1 && s.dyn_groups[0].V == s.Vdyn;
+  if (s.fuse_box) YSIM_TRY(s.alloc(&s.cloth_parts, cloth_box_parts_words(s.Vdyn)));
   YSIM_HIP_TRY(hipHostMalloc((void**)&s.h_stale, 64, 0));
   *s.h_stale = 0u;
   YSIM_HIP_TRY(hipHostGetDevicePointer((void**)&s.d_stale, s.h_stale, 0));
@@ -1792,6 +1815,7 @@ int ysim_hip_scene_set_state(ysim_scene* s, const float* h_x, const float* h_v, 
   if (h_xprev) YSIM_HIP_TRY(hipMemcpyAsync(s->x[s->parity ^ 1], h_xprev, b, hipMemcpyHostToDevice, s->st));
   YSIM_HIP_TRY(hipStreamSynchronize(s->st));
   s->full_next = true;
+  s->box_fresh = false;
   return YSIM_OK;
 }
 
@@ -1916,9 +1940,10 @@ int ysim_hip_scene_get_tree(ysim_scene* s, uint32_t mesh, ysim_bvh_node* h_nodes
   if (h_parent) YSIM_HIP_TRY(hipMemcpyAsync(h_parent, s->parent + s->noff[mesh], nn * 4, hipMemcpyDeviceToHost, s->st));
   YSIM_HIP_TRY(hipStreamSynchronize(s->st));
   if (h_sorted_keys || h_sorted_idx) {
-    // the sort buffers hold the last build of the first dynamic group
+    // the sort buffers hold the build of the last dynamic group (groups are
+    // built in order and share them)
     if (s->dyn_groups.empty()) return fail(YSIM_ERR_STATE, "scene_get_tree: no dynamic build");
-    const BuildGroup& g = s->dyn_groups[0];
+    const BuildGroup& g = s->dyn_groups.back();
     auto it = std::find(g.meshes.begin(), g.meshes.end(), mesh);
     if (it == g.meshes.end()) return fail(YSIM_ERR_STATE, "scene_get_tree: keys of this mesh are not resident");
     uint32_t start = 0;
@@ -1983,6 +2008,7 @@ int ysim_hip_scene_restore(ysim_scene* s) {
   s->parity = par;
   s->step = s->snap_step;
   s->full_next = true;
+  s->box_fresh = false;
   return YSIM_OK;
 }
 

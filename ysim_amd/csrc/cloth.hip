@@ -15,11 +15,9 @@
 
 namespace ysim {
 
-// BS threads per block (128, 256 or 512; chosen by the autotuner).
-template <int BS>
-__global__ __launch_bounds__(BS) void k_cloth(ClothArgs a) {
-  const uint32_t j = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
-  if (j >= a.nitems) return;
+// One vertex j < a.nitems: forces, integrate, contacts, guard; stores v_out
+// and x_out and returns the new position.
+__device__ __forceinline__ f3 cloth_vertex(const ClothArgs& a, uint32_t j) {
   const uint32_t sg = seg_of(a.dsegs, j);
   const uint32_t m = a.dsegs.mesh[sg];
   const uint32_t i = j - a.dsegs.off[sg];
@@ -101,9 +99,29 @@ __global__ __launch_bounds__(BS) void k_cloth(ClothArgs a) {
   if (sanitize(pos, vel, pos0)) atomicOr(a.flags, 1u);
   st3(a.v_out, g, vel);
   st3(a.x_out, g, pos);
+  return pos;
+}
+
+// BS threads per block (128, 256 or 512; chosen by the autotuner).  With
+// box_parts set, each block also writes the box of its new positions (6
+// ordered keys at box_parts + 6 blockIdx.x); the next build's Morton kernel
+// reduces them, so the build needs no box kernel of its own.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_cloth(ClothArgs a) {
+  const uint32_t j = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
+  if (!a.box_parts) {
+    if (j < a.nitems) cloth_vertex(a, j);
+    return;
+  }
+  const float INF = __uint_as_float(0x7f800000u);
+  f3 lo = {INF, INF, INF}, hi = {-INF, -INF, -INF};
+  if (j < a.nitems) lo = hi = cloth_vertex(a, j);
+  block_box_store<BS>(lo, hi, a.box_parts + 6 * (size_t)blockIdx.x);
 }
 
 bool cloth_block_ok(uint32_t b) { return b == 128 || b == 256 || b == 512; }
+uint32_t cloth_box_parts_words(uint32_t nitems) { return 6 * ceil_div(nitems, 128); }
+uint32_t cloth_box_parts(uint32_t nitems, uint32_t block) { return ceil_div(nitems, block); }
 
 int launch_cloth(hipStream_t s, const ClothArgs& a) {
   if (!a.nitems) return YSIM_OK;

@@ -272,4 +272,141 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblocks) {
   return x * per + min(x, rem) + k;
 }
 
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o));
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
+  return v;
+}
+
+// Box of a whole grid of NT-thread blocks into acc[0..5] (ordered keys fkey:
+// min x, y, z, max x, y, z), from one (lo, hi) per thread (+inf / -inf for a
+// thread with no point).  Two levels of last-arrival reduction, so no counter
+// takes more than BOX_GROUP same-address atomics (they serialize at the memory
+// side): each block writes its box to parts with device-coherent stores and
+// arrives at its group's counter; the last block of a group reduces the
+// group's boxes, writes the group box and arrives at the top counter; the last
+// group reduces the group boxes into acc.  Each last arrival resets the
+// counter it read.  parts layout (grid_box_parts_words(gridDim.x) words, zero
+// before the first launch): top counter, BOX_MAX_GROUPS group counters (at
+// fixed places, so launches of other grid sizes sharing the buffer find them
+// zero), 6 words per block, 6 words per group.  At most BOX_GROUP x
+// BOX_MAX_GROUPS blocks.  Every thread of the block must call it.
+constexpr uint32_t BOX_GROUP = 32, BOX_MAX_GROUPS = 4096;
+inline uint32_t grid_box_parts_words(uint32_t nblocks) {
+  const uint32_t ng = (nblocks + BOX_GROUP - 1) / BOX_GROUP;
+  return 1 + BOX_MAX_GROUPS + 6 * nblocks + 6 * ng;
+}
+inline bool grid_box_fits(uint32_t nblocks) { return nblocks <= BOX_GROUP * BOX_MAX_GROUPS; }
+// min (k < 3) or max (k >= 3) of 6-word records src[0 .. count) into threads
+// 0..5's return value (other threads: unspecified); a block-wide step.  Each
+// thread folds whole records, then wave reductions and one LDS pass over the
+// waves (s_acc: at least 6 NT / 64 words).  COHERENT: records written by other
+// workgroups of the running grid.
+template <int NT, bool COHERENT = true>
+__device__ __forceinline__ uint32_t reduce_box_records(const uint32_t* src, uint32_t count, uint32_t* s_acc) {
+  constexpr uint32_t NW = NT / 64;
+  uint32_t v[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+  for (uint32_t r = threadIdx.x; r < count; r += NT)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const uint32_t u = COHERENT ? ld_agent_u32(src + 6 * (size_t)r + k) : src[6 * (size_t)r + k];
+      v[k] = k < 3 ? min(v[k], u) : max(v[k], u);
+    }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) v[k] = k < 3 ? wave_min_u32(v[k]) : wave_max_u32(v[k]);
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  __syncthreads();  // s_acc may still be read by an earlier call
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s_acc[k * NW + w] = v[k];
+  __syncthreads();
+  uint32_t out = 0;
+  if (threadIdx.x < 6) {
+    const uint32_t k = threadIdx.x;
+    out = s_acc[k * NW];
+    for (uint32_t ww = 1; ww < NW; ++ww) out = k < 3 ? min(out, s_acc[k * NW + ww]) : max(out, s_acc[k * NW + ww]);
+  }
+  return out;
+}
+// The block's box (ordered keys, as grid_box_reduce) written by threads 0..5
+// to out[0..5] with plain stores.  Every thread of the block must call it.
+template <int NT>
+__device__ __forceinline__ void block_box_store(f3 lo, f3 hi, uint32_t* out) {
+  constexpr uint32_t NW = NT / 64;
+  __shared__ uint32_t s_red[6][NW];
+  const uint32_t r[6] = {wave_min_u32(fkey(lo.x)), wave_min_u32(fkey(lo.y)), wave_min_u32(fkey(lo.z)),
+                         wave_max_u32(fkey(hi.x)), wave_max_u32(fkey(hi.y)), wave_max_u32(fkey(hi.z))};
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int k = 0; k < 6; ++k) s_red[k][w] = r[k];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const uint32_t k = threadIdx.x;
+    uint32_t v = s_red[k][0];
+    for (uint32_t ww = 1; ww < NW; ++ww) v = k < 3 ? min(v, s_red[k][ww]) : max(v, s_red[k][ww]);
+    out[k] = v;
+  }
+}
+
+template <int NT>
+__device__ __forceinline__ void grid_box_reduce(f3 lo, f3 hi, uint32_t* parts, uint32_t* acc) {
+  constexpr uint32_t NW = NT / 64;
+  __shared__ uint32_t s_red[6][NW];
+  __shared__ uint32_t s_acc[NT];
+  __shared__ uint32_t s_last;
+  const uint32_t nb = gridDim.x, ng = (nb + BOX_GROUP - 1) / BOX_GROUP;
+  const uint32_t b = blockIdx.x, g = b / BOX_GROUP;
+  uint32_t* top = parts;
+  uint32_t* gcnt = parts + 1;
+  uint32_t* bbox = parts + 1 + BOX_MAX_GROUPS;
+  uint32_t* gbox = bbox + 6 * (size_t)nb;
+  const uint32_t r[6] = {wave_min_u32(fkey(lo.x)), wave_min_u32(fkey(lo.y)), wave_min_u32(fkey(lo.z)),
+                         wave_max_u32(fkey(hi.x)), wave_max_u32(fkey(hi.y)), wave_max_u32(fkey(hi.z))};
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int k = 0; k < 6; ++k) s_red[k][w] = r[k];
+  __syncthreads();
+  if (threadIdx.x < 64) {  // wave 0 publishes the block's box, then arrives
+    if (threadIdx.x < 6) {
+      const uint32_t k = threadIdx.x;
+      uint32_t v = s_red[k][0];
+      for (uint32_t ww = 1; ww < NW; ++ww) v = k < 3 ? min(v, s_red[k][ww]) : max(v, s_red[k][ww]);
+      st_agent_u32(bbox + 6 * (size_t)b + k, v);
+    }
+    wait_all_mem();
+    if (threadIdx.x == 0) {
+      const uint32_t gsize = min(BOX_GROUP, nb - g * BOX_GROUP);
+      const uint32_t old = __hip_atomic_fetch_add(gcnt + g, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_last = (old == gsize - 1) ? 1u : 0u;
+    }
+  }
+  __syncthreads();
+  if (!s_last) return;
+  // last block of group g
+  {
+    const uint32_t gsize = min(BOX_GROUP, nb - g * BOX_GROUP);
+    const uint32_t v = reduce_box_records<NT>(bbox + 6 * (size_t)g * BOX_GROUP, gsize, s_acc);
+    if (threadIdx.x < 64) {
+      if (threadIdx.x < 6) st_agent_u32(gbox + 6 * (size_t)g + threadIdx.x, v);
+      wait_all_mem();
+      if (threadIdx.x == 0) {
+        gcnt[g] = 0u;
+        const uint32_t old = __hip_atomic_fetch_add(top, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (old == ng - 1) ? 1u : 0u;
+      }
+    }
+    __syncthreads();
+    if (!s_last) return;
+  }
+  // last group
+  const uint32_t v = reduce_box_records<NT>(gbox, ng, s_acc);
+  if (threadIdx.x < 6) acc[threadIdx.x] = v;
+  if (threadIdx.x == 0) *top = 0u;
+}
+
 }  // namespace ysim

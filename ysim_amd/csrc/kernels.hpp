@@ -54,13 +54,22 @@ int launch_morton(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x,
                   uint32_t* hist, uint32_t* counters, uint32_t* epoch, const uint32_t* run_flag = nullptr);
 // Incremental S1: keys in the order perm (previous sorted facets), each
 // 2048-key window sorted by (key, facet); zeroes *need_full.
+// With nparts > 0 (one segment only) the box is the reduction of nparts
+// 6-word records at box_parts (the cloth kernel's per-block boxes); every
+// block reduces them and block 0 writes the box to box_acc for the build's
+// later kernels.  Otherwise box_acc holds the box per segment.
 int launch_morton_inc(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x, const uint32_t* facets,
-                      const uint32_t* box_acc, uint32_t code_bits, bool key64, const uint32_t* perm, void* keys,
-                      uint32_t* vals, uint32_t nitems, uint32_t* need_full);
+                      uint32_t* box_acc, uint32_t code_bits, bool key64, const uint32_t* perm, void* keys,
+                      uint32_t* vals, uint32_t nitems, uint32_t* need_full, const uint32_t* box_parts = nullptr,
+                      uint32_t nparts = 0);
 // Extra links of a scene build for the LDS refit, written by the hierarchy
 // kernel (null: not written, and the refit takes the device-wide path only).
+// With fid set, the hierarchy writes no tree nodes: the LDS refit that follows
+// writes each node whole (box and child links) in one pass, so no node is
+// written in part twice per rebuild.
 struct RefitLinks {
-  uint32_t* up = nullptr;  // [node window + node] parent's leaf count << 1 | (node is child_b)
+  uint32_t* up = nullptr;   // [node window + node] parent's leaf count << 1 | (node is child_b)
+  uint32_t* fid = nullptr;  // [node window + leaf slot] the leaf's facet id
 };
 // S3: Karras hierarchy per segment; child and parent links (and rl.up when
 // set); zeroes the refit visit counters; resets box_acc (nbox segments) and the
@@ -251,8 +260,13 @@ struct ClothArgs {
   uint32_t nitems = 0;
   uint32_t* flags = nullptr;          // bit0 anomaly
   uint32_t block = 256;               // threads per block: 128, 256 or 512
+  // optional: per block, the box of its x_out (6 ordered keys) at
+  // box_parts + 6 block; cloth_box_parts(nitems, block) records
+  uint32_t* box_parts = nullptr;
 };
 bool cloth_block_ok(uint32_t b);
+uint32_t cloth_box_parts_words(uint32_t nitems);  // enough for every block size
+uint32_t cloth_box_parts(uint32_t nitems, uint32_t block);
 int launch_cloth(hipStream_t s, const ClothArgs& a);
 
 // ---------------------------------------------------------------- analytic.hip

@@ -136,26 +136,11 @@ __global__ __launch_bounds__(BOX1_THREADS) void k_box1(MeshTables mt, uint32_t m
   }
 }
 
-// One mesh over every CU: 1024 vertices per 256-thread block.  Each block
-// writes its box (ordered keys) with device-coherent stores and arrives at a
-// counter; the last block to arrive reduces every block's box into acc and
-// resets the counter.  One same-address atomic per block instead of six.
+// One mesh over every CU: 1024 vertices per 256-thread block, the blocks'
+// boxes reduced by last arrivals (grid_box_reduce).
 constexpr uint32_t BOXP_THREADS = 256, BOXP_ITEMS = 4;
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o));
-  return v;
-}
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
-  return v;
-}
 __global__ __launch_bounds__(BOXP_THREADS) void k_box_parts(MeshTables mt, uint32_t mesh, const float* __restrict__ x,
                                                             uint32_t n, uint32_t* parts, uint32_t* __restrict__ acc) {
-  constexpr uint32_t NW = BOXP_THREADS / 64;
-  __shared__ uint32_t s_red[6][NW];
-  __shared__ uint32_t s_last;
   const uint32_t start = blockIdx.x * (BOXP_THREADS * BOXP_ITEMS);
   const uint32_t end = min(start + BOXP_THREADS * BOXP_ITEMS, n);
   const uint32_t gbase = mt.voff[mesh];
@@ -170,56 +155,17 @@ __global__ __launch_bounds__(BOXP_THREADS) void k_box_parts(MeshTables mt, uint3
       hi = vmax(hi, p);
     }
   }
-  const uint32_t r[6] = {wave_min_u32(fkey(lo.x)), wave_min_u32(fkey(lo.y)), wave_min_u32(fkey(lo.z)),
-                         wave_max_u32(fkey(hi.x)), wave_max_u32(fkey(hi.y)), wave_max_u32(fkey(hi.z))};
-  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-  if (lane == 0)
-    for (int k = 0; k < 6; ++k) s_red[k][w] = r[k];
-  __syncthreads();
-  uint32_t* counter = parts;  // word 0; block b's box at 1 + 6 b
-  if (threadIdx.x < 64) {  // wave 0 publishes the block's box, then arrives
-    if (threadIdx.x < 6) {
-      const uint32_t k = threadIdx.x;
-      uint32_t v = s_red[k][0];
-      for (uint32_t ww = 1; ww < NW; ++ww) v = k < 3 ? min(v, s_red[k][ww]) : max(v, s_red[k][ww]);
-      st_agent_u32(parts + 1 + 6 * (size_t)blockIdx.x + k, v);
-    }
-    wait_all_mem();
-    if (threadIdx.x == 0) {
-      const uint32_t old = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      s_last = (old == gridDim.x - 1) ? 1u : 0u;
-    }
-  }
-  __syncthreads();
-  if (!s_last) return;
-  uint32_t v[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
-  for (uint32_t b = threadIdx.x; b < gridDim.x; b += BOXP_THREADS)
-    for (int k = 0; k < 6; ++k) {
-      const uint32_t q = ld_agent_u32(parts + 1 + 6 * (size_t)b + k);
-      v[k] = k < 3 ? min(v[k], q) : max(v[k], q);
-    }
-  for (int k = 0; k < 6; ++k) v[k] = k < 3 ? wave_min_u32(v[k]) : wave_max_u32(v[k]);
-  __syncthreads();
-  if (lane == 0)
-    for (int k = 0; k < 6; ++k) s_red[k][w] = v[k];
-  __syncthreads();
-  if (threadIdx.x < 6) {
-    const uint32_t k = threadIdx.x;
-    uint32_t u = s_red[k][0];
-    for (uint32_t ww = 1; ww < NW; ++ww) u = k < 3 ? min(u, s_red[k][ww]) : max(u, s_red[k][ww]);
-    acc[k] = u;
-  }
-  if (threadIdx.x == 0) *counter = 0u;
+  grid_box_reduce<BOXP_THREADS>(lo, hi, parts, acc);
 }
 
 uint32_t box_parts_words(uint32_t nverts) {
-  return 6 * ceil_div(nverts, BOXP_THREADS * BOXP_ITEMS) + 1;
+  return grid_box_parts_words(ceil_div(nverts, BOXP_THREADS * BOXP_ITEMS));
 }
 
 int launch_box(hipStream_t s, SegTable vsegs, MeshTables mt, const float* x, uint32_t nitems,
                uint32_t* box_acc, uint32_t single_mesh, uint32_t* parts) {
   if (!nitems) return YSIM_OK;
-  if (single_mesh != 0xffffffffu && parts) {
+  if (single_mesh != 0xffffffffu && parts && grid_box_fits(ceil_div(nitems, BOXP_THREADS * BOXP_ITEMS))) {
     hipLaunchKernelGGL(k_box_parts, dim3(ceil_div(nitems, BOXP_THREADS * BOXP_ITEMS)), dim3(BOXP_THREADS), 0, s, mt,
                        single_mesh, x, nitems, parts, box_acc);
     YSIM_CHECK_LAUNCH();
@@ -315,14 +261,25 @@ constexpr uint32_t INC_THREADS = 1024;
 
 template <typename K>
 __global__ __launch_bounds__(INC_THREADS) void k_morton_inc(SegTable ts, MeshTables mt, const float* __restrict__ x,
-                                                    const uint32_t* __restrict__ facets,
-                                                    const uint32_t* __restrict__ acc, uint32_t code_bits,
-                                                    const uint32_t* __restrict__ perm, K* __restrict__ keys,
-                                                    uint32_t* __restrict__ vals, uint32_t n,
-                                                    uint32_t* __restrict__ need_full) {
+                                                    const uint32_t* __restrict__ facets, uint32_t* acc,
+                                                    uint32_t code_bits, const uint32_t* __restrict__ perm,
+                                                    K* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t n,
+                                                    uint32_t* __restrict__ need_full,
+                                                    const uint32_t* __restrict__ box_parts, uint32_t nparts) {
   __shared__ K sk[INC_WINDOW];
   __shared__ uint32_t sv[INC_WINDOW];
+  __shared__ uint32_t s_part[INC_THREADS];
+  __shared__ uint32_t s_box[6];
   if (blockIdx.x == 0 && threadIdx.x == 0) *need_full = 0u;
+  if (nparts) {  // one segment: the box from the cloth kernel's block boxes
+    const uint32_t v = reduce_box_records<INC_THREADS, false>(box_parts, nparts, s_part);
+    if (threadIdx.x < 6) {
+      s_box[threadIdx.x] = v;
+      if (blockIdx.x == 0) acc[threadIdx.x] = v;
+    }
+    __syncthreads();
+  }
+  const uint32_t* box = nparts ? s_box : acc;
   const bool wide = code_bits > 32;
   const uint32_t start = blockIdx.x * INC_WINDOW;
   const uint32_t cnt = min(INC_WINDOW, n - start);
@@ -339,7 +296,7 @@ __global__ __launch_bounds__(INC_THREADS) void k_morton_inc(SegTable ts, MeshTab
     const uint32_t* f = facets + 3 * (size_t)(mt.foff[m] + fid);
     const uint32_t vb = mt.voff[m];
     const f3 x0 = ld3(x, vb + f[0]), x1 = ld3(x, vb + f[1]), x2 = ld3(x, vb + f[2]);
-    const uint32_t* a = acc + sg * 6;
+    const uint32_t* a = box + sg * 6;
     const f3 bmin = {funkey(a[0]), funkey(a[1]), funkey(a[2])};
     const f3 bmax = {funkey(a[3]), funkey(a[4]), funkey(a[5])};
     const uint64_t code = morton_of(x0, x1, x2, bmin, bmax, wide);
@@ -355,16 +312,18 @@ __global__ __launch_bounds__(INC_THREADS) void k_morton_inc(SegTable ts, MeshTab
 }
 
 int launch_morton_inc(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x, const uint32_t* facets,
-                      const uint32_t* box_acc, uint32_t code_bits, bool key64, const uint32_t* perm, void* keys,
-                      uint32_t* vals, uint32_t nitems, uint32_t* need_full) {
+                      uint32_t* box_acc, uint32_t code_bits, bool key64, const uint32_t* perm, void* keys,
+                      uint32_t* vals, uint32_t nitems, uint32_t* need_full, const uint32_t* box_parts,
+                      uint32_t nparts) {
   if (!nitems) return YSIM_OK;
+  if (nparts && (!box_parts || tsegs.nseg != 1)) return YSIM_ERR_INVALID;
   const uint32_t blocks = ceil_div(nitems, INC_WINDOW);
   if (key64)
     hipLaunchKernelGGL(k_morton_inc<uint64_t>, dim3(blocks), dim3(INC_THREADS), 0, s, tsegs, mt, x, facets, box_acc,
-                       code_bits, perm, (uint64_t*)keys, vals, nitems, need_full);
+                       code_bits, perm, (uint64_t*)keys, vals, nitems, need_full, box_parts, nparts);
   else
     hipLaunchKernelGGL(k_morton_inc<uint32_t>, dim3(blocks), dim3(INC_THREADS), 0, s, tsegs, mt, x, facets, box_acc,
-                       code_bits, perm, (uint32_t*)keys, vals, nitems, need_full);
+                       code_bits, perm, (uint32_t*)keys, vals, nitems, need_full, box_parts, nparts);
   YSIM_CHECK_LAUNCH();
   return YSIM_OK;
 }
@@ -424,9 +383,14 @@ __global__ __launch_bounds__(256) void k_hierarchy(SegTable ts, MeshTables mt, c
   KarrasSeg<K> K_{keys + s, vals + s, nn, mask, wide};
   // the sorted facet order seeds the next incremental build
   if (perm_out) perm_out[j] = K_.idx[i];
-  // leaf slot nn-1+i: child_a = -1, child_b = facet id
-  T[nn - 1 + i].child_a = -1;
-  T[nn - 1 + i].child_b = (int32_t)K_.idx[i];
+  // leaf slot nn-1+i: child_a = -1, child_b = facet id (the LDS refit writes
+  // the node when rl.fid is set)
+  if (rl.fid) {
+    rl.fid[base + nn - 1 + i] = K_.idx[i];
+  } else {
+    T[nn - 1 + i].child_a = -1;
+    T[nn - 1 + i].child_b = (int32_t)K_.idx[i];
+  }
   if (i == 0) P[0] = -1;
   if (i >= nn - 1) return;
   visit[base + i] = 0;
@@ -454,8 +418,10 @@ __global__ __launch_bounds__(256) void k_hierarchy(SegTable ts, MeshTables mt, c
   const uint32_t split = sp;
   int ca = ((int)split == lo) ? (int)split + nn - 1 : (int)split;
   int cb = ((int)split + 1 == hi) ? (int)split + nn : (int)split + 1;
-  T[i].child_a = ca;
-  T[i].child_b = cb;
+  if (!rl.fid) {
+    T[i].child_a = ca;
+    T[i].child_b = cb;
+  }
   P[ca] = i;
   P[cb] = i;
   if (rl.up) {
@@ -832,12 +798,16 @@ __global__ __launch_bounds__(256) void k_refit(SegTable ts, MeshTables mt, const
 // spanning blocks use the device-wide second-arrival protocol above, with the
 // next level's links loaded while the arrival resolves.  Unions keep
 // (child_a, child_b) operand order; the boxes are identical to k_refit's.
+// Every node is written whole, box and child links: a leaf's links are
+// (-1, facet id from fids), an internal node's are its two children, known to
+// the thread that combines it (its own node and the sibling from the ranges).
 template <bool FORMAL, int RB>
 __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, const float* __restrict__ x,
                                                   const float* __restrict__ v, float dt, int swept,
                                                   const uint32_t* __restrict__ facets, ysim_bvh_node* tree,
                                                   const int32_t* __restrict__ parent,
-                                                  const uint32_t* __restrict__ up_links, uint32_t* visit,
+                                                  const uint32_t* __restrict__ up_links,
+                                                  const uint32_t* __restrict__ fids, uint32_t* visit,
                                                   uint32_t n, uint32_t* err) {
   __shared__ uint32_t s_cnt[RB];
   __shared__ float s_box[RB][2][6];
@@ -862,7 +832,8 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
   if (!valid) return;
 
   int child = nn - 1 + i;
-  const uint32_t fid = (uint32_t)T[child].child_b;
+  const uint32_t fid = fids[base + child];
+  int la = -1, lb = (int)fid;  // child links of the node `child`
   int pnext = P[child];
   uint32_t upnext = UP[child];
   const uint32_t* f = facets + 3 * (size_t)(mt.foff[m] + fid);
@@ -880,23 +851,26 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
       hi = vmax(hi, q);
     }
   }
-  // A box read by another workgroup during this kernel (its parent is combined
-  // through the device-wide protocol) is written with device-coherent stores
-  // in the fast variant; every other box with plain stores, which L2 combines
-  // into whole lines (the traversal reads them after the kernel ends).
-  auto store_box = [&](int node, f3 a, f3 b, bool coherent) {
+  // A node whose box another workgroup reads during this kernel (its parent is
+  // combined through the device-wide protocol) is written with device-coherent
+  // stores in the fast variant; every other node with two plain 16-byte
+  // stores (the traversal reads them after the kernel ends).
+  auto store_node = [&](int node, f3 a, f3 b, int ca, int cb, bool coherent) {
     float* q = reinterpret_cast<float*>(T + node);
     if (FORMAL || !coherent) {
-      q[0] = a.x; q[1] = a.y; q[2] = a.z; q[4] = b.x; q[5] = b.y; q[6] = b.z;
+      float4* q4 = reinterpret_cast<float4*>(q);
+      q4[0] = make_float4(a.x, a.y, a.z, __int_as_float(ca));
+      q4[1] = make_float4(b.x, b.y, b.z, __int_as_float(cb));
     } else {
-      st_agent_u64(reinterpret_cast<uint64_t*>(q + 0), pack2(a.x, a.y));
-      st_agent(q + 2, a.z);
-      st_agent_u64(reinterpret_cast<uint64_t*>(q + 4), pack2(b.x, b.y));
-      st_agent(q + 6, b.z);
+      uint64_t* w = reinterpret_cast<uint64_t*>(q);
+      st_agent_u64(w + 0, pack2(a.x, a.y));
+      st_agent_u64(w + 1, pack2(a.z, __int_as_float(ca)));
+      st_agent_u64(w + 2, pack2(b.x, b.y));
+      st_agent_u64(w + 3, pack2(b.z, __int_as_float(cb)));
     }
   };
   if (nn <= 1) {
-    store_box(child, lo, hi, false);
+    store_node(child, lo, hi, la, lb, false);
     return;
   }
   uint32_t clo = (uint32_t)i, chi = (uint32_t)i;  // leaf range of the current node
@@ -911,7 +885,15 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
     const uint32_t psize = up >> 1;
     const uint32_t plo = side ? chi + 1 - psize : clo, phi = side ? chi : clo + psize - 1;
     const bool local = s + plo >= b0 && s + phi < b0 + RB;
-    store_box(child, lo, hi, !local);
+    store_node(child, lo, hi, la, lb, !local);
+    // sibling: child_b (split + 1) for a child_a, child_a (split) for a child_b
+    const uint32_t sp = side ? clo - 1 : chi;
+    const int sib = side ? (sp == plo ? nn - 1 + (int)sp : (int)sp)
+                         : (sp + 1 == phi ? nn + (int)sp : (int)sp + 1);
+    if (sib < 0 || sib >= 2 * nn - 1) {
+      atomicOr(err, 4u);
+      return;
+    }
     if (local) {
       const uint32_t slot = s + (uint32_t)p - b0;
       if (slot >= (uint32_t)RB) {  // p is an endpoint of its range in a valid tree
@@ -946,14 +928,6 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
         old = __hip_atomic_fetch_add(C + p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
       if ((old & 1u) == 0u) return;
-      // sibling: child_b (split + 1) for a child_a, child_a (split) for a child_b
-      const uint32_t sp = side ? clo - 1 : chi;
-      const int sib = side ? (sp == plo ? nn - 1 + (int)sp : (int)sp)
-                           : (sp + 1 == phi ? nn + (int)sp : (int)sp + 1);
-      if (sib < 0 || sib >= 2 * nn - 1) {
-        atomicOr(err, 4u);
-        return;
-      }
       const float* q = reinterpret_cast<const float*>(T + sib);
       f3 smin, smax;
       if (FORMAL) {
@@ -973,11 +947,13 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
         hi = vmax(smax, hi);
       }
     }
+    la = side ? sib : child;
+    lb = side ? child : sib;
     child = p;
     clo = plo;
     chi = phi;
     if (p == 0) {
-      store_box(0, lo, hi, false);
+      store_node(0, lo, hi, la, lb, false);
       return;
     }
   }
@@ -992,10 +968,10 @@ int launch_refit(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x, c
                  uint32_t block) {
   if (!nitems) return YSIM_OK;
   if (rl && rl->up) {
-    if (!refit_block_ok(block)) return YSIM_ERR_INVALID;
+    if (!refit_block_ok(block) || !rl->fid) return YSIM_ERR_INVALID;
 #define YSIM_R(F, B)                                                                                     \
   hipLaunchKernelGGL((k_refit_lds<F, B>), dim3(ceil_div(nitems, B)), dim3(B), 0, s, tsegs, mt, x, v, dt, \
-                     (int)swept, facets, tree, parent, rl->up, visit, nitems, err)
+                     (int)swept, facets, tree, parent, rl->up, rl->fid, visit, nitems, err)
     if (formal) {
       if (block == 256) YSIM_R(true, 256); else if (block == 512) YSIM_R(true, 512); else YSIM_R(true, 1024);
     } else {

@@ -6,6 +6,7 @@ the reference operation it stands in for.  All compute runs in libysim_hip.so.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from typing import Optional
 
 import numpy as np
@@ -24,9 +25,13 @@ class HipContext:
         h = vp()
         call("ysim_hip_ctx_create", device, 0, C.byref(h))
         self.h = h
+        # scenes and arrays use the context's stream: they are released first
+        self._owned = weakref.WeakSet()
 
     def close(self):
         if getattr(self, "h", None):
+            for o in list(getattr(self, "_owned", ())):
+                o.close()
             load().ysim_hip_ctx_destroy(self.h)
             self.h = None
 
@@ -72,6 +77,8 @@ class DeviceArray:
         p = vp()
         call("ysim_hip_alloc", ctx.h, max(self.nbytes, 1), C.byref(p))
         self.ptr = p
+        if getattr(ctx, "_owned", None) is not None:
+            ctx._owned.add(self)
 
     def get(self) -> np.ndarray:
         out = np.empty(self.shape, self.dtype)
@@ -84,11 +91,14 @@ class DeviceArray:
         assert a.nbytes <= self.nbytes
         call("ysim_hip_upload", self.ctx.h, self.ptr, a.ctypes.data, a.nbytes)
 
+    def close(self):
+        if getattr(self, "ptr", None) and self.ctx.h:
+            load().ysim_hip_free(self.ctx.h, self.ptr)
+        self.ptr = None
+
     def __del__(self):
         try:
-            if getattr(self, "ptr", None) and self.ctx.h:
-                load().ysim_hip_free(self.ctx.h, self.ptr)
-                self.ptr = None
+            self.close()
         except Exception:
             pass
 
@@ -217,6 +227,8 @@ class HipScene:
              pk["v"].ctypes.data, pk["mass"].ctypes.data, pk["ext"].ctypes.data, pk["fixed"].ctypes.data,
              pk["facets"].ctypes.data, C.byref(self.params), C.byref(h))
         self.h = h
+        if getattr(ctx, "_owned", None) is not None:
+            ctx._owned.add(self)
         self.V = scene.num_vertices
         if kspring is not None:
             raise NotImplementedError("per-spring stiffness is served by the adjoint API")

@@ -305,16 +305,15 @@ __attribute__((amdgpu_waves_per_eu(4, 4))) __global__ __launch_bounds__(64) void
               ++nc;
             }
           }
-          uint32_t incl = nc;  // inclusive scan of the candidate counts
-#pragma unroll
-          for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t u = __shfl_up(incl, o, 64);
-            if ((int)lane >= o) incl += u;
-          }
-          const uint32_t tot = __shfl(incl, 63, 64);
+          // exclusive scan of the candidate counts (nc <= 4) from the ballots
+          // of their three bits
+          const uint64_t c1 = __ballot(nc & 1u), c2 = __ballot(nc & 2u), c4 = __ballot(nc & 4u);
+          const uint64_t below = lanes_below();
+          const uint32_t excl = popc64(c1 & below) + 2 * popc64(c2 & below) + 4 * popc64(c4 & below);
+          const uint32_t tot = popc64(c1) + 2 * popc64(c2) + 4 * popc64(c4);
           if (tot) {
             if (np + tot > TQ_PAIRS) flush();
-            const uint32_t o = np + incl - nc;
+            const uint32_t o = np + excl;
             for (uint32_t q = 0; q < nc; ++q) s_pair[o + q] = make_uint2((cand >> (6 * q)) & 63u, fid);
             np += tot;
             npair += nc;

@@ -669,7 +669,7 @@ struct ysim_scene {
       // check failed (its kernels return at once otherwise)
       YSIM_TRY(launch_morton_inc(st, g.tsegs, mt, xc, facets, box_acc, g.code_bits, g.key64, g.perm, keys[0],
                                  vals[0], g.T, need_full, parts_box ? cloth_parts : nullptr,
-                                 parts_box ? cloth_box_parts(Vdyn, tune.cloth_block) : 0u));
+                                 parts_box ? cloth_box_parts(Vdyn, tune.cloth_block) : 0u, links.cut_cnt));
       if (timing) YSIM_HIP_TRY(hipEventRecord(ev[2], st));
       YSIM_TRY(launch_incremental_fix(st, g.key64, keys[0], vals[0], g.T, need_full, inc_bcnt, inc_edges));
       const bool defer = capturing && !agglo;
@@ -692,7 +692,7 @@ struct ysim_scene {
       return YSIM_OK;
     }
     YSIM_TRY(launch_morton(st, g.tsegs, mt, xc, facets, box_acc, g.code_bits, g.key64, keys[0], vals[0], g.T,
-                           g.plan, hist, counters, epoch));
+                           g.plan, hist, counters, epoch, nullptr, links.cut_cnt));
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[2], st));
     if (g.plan.hybrid)
       YSIM_TRY(launch_sort_hybrid(st, g.plan, keys[0], vals[0], keys[1], vals[1], hist, status, counters, epoch,
@@ -733,6 +733,8 @@ struct ysim_scene {
     a.mesh_behavior = d_behavior;
     a.mesh_shape = d_shape;
     a.mesh_grid_n = d_grid_n;
+    a.cut_cnt = links.cut_cnt;
+    a.cut = links.cut;
     a.x = x[par];
     a.xprev = x[par ^ 1];
     a.facets = facets;
@@ -1602,6 +1604,9 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
   YSIM_TRY(s.alloc(&s.visit, s.Ntot));
   YSIM_TRY(s.alloc(&s.links.up, s.Ntot));
   YSIM_TRY(s.alloc(&s.links.fid, s.Ntot));
+  YSIM_TRY(s.alloc(&s.links.cut_cnt, num_meshes));
+  YSIM_TRY(s.zero(s.links.cut_cnt, num_meshes));
+  YSIM_TRY(s.alloc(&s.links.cut, (size_t)num_meshes * CUT_CAP));
   YSIM_TRY(s.zero(s.tree, 8 * (size_t)s.Ntot));
   YSIM_TRY(s.zero(s.visit, s.Ntot));
   // segment tables

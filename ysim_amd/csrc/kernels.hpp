@@ -48,20 +48,23 @@ int launch_box(hipStream_t s, SegTable vsegs, MeshTables mt, const float* x, uin
 // of all sort passes of `plan`; zeroes the sort's block counters and advances
 // its look-back epoch (either may be null when no sort follows).
 // run_flag (optional): the kernel does nothing while *run_flag == 0.
+// cut_reset (optional, per mesh): zeroed for the group's meshes.
 int launch_morton(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x,
                   const uint32_t* facets, const uint32_t* box_acc, uint32_t code_bits,
                   bool key64, void* keys, uint32_t* vals, uint32_t nitems, const SortPlan& plan,
-                  uint32_t* hist, uint32_t* counters, uint32_t* epoch, const uint32_t* run_flag = nullptr);
+                  uint32_t* hist, uint32_t* counters, uint32_t* epoch, const uint32_t* run_flag = nullptr,
+                  uint32_t* cut_reset = nullptr);
 // Incremental S1: keys in the order perm (previous sorted facets), each
 // 2048-key window sorted by (key, facet); zeroes *need_full.
 // With nparts > 0 (one segment only) the box is the reduction of nparts
 // 6-word records at box_parts (the cloth kernel's per-block boxes); every
 // block reduces them and block 0 writes the box to box_acc for the build's
 // later kernels.  Otherwise box_acc holds the box per segment.
+// cut_reset (per mesh): zeroed for the group's meshes (their build starts).
 int launch_morton_inc(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x, const uint32_t* facets,
                       uint32_t* box_acc, uint32_t code_bits, bool key64, const uint32_t* perm, void* keys,
                       uint32_t* vals, uint32_t nitems, uint32_t* need_full, const uint32_t* box_parts = nullptr,
-                      uint32_t nparts = 0);
+                      uint32_t nparts = 0, uint32_t* cut_reset = nullptr);
 // Extra links of a scene build for the LDS refit, written by the hierarchy
 // kernel (null: not written, and the refit takes the device-wide path only).
 // With fid set, the hierarchy writes no tree nodes: the LDS refit that follows
@@ -70,7 +73,15 @@ int launch_morton_inc(hipStream_t s, SegTable tsegs, MeshTables mt, const float*
 struct RefitLinks {
   uint32_t* up = nullptr;   // [node window + node] parent's leaf count << 1 | (node is child_b)
   uint32_t* fid = nullptr;  // [node window + leaf slot] the leaf's facet id
+  // Tree cut per mesh: the nodes of at most S = ceil(leaves / 128) leaves whose
+  // parent has more (a frontier that covers every leaf once), appended in any
+  // order to cut + CUT_CAP mesh, counted in cut_cnt[mesh] (past CUT_CAP: not
+  // listed, and readers walk from the root).  The build's Morton kernel zeroes
+  // the group's counts, so a build that writes no cut leaves them at zero.
+  uint32_t* cut_cnt = nullptr;
+  int32_t* cut = nullptr;
 };
+constexpr uint32_t CUT_CAP = 512;
 // S3: Karras hierarchy per segment; child and parent links (and rl.up when
 // set); zeroes the refit visit counters; resets box_acc (nbox segments) and the
 // sort histogram for the next build.
@@ -137,6 +148,9 @@ struct QueryArgs {
   uint32_t block = 128;               // threads per block: 64, 128 or 256
   uint32_t waves = 5;                 // occupancy target (waves per SIMD): 5, 6 or 8
   uint32_t tile = 1;                  // 1: one wave per 64-vertex tile (k_query_tile); 0: one thread per vertex
+  // optional tree cuts (RefitLinks::cut): the tile walk starts from them
+  const uint32_t* cut_cnt = nullptr;
+  const int32_t* cut = nullptr;
 };
 bool query_block_ok(uint32_t b);
 bool query_waves_ok(uint32_t w);

@@ -191,11 +191,14 @@ __global__ __launch_bounds__(256) void k_morton(SegTable ts, MeshTables mt, cons
                                                 uint32_t n, uint32_t passes, uint32_t hist_shift,
                                                 uint32_t* __restrict__ hist,
                                                 uint32_t* __restrict__ counters, uint32_t* __restrict__ epoch,
-                                                const uint32_t* __restrict__ run_flag) {
+                                                const uint32_t* __restrict__ run_flag,
+                                                uint32_t* __restrict__ cut_reset) {
   constexpr uint32_t DIGITS = 1u << DB;
   constexpr uint32_t HW = NH * DIGITS;
   __shared__ uint32_t s_h[HW];
   if (run_flag && *run_flag == 0u) return;  // the incremental sort succeeded: nothing to do
+  if (cut_reset && blockIdx.x == 0)
+    for (uint32_t t = threadIdx.x; t < ts.nseg; t += 256) cut_reset[ts.mesh[t]] = 0u;
   for (uint32_t i = threadIdx.x; i < passes * DIGITS; i += 256) s_h[i] = 0;
   // the next sort's block counters start at zero and its look-back words carry
   // a fresh epoch (the sort passes run after this kernel completes)
@@ -231,13 +234,13 @@ __global__ __launch_bounds__(256) void k_morton(SegTable ts, MeshTables mt, cons
 int launch_morton(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x,
                   const uint32_t* facets, const uint32_t* box_acc, uint32_t code_bits, bool key64,
                   void* keys, uint32_t* vals, uint32_t nitems, const SortPlan& plan, uint32_t* hist,
-                  uint32_t* counters, uint32_t* epoch, const uint32_t* run_flag) {
+                  uint32_t* counters, uint32_t* epoch, const uint32_t* run_flag, uint32_t* cut_reset) {
   if (!nitems) return YSIM_OK;
   uint32_t blocks = ceil_div(nitems, 256 * MORTON_ITEMS);
 #define YSIM_MORTON(KT, DBV, NHV)                                                                             \
   hipLaunchKernelGGL((k_morton<KT, DBV, NHV>), dim3(blocks), dim3(256), 0, s, tsegs, mt, x, facets, box_acc, \
                      code_bits, (KT*)keys, vals, nitems, plan.passes, plan.hist_shift, hist, counters, epoch, \
-                     run_flag)
+                     run_flag, cut_reset)
   if (plan.hybrid) {
     if (key64) YSIM_MORTON(uint64_t, 11, 1); else YSIM_MORTON(uint32_t, 11, 1);
   } else if (plan.digit_bits == 11) {
@@ -265,12 +268,15 @@ __global__ __launch_bounds__(INC_THREADS) void k_morton_inc(SegTable ts, MeshTab
                                                     uint32_t code_bits, const uint32_t* __restrict__ perm,
                                                     K* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t n,
                                                     uint32_t* __restrict__ need_full,
-                                                    const uint32_t* __restrict__ box_parts, uint32_t nparts) {
+                                                    const uint32_t* __restrict__ box_parts, uint32_t nparts,
+                                                    uint32_t* __restrict__ cut_reset) {
   __shared__ K sk[INC_WINDOW];
   __shared__ uint32_t sv[INC_WINDOW];
   __shared__ uint32_t s_part[INC_THREADS];
   __shared__ uint32_t s_box[6];
   if (blockIdx.x == 0 && threadIdx.x == 0) *need_full = 0u;
+  if (cut_reset && blockIdx.x == 0)
+    for (uint32_t t = threadIdx.x; t < ts.nseg; t += INC_THREADS) cut_reset[ts.mesh[t]] = 0u;
   if (nparts) {  // one segment: the box from the cloth kernel's block boxes
     const uint32_t v = reduce_box_records<INC_THREADS, false>(box_parts, nparts, s_part);
     if (threadIdx.x < 6) {
@@ -314,16 +320,16 @@ __global__ __launch_bounds__(INC_THREADS) void k_morton_inc(SegTable ts, MeshTab
 int launch_morton_inc(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x, const uint32_t* facets,
                       uint32_t* box_acc, uint32_t code_bits, bool key64, const uint32_t* perm, void* keys,
                       uint32_t* vals, uint32_t nitems, uint32_t* need_full, const uint32_t* box_parts,
-                      uint32_t nparts) {
+                      uint32_t nparts, uint32_t* cut_reset) {
   if (!nitems) return YSIM_OK;
   if (nparts && (!box_parts || tsegs.nseg != 1)) return YSIM_ERR_INVALID;
   const uint32_t blocks = ceil_div(nitems, INC_WINDOW);
   if (key64)
     hipLaunchKernelGGL(k_morton_inc<uint64_t>, dim3(blocks), dim3(INC_THREADS), 0, s, tsegs, mt, x, facets, box_acc,
-                       code_bits, perm, (uint64_t*)keys, vals, nitems, need_full, box_parts, nparts);
+                       code_bits, perm, (uint64_t*)keys, vals, nitems, need_full, box_parts, nparts, cut_reset);
   else
     hipLaunchKernelGGL(k_morton_inc<uint32_t>, dim3(blocks), dim3(INC_THREADS), 0, s, tsegs, mt, x, facets, box_acc,
-                       code_bits, perm, (uint32_t*)keys, vals, nitems, need_full, box_parts, nparts);
+                       code_bits, perm, (uint32_t*)keys, vals, nitems, need_full, box_parts, nparts, cut_reset);
   YSIM_CHECK_LAUNCH();
   return YSIM_OK;
 }
@@ -428,6 +434,21 @@ __global__ __launch_bounds__(256) void k_hierarchy(SegTable ts, MeshTables mt, c
     const uint32_t size2 = (uint32_t)(hi - lo + 1) << 1;
     rl.up[base + ca] = size2;       // left child
     rl.up[base + cb] = size2 | 1u;  // right child
+  }
+  if (rl.cut) {  // children of at most S leaves under a node of more
+    const uint32_t S = max(1u, ((uint32_t)nn + 127u) / 128u);
+    if ((uint32_t)(hi - lo + 1) > S) {
+      const uint32_t sa = split - (uint32_t)lo + 1, sb = (uint32_t)hi - split;
+      int32_t* CL = rl.cut + (size_t)m * CUT_CAP;
+      if (sa <= S) {
+        const uint32_t k = atomicAdd(rl.cut_cnt + m, 1u);
+        if (k < CUT_CAP) CL[k] = ca;
+      }
+      if (sb <= S) {
+        const uint32_t k = atomicAdd(rl.cut_cnt + m, 1u);
+        if (k < CUT_CAP) CL[k] = cb;
+      }
+    }
   }
 }
 

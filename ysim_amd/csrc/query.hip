@@ -323,10 +323,44 @@ __attribute__((amdgpu_waves_per_eu(4, 4))) __global__ __launch_bounds__(64) void
       }
     };
     // ---- phase 1: breadth-first walk with the union box, up to 64 pending
-    // nodes per round
+    // nodes per round.  It starts from the tree's cut when the build listed
+    // one (a frontier covering every leaf once, so the leaves found are the
+    // same as from the root), in one round of independent loads instead of
+    // the top levels' dependent rounds; else from the root.
     uint32_t sp = 0, nl = 0;
     bool overflow = false;
-    {
+    const uint32_t ncut = a.cut_cnt ? a.cut_cnt[t] : 0u;
+    if (ncut > 0 && ncut <= CUT_CAP) {
+      const int32_t* CL = a.cut + (size_t)t * CUT_CAP;
+      for (uint32_t c0 = 0; c0 < ncut; c0 += 64) {
+        if (nl + 64 > leaf_cap) {
+          leaves(nl);
+          nl = 0;
+        }
+        const bool have = c0 + lane < ncut;
+        bool pc = false, lc = false;
+        int node = 0;
+        NodeV A;
+        if (have) {
+          node = CL[c0 + lane];
+          A = load_node(T, node);
+          const bool h = overlap(Ulo, Uhi, A.lo, A.hi);
+          pc = h && A.a >= 0;
+          lc = h && A.a < 0;
+        }
+        const uint64_t bp = __ballot(pc), bl = __ballot(lc);
+        if (sp + popc64(bp) > stack_cap) {
+          overflow = true;
+          break;
+        }
+        if (pc) s_stk[sp + popc64(bp & lanes_below())] = make_int2(A.a, A.b);
+        if (lc) s_leaf[nl + popc64(bl & lanes_below())] = make_uint2((uint32_t)node, (uint32_t)A.b);
+        sp += popc64(bp);
+        nl += popc64(bl);
+        wave_sync();
+      }
+      if (overflow) sp = 0;
+    } else {
       const NodeV r = load_node(T, 0);
       if (overlap(Ulo, Uhi, r.lo, r.hi)) {
         if (r.a < 0) {

@@ -285,40 +285,48 @@ __attribute__((amdgpu_waves_per_eu(4, 4))) __global__ __launch_bounds__(64) void
             if (overlap(sl, sh, L.lo, L.hi)) mask |= 1u << s;
           }
         }
-        while (__ballot(mask != 0)) {
-          uint32_t nc = 0, cand = 0;  // up to 4 candidate lanes, 6 bits each
-          if (mask) {
-            const uint32_t s = __builtin_ctz(mask);
-            mask &= mask - 1;
-            const uint32_t b0 = ((s >> 2) * 2) * 8 + (s & 3u) * 2;
+        // the vertices of the overlapping sub-boxes whose own box overlaps the
+        // leaf (each vertex lies in one sub-box), as a mask of tile lanes
+        uint64_t vm = 0;
+        while (mask) {
+          const uint32_t s = __builtin_ctz(mask);
+          mask &= mask - 1;
+          const uint32_t b0 = ((s >> 2) * 2) * 8 + (s & 3u) * 2;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              const uint32_t v = b0 + (q & 1) + (q >> 1) * 8;
-              if (base + v >= a.nitems) continue;
-              const f3 vc = {s_q[v * 6 + 0], s_q[v * 6 + 1], s_q[v * 6 + 2]};
-              const f3 vl = {vc.x - margin, vc.y - margin, vc.z - margin};
-              const f3 vh = {vc.x + margin, vc.y + margin, vc.z + margin};
-              if (!overlap(vl, vh, L.lo, L.hi)) continue;
-              const uint32_t pv = s_p[v];
-              if (self && (pv == f0 || pv == f1 || pv == f2)) continue;
-              cand |= v << (6 * nc);
+          for (int q = 0; q < 4; ++q) {
+            const uint32_t v = b0 + (q & 1) + (q >> 1) * 8;
+            if (base + v >= a.nitems) continue;
+            const f3 vc = {s_q[v * 6 + 0], s_q[v * 6 + 1], s_q[v * 6 + 2]};
+            const f3 vl = {vc.x - margin, vc.y - margin, vc.z - margin};
+            const f3 vh = {vc.x + margin, vc.y + margin, vc.z + margin};
+            if (!overlap(vl, vh, L.lo, L.hi)) continue;
+            const uint32_t pv = s_p[v];
+            if (self && (pv == f0 || pv == f1 || pv == f2)) continue;
+            vm |= 1ull << v;
+          }
+        }
+        // emitted in rounds of at most 4 (vertex, facet) pairs per lane
+        while (__ballot(vm != 0)) {
+          uint32_t nc = 0, cand = 0;  // up to 4 candidate lanes, 6 bits each
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (vm) {
+              cand |= (uint32_t)__builtin_ctzll(vm) << (6 * nc);
+              vm &= vm - 1;
               ++nc;
             }
-          }
           // exclusive scan of the candidate counts (nc <= 4) from the ballots
           // of their three bits
           const uint64_t c1 = __ballot(nc & 1u), c2 = __ballot(nc & 2u), c4 = __ballot(nc & 4u);
           const uint64_t below = lanes_below();
           const uint32_t excl = popc64(c1 & below) + 2 * popc64(c2 & below) + 4 * popc64(c4 & below);
           const uint32_t tot = popc64(c1) + 2 * popc64(c2) + 4 * popc64(c4);
-          if (tot) {
-            if (np + tot > TQ_PAIRS) flush();
-            const uint32_t o = np + excl;
-            for (uint32_t q = 0; q < nc; ++q) s_pair[o + q] = make_uint2((cand >> (6 * q)) & 63u, fid);
-            np += tot;
-            npair += nc;
-            wave_sync();
-          }
+          if (np + tot > TQ_PAIRS) flush();
+          const uint32_t o = np + excl;
+          for (uint32_t q = 0; q < nc; ++q) s_pair[o + q] = make_uint2((cand >> (6 * q)) & 63u, fid);
+          np += tot;
+          npair += nc;
+          wave_sync();
         }
       }
     };

@@ -848,30 +848,40 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
   const int32_t* P = parent + base;
   const uint32_t* UP = up_links + base;
   uint32_t* C = visit + base;
-  if (valid && i < nn - 1) s_link[threadIdx.x] = make_int2(P[i], (int)UP[i]);
-  __syncthreads();
-  if (!valid) return;
-
+  // the block's internal links load while the leaf's box is gathered; they
+  // reach LDS after it
+  const bool has_link = valid && i < nn - 1;
+  int2 link = make_int2(0, 0);
+  if (has_link) link = make_int2(P[i], (int)UP[i]);
   int child = nn - 1 + i;
-  const uint32_t fid = fids[base + child];
-  int la = -1, lb = (int)fid;  // child links of the node `child`
-  int pnext = P[child];
-  uint32_t upnext = UP[child];
-  const uint32_t* f = facets + 3 * (size_t)(mt.foff[m] + fid);
-  const uint32_t vb = mt.voff[m];
-  f3 lo = ld3(x, vb + f[0]), hi = lo;
+  uint32_t fid = 0, upnext = 0;
+  int pnext = 0;
+  f3 lo = {0.f, 0.f, 0.f}, hi = lo;
+  if (valid) {
+    fid = fids[base + child];
+    pnext = P[child];
+    upnext = UP[child];
+    const uint32_t* f = facets + 3 * (size_t)(mt.foff[m] + fid);
+    const uint32_t vb = mt.voff[m];
+    lo = ld3(x, vb + f[0]);
+    hi = lo;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    f3 p = ld3(x, vb + f[c]);
-    lo = vmin(lo, p);
-    hi = vmax(hi, p);
-    if (swept) {
-      f3 vv = ld3(v, vb + f[c]);
-      f3 q = p + vv * dt;
-      lo = vmin(lo, q);
-      hi = vmax(hi, q);
+    for (int c = 0; c < 3; ++c) {
+      f3 p = ld3(x, vb + f[c]);
+      lo = vmin(lo, p);
+      hi = vmax(hi, p);
+      if (swept) {
+        f3 vv = ld3(v, vb + f[c]);
+        f3 q = p + vv * dt;
+        lo = vmin(lo, q);
+        hi = vmax(hi, q);
+      }
     }
   }
+  if (has_link) s_link[threadIdx.x] = link;
+  __syncthreads();
+  if (!valid) return;
+  int la = -1, lb = (int)fid;  // child links of the node `child`
   // A node whose box another workgroup reads during this kernel (its parent is
   // combined through the device-wide protocol) is written with device-coherent
   // stores in the fast variant; every other node with two plain 16-byte

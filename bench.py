@@ -357,10 +357,18 @@ def main():
                "query": "k_query_tile" if int(tuning.get("query_tile", 0))
                else f"k_query<false, {int(tuning['query_block'])}, {int(tuning['query_waves'])}>",
                "integrate": f"k_cloth<{int(tuning['cloth_block'])}>"}
+    if int(tuning.get("query_tile", 0)) and not (scene.analytic is not None and len(scene.analytic)):
+        # the tile query runs each tile's cloth step itself: one kernel for
+        # query and integrate
+        kernels["query"] = "k_query_tile<true>"
+        kernels["integrate"] = "(fused into k_query_tile<true>)"
+        sb["query"] += sb["integrate"]
+        sb["integrate"] = 0
+        launches["integrate"] = 0
     if sum(1 for m in scene.meshes if m.behavior != scenes.BEHAVIOR_FLOAT) == 1:
         # one dynamic mesh: the cloth kernel reduces the box of the positions it
         # writes, and the next build launches no box kernel
-        kernels["box"] = "(fused into k_cloth)"
+        kernels["box"] = "(per-block boxes from the cloth step, reduced in k_morton_inc)"
         launches["box"] = 0
         sb["box"] = 0
     if int(tuning.get("build_mode", 0)):

@@ -210,6 +210,7 @@ struct ysim_scene {
   // launches no box kernel.
   bool fuse_box = false, box_fresh = false;
   uint32_t* cloth_parts = nullptr;
+  uint32_t parts_count = 0;  // box records the last cloth step wrote (per block or per tile)
   uint32_t* h_stale = nullptr;    // pinned host word
   uint32_t* d_stale = nullptr;    // its device address
   uint64_t deferred_builds = 0;
@@ -669,7 +670,7 @@ struct ysim_scene {
       // check failed (its kernels return at once otherwise)
       YSIM_TRY(launch_morton_inc(st, g.tsegs, mt, xc, facets, box_acc, g.code_bits, g.key64, g.perm, keys[0],
                                  vals[0], g.T, need_full, parts_box ? cloth_parts : nullptr,
-                                 parts_box ? cloth_box_parts(Vdyn, tune.cloth_block) : 0u, links.cut_cnt));
+                                 parts_box ? parts_count : 0u, links.cut_cnt));
       if (timing) YSIM_HIP_TRY(hipEventRecord(ev[2], st));
       YSIM_TRY(launch_incremental_fix(st, g.key64, keys[0], vals[0], g.T, need_full, inc_bcnt, inc_edges));
       const bool defer = capturing && !agglo;
@@ -783,8 +784,28 @@ struct ysim_scene {
     return YSIM_OK;
   }
 
+  // The tile query can run each tile's cloth step itself (one kernel for
+  // query and integrate) unless analytic contacts join the slots after it.
+  bool fuse_query_cloth() const { return !hash_mode && tune.query_tile && !analytic_on() && Vdyn > 0; }
+  // box records the cloth step writes in the current configuration
+  uint32_t parts_for_config() const {
+    return fuse_query_cloth() ? query_tiles(Vdyn) : cloth_box_parts(Vdyn, tune.cloth_block);
+  }
+
   int enqueue_substep(uint32_t par, bool rebuild) {
     YSIM_TRY(enqueue_collision(par, rebuild));
+    if (fuse_query_cloth()) {
+      ClothArgs ca = cloth_args(par);
+      if (fuse_box) ca.box_parts = cloth_parts;
+      YSIM_TRY(launch_query_cloth(st, query_args(par), ca));
+      parts_count = query_tiles(Vdyn);
+      if (fuse_box) box_fresh = true;
+      if (timing) {
+        YSIM_HIP_TRY(hipEventRecord(ev[6], st));
+        YSIM_HIP_TRY(hipEventRecord(ev[7], st));
+      }
+      return YSIM_OK;
+    }
     YSIM_TRY(enqueue_query(par));
     return enqueue_integrate(par);
   }
@@ -804,6 +825,16 @@ struct ysim_scene {
   }
 
   int enqueue_integrate(uint32_t par) {
+    ClothArgs ca = cloth_args(par);
+    if (fuse_box) ca.box_parts = cloth_parts;
+    YSIM_TRY(launch_cloth(st, ca));
+    parts_count = cloth_box_parts(Vdyn, tune.cloth_block);
+    if (fuse_box) box_fresh = true;
+    if (timing) YSIM_HIP_TRY(hipEventRecord(ev[7], st));
+    return YSIM_OK;
+  }
+
+  ClothArgs cloth_args(uint32_t par) const {
     ClothArgs ca;
     ca.dsegs = dyn_vsegs;
     ca.mt = mt;
@@ -830,11 +861,7 @@ struct ysim_scene {
     ca.nitems = Vdyn;
     ca.flags = flags;
     ca.block = tune.cloth_block;
-    if (fuse_box) ca.box_parts = cloth_parts;
-    YSIM_TRY(launch_cloth(st, ca));
-    if (fuse_box) box_fresh = true;
-    if (timing) YSIM_HIP_TRY(hipEventRecord(ev[7], st));
-    return YSIM_OK;
+    return ca;
   }
 
   bool rebuild_at(uint64_t s) const { return P.rebuild_every > 0 && (s % P.rebuild_every) == 0; }
@@ -867,12 +894,15 @@ struct ysim_scene {
     // a replay always follows a substep (box_fresh), so its first build
     // skips the box kernel too; the host flag is left as it was
     const bool fresh = box_fresh;
+    const uint32_t parts = parts_count;
     box_fresh = true;
+    parts_count = parts_for_config();
     capturing = true;
     for (uint32_t i = 0; i < period && rc == YSIM_OK; ++i)
       rc = enqueue_substep(i % 2, rebuild_at(i));
     capturing = false;
     box_fresh = fresh;
+    parts_count = parts;
     hipError_t e = hipStreamEndCapture(st, &g);
     if (rc != YSIM_OK) return rc;
     if (e != hipSuccess) return fail(YSIM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -898,7 +928,8 @@ struct ysim_scene {
         ++deferred_builds;
         full_next = true;
       }
-      if (full_next || (fuse_box && !box_fresh) || (step % graph_period) != 0 || n < graph_period) {
+      const bool box_ok = !fuse_box || (box_fresh && parts_count == parts_for_config());
+      if (full_next || !box_ok || (step % graph_period) != 0 || n < graph_period) {
         YSIM_TRY(eager_steps(1));
         full_next = false;
         --n;
@@ -1047,6 +1078,7 @@ int ysim_scene::set_analytic(const ysim_analytic_shape* h, uint32_t n) {
     tune_amark.clear();
     YSIM_TRY(build_targets());
     drop_graph();
+    box_fresh = false;  // the cloth step may change kernels (and box records)
   }
   if (n > ashape_cap) {
     YSIM_TRY(release(d_ashapes));

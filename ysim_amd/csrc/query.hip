@@ -129,6 +129,27 @@ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __global__ __launch_bounds__(QB) 
   if (!EMIT_PAIRS) wave_stats(a, (xcd_block(blockIdx.x, gridDim.x) * QB + threadIdx.x) >> 6, npair);
 }
 
+// After a tile's query (CLOTH variant): the cloth step of its vertices, and
+// the box of their new positions as 6 ordered keys at box_parts + 6 tile.
+// Contact slots were written by any lane of the wave: their stores complete
+// before the wave barrier (the wave's loads of its own slot lines, never read
+// before in this kernel, then come from L2); the count comes in a register.
+__device__ __forceinline__ void tile_cloth(const ClothArgs& ca, bool active, uint32_t j, uint32_t count,
+                                           uint32_t tile) {
+  wait_all_mem();
+  wave_sync();
+  const float INF = __uint_as_float(0x7f800000u);
+  f3 lo = {INF, INF, INF}, hi = {-INF, -INF, -INF};
+  if (active) lo = hi = cloth_vertex(ca, j, (int32_t)count);
+  if (!ca.box_parts) return;
+  const uint32_t r0 = wave_min_u32(fkey(lo.x)), r1 = wave_min_u32(fkey(lo.y)), r2 = wave_min_u32(fkey(lo.z));
+  const uint32_t r3 = wave_max_u32(fkey(hi.x)), r4 = wave_max_u32(fkey(hi.y)), r5 = wave_max_u32(fkey(hi.z));
+  const uint32_t lane = __lane_id();
+  if (lane < 6)
+    ca.box_parts[6 * (size_t)tile + lane] =
+        lane == 0 ? r0 : lane == 1 ? r1 : lane == 2 ? r2 : lane == 3 ? r3 : lane == 4 ? r4 : r5;
+}
+
 // One wave per 64 consecutive query items.  Waves whose items span two query
 // segments take the per-lane walk.  When the leaf list fills up, the leaves
 // found so far go through phases 2-3 and the walk goes on.  When the walk's
@@ -136,7 +157,10 @@ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __global__ __launch_bounds__(QB) 
 // the slot counts it started with (the slots it filled are overwritten).
 // a.tile == 2 shrinks both buffers (128 leaves, 32 pending pairs) so that the
 // leaf flushes and the per-lane redo both run (a test mode for those paths).
-__attribute__((amdgpu_waves_per_eu(4, 4))) __global__ __launch_bounds__(64) void k_query_tile(QueryArgs a) {
+// CLOTH: the tile's cloth step follows its query (tile_cloth).
+template <bool CLOTH>
+__attribute__((amdgpu_waves_per_eu(4, 4))) __global__ __launch_bounds__(64) void k_query_tile(QueryArgs a,
+                                                                                               ClothArgs ca) {
   // s_buf: [0, TQ_STACK) child-link pairs of overlapping internal nodes,
   // [TQ_STACK, TQ_STACK + TQ_PAIRS) (lane, facet) candidates; the whole buffer
   // is the per-lane stack (depth 8) of the fallback walks
@@ -159,15 +183,15 @@ __attribute__((amdgpu_waves_per_eu(4, 4))) __global__ __launch_bounds__(64) void
   int2* fstk = s_buf + lane;
   if (seg_of(a.qsegs, base) != sg) {
     // mixed segments: per-lane walk over this lane's own targets
+    uint32_t jl = 0, cnt = 0;
     if (active) {
       const uint32_t sl = seg_of(a.qsegs, tq);
       const uint32_t ml = a.qsegs.mesh[sl];
       const uint32_t pl = query_vertex(a, sl, ml, tq);
-      const uint32_t jl = a.qsegs.off[sl] + pl;
+      jl = a.qsegs.off[sl] + pl;
       const f3 qc = ld3(a.x, a.mt.voff[ml] + pl), qp = ld3(a.xprev, a.mt.voff[ml] + pl);
       const float mg = a.mesh_margin[ml];
       const f3 qlo = {qc.x - mg, qc.y - mg, qc.z - mg}, qhi = {qc.x + mg, qc.y + mg, qc.z + mg};
-      uint32_t cnt = 0;
       for (uint32_t ti = a.tgt_off[sl]; ti < a.tgt_off[sl + 1]; ++ti)
         if (!lane_target<false, SDEPTH>(a, ml, pl, jl, a.tgt[ti], qc, qp, qlo, qhi, a.mesh_band[ml], fstk, 64,
                                         cnt, npair))
@@ -175,6 +199,7 @@ __attribute__((amdgpu_waves_per_eu(4, 4))) __global__ __launch_bounds__(64) void
       a.ccount[jl] = cnt;
       if (cnt > a.cap) atomicOr(a.flags, 2u);
     }
+    if (CLOTH) tile_cloth(ca, active, jl, cnt, w);  // every lane (wave reductions)
     wave_stats(a, w, npair);
     return;
   }
@@ -445,12 +470,23 @@ __attribute__((amdgpu_waves_per_eu(4, 4))) __global__ __launch_bounds__(64) void
     flush();
   }
   wave_sync();
+  const uint32_t c = s_cnt[lane];
   if (active) {
-    const uint32_t c = s_cnt[lane];
     a.ccount[j] = c;
     if (c > a.cap) atomicOr(a.flags, 2u);
   }
+  if (CLOTH) tile_cloth(ca, active, j, c, w);
   wave_stats(a, w, npair);
+}
+
+uint32_t query_tiles(uint32_t nitems) { return ceil_div(nitems, 64); }
+
+int launch_query_cloth(hipStream_t s, const QueryArgs& a, const ClothArgs& ca) {
+  if (!a.nitems) return YSIM_OK;
+  if (!a.tile || ca.nitems != a.nitems) return YSIM_ERR_INVALID;
+  hipLaunchKernelGGL(k_query_tile<true>, dim3(query_tiles(a.nitems)), dim3(64), 0, s, a, ca);
+  YSIM_CHECK_LAUNCH();
+  return YSIM_OK;
 }
 
 bool query_block_ok(uint32_t b) { return b == 64 || b == 128 || b == 256; }
@@ -459,7 +495,7 @@ bool query_waves_ok(uint32_t w) { return w == 5 || w == 6 || w == 8; }
 int launch_query(hipStream_t s, const QueryArgs& a, bool emit_pairs) {
   if (!a.nitems) return YSIM_OK;
   if (!emit_pairs && a.tile) {
-    hipLaunchKernelGGL(k_query_tile, dim3(ceil_div(a.nitems, 64)), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_query_tile<false>, dim3(ceil_div(a.nitems, 64)), dim3(64), 0, s, a, ClothArgs{});
     YSIM_CHECK_LAUNCH();
     return YSIM_OK;
   }

@@ -357,9 +357,10 @@ def main():
                "query": "k_query_tile" if int(tuning.get("query_tile", 0))
                else f"k_query<false, {int(tuning['query_block'])}, {int(tuning['query_waves'])}>",
                "integrate": f"k_cloth<{int(tuning['cloth_block'])}>"}
-    if int(tuning.get("query_tile", 0)) and not (scene.analytic is not None and len(scene.analytic)):
-        # the tile query runs each tile's cloth step itself: one kernel for
-        # query and integrate
+    one_mesh = sum(1 for m in scene.meshes if m.behavior != scenes.BEHAVIOR_FLOAT) == 1
+    if one_mesh and int(tuning.get("query_tile", 0)) and not (scene.analytic is not None and len(scene.analytic)):
+        # one dynamic mesh: the tile query runs each tile's cloth step itself,
+        # one kernel for query and integrate
         kernels["query"] = "k_query_tile<true>"
         kernels["integrate"] = "(fused into k_query_tile<true>)"
         sb["query"] += sb["integrate"]

@@ -106,7 +106,8 @@ def test_pairing_carry_over(ctx, oracle_threads):
 
 def test_snapshot_autotune_and_replacement(ctx, oracle_threads):
     """Snapshot / restore and autotune keep the pairing flags with the state;
-    removing the shapes puts their meshes back into the triangle path."""
+    removing the shapes puts their meshes back into the triangle path, and
+    installing them again takes them out."""
     sc = scenes.config_analytic(n=32, height=0.7)
     hs, os_ = HipScene(ctx, sc), OracleScene(sc)
     hs.step(60)
@@ -132,6 +133,15 @@ def test_snapshot_autotune_and_replacement(ctx, oracle_threads):
     _same_state(hs.state(), os_.state())
     co = _same_contacts(hs, os_)
     assert np.any(np.isin(co["target_obj"], sc.analytic["obj_index"]) & (co["triangle"] > 0))
+    # and back: the cloth step leaves the query kernel again (another count of
+    # box records for the next build)
+    hs.set_analytic(sc.analytic)
+    os_.set_analytic(sc.analytic)
+    hs.step(30, graph=True)
+    os_.step(30)
+    _same_state(hs.state(), os_.state())
+    _same_contacts(hs, os_)
+    assert hs.flags() == 0
 
 
 def test_hash_mode_keeps_triangles(ctx, oracle_threads):

@@ -784,9 +784,14 @@ struct ysim_scene {
     return YSIM_OK;
   }
 
-  // The tile query can run each tile's cloth step itself (one kernel for
-  // query and integrate) unless analytic contacts join the slots after it.
-  bool fuse_query_cloth() const { return !hash_mode && tune.query_tile && !analytic_on() && Vdyn > 0; }
+  // With one dynamic mesh the tile query runs each tile's cloth step itself
+  // (one kernel for query and integrate) unless analytic contacts join the
+  // slots after it.  Scenes of many meshes keep the cloth kernel: its
+  // occupancy is higher than the query's, which matters when many waves
+  // take the per-lane walk (measured on the batched scene).
+  bool fuse_query_cloth() const {
+    return fuse_box && !hash_mode && tune.query_tile && !analytic_on() && Vdyn > 0;
+  }
   // box records the cloth step writes in the current configuration
   uint32_t parts_for_config() const {
     return fuse_query_cloth() ? query_tiles(Vdyn) : cloth_box_parts(Vdyn, tune.cloth_block);

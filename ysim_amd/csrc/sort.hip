@@ -566,8 +566,11 @@ int launch_sort_hybrid(hipStream_t s, const SortPlan& p, void* keys_a, uint32_t*
 // facet) and arrives at its two boundaries (counters advance by two per pass,
 // so they need no clearing); the second arrival compares the pair and
 // requests the full sort if it is not strictly increasing.
+// 1024 threads per window: two keys per thread keep enough loads in flight
+// for one window per CU.
+constexpr uint32_t WS_THREADS = 1024;
 template <typename K>
-__global__ __launch_bounds__(256) void k_window_sort_shifted(K* __restrict__ keys, uint32_t* __restrict__ vals,
+__global__ __launch_bounds__(WS_THREADS) void k_window_sort_shifted(K* __restrict__ keys, uint32_t* __restrict__ vals,
                                                              uint32_t n, uint32_t* __restrict__ need_full,
                                                              uint32_t* __restrict__ bcnt, uint64_t* edges) {
   __shared__ K sk[INC_WINDOW];
@@ -578,13 +581,13 @@ __global__ __launch_bounds__(256) void k_window_sort_shifted(K* __restrict__ key
   const uint32_t hi = (uint32_t)min<int64_t>(s0 + INC_WINDOW, (int64_t)n);
   if (lo >= hi) return;
   const uint32_t cnt = hi - lo;
-  for (uint32_t i = threadIdx.x; i < INC_WINDOW; i += 256) {
+  for (uint32_t i = threadIdx.x; i < INC_WINDOW; i += WS_THREADS) {
     sk[i] = i < cnt ? keys[lo + i] : (K)~(K)0;
     sv[i] = i < cnt ? vals[lo + i] : ~0u;
   }
   __syncthreads();
-  if (window_sort<K, INC_WINDOW>(sk, sv, cnt))
-    for (uint32_t i = threadIdx.x; i < cnt; i += 256) {
+  if (window_sort<K, INC_WINDOW, WS_THREADS>(sk, sv, cnt))
+    for (uint32_t i = threadIdx.x; i < cnt; i += WS_THREADS) {
       keys[lo + i] = sk[i];
       vals[lo + i] = sv[i];
     }
@@ -626,10 +629,10 @@ int launch_incremental_fix(hipStream_t s, bool wide, void* keys, uint32_t* vals,
   if (!n) return YSIM_OK;
   const uint32_t wblocks = ceil_div(n, INC_WINDOW) + 1;
   if (wide)
-    hipLaunchKernelGGL(k_window_sort_shifted<uint64_t>, dim3(wblocks), dim3(256), 0, s, (uint64_t*)keys, vals, n,
+    hipLaunchKernelGGL(k_window_sort_shifted<uint64_t>, dim3(wblocks), dim3(WS_THREADS), 0, s, (uint64_t*)keys, vals, n,
                        need_full, bcnt, edges);
   else
-    hipLaunchKernelGGL(k_window_sort_shifted<uint32_t>, dim3(wblocks), dim3(256), 0, s, (uint32_t*)keys, vals, n,
+    hipLaunchKernelGGL(k_window_sort_shifted<uint32_t>, dim3(wblocks), dim3(WS_THREADS), 0, s, (uint32_t*)keys, vals, n,
                        need_full, bcnt, edges);
   YSIM_CHECK_LAUNCH();
   return YSIM_OK;

@@ -303,7 +303,10 @@ int ysim_hip_scene_get_pairs(ysim_scene* scene, ysim_broad_collision* h_pairs,
 int ysim_hip_scene_get_tree(ysim_scene* scene, uint32_t mesh, ysim_bvh_node* h_nodes,
                             int32_t* h_parent, uint64_t* h_sorted_keys, uint32_t* h_sorted_idx);
 /* Per-stage average device time (ms) over the last n_steps eager substeps run
- * with timing enabled; out has YSIM_STAGE_COUNT entries. */
+ * with timing enabled; out has YSIM_STAGE_COUNT entries.  With one dynamic
+ * mesh (tile query, no analytic colliders) one kernel runs the query and the
+ * integrate: the query entry covers both and the integrate entry is ~0; the
+ * box entry is ~0 too while the cloth step supplies the Morton box. */
 int ysim_hip_scene_stage_timings(ysim_scene* scene, uint32_t n_steps, float* h_ms);
 /* Sticky flags: bit0 anomaly (world-bounds guard), bit1 contact overflow,
  * bit2 traversal guard. */

@@ -30,9 +30,12 @@ namespace ysim {
 
 namespace {
 constexpr int SDEPTH = 8;          // LDS stack entries per thread (deeper stacks overflow no less on cloth)
-constexpr uint32_t TQ_STACK = 256; // tile walk: pending child pairs
-constexpr uint32_t TQ_LEAVES = 256;
-constexpr uint32_t TQ_PAIRS = 256;
+constexpr uint32_t TQ_STACK = 128; // tile walk: pending child pairs
+constexpr uint32_t TQ_LEAVES = 192;
+constexpr uint32_t TQ_PAIRS = 256;  // one emission round adds up to 4 pairs per lane
+static_assert(TQ_PAIRS >= 64 * 4, "an emission round must fit the pair buffer after a flush");
+static_assert(TQ_LEAVES >= 2 * 64, "a walk round adds up to 2 leaves per lane");
+constexpr int TQ_SDEPTH = (TQ_STACK + TQ_PAIRS) / 64;  // fallback per-lane stack depth of the tile kernel
 
 __device__ __forceinline__ uint64_t lanes_below() { return (1ull << __lane_id()) - 1ull; }
 __device__ __forceinline__ uint32_t popc64(uint64_t v) { return (uint32_t)__popcll(v); }
@@ -157,19 +160,21 @@ __device__ __forceinline__ void tile_cloth(const ClothArgs& ca, bool active, uin
 // the slot counts it started with (the slots it filled are overwritten).
 // a.tile == 2 shrinks both buffers (128 leaves, 32 pending pairs) so that the
 // leaf flushes and the per-lane redo both run (a test mode for those paths).
-// CLOTH: the tile's cloth step follows its query (tile_cloth).
+// CLOTH: the tile's cloth step follows its query (tile_cloth); its registers
+// keep that variant at 4 waves per SIMD (at 5 it spills), the query alone
+// runs 5 (LDS 7.3 KB per wave).
 template <bool CLOTH>
-__attribute__((amdgpu_waves_per_eu(4, 4))) __global__ __launch_bounds__(64) void k_query_tile(QueryArgs a,
+__attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __launch_bounds__(64) void k_query_tile(QueryArgs a,
                                                                                                ClothArgs ca) {
   // s_buf: [0, TQ_STACK) child-link pairs of overlapping internal nodes,
   // [TQ_STACK, TQ_STACK + TQ_PAIRS) (lane, facet) candidates; the whole buffer
-  // is the per-lane stack (depth 8) of the fallback walks
+  // is the per-lane stack (TQ_SDEPTH entries) of the fallback walks
   __shared__ int2 s_buf[TQ_STACK + TQ_PAIRS];
   __shared__ uint2 s_leaf[TQ_LEAVES];  // (leaf node, facet) overlapping the union box
   __shared__ float s_q[64 * 6];        // qcur, qprev per lane
   __shared__ float s_sub[16 * 6];      // 2 x 2-vertex sub-boxes
   __shared__ uint32_t s_cnt[64], s_p[64], s_j[64];
-  static_assert(TQ_STACK + TQ_PAIRS == 64 * SDEPTH, "fallback stack: SDEPTH entries per lane");
+  static_assert(TQ_STACK + TQ_PAIRS == 64 * TQ_SDEPTH, "fallback stack: TQ_SDEPTH entries per lane");
   int2* s_stk = s_buf;
   uint2* s_pair = reinterpret_cast<uint2*>(s_buf + TQ_STACK);
   const uint32_t lane = threadIdx.x;
@@ -193,7 +198,7 @@ __attribute__((amdgpu_waves_per_eu(4, 4))) __global__ __launch_bounds__(64) void
       const float mg = a.mesh_margin[ml];
       const f3 qlo = {qc.x - mg, qc.y - mg, qc.z - mg}, qhi = {qc.x + mg, qc.y + mg, qc.z + mg};
       for (uint32_t ti = a.tgt_off[sl]; ti < a.tgt_off[sl + 1]; ++ti)
-        if (!lane_target<false, SDEPTH>(a, ml, pl, jl, a.tgt[ti], qc, qp, qlo, qhi, a.mesh_band[ml], fstk, 64,
+        if (!lane_target<false, TQ_SDEPTH>(a, ml, pl, jl, a.tgt[ti], qc, qp, qlo, qhi, a.mesh_band[ml], fstk, 64,
                                         cnt, npair))
           atomicOr(a.flags, 4u);
       a.ccount[jl] = cnt;
@@ -458,7 +463,7 @@ __attribute__((amdgpu_waves_per_eu(4, 4))) __global__ __launch_bounds__(64) void
       const f3 qc = {s_q[lane * 6 + 0], s_q[lane * 6 + 1], s_q[lane * 6 + 2]};
       const f3 qp = {s_q[lane * 6 + 3], s_q[lane * 6 + 4], s_q[lane * 6 + 5]};
       const f3 qlo = {qc.x - margin, qc.y - margin, qc.z - margin}, qhi = {qc.x + margin, qc.y + margin, qc.z + margin};
-      if (active && !lane_target<false, SDEPTH>(a, m, s_p[lane], s_j[lane], t, qc, qp, qlo, qhi, band, fstk, 64, c,
+      if (active && !lane_target<false, TQ_SDEPTH>(a, m, s_p[lane], s_j[lane], t, qc, qp, qlo, qhi, band, fstk, 64, c,
                                                 npair))
         atomicOr(a.flags, 4u);
       wave_sync();

@@ -415,8 +415,8 @@ def sim_params(subh, margin, morton_bits=30, rebuild_every=0, self_collision=1, 
 # (simulator.hpp:706-713).  A total mass of 1 spread over the grid makes
 # dt*sqrt(k/m) = 9.5 at 512^2, dt = 1/1200, k = 500, far beyond the symplectic
 # Euler stability bound of 2; the sheet explodes within a few substeps.  The
-# configs therefore use mass 1 per vertex (see DESIGN.md for the measured
-# blow-up at the literal setting, reproducible with vertex_mass="total").
+# configs therefore use mass 1 per vertex (the blow-up at the literal setting
+# is reproducible with vertex_mass="total").
 def vertex_mass(n_vertices: int, mode: str = "per_vertex") -> float:
     return 1.0 / n_vertices if mode == "total" else 1.0
 

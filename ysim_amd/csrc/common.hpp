@@ -100,7 +100,7 @@ YSIM_HD uint32_t expand10(uint32_t v) {
   v = (v * 0x00000005u) & 0x49249249u;
   return v;
 }
-// 21 -> 63 bit spread (the 63-bit variant of this build).
+// 21 -> 63 bit spread (the 63-bit Morton codes).
 YSIM_HD uint64_t expand21(uint64_t v) {
   v &= 0x1fffffull;
   v = (v | (v << 32)) & 0x001f00000000ffffull;

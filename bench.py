@@ -322,11 +322,15 @@ def main():
 
     # timed region: exactly `steps` substeps, synchronised on both sides
     log(f"timed region: {args.steps} substeps")
+    deferred0 = hs.deferred_builds()
     ctx.sync()
     t0 = time.perf_counter()
     run(args.steps)
     ctx.sync()
     elapsed = time.perf_counter() - t0
+    # replays of the timed region whose Karras pass kept the previous topology
+    # (failed incremental order check; the following substep rebuilt eagerly)
+    deferred_timed = hs.deferred_builds() - deferred0
     cnt = hs.counters(reset=True)
     flags = hs.flags()
     ms_per_step = 1e3 * elapsed / args.steps
@@ -425,6 +429,8 @@ def main():
                       "items_per_thread": int(tuning["sort_items"])}),
             "tuning": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in tuning.items()},
             "hipgraph": use_graph, "vertex_mass": 1.0, "episode_substeps": args.episode,
+            "deferred_builds_timed": deferred_timed,
+            "substeps_rebuilt_timed": args.steps - deferred_timed,
             "candidate_pairs_per_substep": round(pairs_per_step, 1), "contacts_last_substep": contacts,
             "flags": flags,
         },

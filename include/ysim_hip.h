@@ -328,6 +328,32 @@ int ysim_hip_scene_counters(ysim_scene* scene, uint64_t* h_out, uint32_t reset);
  * through this API (set_state, restore, tuning) does the same without a
  * failed check.  *h_out: how many replays kept a previous topology. */
 int ysim_hip_scene_deferred_builds(ysim_scene* scene, uint64_t* h_out);
+/* No counterpart in the reference (parity audit of the kernel a substep
+ * actually runs).  With capacity > 0, every later substep's query (the tile
+ * walk fused with the cloth step, the per-vertex walk or the hash query,
+ * whichever the tuning selects; eager or graph replay) also records each
+ * candidate pair it reports, into a device buffer of `capacity` records
+ * (16 B each) that each substep's query clears first.  capacity 0 turns the
+ * audit off and frees the buffer.  Turning it on or off drops the captured
+ * graph. */
+int ysim_hip_scene_set_audit(ysim_scene* scene, uint32_t capacity);
+/* The candidate pairs recorded by the last audited query, as BroadCollision
+ * records sorted by (query_obj, point, target_obj, triangle), duplicates kept
+ * (a correct walk reports each pair once).  The records of a tile walk that
+ * overflowed and was redone per lane are dropped (*h_redo, optional, counts
+ * those walks).  YSIM_ERR_CAPACITY when the device buffer overflowed
+ * (*h_count = records needed: set_audit with that and run the substep again)
+ * or when `capacity` is smaller than *h_count; h_pairs may be null to ask for
+ * the count.  YSIM_ERR_STATE when the audit is off. */
+int ysim_hip_scene_get_audit(ysim_scene* scene, ysim_broad_collision* h_pairs, uint32_t capacity,
+                             uint32_t* h_count, uint32_t* h_redo);
+/* No counterpart in the reference (the tile walk's start frontier).  The tree
+ * cut of one mesh that its last Karras build listed: *h_count nodes (0 when
+ * no cut was listed; the walk then starts from the root), written to h_nodes
+ * when capacity allows (YSIM_CUT_CAP at most). */
+#define YSIM_CUT_CAP 512u
+int ysim_hip_scene_get_cut(ysim_scene* scene, uint32_t mesh, int32_t* h_nodes, uint32_t capacity,
+                           uint32_t* h_count);
 
 /* ---- analytic colliders -----------------------------------------------------
  * ColliderKind (include/sim/collision_pod.hpp:32, common_types.metalh:80-84):

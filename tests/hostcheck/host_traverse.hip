@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <vector>
 
+#include "../../ysim_amd/csrc/audit.hpp"
 #include "../../ysim_amd/csrc/traverse.hpp"
 
 using namespace ysim;
@@ -44,4 +45,17 @@ extern "C" int yt_query(const float* x, uint32_t nq, uint32_t qobj, const ysim_b
   *out_iters = iters;
   *out_max_iters = maxit;
   return 0;
+}
+
+// The library's audit filter (ysim_amd/csrc/audit.hpp) on host records:
+// returns the kept count, writes up to cap (j, t, fid) triples.
+extern "C" uint64_t yt_audit_collect(const uint32_t* rec, uint64_t n, uint32_t* out3, uint64_t cap,
+                                     uint32_t* n_retract, uint32_t* n_bad) {
+  const std::vector<AuditTriple> kept = audit_collect(rec, n, n_retract, n_bad);
+  for (uint64_t i = 0; i < kept.size() && i < cap; ++i) {
+    out3[3 * i] = kept[i].j;
+    out3[3 * i + 1] = kept[i].t;
+    out3[3 * i + 2] = kept[i].fid;
+  }
+  return kept.size();
 }

@@ -68,6 +68,9 @@ _SIG = {
     "ysim_hip_scene_deferred_builds": (C.c_int, [vp, vp]),
     "ysim_hip_scene_get_contacts": (C.c_int, [vp, vp, vp, u32, u32p]),
     "ysim_hip_scene_get_pairs": (C.c_int, [vp, vp, u32, u32p]),
+    "ysim_hip_scene_set_audit": (C.c_int, [vp, u32]),
+    "ysim_hip_scene_get_audit": (C.c_int, [vp, vp, u32, u32p, u32p]),
+    "ysim_hip_scene_get_cut": (C.c_int, [vp, u32, vp, u32, u32p]),
     "ysim_hip_scene_get_tree": (C.c_int, [vp, u32, vp, vp, vp, vp]),
     "ysim_hip_scene_stage_timings": (C.c_int, [vp, u32, f32p]),
     "ysim_hip_scene_flags": (C.c_int, [vp, u32p]),

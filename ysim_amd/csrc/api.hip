@@ -226,6 +226,10 @@ struct ysim_scene {
   uint32_t* flags = nullptr;   // [0] sticky flags
   uint32_t* pairs = nullptr;
   uint32_t pair_cap = 0;
+  // candidate audit (scene_set_audit): records of the last audited query
+  uint4* audit = nullptr;
+  uint32_t* audit_cnt = nullptr;
+  uint32_t audit_cap = 0;
   std::vector<void*> allocs;
   uint32_t parity = 0;
   uint64_t step = 0;
@@ -755,6 +759,18 @@ struct ysim_scene {
     return a;
   }
 
+  // The audit fields of a query launch; clears the record count first (a
+  // kernel, so graph replays clear it too).
+  int audited(QueryArgs& a) {
+    if (!audit) return YSIM_OK;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, audit_cnt, 1u);
+    YSIM_CHECK_LAUNCH();
+    a.audit = audit;
+    a.audit_cnt = audit_cnt;
+    a.audit_cap = audit_cap;
+    return YSIM_OK;
+  }
+
   int enqueue_collision(uint32_t par, bool rebuild) {
     const float* xc = x[par];
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[0], st));
@@ -802,7 +818,9 @@ struct ysim_scene {
     if (fuse_query_cloth()) {
       ClothArgs ca = cloth_args(par);
       if (fuse_box) ca.box_parts = cloth_parts;
-      YSIM_TRY(launch_query_cloth(st, query_args(par), ca));
+      QueryArgs qa = query_args(par);
+      YSIM_TRY(audited(qa));
+      YSIM_TRY(launch_query_cloth(st, qa, ca));
       parts_count = query_tiles(Vdyn);
       if (fuse_box) box_fresh = true;
       if (timing) {
@@ -817,6 +835,7 @@ struct ysim_scene {
 
   int enqueue_query(uint32_t par) {
     QueryArgs qa = query_args(par);
+    YSIM_TRY(audited(qa));
     if (hash_mode) {
       HashArgs h = hargs;
       h.x = x[par];
@@ -2086,6 +2105,77 @@ int ysim_hip_scene_deferred_builds(ysim_scene* s, uint64_t* h_out) {
     s->full_next = true;
   }
   *h_out = s->deferred_builds;
+  return YSIM_OK;
+}
+
+int ysim_hip_scene_set_audit(ysim_scene* s, uint32_t capacity) {
+  YSIM_ARG(s, "null scene");
+  if (s->stage_phase != 0) return fail(YSIM_ERR_STATE, "scene_set_audit: a staged substep is in progress");
+  YSIM_TRY(s->release(s->audit));
+  YSIM_TRY(s->release(s->audit_cnt));
+  s->audit = nullptr;
+  s->audit_cnt = nullptr;
+  s->audit_cap = 0;
+  s->drop_graph();
+  if (!capacity) return YSIM_OK;
+  YSIM_TRY(s->alloc(&s->audit, capacity));
+  YSIM_TRY(s->alloc(&s->audit_cnt, 1));
+  YSIM_TRY(s->zero(s->audit_cnt, 1));
+  s->audit_cap = capacity;
+  return YSIM_OK;
+}
+
+int ysim_hip_scene_get_audit(ysim_scene* s, ysim_broad_collision* h_pairs, uint32_t capacity, uint32_t* h_count,
+                             uint32_t* h_redo) {
+  YSIM_ARG(s && h_count, "scene_get_audit: null argument");
+  if (!s->audit) return fail(YSIM_ERR_STATE, "scene_get_audit: the audit is off (scene_set_audit)");
+  uint32_t n = 0;
+  YSIM_HIP_TRY(hipMemcpyAsync(&n, s->audit_cnt, 4, hipMemcpyDeviceToHost, s->st));
+  YSIM_HIP_TRY(hipStreamSynchronize(s->st));
+  if (n > s->audit_cap) {
+    *h_count = n;
+    return fail(YSIM_ERR_CAPACITY, "scene_get_audit: the audit buffer overflowed (records needed in *h_count)");
+  }
+  std::vector<uint32_t> rec(4 * (size_t)n);
+  if (n) YSIM_HIP_TRY(hipMemcpy(rec.data(), s->audit, 16 * (size_t)n, hipMemcpyDeviceToHost));
+  uint32_t nr = 0, nb = 0;
+  const std::vector<AuditTriple> kept = audit_collect(rec.data(), n, &nr, &nb);
+  if (nb) return fail(YSIM_ERR_STATE, "scene_get_audit: records of an unknown kind");
+  if (h_redo) *h_redo = nr;
+  *h_count = (uint32_t)kept.size();
+  if (!h_pairs) return YSIM_OK;
+  if (kept.size() > capacity) return fail(YSIM_ERR_CAPACITY, "scene_get_audit: capacity too small");
+  // query item j -> (dynamic mesh, mesh-local vertex): dyn_voff holds the
+  // query segments' starts
+  for (size_t i = 0; i < kept.size(); ++i) {
+    const AuditTriple& a = kept[i];
+    const size_t k = (size_t)(std::upper_bound(s->dyn_voff.begin(), s->dyn_voff.end(), a.j) - s->dyn_voff.begin()) - 1;
+    if (k >= s->dyn.size() || a.t >= s->nm) return fail(YSIM_ERR_STATE, "scene_get_audit: record out of range");
+    ysim_broad_collision& r = h_pairs[i];
+    r.query_obj = s->dyn[k];
+    r.point = a.j - s->dyn_voff[k];
+    r.target_obj = a.t;
+    r.triangle = a.fid;
+    r.query_behavior = s->meshes[r.query_obj].behavior;
+    r.target_behavior = s->meshes[r.target_obj].behavior;
+    r.query_shape = s->meshes[r.query_obj].shape;
+    r.target_shape = s->meshes[r.target_obj].shape;
+  }
+  return YSIM_OK;
+}
+
+static_assert(YSIM_CUT_CAP == CUT_CAP, "the header's cut capacity is the build's");
+
+int ysim_hip_scene_get_cut(ysim_scene* s, uint32_t mesh, int32_t* h_nodes, uint32_t capacity, uint32_t* h_count) {
+  YSIM_ARG(s && h_count && mesh < s->nm, "scene_get_cut: bad argument");
+  uint32_t n = 0;
+  YSIM_HIP_TRY(hipMemcpyAsync(&n, s->links.cut_cnt + mesh, 4, hipMemcpyDeviceToHost, s->st));
+  YSIM_HIP_TRY(hipStreamSynchronize(s->st));
+  if (n > CUT_CAP) n = 0;  // not listed: walks start from the root
+  *h_count = n;
+  if (!h_nodes || !n) return YSIM_OK;
+  if (n > capacity) return fail(YSIM_ERR_CAPACITY, "scene_get_cut: capacity too small");
+  YSIM_HIP_TRY(hipMemcpy(h_nodes, s->links.cut + (size_t)mesh * CUT_CAP, 4 * (size_t)n, hipMemcpyDeviceToHost));
   return YSIM_OK;
 }
 

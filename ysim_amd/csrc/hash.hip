@@ -235,6 +235,7 @@ __global__ __launch_bounds__(128) void k_hash_query(HashArgs h, QueryArgs a) {
             const uint32_t f0 = f[0], f1 = f[1], f2 = f[2];
             if (self && (p == f0 || p == f1 || p == f2)) continue;
             ++npair;
+            if (a.audit) audit_lane(a, j, r.mesh, r.fid);
             if (EMIT_PAIRS) {
               uint32_t slot = atomicAdd(a.npairs, 1u);
               if (slot < a.pair_cap) reinterpret_cast<uint4*>(a.pairs)[slot] = make_uint4(m, p, r.mesh, r.fid);

@@ -3,6 +3,7 @@
 // a scene (one segment per mesh), so the plain single-cloth substep and the
 // batched multi-instance substep run the same code.
 #pragma once
+#include "audit.hpp"
 #include "common.hpp"
 #include "sort.hpp"
 
@@ -151,7 +152,18 @@ struct QueryArgs {
   // optional tree cuts (RefitLinks::cut): the tile walk starts from them
   const uint32_t* cut_cnt = nullptr;
   const int32_t* cut = nullptr;
+  // candidate audit (off when null): every candidate the walk reports is also
+  // recorded as {query item, target mesh, facet, tag} (audit.hpp: tags, and
+  // the host filter that drops the records of tile walks redone per lane)
+  uint4* audit = nullptr;
+  uint32_t* audit_cnt = nullptr;  // records reserved (may exceed audit_cap)
+  uint32_t audit_cap = 0;
 };
+// Audit record of one candidate found by a per-lane walk (a.audit set).
+__device__ __forceinline__ void audit_lane(const QueryArgs& a, uint32_t j, uint32_t t, uint32_t fid) {
+  const uint32_t s = atomicAdd(a.audit_cnt, 1u);
+  if (s < a.audit_cap) a.audit[s] = make_uint4(j, t, fid, audit_tag(AUDIT_LANE, 0u));
+}
 bool query_block_ok(uint32_t b);
 bool query_waves_ok(uint32_t w);
 int launch_query(hipStream_t s, const QueryArgs& a, bool emit_pairs);

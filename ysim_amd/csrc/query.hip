@@ -75,6 +75,7 @@ __device__ __forceinline__ bool lane_target(const QueryArgs& a, uint32_t m, uint
     const uint32_t f0 = f[0], f1 = f[1], f2 = f[2];
     if (self && (p == f0 || p == f1 || p == f2)) return;
     ++npair;
+    if (a.audit) audit_lane(a, j, t, (uint32_t)fid);
     if (EMIT_PAIRS) {
       uint32_t slot = atomicAdd(a.npairs, 1u);
       if (slot < a.pair_cap) reinterpret_cast<uint4*>(a.pairs)[slot] = make_uint4(m, p, t, (uint32_t)fid);
@@ -354,6 +355,14 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
           if (np + tot > TQ_PAIRS) flush();
           const uint32_t o = np + excl;
           for (uint32_t q = 0; q < nc; ++q) s_pair[o + q] = make_uint2((cand >> (6 * q)) & 63u, fid);
+          if (a.audit) {  // one reservation per round for the wave's tot records
+            uint32_t r = 0;
+            if (lane == 0) r = atomicAdd(a.audit_cnt, tot);
+            r = __shfl(r, 0, 64) + excl;
+            for (uint32_t q = 0; q < nc; ++q)
+              if (r + q < a.audit_cap)
+                a.audit[r + q] = make_uint4(s_j[(cand >> (6 * q)) & 63u], t, fid, audit_tag(AUDIT_TILE, w));
+          }
           np += tot;
           npair += nc;
           wave_sync();
@@ -457,6 +466,10 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
     }
     if (overflow) {
       // too wide: this target again per lane, from the slot counts it started with
+      if (a.audit && lane == 0) {  // the walk's audit records of t are void
+        const uint32_t r = atomicAdd(a.audit_cnt, 1u);
+        if (r < a.audit_cap) a.audit[r] = make_uint4(0u, t, 0u, audit_tag(AUDIT_RETRACT, w));
+      }
       wave_sync();
       uint32_t c = cnt0;
       npair = npair0;

@@ -290,6 +290,30 @@ class HipScene:
         call("ysim_hip_scene_get_pairs", self.h, rec.ctypes.data, n.value, C.byref(n))
         return np.stack([rec["query_obj"], rec["point"], rec["target_obj"], rec["triangle"]], 1)
 
+    def set_audit(self, capacity: int):
+        """Record every candidate pair of each later substep's query (the
+        kernel the tuning selects, eager or graph); 0 turns it off."""
+        call("ysim_hip_scene_set_audit", self.h, int(capacity))
+
+    def audit(self):
+        """Candidate pairs the last audited query reported, as (n, 4) rows
+        (qobj, point, tobj, tri) sorted, duplicates kept, and the number of
+        tile walks that were redone per lane."""
+        n = C.c_uint32(0)
+        redo = C.c_uint32(0)
+        call("ysim_hip_scene_get_audit", self.h, None, 0, C.byref(n), C.byref(redo))
+        rec = np.zeros(n.value, abi.BROAD)
+        call("ysim_hip_scene_get_audit", self.h, rec.ctypes.data, n.value, C.byref(n), C.byref(redo))
+        rows = np.stack([rec["query_obj"], rec["point"], rec["target_obj"], rec["triangle"]], 1)
+        return rows.astype(np.uint32).reshape(-1, 4), int(redo.value)
+
+    def cut(self, mesh: int) -> np.ndarray:
+        """The tree cut the last Karras build of `mesh` listed (empty: none)."""
+        n = C.c_uint32(0)
+        out = np.zeros(512, np.int32)
+        call("ysim_hip_scene_get_cut", self.h, mesh, out.ctypes.data, out.size, C.byref(n))
+        return out[: n.value].copy()
+
     def tree(self, mesh: int, keys: bool = False):
         T = self.scene.meshes[mesh].facets.shape[0]
         nodes = np.zeros(2 * T - 1, abi.BVH_NODE)

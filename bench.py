@@ -88,18 +88,74 @@ def pmc_traffic(kernel: str):
 
 
 def cpu_baseline(scene, sample_steps: int, warm: int = 1):
+    """The strict-IEEE oracle's substep on the same inputs, single thread and on
+    all cores: steady-state median substep time over `sample_steps` substeps
+    after `warm` excluded ones, and each stage's median."""
     from oracle import oracle as O
     cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    stages = ("box", "morton", "sort", "hierarchy", "refit", "query", "integrate")
     res = {}
     for label, threads in (("single", 1), ("all", cores)):
         O.set_threads(threads)
         S = O.OracleScene(scene)
         S.step(warm)
+        S.record_times(True)
         t0 = time.perf_counter()
         S.step(sample_steps)
-        dt = time.perf_counter() - t0
-        res[label] = (sample_steps / dt, threads)
+        wall = time.perf_counter() - t0
+        st = S.step_times()
+        per = st.sum(axis=1)
+        res[label] = {"substeps_per_s": 1.0 / float(np.median(per)), "threads": threads,
+                      "median_substep_ms": round(1e3 * float(np.median(per)), 3),
+                      "mean_substeps_per_s": round(sample_steps / wall, 3),
+                      "stage_median_ms": {k: round(1e3 * float(np.median(st[:, i])), 3) for i, k in enumerate(stages)}}
     return res
+
+
+def parity_report(ctx, log):
+    """Each configuration run on the GPU and in the oracle with its FP64 twin and
+    threshold-tie statistics: whether the GPU state equals the FP32 oracle's bit
+    for bit, the GPU positions' largest error against the FP64 twin (absolute,
+    and relative to the largest FP64 coordinate), and the candidates whose
+    narrow decision is a tie (FP32 and FP64 decisions differ, or the FP64
+    distance lies within 4 FP32 ulps of the band).  The twin follows the FP32
+    contact set (the same piece of the piecewise-smooth map)."""
+    from oracle import oracle as O
+    from ysim_amd import scenes
+    from ysim_amd.hip import HipScene
+    O.set_threads(int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1))
+    cfgs = (("c1", scenes.config_c1(), 20), ("c2", scenes.config_c2(), 100),
+            ("c3_subset", scenes.config_c3(count=8, lo=16, hi=256), 50),
+            ("c5_small", scenes.config_c5(n=256, level=4, height=0.93), 50))
+    out = {}
+    for name, sc, n in cfgs:
+        t0 = time.perf_counter()
+        hs = HipScene(ctx, sc)
+        hs.step(n, graph=True)
+        xg, vg, pg = hs.state()
+        flags = hs.flags()
+        del hs
+        o = O.OracleScene(sc, twin=True)
+        o.enable_ties()
+        o.step(n)
+        xo, vo, po = o.state()
+        x64, v64 = o.twin_state()
+        dyn = np.concatenate([np.full(m.x.shape[0], m.behavior != scenes.BEHAVIOR_FLOAT) for m in sc.meshes])
+        err = np.abs(xg[dyn].astype(np.float64) - x64[dyn])
+        scale = float(np.abs(x64[dyn]).max())
+        ties = o.ties()
+        out[name] = {
+            "workload": sc.name, "vertices": sc.num_vertices, "triangles": sc.num_facets, "substeps": n,
+            "bit_exact_vs_fp32_oracle": bool(np.array_equal(xg, xo) and np.array_equal(vg, vo) and
+                                             np.array_equal(pg, po)),
+            "fp64_max_abs": float(err.max()), "fp64_max_rel": float(err.max() / scale),
+            "candidates_classified": ties["classified"], "narrow_ties": ties["decision"],
+            "band_ties": ties["band"], "contacts_last_substep": int(len(o.contacts()[1])), "flags": flags,
+        }
+        log(f"parity {name}: bit-exact={out[name]['bit_exact_vs_fp32_oracle']}, fp64 max rel "
+            f"{out[name]['fp64_max_rel']:.2e}, ties {ties['decision']}/{ties['band']} of {ties['classified']} "
+            f"({time.perf_counter() - t0:.1f} s)")
+    return out
 
 
 def batched_vs_sequential(ctx, substeps: int, log):
@@ -252,7 +308,8 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=512, help="cloth side (512 = the benchmark workload)")
-    ap.add_argument("--cpu-steps", type=int, default=12, help="oracle substeps timed for cpu_baseline")
+    ap.add_argument("--cpu-steps", type=int, default=40, help="oracle substeps timed for cpu_baseline")
+    ap.add_argument("--no-parity", action="store_true", help="skip the per-configuration FP64 / tie report")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--episode", type=int, default=100, help="substeps per restored episode (config: 100)")
@@ -449,14 +506,23 @@ def main():
     if not args.no_extras:
         out["c5_lbvh_vs_hash"] = lbvh_vs_hash(ctx, 20, log)
         out["adjoint"] = adjoint_timing(ctx, log)
+    if not args.no_parity and args.n == 512:
+        out["parity"] = parity_report(ctx, log)
+        p2 = out["parity"]["c2"]
+        out["config"].update({k: p2[k] for k in ("fp64_max_abs", "fp64_max_rel", "narrow_ties", "band_ties",
+                                                  "bit_exact_vs_fp32_oracle")})
     if not args.no_cpu:
         log(f"cpu baseline: {args.cpu_steps} oracle substeps, 1 thread and all cores")
         cb = cpu_baseline(scene, args.cpu_steps)
         out["cpu_baseline"] = {
-            "value": round(cb["single"][0], 3), "unit": "substeps/s", "cores": 1, "kind": "port",
-            "sample": f"{args.cpu_steps} substeps of the same {args.n}x{args.n} workload after 1 warm-up "
-                      "substep, strict-IEEE C++ oracle (-O3 -ffp-contract=off), single thread",
-            "all_cores": {"value": round(cb["all"][0], 3), "cores": cb["all"][1]},
+            "value": round(cb["single"]["substeps_per_s"], 3), "unit": "substeps/s", "cores": 1, "kind": "port",
+            "sample": f"median substep of {args.cpu_steps} substeps of the same {args.n}x{args.n} workload after 1 "
+                      "excluded warm-up substep, strict-IEEE C++ oracle (-O3 -ffp-contract=off), single thread",
+            "median_substep_ms": cb["single"]["median_substep_ms"],
+            "stage_median_ms": cb["single"]["stage_median_ms"],
+            "all_cores": {"value": round(cb["all"]["substeps_per_s"], 3), "cores": cb["all"]["threads"],
+                          "median_substep_ms": cb["all"]["median_substep_ms"],
+                          "stage_median_ms": cb["all"]["stage_median_ms"]},
         }
     print(json.dumps(out))
 

@@ -71,6 +71,11 @@ def lib():
         "yo_scene_create": (vp, [C.c_uint32, vp, f32p, f32p, f32p, f32p, f32p, u32p, vp]),
         "yo_scene_destroy": (None, [vp]),
         "yo_scene_enable_twin": (None, [vp]),
+        "yo_scene_enable_ties": (None, [vp]),
+        "yo_scene_ties": (None, [vp, u64p]),
+        "yo_scene_record_times": (None, [vp, C.c_int]),
+        "yo_scene_step_times": (C.c_uint32, [vp, f64p, C.c_uint32]),
+        "yo_tie_bits": (C.c_uint32, [f32p, f32p, f32p, f32p, f32p, C.c_float, C.c_int]),
         "yo_scene_set_kspring": (None, [vp, f32p, C.c_uint32]),
         "yo_scene_step": (None, [vp, C.c_uint32]),
         "yo_scene_pairs": (C.c_uint64, [vp, u32p, C.c_uint64]),
@@ -265,6 +270,13 @@ def tri_forces(facets, x0, x, v, ext, kstretch, kbend, kair, kd, f64=False):
     return out.reshape(-1, 3)
 
 
+def tie_bits(qc, qp, t0, t1, t2, band: float, fired32: bool) -> int:
+    """Threshold-tie bits of one candidate: 1 = FP32 and FP64 narrow decisions
+    differ, 2 = FP64 distance within 4 FP32 ulps of the band."""
+    a = [np.ascontiguousarray(t, np.float32).reshape(3) for t in (qc, qp, t0, t1, t2)]
+    return int(lib().yo_tie_bits(*(P(t, f32p) for t in a), float(band), int(bool(fired32))))
+
+
 def spring_count(n: int) -> int:
     return lib().yo_spring_count(n)
 
@@ -358,6 +370,30 @@ class OracleScene:
     def set_state(self, x=None, v=None, xprev=None):
         a = [None if t is None else np.ascontiguousarray(t, np.float32).reshape(-1) for t in (x, v, xprev)]
         lib().yo_scene_set_state(self.h, P(a[0], f32p), P(a[1], f32p), P(a[2], f32p))
+
+    def record_times(self, on: bool = True):
+        """Record each later substep's per-stage wall time (clears the record)."""
+        lib().yo_scene_record_times(self.h, int(on))
+
+    def step_times(self) -> np.ndarray:
+        """(substeps, 7) stage seconds (box, morton, sort, hierarchy, refit,
+        query incl. narrow and contacts, integrate) of the recorded substeps."""
+        L = lib()
+        n = L.yo_scene_step_times(self.h, None, 0)
+        out = np.zeros(7 * n, np.float64)
+        L.yo_scene_step_times(self.h, P(out, f64p), n)
+        return out.reshape(-1, 7)
+
+    def enable_ties(self):
+        """Classify every later candidate's narrow decision in FP64 too."""
+        lib().yo_scene_enable_ties(self.h)
+
+    def ties(self) -> dict:
+        """Threshold ties since enable_ties: candidates classified, FP32/FP64
+        decision differences, FP64 distances within 4 FP32 ulps of the band."""
+        out = np.zeros(3, np.uint64)
+        lib().yo_scene_ties(self.h, P(out, u64p))
+        return dict(classified=int(out[0]), decision=int(out[1]), band=int(out[2]))
 
     def twin_state(self):
         x = np.zeros(3 * self.V, np.float64)

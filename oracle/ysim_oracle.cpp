@@ -19,6 +19,8 @@
 // twin and closed-form free fall, CSR order vs lexsort.
 
 #include <algorithm>
+#include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -334,6 +336,41 @@ inline bool narrow_test(V3<R> qcur, V3<R> qprev, V3<R> t0, V3<R> t1, V3<R> t2, R
   n_out = n;
   d_out = dcur;
   return true;
+}
+
+// Oriented current distance of narrow_test (the value compared with the band
+// at src/metal/bruteforce.metal:149-151), or false for a degenerate facet.
+template <class R>
+inline bool narrow_dcur(V3<R> qcur, V3<R> qprev, V3<R> t0, V3<R> t1, V3<R> t2, R& d_out) {
+  V3<R> n = cross(t1 - t0, t2 - t0);
+  R nlen2 = dot(n, n);
+  if (nlen2 < (R)1e-12f) return false;
+  n = n * ((R)1 / sqrt_(nlen2));
+  R dprev = dot(n, qprev - t0);
+  R dcur = dot(n, qcur - t0);
+  R dref = (dprev != (R)0) ? dprev : dcur;
+  d_out = dref < (R)0 ? -dcur : dcur;
+  return true;
+}
+
+inline V3<double> up(V3<float> a) { return {(double)a.x, (double)a.y, (double)a.z}; }
+
+// Threshold ties of one candidate (the FP32 inputs evaluated again in FP64):
+// bit 0 when the FP32 and FP64 narrow decisions differ, bit 1 when the FP64
+// distance lies within 4 FP32 ulps of the band (the strict compare could go
+// either way under another rounding).
+inline uint32_t tie_bits(V3<float> qc, V3<float> qp, V3<float> t0, V3<float> t1, V3<float> t2, float band,
+                         bool fired32) {
+  V3<double> n;
+  double d = 0.0;
+  const bool fired64 = narrow_test<double>(up(qc), up(qp), up(t0), up(t1), up(t2), (double)band, n, d);
+  uint32_t b = (fired32 != fired64) ? 1u : 0u;
+  double dc = 0.0;
+  if (narrow_dcur<double>(up(qc), up(qp), up(t0), up(t1), up(t2), dc)) {
+    const double ulp = (double)std::nextafter(band, 2.f * band + 1.f) - (double)band;
+    if (std::fabs(dc - (double)band) <= 4.0 * ulp) b |= 2u;
+  }
+  return b;
 }
 
 inline bool box_overlap(const float* qmin, const float* qmax, const ysim_bvh_node& nd) {
@@ -1231,13 +1268,21 @@ struct OScene {
   // last substep contacts as CSR
   std::vector<uint32_t> csr_off;
   std::vector<ysim_narrow_collision> csr;
+  // threshold-tie statistics of every candidate of every substep (FP32
+  // decision vs FP64 on the same inputs): candidates classified, decisions
+  // that differ, FP64 distances within 4 FP32 ulps of the band
+  bool tie_stats = false;
+  uint64_t ties[3] = {0, 0, 0};
   // double twin of the dynamics (same contacts)
   bool twin = false;
   std::vector<double> xd, vd, xprevd, fd;
   std::vector<double> twin_nd4;  // FP64 contact normals/distances, same records as csr
   std::vector<double> kspring_d;
   std::vector<float> kspring;    // optional per-spring k for mesh 0 (adjoint configs)
-  double timing[YSIM_STAGE_COUNT] = {0};
+  double timing[YSIM_STAGE_COUNT] = {0};  // seconds per stage, accumulated
+  // per-substep stage seconds of yo_scene_step (when record_times is set)
+  bool record_times = false;
+  std::vector<std::array<double, YSIM_STAGE_COUNT>> step_times;
   // analytic colliders: shapes, their per-shape constants, the shape of each
   // mesh (-1: none), the (cloth, shape) pairs and the previous substep's
   // pairing of each (PackedCollisionData::endAnalyticPairs: held = last fresh)
@@ -1255,6 +1300,10 @@ bool is_dynamic(const ysim_mesh_desc& d) {
   return d.behavior == YSIM_BEHAVIOR_FASTGRID_CLOTH || d.behavior == YSIM_BEHAVIOR_TRIANGULAR_CLOTH;
 }
 
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 void build_tree(OScene& S, OMesh& M) {
   uint32_t n = M.d.num_facets;
   M.keys.resize(n);
@@ -1265,13 +1314,18 @@ void build_tree(OScene& S, OMesh& M) {
   const float* x = S.x.data() + 3 * M.voff;
   const uint32_t* f = S.facets.data() + 3 * M.foff;
   float box[6];
+  double t = now_s(), u;
   yo_scene_box(x, M.d.num_vertices, box);
+  S.timing[YSIM_STAGE_BOX] += (u = now_s()) - t;
   yo_morton(x, f, n, box, S.P.morton_bits, M.keys.data(), M.idx.data());
+  S.timing[YSIM_STAGE_MORTON] += (t = now_s()) - u;
   yo_sort_pairs(M.keys.data(), M.idx.data(), n, S.P.morton_bits == 30 ? 30 : 63);
+  S.timing[YSIM_STAGE_SORT] += (u = now_s()) - t;
   if (S.build_mode == 1 && is_dynamic(M.d) && M.built)
     yo_agglomerative(M.keys.data(), M.idx.data(), n, S.P.morton_bits, M.tree.data(), M.parent.data());
   else
     yo_karras(M.keys.data(), M.idx.data(), n, S.P.morton_bits, M.tree.data(), M.parent.data());
+  S.timing[YSIM_STAGE_HIERARCHY] += now_s() - u;
   M.built = true;
 }
 
@@ -1322,6 +1376,21 @@ void* yo_scene_create(uint32_t nm, const ysim_mesh_desc* meshes, const float* x,
 }
 
 void yo_scene_destroy(void* s) { delete (OScene*)s; }
+
+// tie_bits of one candidate given as float triples (checks of the classifier).
+uint32_t yo_tie_bits(const float* qc, const float* qp, const float* t0, const float* t1, const float* t2,
+                     float band, int fired32) {
+  auto v = [](const float* a) { return V3<float>{a[0], a[1], a[2]}; };
+  return tie_bits(v(qc), v(qp), v(t0), v(t1), v(t2), band, fired32 != 0);
+}
+
+// Threshold-tie statistics from now on (each substep's candidates): out3 =
+// {candidates classified, FP32/FP64 decision differences, band ties}.
+void yo_scene_enable_ties(void* s) { ((OScene*)s)->tie_stats = true; }
+void yo_scene_ties(void* s, uint64_t* out3) {
+  OScene* S = (OScene*)s;
+  for (int k = 0; k < 3; ++k) out3[k] = S->ties[k];
+}
 
 void yo_scene_enable_twin(void* s) {
   OScene* S = (OScene*)s;
@@ -1394,9 +1463,12 @@ static void collide(OScene& S, uint64_t* npairs_out, std::vector<uint32_t>* pair
   for (auto& M : S.meshes) {
     if (!is_dynamic(M.d)) continue;
     if (rebuild || !M.built) build_tree(S, M);
+    const double t = now_s();
     yo_refit(S.x.data() + 3 * M.voff, S.v.data() + 3 * M.voff, P.subh, P.swept,
              S.facets.data() + 3 * M.foff, M.d.num_facets, M.tree.data());
+    S.timing[YSIM_STAGE_REFIT] += now_s() - t;
   }
+  const double t_query = now_s();
   std::vector<Contact> contacts;
   uint64_t npairs = 0;
   for (uint32_t q = 0; q < S.meshes.size(); ++q) {
@@ -1440,7 +1512,21 @@ static void collide(OScene& S, uint64_t* npairs_out, std::vector<uint32_t>* pair
             if (pairs_out) { pp[p].push_back(p); pp[p].push_back(fid); }
             V3<float> n;
             float d;
-            if (narrow_test(ld(xq, p), ld(xqp, p), ld(xt, f[0]), ld(xt, f[1]), ld(xt, f[2]), band, n, d)) {
+            const bool fired = narrow_test(ld(xq, p), ld(xqp, p), ld(xt, f[0]), ld(xt, f[1]), ld(xt, f[2]), band, n, d);
+            if (S.tie_stats) {
+              const uint32_t tb = tie_bits(ld(xq, p), ld(xqp, p), ld(xt, f[0]), ld(xt, f[1]), ld(xt, f[2]), band, fired);
+#pragma omp atomic
+              S.ties[0]++;
+              if (tb & 1u) {
+#pragma omp atomic
+                S.ties[1]++;
+              }
+              if (tb & 2u) {
+#pragma omp atomic
+                S.ties[2]++;
+              }
+            }
+            if (fired) {
               Contact c;
               c.gv = Q.voff + p; c.tobj = t; c.tri = fid; c.qobj = q; c.point = p;
               c.n[0] = n.x; c.n[1] = n.y; c.n[2] = n.z; c.d = d;
@@ -1486,6 +1572,7 @@ static void collide(OScene& S, uint64_t* npairs_out, std::vector<uint32_t>* pair
   }
   for (uint32_t i = 0; i < S.Vtot; ++i) S.csr_off[i + 1] += S.csr_off[i];
   if (npairs_out) *npairs_out = npairs;
+  S.timing[YSIM_STAGE_QUERY] += now_s() - t_query;
 }
 
 // FP64 twin of the contact geometry: recompute each contact record of the FP32
@@ -1601,11 +1688,36 @@ static void dynamics(OScene& S) {
 void yo_scene_step(void* s, uint32_t n) {
   OScene* S = (OScene*)s;
   for (uint32_t i = 0; i < n; ++i) {
+    std::array<double, YSIM_STAGE_COUNT> t0;
+    std::copy(S->timing, S->timing + YSIM_STAGE_COUNT, t0.begin());
     collide(*S, nullptr, nullptr);
     if (S->twin) twin_contacts(*S);
+    const double t = now_s();
     dynamics(*S);
+    S->timing[YSIM_STAGE_INTEGRATE] += now_s() - t;
     S->step++;
+    if (S->record_times) {
+      std::array<double, YSIM_STAGE_COUNT> d;
+      for (int k = 0; k < YSIM_STAGE_COUNT; ++k) d[k] = S->timing[k] - t0[k];
+      S->step_times.push_back(d);
+    }
   }
+}
+
+// Per-substep stage times (seconds, YSIM_STAGE_COUNT per substep) of the
+// substeps run since record_times was switched on; returns the substep count
+// and copies up to cap substeps.  on = 0 stops recording and clears.
+void yo_scene_record_times(void* s, int on) {
+  OScene* S = (OScene*)s;
+  S->record_times = on != 0;
+  S->step_times.clear();
+}
+uint32_t yo_scene_step_times(void* s, double* out, uint32_t cap) {
+  OScene* S = (OScene*)s;
+  const uint32_t n = (uint32_t)S->step_times.size();
+  for (uint32_t i = 0; i < n && i < cap; ++i)
+    std::copy(S->step_times[i].begin(), S->step_times[i].end(), out + (size_t)YSIM_STAGE_COUNT * i);
+  return n;
 }
 
 // Collision pipeline only (no step): candidate pairs as (qobj, point, tobj, tri)

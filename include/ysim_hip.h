@@ -179,6 +179,19 @@ int ysim_hip_sort_pairs(ysim_ctx* ctx, void* d_keys, uint32_t* d_vals, uint32_t 
 int ysim_hip_sort_pairs_ex(ysim_ctx* ctx, void* d_keys, uint32_t* d_vals, uint32_t n, uint32_t key_bits,
                            uint32_t digit_bits, uint32_t items, uint32_t hybrid);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  ysim_hip_sort_pairs (same plan, same kernels, same
+ * permutation) with its scratch in a static device array of
+ * YSIM_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends: it calls no
+ * allocator.  Inputs needing more scratch than that (about 2^20 keys of
+ * 64 bits, more of 32) return YSIM_ERR_CAPACITY before any launch.  All calls
+ * on one GPU share the array, so calls that overlap on one GPU (two host
+ * threads, two contexts of the same device) race: serialize them. */
+#define YSIM_NOALLOC_SCRATCH_BYTES (24u << 20)
+int ysim_hip_sort_pairs_noalloc(ysim_ctx* ctx, void* d_keys, uint32_t* d_vals, uint32_t n,
+                                uint32_t key_bits);
+
 /* Karras hierarchy over sorted codes: buildTree_Tri (src/metal/bvh.metal:197-258,
  * 352-393).  Writes child links of the 2n-1 node slots and d_parent
  * (root parent = -1); leaf slots get child_a = -1, child_b = facet id.

@@ -79,6 +79,7 @@ _SIG = {
     "ysim_hip_scene_restore": (C.c_int, [vp]),
     "ysim_hip_copy_bandwidth": (C.c_int, [vp, C.c_size_t, u32, f32p]),
     "ysim_hip_sort_pairs_ex": (C.c_int, [vp, vp, vp, u32, u32, u32, u32, u32]),
+    "ysim_hip_sort_pairs_noalloc": (C.c_int, [vp, vp, vp, u32, u32]),
     "ysim_hip_scene_get_tuning": (C.c_int, [vp, vp]),
     "ysim_hip_scene_set_tuning": (C.c_int, [vp, vp]),
     "ysim_hip_scene_autotune": (C.c_int, [vp, u32, vp]),

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -1128,6 +1129,41 @@ int ysim_hip_device_count(int* out) {
   return YSIM_OK;
 }
 
+// ------------------------------------------------------------------ static scratch
+namespace {
+// The allocation-free sort's scratch (histograms, look-back status words and
+// the ping-pong copy of keys and values): a file-scope device array, so the
+// entry point that uses it calls no allocator.  Each GPU the library runs on
+// holds its own copy, at its own address, until the process ends.  In host
+// code the array's name is only a stand-in: kernels and copies get the address
+// hipGetSymbolAddress returned for the current device (noalloc_ptr).
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)YSIM_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char noalloc_scratch[NOALLOC_SCRATCH_BYTES];
+std::mutex noalloc_mu;
+constexpr int NOALLOC_DEVICES = 64;
+void* noalloc_ptr[NOALLOC_DEVICES] = {};
+hipError_t noalloc_rc[NOALLOC_DEVICES] = {};
+
+// Looks up this device's copy once (the device must be current).
+void noalloc_setup(ysim_ctx* c) {
+  std::lock_guard<std::mutex> lk(noalloc_mu);
+  if (c->device >= NOALLOC_DEVICES) {
+    c->noalloc_err = (int)hipErrorInvalidDevice;
+    return;
+  }
+  if (!noalloc_ptr[c->device] && noalloc_rc[c->device] == hipSuccess) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch));
+    if (e == hipSuccess && p)
+      noalloc_ptr[c->device] = p;
+    else
+      noalloc_rc[c->device] = e == hipSuccess ? hipErrorInvalidSymbol : e;
+  }
+  c->noalloc = noalloc_ptr[c->device];
+  c->noalloc_err = (int)noalloc_rc[c->device];
+}
+}  // namespace
+
 int ysim_hip_ctx_create(int device, uint32_t flags, ysim_ctx** out) {
   YSIM_ARG(out, "null out");
   *out = nullptr;
@@ -1147,6 +1183,7 @@ int ysim_hip_ctx_create(int device, uint32_t flags, ysim_ctx** out) {
     delete c;
     return fail(YSIM_ERR_HIP, "hipStreamCreate failed");
   }
+  noalloc_setup(c);
   *out = c;
   return YSIM_OK;
 }
@@ -1279,22 +1316,22 @@ int ysim_hip_sort_pairs(ysim_ctx* ctx, void* d_keys, uint32_t* d_vals, uint32_t 
   return ysim_hip_sort_pairs_ex(ctx, d_keys, d_vals, n, key_bits, 8, 8, 0);
 }
 
-int ysim_hip_sort_pairs_ex(ysim_ctx* ctx, void* d_keys, uint32_t* d_vals, uint32_t n, uint32_t key_bits,
-                           uint32_t digit_bits, uint32_t items, uint32_t hybrid) {
-  YSIM_ARG(ctx && (n == 0 || (d_keys && d_vals)), "sort_pairs: null argument");
-  YSIM_ARG(key_bits >= 1 && key_bits <= 64, "sort_pairs: key_bits in [1, 64]");
-  YSIM_ARG(hybrid ? (items == 8 || items == 16) : sort_config_ok(digit_bits, items),
-           "sort_pairs: digit_bits 8 or 11, items 8 or 16");
-  if (n == 0) return YSIM_OK;
-  YSIM_ARG(n < (1u << 30), "sort_pairs: n must be < 2^30");
-  SortPlan p = hybrid ? sort_plan_hybrid(n, key_bits, items) : sort_plan(n, key_bits, digit_bits, items);
+namespace {
+// Scratch layout of one sort of plan p:
+// [4096 words misc][hist][counters][err 16][epoch 16][status (64-bit words)][pad to 256 B]
+// [keys_b, 256-B rounded][vals_b]
+size_t sort_scratch_bytes(const SortPlan& p) {
   const size_t kb = (p.wide ? 8 : 4);
-  // layout: [4096 words misc][hist][counters][err 16][epoch 16][status (64-bit words)][pad to 256 B]
-  //         [keys_b, 256-B rounded][vals_b]
-  const size_t tmp_bytes = 256 + (((size_t)n * kb + 255) & ~(size_t)255) + (size_t)n * 4 + 256;
-  const size_t words = 4096 + p.hist_words + p.counter_words + 32 + 2 * p.status_words + tmp_bytes / 4 + 2;
-  YSIM_TRY(ctx->ensure_scratch(words * 4));
-  uint32_t* base = (uint32_t*)ctx->scratch;
+  const size_t tmp_bytes = 256 + (((size_t)p.n * kb + 255) & ~(size_t)255) + (size_t)p.n * 4 + 256;
+  return 4 * (4096 + p.hist_words + p.counter_words + 32 + 2 * p.status_words + tmp_bytes / 4 + 2);
+}
+
+// The sort of plan p on d_keys / d_vals with its scratch at base (at least
+// sort_scratch_bytes(p), any contents: the sort clears what must start at zero).
+int sort_in(ysim_ctx* ctx, void* base_, const SortPlan& p, void* d_keys, uint32_t* d_vals) {
+  const uint32_t n = p.n;
+  const size_t kb = (p.wide ? 8 : 4);
+  uint32_t* base = (uint32_t*)base_;
   uint32_t* hist = base + 4096;
   uint32_t* counters = hist + p.hist_words;
   uint32_t* err = counters + p.counter_words;
@@ -1320,6 +1357,39 @@ int ysim_hip_sort_pairs_ex(ysim_ctx* ctx, void* d_keys, uint32_t* d_vals, uint32
   YSIM_HIP_TRY(hipStreamSynchronize(ctx->stream));
   if (herr) return fail(YSIM_ERR_HIP, "sort_pairs: look-back guard fired");
   return YSIM_OK;
+}
+}  // namespace
+
+int ysim_hip_sort_pairs_ex(ysim_ctx* ctx, void* d_keys, uint32_t* d_vals, uint32_t n, uint32_t key_bits,
+                           uint32_t digit_bits, uint32_t items, uint32_t hybrid) {
+  YSIM_ARG(ctx && (n == 0 || (d_keys && d_vals)), "sort_pairs: null argument");
+  YSIM_ARG(key_bits >= 1 && key_bits <= 64, "sort_pairs: key_bits in [1, 64]");
+  YSIM_ARG(hybrid ? (items == 8 || items == 16) : sort_config_ok(digit_bits, items),
+           "sort_pairs: digit_bits 8 or 11, items 8 or 16");
+  if (n == 0) return YSIM_OK;
+  YSIM_ARG(n < (1u << 30), "sort_pairs: n must be < 2^30");
+  const SortPlan p = hybrid ? sort_plan_hybrid(n, key_bits, items) : sort_plan(n, key_bits, digit_bits, items);
+  YSIM_TRY(ctx->ensure_scratch(sort_scratch_bytes(p)));
+  return sort_in(ctx, ctx->scratch, p, d_keys, d_vals);
+}
+
+int ysim_hip_sort_pairs_noalloc(ysim_ctx* ctx, void* d_keys, uint32_t* d_vals, uint32_t n, uint32_t key_bits) {
+  YSIM_ARG(ctx && (n == 0 || (d_keys && d_vals)), "sort_pairs_noalloc: null argument");
+  YSIM_ARG(key_bits >= 1 && key_bits <= 64, "sort_pairs_noalloc: key_bits in [1, 64]");
+  if (n == 0) return YSIM_OK;
+  YSIM_ARG(n < (1u << 30), "sort_pairs_noalloc: n must be < 2^30");
+  if (!ctx->noalloc)
+    return fail(YSIM_ERR_HIP, std::string("sort_pairs_noalloc: the static scratch of this device has no address (") +
+                                  hipGetErrorString((hipError_t)ctx->noalloc_err) + ")");
+  // the same plan as ysim_hip_sort_pairs (8-bit digits, 8 keys per thread)
+  const SortPlan p = sort_plan(n, key_bits, 8, 8);
+  const size_t need = sort_scratch_bytes(p);
+  if (need > NOALLOC_SCRATCH_BYTES)
+    return fail(YSIM_ERR_CAPACITY, "sort_pairs_noalloc: " + std::to_string(n) + " keys of " +
+                                       std::to_string(key_bits) + " bits need " + std::to_string(need) +
+                                       " bytes of scratch; the static scratch holds " +
+                                       std::to_string(NOALLOC_SCRATCH_BYTES) + " (use ysim_hip_sort_pairs)");
+  return sort_in(ctx, ctx->noalloc, p, d_keys, d_vals);
 }
 
 int ysim_hip_lbvh_hierarchy(ysim_ctx* ctx, const void* d_sorted_keys, const uint32_t* d_sorted_idx, uint32_t n,

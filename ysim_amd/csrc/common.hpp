@@ -51,6 +51,10 @@ struct Ctx {
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
   int ensure_scratch(size_t bytes);
+  // device address of this device's static scratch array (the allocation-free
+  // sort), or null when its lookup failed (noalloc_err says why)
+  void* noalloc = nullptr;
+  int noalloc_err = 0;
 };
 
 // ------------------------------------------------------------------ vectors

@@ -72,3 +72,32 @@ def test_drape_sphere_query(L, small):
     got, _ = host_pairs(L, xc, nodes, parent, sp.facets, False, sc.params["margin"], small)
     exp = O.broad_query(xc, 0, nodes, sp.facets, 1, sc.params["margin"])
     assert set(map(tuple, got.tolist())) == set(map(tuple, exp.tolist()))
+
+
+@pytest.mark.parametrize("corrupt", ["self_loop", "out_of_range", "two_cycle"])
+def test_corrupt_parent_links_fire_the_guard(L, corrupt):
+    """The walk's stackless continuation climbs through parent links (a
+    2-entry stack forces it).  Corrupted links must end the walk through its
+    guard (return -1) rather than loop or read out of range; checked on the
+    host build only."""
+    sc = scenes.config_c2(n=32)
+    S = O.OracleScene(sc)
+    S.step(1)
+    nodes, parent, _, _ = S.tree(0)
+    x, _, _ = S.state()
+    m = sc.meshes[0]
+    T = m.facets.shape[0]
+    _, iters = host_pairs(L, x, nodes, parent, m.facets, True, 4 * sc.params["margin"], 1)
+    bad = parent.copy()
+    internal = np.arange(1, T - 1)
+    if corrupt == "self_loop":
+        bad[internal] = internal
+    elif corrupt == "out_of_range":
+        bad[internal] = 10 ** 6
+    else:  # every internal node's parent is its own child_a
+        bad[internal] = nodes["child_a"][internal]
+    args = [x.ctypes.data_as(C.c_void_p), C.c_uint32(len(x)), 0, nodes.ctypes.data_as(C.c_void_p),
+            bad.ctypes.data_as(C.c_void_p), C.c_uint32(T), m.facets.ctypes.data_as(C.c_void_p), 0,
+            C.c_float(4 * sc.params["margin"]), 1]
+    cnt, it, mx = C.c_uint64(), C.c_uint64(), C.c_uint32()
+    assert L.yt_query(*args, None, C.c_uint64(0), C.byref(cnt), C.byref(it), C.byref(mx)) == -1

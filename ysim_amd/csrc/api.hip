@@ -1836,6 +1836,8 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
     for (uint32_t m : targets) h.ntri += s.meshes[m].num_facets;
     const float cell = P.hash_cell > 0.f ? P.hash_cell : 2.f * max_margin;
     YSIM_ARG(cell > 0.f, "scene_create: hash cell must be positive");
+    // a query box spans at most 2 margin / cell + 2 cells per axis: keep that small
+    YSIM_ARG(cell * 7.f >= max_margin, "scene_create: hash cell must be at least margin / 7");
     h.inv_h = 1.0f / cell;
     h.origin = f3{0.f, 0.f, 0.f};
     h.mask = (1u << P.hash_slots_log2) - 1u;
@@ -1870,6 +1872,9 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
     YSIM_TRY(s.alloc(&h.cursor, (size_t)h.mask + 1));
     YSIM_TRY(s.alloc(&h.offsets, (size_t)h.mask + 2));
     YSIM_TRY(s.alloc(&h.scan_tmp, 1025));
+    YSIM_TRY(s.alloc(&h.big, h.ntri));
+    YSIM_TRY(s.alloc(&h.big_cnt, 1));
+    YSIM_TRY(s.zero(h.big_cnt, 1));
   }
   // initial build of every tree (unswept), Simulator::initialize -> broadPhase.build
   for (auto* gs : {&s.stat_groups, &s.dyn_groups})

@@ -70,6 +70,12 @@ __global__ __launch_bounds__(256) void k_hash_boxes_count(HashArgs a) {
   const int x0 = cell_of(lo.x, a.origin.x, a.inv_h), x1 = cell_of(hi.x, a.origin.x, a.inv_h);
   const int y0 = cell_of(lo.y, a.origin.y, a.inv_h), y1 = cell_of(hi.y, a.origin.y, a.inv_h);
   const int z0 = cell_of(lo.z, a.origin.z, a.inv_h), z1 = cell_of(hi.z, a.origin.z, a.inv_h);
+  const int64_t span = ((int64_t)x1 - x0 + 1) * ((int64_t)y1 - y0 + 1) * ((int64_t)z1 - z0 + 1);
+  if (!(span >= 1 && span <= HASH_MAX_CELLS)) {  // direct list (ntri entries: never full)
+    a.big[atomicAdd(a.big_cnt, 1u)] = gt;
+    a.ncells[gt] = 0;
+    return;
+  }
   uint32_t n = 0;
   for (int cz = z0; cz <= z1; ++cz)
     for (int cy = y0; cy <= y1; ++cy)
@@ -85,6 +91,7 @@ __global__ __launch_bounds__(256) void k_hash_fill(HashArgs a) {
   const uint32_t gt = blockIdx.x * 256 + threadIdx.x;
   if (gt >= a.ntri) return;
   const TriRef r = tri_ref(a, gt);
+  if (a.ncells[gt] == 0) return;  // on the direct list
   const float* B = a.boxes + 6 * (size_t)gt;
   const uint32_t grp = a.mesh_group[r.mesh];
   const int x0 = cell_of(B[0], a.origin.x, a.inv_h), x1 = cell_of(B[3], a.origin.x, a.inv_h);
@@ -106,6 +113,8 @@ int launch_hash_build(hipStream_t s, const HashArgs& a) {
   hipLaunchKernelGGL(k_zero_u32, dim3(zb), dim3(256), 0, s, a.count, nslot);
   YSIM_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_zero_u32, dim3(zb), dim3(256), 0, s, a.cursor, nslot);
+  YSIM_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, a.big_cnt, 1u);
   YSIM_CHECK_LAUNCH();
   if (!a.ntri) return YSIM_OK;
   hipLaunchKernelGGL(k_hash_boxes_count, dim3(ceil_div(a.ntri, 256)), dim3(256), 0, s, a);
@@ -210,6 +219,34 @@ __global__ __launch_bounds__(128) void k_hash_query(HashArgs h, QueryArgs a) {
     const int x0 = cell_of(qmin.x, h.origin.x, h.inv_h), x1 = cell_of(qmax.x, h.origin.x, h.inv_h);
     const int y0 = cell_of(qmin.y, h.origin.y, h.inv_h), y1 = cell_of(qmax.y, h.origin.y, h.inv_h);
     const int z0 = cell_of(qmin.z, h.origin.z, h.inv_h), z1 = cell_of(qmax.z, h.origin.z, h.inv_h);
+    // a triangle whose box overlaps the query box: the group, self-collision
+    // and incident-facet rules, then the narrow phase (or a pair record)
+    auto candidate = [&](uint32_t gt) {
+      const TriRef r = tri_ref(h, gt);
+      if (h.mesh_group[r.mesh] != grp) return;  // the key keeps only 4 group bits
+      const bool self = (r.mesh == m);
+      if (self && !h.self_collision) return;
+      const uint32_t* f = a.facets + 3 * (size_t)(a.mt.foff[r.mesh] + r.fid);
+      const uint32_t f0 = f[0], f1 = f[1], f2 = f[2];
+      if (self && (p == f0 || p == f1 || p == f2)) return;
+      ++npair;
+      if (a.audit) audit_lane(a, j, r.mesh, r.fid);
+      if (EMIT_PAIRS) {
+        uint32_t slot = atomicAdd(a.npairs, 1u);
+        if (slot < a.pair_cap) reinterpret_cast<uint4*>(a.pairs)[slot] = make_uint4(m, p, r.mesh, r.fid);
+        return;
+      }
+      const uint32_t tvb = a.mt.voff[r.mesh];
+      f3 nrm;
+      float d;
+      if (narrow_test(qcur, qprev, ld3(a.x, tvb + f0), ld3(a.x, tvb + f1), ld3(a.x, tvb + f2), band, nrm, d)) {
+        if (cnt < cap) {
+          a.cnd[(size_t)j * cap + cnt] = make_float4(nrm.x, nrm.y, nrm.z, d);
+          a.cid[(size_t)j * cap + cnt] = make_uint2(r.mesh, r.fid);
+        }
+        ++cnt;
+      }
+    };
     for (int cz = z0; collidable && cz <= z1; ++cz)
       for (int cy = y0; cy <= y1; ++cy)
         for (int cx = x0; cx <= x1; ++cx) {
@@ -227,33 +264,16 @@ __global__ __launch_bounds__(128) void k_hash_query(HashArgs h, QueryArgs a) {
             if (cell_of(imin.x, h.origin.x, h.inv_h) != cx || cell_of(imin.y, h.origin.y, h.inv_h) != cy ||
                 cell_of(imin.z, h.origin.z, h.inv_h) != cz)
               continue;
-            const TriRef r = tri_ref(h, E.tri);
-            if (h.mesh_group[r.mesh] != grp) continue;  // the key keeps only 4 group bits
-            const bool self = (r.mesh == m);
-            if (self && !h.self_collision) continue;
-            const uint32_t* f = a.facets + 3 * (size_t)(a.mt.foff[r.mesh] + r.fid);
-            const uint32_t f0 = f[0], f1 = f[1], f2 = f[2];
-            if (self && (p == f0 || p == f1 || p == f2)) continue;
-            ++npair;
-            if (a.audit) audit_lane(a, j, r.mesh, r.fid);
-            if (EMIT_PAIRS) {
-              uint32_t slot = atomicAdd(a.npairs, 1u);
-              if (slot < a.pair_cap) reinterpret_cast<uint4*>(a.pairs)[slot] = make_uint4(m, p, r.mesh, r.fid);
-              continue;
-            }
-            const uint32_t tvb = a.mt.voff[r.mesh];
-            f3 nrm;
-            float d;
-            if (narrow_test(qcur, qprev, ld3(a.x, tvb + f0), ld3(a.x, tvb + f1), ld3(a.x, tvb + f2), band, nrm,
-                            d)) {
-              if (cnt < cap) {
-                a.cnd[(size_t)j * cap + cnt] = make_float4(nrm.x, nrm.y, nrm.z, d);
-                a.cid[(size_t)j * cap + cnt] = make_uint2(r.mesh, r.fid);
-              }
-              ++cnt;
-            }
+            candidate(E.tri);
           }
         }
+    // triangles too large for the table, tested directly (each reported once)
+    const uint32_t nbig = collidable ? min(*h.big_cnt, h.ntri) : 0u;
+    for (uint32_t k = 0; k < nbig; ++k) {
+      const uint32_t gt = h.big[k];
+      const float* B = h.boxes + 6 * (size_t)gt;
+      if (overlap(qmin, qmax, f3{B[0], B[1], B[2]}, f3{B[3], B[4], B[5]})) candidate(gt);
+    }
     if (!EMIT_PAIRS) {
       a.ccount[j] = cnt;
       if (cnt > cap) atomicOr(a.flags, 2u);

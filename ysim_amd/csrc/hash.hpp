@@ -34,7 +34,14 @@ struct HashArgs {
   f3 origin = {0.f, 0.f, 0.f};
   float inv_h = 1.f;                   // 1 / cell edge
   uint32_t self_collision = 1;
+  // triangles whose box touches more than HASH_MAX_CELLS cells are not put in
+  // the table: they are listed here and every query tests them directly
+  uint32_t* big = nullptr;             // ntri entries
+  uint32_t* big_cnt = nullptr;         // [0] listed this build
 };
+// Cells a triangle may occupy in the table (a cloth thrown to the world bound
+// spans ~10^7 cells per axis; such boxes take the direct list instead).
+constexpr int64_t HASH_MAX_CELLS = 4096;
 
 int launch_hash_build(hipStream_t s, const HashArgs& a);
 int launch_hash_query(hipStream_t s, const HashArgs& h, const QueryArgs& a, bool emit_pairs);

@@ -155,7 +155,7 @@ __device__ __forceinline__ void tile_cloth(const ClothArgs& ca, bool active, uin
 }
 
 // One wave per 64 consecutive query items.  Waves whose items span two query
-// segments take the per-lane walk.  When the leaf list fills up, the leaves
+// segments, or that hold a non-finite query box, take the per-lane walk.  When the leaf list fills up, the leaves
 // found so far go through phases 2-3 and the walk goes on.  When the walk's
 // pending list outgrows its buffer, the (wave, target) is redone per lane from
 // the slot counts it started with (the slots it filled are overwritten).
@@ -187,8 +187,20 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
   const uint32_t sg = seg_of(a.qsegs, last);  // wave-uniform from here on
   uint32_t npair = 0;
   int2* fstk = s_buf + lane;
-  if (seg_of(a.qsegs, base) != sg) {
-    // mixed segments: per-lane walk over this lane's own targets
+  // a wave whose items span two query segments, or that holds a non-finite
+  // query box (a NaN box overlaps every box under the closed test; the union
+  // box would ignore it), walks per lane over each lane's own targets
+  bool per_lane = seg_of(a.qsegs, base) != sg;
+  if (!per_lane) {
+    const uint32_t mw = a.qsegs.mesh[sg];
+    const f3 q = active ? ld3(a.x, a.mt.voff[mw] + query_vertex(a, sg, mw, tq)) : f3{0.f, 0.f, 0.f};
+    const float mg = a.mesh_margin[mw];
+    const bool finite = __builtin_isfinite(q.x - mg) && __builtin_isfinite(q.y - mg) &&
+                        __builtin_isfinite(q.z - mg) && __builtin_isfinite(q.x + mg) &&
+                        __builtin_isfinite(q.y + mg) && __builtin_isfinite(q.z + mg);
+    per_lane = __ballot(!finite) != 0;
+  }
+  if (per_lane) {
     uint32_t jl = 0, cnt = 0;
     if (active) {
       const uint32_t sl = seg_of(a.qsegs, tq);

@@ -332,6 +332,18 @@ int ysim_hip_scene_restore(ysim_scene* scene);
 /* Counters since the last reset: out[0] candidate pairs tested by the fused
  * query, out[1] substeps run, out[2] contacts of the last substep. */
 int ysim_hip_scene_counters(ysim_scene* scene, uint64_t* h_out, uint32_t reset);
+/* The per-call collision arguments of a substep: BVH::detectCollisions(margin,
+ * self, ...) (include/sim/bvh.hpp:2853) and
+ * BruteForce::narrowAndSortByVertices(radius, thickness, ...)
+ * (include/sim/bruteforce.hpp:311).  Sets the scene's margin, radius,
+ * narrow_thickness and self_collision (ysim_sim_params meanings; margin <= 0
+ * keeps each cloth's own thickness) for this and later substeps; callable
+ * before the substep's query (YSIM_ERR_STATE after it).  Unchanged values
+ * cost nothing; new values are written into device arrays that a captured
+ * graph reads, and a change of self_collision rebuilds the target lists and
+ * drops the graph.  Hash scenes reject a margin above 7 cells. */
+int ysim_hip_scene_set_collision(ysim_scene* scene, float margin, float radius, float narrow_thickness,
+                                 uint32_t self_collision);
 /* No counterpart in the reference (diagnostics).  Graph replays
  * (scene_step with use_graph) leave out the incremental sort's fallback
  * kernels: a rebuild whose order check fails keeps the previous topology,
@@ -410,7 +422,8 @@ typedef struct ysim_analytic_shape {
  * (moving colliders: pass the previous pose in prev_*) and keeps the
  * previous-substep pairing; another sequence resets it; n = 0 removes them
  * all.  With broad_mode 1 (spatial hash) the shapes are stored but inactive
- * and the meshes collide through their triangles, as in the reference. */
+ * and the meshes collide through their triangles, as in the reference.  In a
+ * staged substep it may be called until scene_query (YSIM_ERR_STATE after). */
 int ysim_hip_scene_set_analytic(ysim_scene* scene, const ysim_analytic_shape* h_shapes, uint32_t n);
 
 /* ---- measurement ------------------------------------------------------------

@@ -72,6 +72,7 @@ def lib():
         "yo_scene_destroy": (None, [vp]),
         "yo_scene_enable_twin": (None, [vp]),
         "yo_scene_enable_ties": (None, [vp]),
+        "yo_scene_set_collision": (None, [vp, C.c_float, C.c_float, C.c_float, C.c_uint32]),
         "yo_scene_ties": (None, [vp, u64p]),
         "yo_scene_record_times": (None, [vp, C.c_int]),
         "yo_scene_step_times": (C.c_uint32, [vp, f64p, C.c_uint32]),
@@ -370,6 +371,10 @@ class OracleScene:
     def set_state(self, x=None, v=None, xprev=None):
         a = [None if t is None else np.ascontiguousarray(t, np.float32).reshape(-1) for t in (x, v, xprev)]
         lib().yo_scene_set_state(self.h, P(a[0], f32p), P(a[1], f32p), P(a[2], f32p))
+
+    def set_collision(self, margin: float, radius: float, narrow_thickness: float, self_collision: int):
+        """Collision arguments of later substeps (the library's scene_set_collision)."""
+        lib().yo_scene_set_collision(self.h, margin, radius, narrow_thickness, int(self_collision))
 
     def record_times(self, on: bool = True):
         """Record each later substep's per-stage wall time (clears the record)."""

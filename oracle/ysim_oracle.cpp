@@ -1377,6 +1377,16 @@ void* yo_scene_create(uint32_t nm, const ysim_mesh_desc* meshes, const float* x,
 
 void yo_scene_destroy(void* s) { delete (OScene*)s; }
 
+// The per-call collision arguments of later substeps (the library's
+// scene_set_collision): margin, radius, narrow thickness, self collision.
+void yo_scene_set_collision(void* s, float margin, float radius, float narrow_thickness, uint32_t self_collision) {
+  OScene* S = (OScene*)s;
+  S->P.margin = margin;
+  S->P.radius = radius;
+  S->P.narrow_thickness = narrow_thickness;
+  S->P.self_collision = self_collision;
+}
+
 // tie_bits of one candidate given as float triples (checks of the classifier).
 uint32_t yo_tie_bits(const float* qc, const float* qp, const float* t0, const float* t1, const float* t2,
                      float band, int fired32) {

@@ -12,6 +12,18 @@
 
 #include "../../ysim_amd/host/ysim_hip_backend.hpp"
 
+// The CPU oracle (oracle/liboracle.so, test infrastructure): the same packed
+// scene stepped on the host, the reference for bit-exact comparisons.
+extern "C" {
+void* yo_scene_create(uint32_t nm, const ysim_mesh_desc* meshes, const float* x, const float* v, const float* mass,
+                      const float* ext, const float* fixed, const uint32_t* facets, const ysim_sim_params* P);
+void yo_scene_destroy(void* s);
+void yo_scene_step(void* s, uint32_t n);
+void yo_scene_get_state(void* s, float* x, float* v, float* xprev);
+int yo_scene_set_analytic(void* s, const ysim_analytic_shape* shapes, uint32_t n);
+void yo_scene_set_collision(void* s, float margin, float radius, float narrow_thickness, uint32_t self_collision);
+}
+
 static int failures = 0;
 static void report(bool ok, const char* name) {
   std::fprintf(stderr, "%s %s\n", ok ? "PASS" : "FAIL", name);
@@ -214,6 +226,68 @@ int main() {
     T2.update();
     std::vector<float> z1 = T1.scene.positions(), z2 = T2.scene.positions();
     report(before == after && z1 == z2 && t.ms_per_substep > 0.f, "Simulator<HIP>::autotune / results unchanged");
+  }
+
+  // The reference's default scene: a cloth over a sphere whose collider is the
+  // analytic sphere (scene.hpp:326-331), at the reference's rebuild cadence
+  // (frame % 10 == 0, substep 0: simulator.hpp:3253), with the per-call
+  // margin and radius changed mid-run; bit-exact against the oracle.
+  {
+    SymplecticSystem<HIP, float> sysA2(1.0f / 60.f, 10);
+    Simulator<HIP, float> S(sysA2);
+    const float l = 1.0f / (n - 1);
+    const float center[3] = {0.5f, 0.8f, 0.5f};
+    std::vector<float> jitter(n * n);
+    for (uint32_t i = 0; i < n * n; ++i) jitter[i] = 1e-3f * l * std::cos(0.3f * i);
+    S.scene.addClothGridFastAt(n, 1.0f, center, 500.f, 50.f, 5.f, 0.5f * l, 1.0f, &jitter);
+    std::vector<float> sx;
+    std::vector<uint32_t> sf;
+    const float sc[3] = {0.5f, 0.5f, 0.5f};
+    uv_sphere(24, 12, 0.3f, sc, sx, sf);
+    S.scene.addFloatMesh(sx, sf, YSIM_COLLIDER_SPHERE);
+    S.scene.addAnalyticSphere(1, sc, 0.3f);
+    S.margin = 0.5f * l;
+    S.radius = 0.f;
+    S.system.kd = 0.01f;
+    S.enableRefit = true;  // the reference cadence
+    S.initialize();
+    const Scene<HIP, float>::Packed pk = S.scene.packed();
+    const ysim_sim_params P = S.params();
+    void* O = yo_scene_create((uint32_t)pk.descs.size(), pk.descs.data(), pk.x.data(), pk.v.data(), pk.mass.data(),
+                              pk.ext.data(), pk.fixed.data(), pk.facets.data(), &P);
+    report(O && yo_scene_set_analytic(O, S.scene.analyticShapes.data(), (uint32_t)S.scene.analyticShapes.size()) == 0,
+           "analytic sphere / oracle scene created");
+    std::vector<float> xo(pk.x.size());
+    size_t fired = 0;
+    bool same = true;
+    for (int frame = 0; frame < 12; ++frame) {
+      if (frame == 6) {  // a caller changes Simulator::margin and radius between frames
+        S.margin = 0.75f * l;
+        S.radius = 0.002f;
+        yo_scene_set_collision(O, S.margin, S.radius, S.margin, 1u);
+      }
+      S.update();
+      yo_scene_step(O, (uint32_t)S.system.subSteps);
+      std::vector<uint32_t> off;
+      std::vector<ysim_narrow_collision> c;
+      S.narrowPhase.contacts(off, c);
+      fired += c.size();
+      yo_scene_get_state(O, xo.data(), nullptr, nullptr);
+      std::vector<float> xg = S.scene.positions();
+      same = same && std::memcmp(xg.data(), xo.data(), 4 * xo.size()) == 0;
+    }
+    yo_scene_destroy(O);
+    report(P.rebuild_every == 100u, "Simulator<HIP> / enableRefit maps to the frame % 10 rebuild cadence");
+    report(fired > 0, "analytic sphere / contacts fire");
+    report(same, "analytic sphere / 12 frames across the frame-10 rebuild and a margin change, bit-exact vs oracle");
+    bool threw = false;
+    try {
+      S.broadPhase.refit();
+      S.broadPhase.detectCollisions(S.margin, true, true, /*resetCounter*/ false);
+    } catch (const std::invalid_argument&) {
+      threw = true;
+    }
+    report(threw, "BVH<HIP>::detectCollisions / appending to another pass's pairs is refused");
   }
 
   std::fprintf(stderr, "%s: %d failure(s)\n", failures ? "FAILED" : "ALL PASSED", failures);

@@ -100,3 +100,22 @@ def test_snapshot_restore_replays_identically(ctx):
     b = hs.state()
     for u, w in zip(a, b):
         assert np.array_equal(u, w)
+
+
+def test_set_collision_between_replays(ctx, oracle_threads):
+    """Per-call margin / radius / thickness / self collision
+    (detectCollisions, bvh.hpp:2853; narrowAndSortByVertices, bruteforce.hpp:311)
+    changed between graph replays: the replayed graph reads the new values, a
+    self-collision toggle re-captures it; states equal the oracle's."""
+    sc = scenes.config_drape(n=32, height=0.8, rebuild_every=1)
+    hs, os_ = HipScene(ctx, sc), OracleScene(sc)
+    m0 = sc.params["margin"]
+    for margin, radius, thick, self_ in ((m0, 0.0, m0, 1), (1.5 * m0, 0.002, 1.2 * m0, 1), (1.5 * m0, 0.002, m0, 0),
+                                         (m0, 0.0, m0, 1)):
+        hs.set_collision(margin, radius, thick, self_)
+        os_.set_collision(margin, radius, thick, self_)
+        hs.step(10, graph=True)
+        os_.step(10)
+        assert_same_state(hs, os_)
+        assert_same_contacts(hs, os_)
+    assert hs.flags() == 0

@@ -127,6 +127,8 @@ def test_run_config_engine_and_profile():
     assert cfg["profile"]["enabled"] and cfg["profile"]["frames"] == 20 and cfg["profile"]["level"] == "in_frame"
     with pytest.raises(S.SceneFormatError, match="unknown engine.backend 'TPU'"):
         S.parse_run_config(dict(_scene(), engine={"backend": "TPU"}))
+    # the HIP tag is accepted where the reference's isKnownBackend is extended
+    assert S.parse_run_config(dict(_scene(), engine={"backend": "HIP"}))["engine"]["backend"] == "HIP"
     with pytest.raises(S.SceneFormatError, match="profile.frames must be >= 1"):
         S.parse_run_config(dict(_scene(), profile={"frames": 0}))
 

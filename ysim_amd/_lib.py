@@ -69,6 +69,7 @@ _SIG = {
     "ysim_hip_scene_get_contacts": (C.c_int, [vp, vp, vp, u32, u32p]),
     "ysim_hip_scene_get_pairs": (C.c_int, [vp, vp, u32, u32p]),
     "ysim_hip_scene_set_audit": (C.c_int, [vp, u32]),
+    "ysim_hip_scene_set_collision": (C.c_int, [vp, C.c_float, C.c_float, C.c_float, u32]),
     "ysim_hip_scene_get_audit": (C.c_int, [vp, vp, u32, u32p, u32p]),
     "ysim_hip_scene_get_cut": (C.c_int, [vp, u32, vp, u32, u32p]),
     "ysim_hip_scene_get_tree": (C.c_int, [vp, u32, vp, vp, vp, vp]),

@@ -10,6 +10,7 @@
 // is exactly the xPrev the swept narrow phase needs.  A hipGraph captures a
 // whole period of substeps (even length, so buffer roles repeat).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -574,6 +575,54 @@ struct ysim_scene {
 
   int set_analytic(const ysim_analytic_shape* h, uint32_t n);
 
+  // Per-query-mesh broad margin, narrow band and contact thickness from the
+  // scene parameters (margin <= 0: each cloth's own thickness).
+  void collision_arrays(std::vector<float>& margin, std::vector<float>& band, std::vector<float>& thick) const {
+    margin.resize(nm);
+    band.resize(nm);
+    thick.resize(nm);
+    for (uint32_t m = 0; m < nm; ++m) {
+      margin[m] = P.margin > 0.f ? P.margin : meshes[m].cloth.thickness;
+      band[m] = P.radius + (P.margin > 0.f ? P.narrow_thickness : meshes[m].cloth.thickness);
+      thick[m] = P.margin > 0.f ? P.narrow_thickness : meshes[m].cloth.thickness;
+    }
+  }
+
+  // detectCollisions(margin, self) / narrowAndSortByVertices(radius, thickness)
+  // arguments of one substep: new values are written into the device arrays
+  // the query reads (a captured graph stays valid); a change of self
+  // collision rebuilds the target lists and drops the graph.
+  int set_collision(float margin, float radius, float thick, uint32_t self) {
+    if (stage_phase == 2) return fail(YSIM_ERR_STATE, "scene_set_collision: the substep's query already ran");
+    YSIM_ARG(std::isfinite(margin) && std::isfinite(radius) && std::isfinite(thick) && radius >= 0.f && thick >= 0.f,
+             "scene_set_collision: margin, radius and thickness must be finite, radius and thickness >= 0");
+    YSIM_ARG(self <= 1, "scene_set_collision: self_collision must be 0 or 1");
+    if (margin == P.margin && radius == P.radius && thick == P.narrow_thickness && self == P.self_collision)
+      return YSIM_OK;
+    if (hash_mode) {
+      float mx = 0.f;
+      for (uint32_t m : dyn) mx = std::max(mx, margin > 0.f ? margin : meshes[m].cloth.thickness);
+      YSIM_ARG(7.f / hargs.inv_h >= mx, "scene_set_collision: margin exceeds 7 hash cells");
+    }
+    const bool self_changed = self != P.self_collision;
+    P.margin = margin;
+    P.radius = radius;
+    P.narrow_thickness = thick;
+    P.self_collision = self;
+    std::vector<float> mg, bd, th;
+    collision_arrays(mg, bd, th);
+    YSIM_HIP_TRY(hipMemcpyAsync(d_margin, mg.data(), 4 * (size_t)nm, hipMemcpyHostToDevice, st));
+    YSIM_HIP_TRY(hipMemcpyAsync(d_band, bd.data(), 4 * (size_t)nm, hipMemcpyHostToDevice, st));
+    YSIM_HIP_TRY(hipMemcpyAsync(d_thick, th.data(), 4 * (size_t)nm, hipMemcpyHostToDevice, st));
+    YSIM_HIP_TRY(hipStreamSynchronize(st));
+    if (self_changed) {
+      hargs.self_collision = self;
+      YSIM_TRY(build_targets());
+      drop_graph();
+    }
+    return YSIM_OK;
+  }
+
   int enqueue_analytic(uint32_t par) {
     if (!analytic_on() || !napairs) return YSIM_OK;
     AnalyticArgs a;
@@ -1058,7 +1107,8 @@ AShape analytic_consts(const ysim_analytic_shape& sh) {
 }  // namespace
 
 int ysim_scene::set_analytic(const ysim_analytic_shape* h, uint32_t n) {
-  if (stage_phase != 0) return fail(YSIM_ERR_STATE, "scene_set_analytic: a staged substep is in progress");
+  // shapes feed the query, not the refit: a staged substep may set them until its query
+  if (stage_phase == 2) return fail(YSIM_ERR_STATE, "scene_set_analytic: the substep's query already ran");
   YSIM_ARG(n == 0 || h, "scene_set_analytic: null shapes");
   std::vector<int> so(nm, -1);
   for (uint32_t k = 0; k < n; ++k) {
@@ -1661,16 +1711,15 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
   s.mt = MeshTables{dv, df, dn};
   std::vector<ysim_fastgrid_params> cloth(num_meshes);
   std::vector<uint32_t> nf(num_meshes), beh(num_meshes), shp(num_meshes);
-  std::vector<float> margin(num_meshes), band(num_meshes);
+  std::vector<float> margin, band, thick;
   for (uint32_t m = 0; m < num_meshes; ++m) {
     cloth[m] = s.meshes[m].cloth;
     nf[m] = s.meshes[m].num_facets;
     beh[m] = s.meshes[m].behavior;
     shp[m] = s.meshes[m].shape;
-    // margin <= 0: per-mesh margin and band = the cloth's thickness (batching)
-    margin[m] = P.margin > 0.f ? P.margin : s.meshes[m].cloth.thickness;
-    band[m] = P.radius + (P.margin > 0.f ? P.narrow_thickness : s.meshes[m].cloth.thickness);
   }
+  // margin <= 0: per-mesh margin and band = the cloth's thickness (batching)
+  s.collision_arrays(margin, band, thick);
   YSIM_TRY(s.put(&s.d_cloth, cloth));
   YSIM_TRY(s.put(&s.d_nfacets, nf));
   YSIM_TRY(s.put(&s.d_behavior, beh));
@@ -1719,12 +1768,7 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
   YSIM_TRY(s.put(&s.d_shape, shp));
   YSIM_TRY(s.put(&s.d_margin, margin));
   YSIM_TRY(s.put(&s.d_band, band));
-  {
-    std::vector<float> thick(num_meshes);
-    for (uint32_t m = 0; m < num_meshes; ++m)
-      thick[m] = P.margin > 0.f ? P.narrow_thickness : s.meshes[m].cloth.thickness;
-    YSIM_TRY(s.put(&s.d_thick, thick));
-  }
+  YSIM_TRY(s.put(&s.d_thick, thick));
   YSIM_TRY(s.alloc(&s.tree, s.Ntot));
   YSIM_TRY(s.alloc(&s.parent, s.Ntot));
   YSIM_TRY(s.alloc(&s.visit, s.Ntot));
@@ -2257,6 +2301,12 @@ int ysim_hip_scene_get_cut(ysim_scene* s, uint32_t mesh, int32_t* h_nodes, uint3
 int ysim_hip_scene_set_analytic(ysim_scene* s, const ysim_analytic_shape* h_shapes, uint32_t n) {
   YSIM_ARG(s, "null scene");
   return s->set_analytic(h_shapes, n);
+}
+
+int ysim_hip_scene_set_collision(ysim_scene* s, float margin, float radius, float narrow_thickness,
+                                 uint32_t self_collision) {
+  YSIM_ARG(s, "null scene");
+  return s->set_collision(margin, radius, narrow_thickness, self_collision);
 }
 
 int ysim_hip_scene_flags(ysim_scene* s, uint32_t* h_flags) {

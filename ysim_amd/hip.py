@@ -290,6 +290,11 @@ class HipScene:
         call("ysim_hip_scene_get_pairs", self.h, rec.ctypes.data, n.value, C.byref(n))
         return np.stack([rec["query_obj"], rec["point"], rec["target_obj"], rec["triangle"]], 1)
 
+    def set_collision(self, margin: float, radius: float, narrow_thickness: float, self_collision: int):
+        """detectCollisions / narrowAndSortByVertices arguments of later substeps."""
+        call("ysim_hip_scene_set_collision", self.h, float(margin), float(radius), float(narrow_thickness),
+             int(self_collision))
+
     def set_audit(self, capacity: int):
         """Record every candidate pair of each later substep's query (the
         kernel the tuning selects, eager or graph); 0 turns it off."""

@@ -195,8 +195,14 @@ struct Scene<HIP, PR> {
     float mass = 1.0f;             // per vertex
     bool collidable = true;
     uint32_t group = 0;
+    uint32_t shape = YSIM_COLLIDER_MESH;  // ColliderKind lane (collision_pod.hpp:32)
   };
   std::vector<Mesh> meshes;
+  // AnalyticShape records of the analytic colliders (Scene::refreshAnalyticShapes,
+  // scene.hpp:659-745): set them (poses included) each frame; the broad phase
+  // installs them when detectCollisions is asked for analytic colliders.
+  std::vector<ysim_analytic_shape> analyticShapes;
+  bool analyticActive() const { return !analyticShapes.empty(); }
   float gravity[3] = {0.f, -9.81f, 0.f};   // SceneEnvironment.gravity (scene.hpp:187)
   ysim_scene* handle = nullptr;
   uint32_t numMeshes() const { return (uint32_t)meshes.size(); }
@@ -259,44 +265,77 @@ struct Scene<HIP, PR> {
     m.cloth.thickness = thickness;
     meshes.push_back(std::move(m));
   }
-  // A static triangle obstacle (BehaviorType::Float, ColliderKind::Mesh).
-  void addFloatMesh(std::vector<float> x, std::vector<uint32_t> facets) {
+  // A static triangle obstacle (BehaviorType::Float).  shape: its ColliderKind
+  // (YSIM_COLLIDER_*; not Mesh for a mesh answered by an analytic collider).
+  void addFloatMesh(std::vector<float> x, std::vector<uint32_t> facets, uint32_t shape = YSIM_COLLIDER_MESH) {
     Mesh m;
     m.x = std::move(x);
     m.facets = std::move(facets);
+    m.shape = shape;
     meshes.push_back(std::move(m));
   }
+  // The analytic sphere of a static mesh (the reference's default collider of
+  // a sphere object, scene.hpp:326-331): centre c and radius r, at rest.
+  void addAnalyticSphere(uint32_t meshIndex, const float c[3], float r) {
+    if (meshIndex >= meshes.size() || meshes[meshIndex].shape != YSIM_COLLIDER_SPHERE)
+      throw std::invalid_argument("Scene<HIP>::addAnalyticSphere: the mesh must be a static sphere-collider mesh");
+    ysim_analytic_shape a{};
+    a.kind = YSIM_COLLIDER_SPHERE;
+    a.obj_index = meshIndex;
+    a.behavior_type = YSIM_BEHAVIOR_FLOAT;
+    for (int k = 0; k < 3; ++k) {
+      a.center_radius[k] = c[k];
+      a.prev_center_pad[k] = c[k];
+      a.half_ext_height[k] = r;
+    }
+    a.rot_quat[0] = a.prev_rot_quat[0] = 1.f;
+    a.flags = 1u;  // collidable
+    analyticShapes.push_back(a);
+  }
 
-  // Scene::pack (scene.hpp:783-1182) + creation of the resident scene.
-  void pack(const ysim_sim_params& params) {
-    release();
-    std::vector<ysim_mesh_desc> d(meshes.size());
+  // Scene::pack (scene.hpp:783-1182): the packed buffers the resident scene
+  // is created from.
+  struct Packed {
+    std::vector<ysim_mesh_desc> descs;
     std::vector<float> x, v, mass, ext, fixed;
     std::vector<uint32_t> facets;
+  };
+  Packed packed() const {
+    Packed pk;
+    pk.descs.resize(meshes.size());
     for (size_t i = 0; i < meshes.size(); ++i) {
       const Mesh& m = meshes[i];
       const uint32_t nv = (uint32_t)(m.x.size() / 3);
-      d[i] = ysim_mesh_desc{};
-      d[i].num_vertices = nv;
-      d[i].num_facets = (uint32_t)(m.facets.size() / 3);
-      d[i].behavior = m.behaviorType == BehaviorType::FastGridCloth     ? YSIM_BEHAVIOR_FASTGRID_CLOTH
-                      : m.behaviorType == BehaviorType::TriangularCloth ? YSIM_BEHAVIOR_TRIANGULAR_CLOTH
-                                                                        : YSIM_BEHAVIOR_FLOAT;
-      d[i].collidable = m.collidable ? 1u : 0u;
-      d[i].group = m.group;
-      d[i].cloth = FastGridClothBehavior<HIP, PR>::params(m);
-      x.insert(x.end(), m.x.begin(), m.x.end());
-      v.insert(v.end(), m.x.size(), 0.f);
-      mass.insert(mass.end(), nv, m.mass);
-      fixed.insert(fixed.end(), nv, 1.f);
+      ysim_mesh_desc& d = pk.descs[i];
+      d = ysim_mesh_desc{};
+      d.num_vertices = nv;
+      d.num_facets = (uint32_t)(m.facets.size() / 3);
+      d.behavior = m.behaviorType == BehaviorType::FastGridCloth     ? YSIM_BEHAVIOR_FASTGRID_CLOTH
+                   : m.behaviorType == BehaviorType::TriangularCloth ? YSIM_BEHAVIOR_TRIANGULAR_CLOTH
+                                                                     : YSIM_BEHAVIOR_FLOAT;
+      d.collidable = m.collidable ? 1u : 0u;
+      d.group = m.group;
+      d.shape = m.shape;
+      d.cloth = FastGridClothBehavior<HIP, PR>::params(m);
+      pk.x.insert(pk.x.end(), m.x.begin(), m.x.end());
+      pk.v.insert(pk.v.end(), m.x.size(), 0.f);
+      pk.mass.insert(pk.mass.end(), nv, m.mass);
+      pk.fixed.insert(pk.fixed.end(), nv, 1.f);
       for (uint32_t k = 0; k < nv; ++k)  // applyEnvironmentForces: ext = g * m (simulator.hpp:3148-3164)
         for (int a = 0; a < 3; ++a)
-          ext.push_back(d[i].behavior != YSIM_BEHAVIOR_FLOAT ? gravity[a] * m.mass : 0.f);
-      facets.insert(facets.end(), m.facets.begin(), m.facets.end());
+          pk.ext.push_back(d.behavior != YSIM_BEHAVIOR_FLOAT ? gravity[a] * m.mass : 0.f);
+      pk.facets.insert(pk.facets.end(), m.facets.begin(), m.facets.end());
     }
-    ysim_hip_detail::check(ysim_hip_scene_create(ysim_hip_detail::global_ctx(), (uint32_t)d.size(), d.data(), x.data(),
-                                                 v.data(), mass.data(), ext.data(), fixed.data(), facets.data(),
-                                                 &params, &handle), "scene_create");
+    return pk;
+  }
+  // ... and the creation of the resident scene from them.
+  void pack(const ysim_sim_params& params) {
+    release();
+    const Packed pk = packed();
+    ysim_hip_detail::check(ysim_hip_scene_create(ysim_hip_detail::global_ctx(), (uint32_t)pk.descs.size(),
+                                                 pk.descs.data(), pk.x.data(), pk.v.data(), pk.mass.data(),
+                                                 pk.ext.data(), pk.fixed.data(), pk.facets.data(), &params, &handle),
+                           "scene_create");
   }
   void release() {
     if (handle) ysim_hip_scene_destroy(handle);
@@ -339,8 +378,10 @@ struct FastGridClothBehavior<HIP, PR> {
 // The broad phase of the HIP scene: per-mesh LBVHs in HBM.  refit() and
 // refitSwept() run the swept leaf + bottom-up pass (a full Morton/sort/Karras
 // rebuild when the scene's rebuild cadence says so); enlargeTrajectory() is
-// folded into that pass.  detectCollisions() only records the request: the
-// traversal runs fused with the narrow phase in BruteForce::narrowAndSortByVertices.
+// folded into that pass.  detectCollisions() records its arguments and
+// installs or removes the analytic colliders; the traversal runs fused with
+// the narrow phase in BruteForce::narrowAndSortByVertices, which hands the
+// substep's margin, radius, thickness and self flag to the scene.
 template <typename PR>
 struct BVH<HIP, PR, SCENE, OBJECT> {
   Scene<HIP, PR>* scene = nullptr;
@@ -348,6 +389,9 @@ struct BVH<HIP, PR, SCENE, OBJECT> {
   bool refitted = false;
   bool detectRequested = false;
   float margin = 0.f;
+  bool self = true;
+  bool analytic = false;
+  bool analyticInstalled = false;
   void build(Scene<HIP, PR>& s) { scene = &s; }  // trees are built by Scene::pack
   void refit() {
     ysim_hip_detail::check(ysim_hip_scene_refit(scene->handle), "scene_refit");
@@ -357,9 +401,30 @@ struct BVH<HIP, PR, SCENE, OBJECT> {
     if (!refitted) refit();  // the swept hull is part of the refit pass
   }
   void refitSwept(PR) { refit(); }
-  void detectCollisions(PR m, bool /*self*/ = true, bool analytic = false, bool /*resetCounter*/ = true) {
-    if (analytic) throw std::invalid_argument("BVH<HIP>: analytic colliders are not supported");
-    margin = m;
+  // detectCollisions(margin, self, analytic, resetCounter) (bvh.hpp:2853).
+  // resetCounter = false asks to append this pass to pairs another source
+  // already wrote in the substep (the reference's CPU self-collision rows);
+  // the fused query produces the substep's whole candidate set itself, so
+  // that is refused.
+  void detectCollisions(PR m, bool selfCollide = true, bool analyticColliders = false, bool resetCounter = true) {
+    if (!resetCounter)
+      throw std::invalid_argument("BVH<HIP>::detectCollisions: resetCounter = false (appending to another "
+                                  "detection's pairs) is not supported; the fused query covers every pair");
+    if (analyticColliders && !scene->analyticActive())
+      throw std::invalid_argument("BVH<HIP>::detectCollisions: analytic colliders requested but the scene has none");
+    if (analyticColliders) {
+      // poses of this frame (same mesh sequence: a captured graph stays valid)
+      ysim_hip_detail::check(ysim_hip_scene_set_analytic(scene->handle, scene->analyticShapes.data(),
+                                                         (uint32_t)scene->analyticShapes.size()),
+                             "scene_set_analytic");
+      analyticInstalled = true;
+    } else if (analyticInstalled) {
+      ysim_hip_detail::check(ysim_hip_scene_set_analytic(scene->handle, nullptr, 0), "scene_set_analytic");
+      analyticInstalled = false;
+    }
+    margin = (float)m;
+    self = selfCollide;
+    analytic = analyticColliders;
     detectRequested = true;
   }
   // Candidate pairs at the current state (for inspection / parity).
@@ -377,9 +442,17 @@ template <typename PR>
 struct BruteForce<HIP, PR> {
   BVH<HIP, PR, SCENE, OBJECT>* broad = nullptr;
   bool syncEachPhase = false;
-  void narrowAndSortByVertices(PR /*radius*/, PR /*thickness*/, bool analytic = false) {
-    if (analytic) throw std::invalid_argument("BruteForce<HIP>: analytic colliders are not supported");
+  // narrowAndSortByVertices(radius, thickness, analytic) (bruteforce.hpp:311):
+  // the band is radius + thickness (strict <), the query box the margin of
+  // the preceding detectCollisions.
+  void narrowAndSortByVertices(PR radius, PR thickness, bool analyticColliders = false) {
     if (!broad || !broad->refitted) throw std::logic_error("BruteForce<HIP>: refit the broad phase first");
+    if (!broad->detectRequested) throw std::logic_error("BruteForce<HIP>: call detectCollisions first");
+    if (analyticColliders != broad->analytic)
+      throw std::invalid_argument("BruteForce<HIP>: the analytic flag differs from the substep's detectCollisions");
+    ysim_hip_detail::check(ysim_hip_scene_set_collision(broad->scene->handle, broad->margin, (float)radius,
+                                                        (float)thickness, broad->self ? 1u : 0u),
+                           "scene_set_collision");
     ysim_hip_detail::check(ysim_hip_scene_query(broad->scene->handle), "scene_query");
     broad->refitted = false;
     broad->detectRequested = false;
@@ -430,12 +503,15 @@ struct Simulator<HIP, PR> {
   PR margin = PR(0.015);
   PR radius = PR(0.012);
   bool enableSelfCollisions = true;
-  bool enableRefit = true;      // false: rebuild the dynamic trees every substep
+  // true: the reference's cadence, a full rebuild at substep 0 of every 10th
+  // frame and refits otherwise (simulator.hpp:3253); false: rebuild the
+  // dynamic trees every substep
+  bool enableRefit = true;
   uint32_t mortonBits = 30;
   explicit Simulator(SymplecticSystem<HIP, PR>& s) : system(s) {}
 
-  // Simulator::initialize (simulator.hpp:2942-3115): pack + initial build.
-  void initialize() {
+  // The scene parameters initialize() packs with.
+  ysim_sim_params params() const {
     ysim_sim_params p{};
     p.subh = (float)system.subh;
     p.kair = (float)system.kair;
@@ -444,10 +520,15 @@ struct Simulator<HIP, PR> {
     p.radius = (float)radius;
     p.narrow_thickness = (float)margin;  // narrowAndSortByVertices(radius, margin) (simulator.hpp:3755)
     p.morton_bits = mortonBits;
-    p.rebuild_every = enableRefit ? 0u : 1u;
+    p.rebuild_every = enableRefit ? (uint32_t)(10 * system.subSteps) : 1u;
     p.self_collision = enableSelfCollisions ? 1u : 0u;
     p.swept = 1;
-    scene.pack(p);
+    return p;
+  }
+
+  // Simulator::initialize (simulator.hpp:2942-3115): pack + initial build.
+  void initialize() {
+    scene.pack(params());
     broadPhase.build(scene);
     narrowPhase.broad = &broadPhase;
   }
@@ -465,8 +546,8 @@ struct Simulator<HIP, PR> {
     for (size_t i = 0; i < system.subSteps; ++i) {
       broadPhase.refit();
       broadPhase.enlargeTrajectory(system.subh);
-      broadPhase.detectCollisions(margin, enableSelfCollisions, false, true);
-      narrowPhase.narrowAndSortByVertices(radius, margin, false);
+      broadPhase.detectCollisions(margin, enableSelfCollisions, scene.analyticActive(), true);
+      narrowPhase.narrowAndSortByVertices(radius, margin, scene.analyticActive());
       system.snapshotXPrev(scene);
       system.update(scene);
     }

@@ -62,7 +62,9 @@ FORMAT_VERSION = 1
 KNOWN_BEHAVIORS = ("Float", "TriangularCloth", "FastGridCloth", "Rigid")
 RESERVED_BEHAVIORS = ("Elastic", "Fluid", "Generator")
 KNOWN_SHAPES = ("grid", "sphere", "cube", "cylinder")
-KNOWN_BACKENDS = ("METAL", "CPU", "CUDA")
+# the reference's isKnownBackend (include/sim_config.hpp:100-102) plus the tag
+# this backend adds beside them
+KNOWN_BACKENDS = ("METAL", "CPU", "CUDA", "HIP")
 KNOWN_SYSTEMS = ("Explicit", "PBD", "PD", "LargeSteps")
 PROFILE_LEVELS = ("none", "per_frame", "in_frame")
 COLLIDER_NAMES = {"mesh": 0, "sphere": 1, "box": 2, "cylinder": 3, "plane": 4}
@@ -370,7 +372,7 @@ def parse_run_config(data) -> dict:
             if not isinstance(e["backend"], str):
                 raise SceneFormatError("engine.backend must be a string")
             if e["backend"] not in KNOWN_BACKENDS:
-                raise SceneFormatError(f"unknown engine.backend '{e['backend']}' (expected METAL|CPU|CUDA)")
+                raise SceneFormatError(f"unknown engine.backend '{e['backend']}' (expected METAL|CPU|CUDA|HIP)")
             cfg["engine"]["backend"] = e["backend"]
         if "system" in e:
             if not isinstance(e["system"], str):

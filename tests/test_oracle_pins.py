@@ -147,6 +147,126 @@ def test_karras_matches_topdown(bits, dups):
     assert np.array_equal(tree[n - 1:]["child_b"], i.astype(np.int32))
 
 
+class MidpointKarras:
+    """The reference's own build rule, restated for n small: delta and
+    determineRange (src/metal/bvh.metal:226-258) with the index tie-break, and
+    findSplit (bvh.metal:197-224), which returns the midpoint when the range's
+    end codes are equal and otherwise measures prefixes on the codes alone.
+    The child numbering follows buildTree_Tri (bvh.metal:383-392)."""
+
+    def __init__(self, keys, idx):
+        self.k = [int(v) for v in keys]
+        self.i = [int(v) for v in idx]
+        self.n = len(self.k)
+
+    @staticmethod
+    def clz(v):
+        return 32 - int(v).bit_length()
+
+    def delta(self, a, b):
+        if b < 0 or b >= self.n:
+            return -1
+        if self.k[a] != self.k[b]:
+            return self.clz(self.k[a] ^ self.k[b])
+        return 32 + self.clz(self.i[a] ^ self.i[b])
+
+    def range(self, x):
+        d = 1 if self.delta(x, x + 1) - self.delta(x, x - 1) >= 0 else -1
+        dmin = self.delta(x, x - d)
+        lmax = 2
+        while self.delta(x, x + lmax * d) > dmin:
+            lmax <<= 1
+        length, t = 0, lmax >> 1
+        while t >= 1:
+            if self.delta(x, x + (length + t) * d) > dmin:
+                length += t
+            t >>= 1
+        j = x + length * d
+        return (j, x) if d < 0 else (x, j)
+
+    def split(self, lo, hi):
+        c0, c1 = self.k[lo], self.k[hi]
+        if c0 == c1:
+            return (lo + hi) >> 1
+        common = self.clz(c0 ^ c1)
+        s, step = lo, hi - lo
+        while True:
+            step = (step + 1) >> 1
+            ns = s + step
+            if ns < hi and self.clz(c0 ^ self.k[ns]) > common:
+                s = ns
+            if step <= 1:
+                return s
+
+    def children(self, node):
+        lo, hi = self.range(node)
+        s = self.split(lo, hi)
+        ca = s + self.n - 1 if s == lo else s
+        cb = s + self.n if s + 1 == hi else s + 1
+        return (lo, hi), (ca, cb)
+
+
+def claims(children, n):
+    """Parent count of every node slot 0 .. 2n-2."""
+    c = np.zeros(2 * n - 1, np.int64)
+    for ca, cb in children:
+        c[ca] += 1
+        c[cb] += 1
+    return c
+
+
+def test_midpoint_findsplit_double_claims_on_equal_runs():
+    """Five equal codes: the reference's midpoint split makes internal node 2
+    a child of both node 0 and node 3 and leaves leaf 4 without a parent; the
+    oracle (prefix by delta, index tie-break) gives every slot but the root
+    exactly one parent.  Runs of 2 to 4 equal codes happen to agree."""
+    n = 5
+    k = np.zeros(n, np.uint64)
+    i = np.arange(n, dtype=np.uint32)
+    ref = MidpointKarras(k, i)
+    kids = [ref.children(x)[1] for x in range(n - 1)]
+    assert kids[0] == (2, 3) and kids[3] == (1, 2)
+    c = claims(kids, n)
+    assert c[2] == 2 and c[n - 1 + 4] == 0
+    tree, parent = O.karras(k, i, 30)
+    got = claims(zip(tree[: n - 1]["child_a"], tree[: n - 1]["child_b"]), n)
+    assert got[0] == 0 and np.all(got[1:] == 1)
+    assert (tree[0]["child_a"], tree[0]["child_b"]) == (3, n - 1 + 4)
+    for run in range(2, 5):
+        kr = np.zeros(run, np.uint64)
+        ir = np.arange(run, dtype=np.uint32)
+        mk = MidpointKarras(kr, ir)
+        tr, _ = O.karras(kr, ir, 30)
+        for x in range(run - 1):
+            assert mk.children(x)[1] == (tr[x]["child_a"], tr[x]["child_b"])
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_midpoint_findsplit_agrees_when_end_codes_differ(seed):
+    """On every range whose end codes differ the reference's findSplit and the
+    oracle's pick the same child pair, duplicates inside the range included;
+    with equal end codes and runs of >= 5 some node gets another child."""
+    rng = np.random.default_rng(seed)
+    n = 300
+    k = np.sort(rng.integers(0, 40, n).astype(np.uint64))  # runs of about 7 equal codes
+    i = rng.permutation(n).astype(np.uint32)
+    order = np.lexsort((i, k))
+    k, i = k[order], i[order]
+    ref = MidpointKarras(k, i)
+    tree, _ = O.karras(k, i, 30)
+    differ = same_end = 0
+    for x in range(n - 1):
+        (lo, hi), kids = ref.children(x)
+        mine = (int(tree[x]["child_a"]), int(tree[x]["child_b"]))
+        if k[lo] != k[hi]:
+            assert kids == mine, x
+        else:
+            same_end += 1
+            differ += kids != mine
+    assert same_end > 0 and differ > 0
+    assert np.count_nonzero(claims([ref.children(x)[1] for x in range(n - 1)], n)[1:] != 1) > 0
+
+
 def test_refit_is_exact_union():
     sc = scenes.config_c1(n=24)
     m = sc.meshes[0]

@@ -35,7 +35,12 @@ constexpr uint32_t TQ_LEAVES = 192;
 constexpr uint32_t TQ_PAIRS = 256;  // one emission round adds up to 4 pairs per lane
 static_assert(TQ_PAIRS >= 64 * 4, "an emission round must fit the pair buffer after a flush");
 static_assert(TQ_LEAVES >= 2 * 64, "a walk round adds up to 2 leaves per lane");
-constexpr int TQ_SDEPTH = (TQ_STACK + TQ_PAIRS) / 64;  // fallback per-lane stack depth of the tile kernel
+// LDS words of the tile kernel's shared buffer: pending child pairs (2 each),
+// candidate pairs (1 each), the triangles of a batch of 64 leaves (9 floats
+// each) and their facet ids; the whole buffer is also the fallback per-lane stack
+constexpr uint32_t TQ_BUF_WORDS = 2 * TQ_STACK + TQ_PAIRS + 64 * 9 + 64;
+static_assert(TQ_BUF_WORDS % 128 == 0, "whole per-lane stack entries");
+constexpr int TQ_SDEPTH = TQ_BUF_WORDS / 128;  // fallback per-lane stack depth of the tile kernel
 
 __device__ __forceinline__ uint64_t lanes_below() { return (1ull << __lane_id()) - 1ull; }
 __device__ __forceinline__ uint32_t popc64(uint64_t v) { return (uint32_t)__popcll(v); }
@@ -154,7 +159,8 @@ __device__ __forceinline__ void tile_cloth(const ClothArgs& ca, bool active, uin
         lane == 0 ? r0 : lane == 1 ? r1 : lane == 2 ? r2 : lane == 3 ? r3 : lane == 4 ? r4 : r5;
 }
 
-// One wave per 64 consecutive query items.  Waves whose items span two query
+// One wave per 64 consecutive query items (lane 8 r + c: row r, column c of
+// an 8 x 8 vertex tile for grid cloths).  Waves whose items span two query
 // segments, or that hold a non-finite query box, take the per-lane walk.  When the leaf list fills up, the leaves
 // found so far go through phases 2-3 and the walk goes on.  When the walk's
 // pending list outgrows its buffer, the (wave, target) is redone per lane from
@@ -167,17 +173,19 @@ __device__ __forceinline__ void tile_cloth(const ClothArgs& ca, bool active, uin
 template <bool CLOTH>
 __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __launch_bounds__(64) void k_query_tile(QueryArgs a,
                                                                                                ClothArgs ca) {
-  // s_buf: [0, TQ_STACK) child-link pairs of overlapping internal nodes,
-  // [TQ_STACK, TQ_STACK + TQ_PAIRS) (lane, facet) candidates; the whole buffer
-  // is the per-lane stack (TQ_SDEPTH entries) of the fallback walks
-  __shared__ int2 s_buf[TQ_STACK + TQ_PAIRS];
+  // s_buf: TQ_STACK child-link pairs of overlapping internal nodes, TQ_PAIRS
+  // candidates (lane | batch slot << 6), the corners and facet ids of the
+  // current batch of leaves; the whole buffer is the per-lane stack
+  // (TQ_SDEPTH entries) of the fallback walks
+  __shared__ int2 s_buf[TQ_BUF_WORDS / 2];
   __shared__ uint2 s_leaf[TQ_LEAVES];  // (leaf node, facet) overlapping the union box
   __shared__ float s_q[64 * 6];        // qcur, qprev per lane
-  __shared__ float s_sub[16 * 6];      // 2 x 2-vertex sub-boxes
+  __shared__ float s_rc[16 * 6];       // boxes of the tile's 8 rows, then of its 8 columns
   __shared__ uint32_t s_cnt[64], s_p[64], s_j[64];
-  static_assert(TQ_STACK + TQ_PAIRS == 64 * TQ_SDEPTH, "fallback stack: TQ_SDEPTH entries per lane");
   int2* s_stk = s_buf;
-  uint2* s_pair = reinterpret_cast<uint2*>(s_buf + TQ_STACK);
+  uint32_t* s_pair = reinterpret_cast<uint32_t*>(s_buf + TQ_STACK);
+  float* s_tri = reinterpret_cast<float*>(s_pair + TQ_PAIRS);
+  uint32_t* s_bfid = reinterpret_cast<uint32_t*>(s_tri + 64 * 9);
   const uint32_t lane = threadIdx.x;
   const uint32_t w = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t base = w * 64;
@@ -249,24 +257,31 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
     hi = {fmaxf(hi.x, __shfl_xor(hi.x, o, 64)), fmaxf(hi.y, __shfl_xor(hi.y, o, 64)),
           fmaxf(hi.z, __shfl_xor(hi.z, o, 64))};
   };
-  // sub-box s holds lanes {b, b+1, b+8, b+9}, b = 16 (s / 4) + 2 (s % 4): a
-  // 2 x 2 block of an 8 x 8 tile
+  // row r holds lanes 8 r .. 8 r + 7, column c lanes c, c + 8, .., c + 56; a
+  // vertex whose box overlaps a leaf's lies in a row and a column that do too
+  auto put = [&](uint32_t b) {
+    s_rc[b * 6 + 0] = lo.x;
+    s_rc[b * 6 + 1] = lo.y;
+    s_rc[b * 6 + 2] = lo.z;
+    s_rc[b * 6 + 3] = hi.x;
+    s_rc[b * 6 + 4] = hi.y;
+    s_rc[b * 6 + 5] = hi.z;
+  };
+  const f3 lo1 = lo, hi1 = hi;
   reduce(1);
-  reduce(8);
-  if ((lane & 9u) == 0) {
-    const uint32_t s = ((lane >> 4) << 2) | ((lane & 7u) >> 1);
-    s_sub[s * 6 + 0] = lo.x;
-    s_sub[s * 6 + 1] = lo.y;
-    s_sub[s * 6 + 2] = lo.z;
-    s_sub[s * 6 + 3] = hi.x;
-    s_sub[s * 6 + 4] = hi.y;
-    s_sub[s * 6 + 5] = hi.z;
-  }
   reduce(2);
   reduce(4);
+  if ((lane & 7u) == 0) put(lane >> 3);
+  reduce(8);
   reduce(16);
   reduce(32);
   const f3 Ulo = lo, Uhi = hi;  // union of the wave's boxes
+  lo = lo1;
+  hi = hi1;
+  reduce(8);
+  reduce(16);
+  reduce(32);
+  if (lane < 8) put(8 + lane);
   const uint32_t leaf_cap = a.tile == 2 ? 128u : TQ_LEAVES;  // >= 2 x 64
   const uint32_t stack_cap = a.tile == 2 ? 32u : TQ_STACK;
   wave_sync();
@@ -281,19 +296,21 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
     const bool self = (t == m);
     const uint32_t cnt0 = s_cnt[lane], npair0 = npair;  // for a per-lane redo of this target
     uint32_t np = 0;
-    // ---- phase 3 (whenever the pair buffer fills): narrow phase per pair
+    // ---- phase 3 (after each batch of leaves, and whenever the pair buffer
+    // fills): narrow phase per pair, the triangle's corners from LDS
     auto flush = [&]() {
       for (uint32_t c = 0; c < np; c += 64) {
         if (c + lane < np) {
-          const uint2 pr = s_pair[c + lane];
-          const uint32_t v = pr.x, fid = pr.y;
-          const uint32_t* f = F + 3 * (size_t)fid;
-          const uint32_t f0 = f[0], f1 = f[1], f2 = f[2];
+          const uint32_t pr = s_pair[c + lane];
+          const uint32_t v = pr & 63u, sl = pr >> 6;
+          const uint32_t fid = s_bfid[sl];
+          const float* tp = s_tri + 9 * sl;
+          const f3 t0 = {tp[0], tp[1], tp[2]}, t1 = {tp[3], tp[4], tp[5]}, t2 = {tp[6], tp[7], tp[8]};
           const f3 vc = {s_q[v * 6 + 0], s_q[v * 6 + 1], s_q[v * 6 + 2]};
           const f3 vp = {s_q[v * 6 + 3], s_q[v * 6 + 4], s_q[v * 6 + 5]};
           f3 nrm;
           float d;
-          if (narrow_test(vc, vp, ld3(a.x, tvb + f0), ld3(a.x, tvb + f1), ld3(a.x, tvb + f2), band, nrm, d)) {
+          if (narrow_test(vc, vp, t0, t1, t2, band, nrm, d)) {
             const uint32_t slot = atomicAdd(&s_cnt[v], 1u);
             if (slot < a.cap) {
               const uint32_t jv = s_j[v];
@@ -307,12 +324,14 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
       wave_sync();
     };
     // ---- phase 2 (whenever the leaf list fills): leaves -> (vertex, facet)
-    // candidates through the sub-boxes
+    // candidates, in batches of 64 leaves (one per lane, batch slot = lane)
     auto leaves = [&](uint32_t nl) {
       for (uint32_t c0 = 0; c0 < nl; c0 += 64) {
         const bool have = c0 + lane < nl;
-        uint32_t mask = 0, fid = 0, f0 = 0, f1 = 0, f2 = 0;
+        uint32_t fid = 0, f0 = 0, f1 = 0, f2 = 0;
+        uint64_t cand = 0;
         NodeV L;
+        f3 p0, p1, p2;
         if (have) {
           const uint2 le = s_leaf[c0 + lane];
           L = load_node(T, (int)le.x);
@@ -321,33 +340,54 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
           f0 = f[0];
           f1 = f[1];
           f2 = f[2];
-#pragma unroll 4
-          for (int s = 0; s < 16; ++s) {
-            const f3 sl = {s_sub[s * 6 + 0], s_sub[s * 6 + 1], s_sub[s * 6 + 2]};
-            const f3 sh = {s_sub[s * 6 + 3], s_sub[s * 6 + 4], s_sub[s * 6 + 5]};
-            if (overlap(sl, sh, L.lo, L.hi)) mask |= 1u << s;
+          p0 = ld3(a.x, tvb + f0);  // for the narrow phase; in flight during the tests
+          p1 = ld3(a.x, tvb + f1);
+          p2 = ld3(a.x, tvb + f2);
+          // rows and columns overlapping the leaf; candidates = their crossings
+          uint64_t rows = 0, cols = 0;
+#pragma unroll 2
+          for (int r = 0; r < 8; ++r) {
+            const f3 bl = {s_rc[r * 6 + 0], s_rc[r * 6 + 1], s_rc[r * 6 + 2]};
+            const f3 bh = {s_rc[r * 6 + 3], s_rc[r * 6 + 4], s_rc[r * 6 + 5]};
+            if (overlap(bl, bh, L.lo, L.hi)) rows |= 0xffull << (8 * r);
           }
+#pragma unroll 2
+          for (int c = 0; c < 8; ++c) {
+            const f3 bl = {s_rc[48 + c * 6 + 0], s_rc[48 + c * 6 + 1], s_rc[48 + c * 6 + 2]};
+            const f3 bh = {s_rc[48 + c * 6 + 3], s_rc[48 + c * 6 + 4], s_rc[48 + c * 6 + 5]};
+            if (overlap(bl, bh, L.lo, L.hi)) cols |= 0x0101010101010101ull << c;
+          }
+          cand = rows & cols;
         }
-        // the vertices of the overlapping sub-boxes whose own box overlaps the
-        // leaf (each vertex lies in one sub-box), as a mask of tile lanes
+        // the candidate vertices whose own box overlaps the leaf, as a mask
+        // of tile lanes
         uint64_t vm = 0;
-        while (mask) {
-          const uint32_t s = __builtin_ctz(mask);
-          mask &= mask - 1;
-          const uint32_t b0 = ((s >> 2) * 2) * 8 + (s & 3u) * 2;
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const uint32_t v = b0 + (q & 1) + (q >> 1) * 8;
-            if (base + v >= a.nitems) continue;
-            const f3 vc = {s_q[v * 6 + 0], s_q[v * 6 + 1], s_q[v * 6 + 2]};
-            const f3 vl = {vc.x - margin, vc.y - margin, vc.z - margin};
-            const f3 vh = {vc.x + margin, vc.y + margin, vc.z + margin};
-            if (!overlap(vl, vh, L.lo, L.hi)) continue;
-            const uint32_t pv = s_p[v];
-            if (self && (pv == f0 || pv == f1 || pv == f2)) continue;
-            vm |= 1ull << v;
-          }
+        while (cand) {
+          const uint32_t v = __builtin_ctzll(cand);
+          cand &= cand - 1;
+          if (base + v >= a.nitems) continue;
+          const f3 vc = {s_q[v * 6 + 0], s_q[v * 6 + 1], s_q[v * 6 + 2]};
+          const f3 vl = {vc.x - margin, vc.y - margin, vc.z - margin};
+          const f3 vh = {vc.x + margin, vc.y + margin, vc.z + margin};
+          if (!overlap(vl, vh, L.lo, L.hi)) continue;
+          const uint32_t pv = s_p[v];
+          if (self && (pv == f0 || pv == f1 || pv == f2)) continue;
+          vm |= 1ull << v;
         }
+        if (have) {
+          float* tp = s_tri + 9 * lane;
+          tp[0] = p0.x;
+          tp[1] = p0.y;
+          tp[2] = p0.z;
+          tp[3] = p1.x;
+          tp[4] = p1.y;
+          tp[5] = p1.z;
+          tp[6] = p2.x;
+          tp[7] = p2.y;
+          tp[8] = p2.z;
+          s_bfid[lane] = fid;
+        }
+        wave_sync();
         // emitted in rounds of at most 4 (vertex, facet) pairs per lane
         while (__ballot(vm != 0)) {
           uint32_t nc = 0, cand = 0;  // up to 4 candidate lanes, 6 bits each
@@ -366,7 +406,7 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
           const uint32_t tot = popc64(c1) + 2 * popc64(c2) + 4 * popc64(c4);
           if (np + tot > TQ_PAIRS) flush();
           const uint32_t o = np + excl;
-          for (uint32_t q = 0; q < nc; ++q) s_pair[o + q] = make_uint2((cand >> (6 * q)) & 63u, fid);
+          for (uint32_t q = 0; q < nc; ++q) s_pair[o + q] = ((cand >> (6 * q)) & 63u) | (lane << 6);
           if (a.audit) {  // one reservation per round for the wave's tot records
             uint32_t r = 0;
             if (lane == 0) r = atomicAdd(a.audit_cnt, tot);
@@ -379,6 +419,7 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
           npair += nc;
           wave_sync();
         }
+        flush();  // the next batch overwrites the triangles
       }
     };
     // ---- phase 1: breadth-first walk with the union box, up to 64 pending
@@ -497,7 +538,6 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
       continue;
     }
     leaves(nl);
-    flush();
   }
   wave_sync();
   const uint32_t c = s_cnt[lane];

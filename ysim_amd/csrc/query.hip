@@ -161,15 +161,16 @@ __device__ __forceinline__ void tile_cloth(const ClothArgs& ca, bool active, uin
 
 // One wave per 64 consecutive query items (lane 8 r + c: row r, column c of
 // an 8 x 8 vertex tile for grid cloths).  Waves whose items span two query
-// segments, or that hold a non-finite query box, take the per-lane walk.  When the leaf list fills up, the leaves
-// found so far go through phases 2-3 and the walk goes on.  When the walk's
-// pending list outgrows its buffer, the (wave, target) is redone per lane from
-// the slot counts it started with (the slots it filled are overwritten).
-// a.tile == 2 shrinks both buffers (128 leaves, 32 pending pairs) so that the
-// leaf flushes and the per-lane redo both run (a test mode for those paths).
-// CLOTH: the tile's cloth step follows its query (tile_cloth); its registers
-// keep that variant at 4 waves per SIMD (at 5 it spills), the query alone
-// runs 5 (LDS 7.3 KB per wave).
+// segments, or that hold a non-finite query box, take the per-lane walk.
+// When the leaf list fills up, the leaves found so far go through phases 2-3
+// and the walk goes on.  When the walk's pending list outgrows its buffer, the
+// (wave, target) is redone per lane from the slot counts it started with (the
+// slots it filled are overwritten).  a.tile == 2 shrinks both buffers (128
+// leaves, 32 pending pairs) so that the leaf flushes and the per-lane redo
+// both run (a test mode for those paths).
+// CLOTH: the tile's cloth step follows its query (tile_cloth, one call site
+// for both paths); its registers keep that variant at 4 waves per SIMD (at 5
+// it spills), the query alone runs 5 (LDS 8.6 KB per wave).
 template <bool CLOTH>
 __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __launch_bounds__(64) void k_query_tile(QueryArgs a,
                                                                                                ClothArgs ca) {
@@ -199,22 +200,64 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
   // query box (a NaN box overlaps every box under the closed test; the union
   // box would ignore it), walks per lane over each lane's own targets
   bool per_lane = seg_of(a.qsegs, base) != sg;
+  // the cooperative path's lane data (idle lanes take the wave's last item)
+  const uint32_t m = a.qsegs.mesh[sg];
+  const uint32_t p = query_vertex(a, sg, m, active ? tq : last);
+  const uint32_t j = a.qsegs.off[sg] + p;
+  const uint32_t g = a.mt.voff[m] + p;
+  const uint32_t ti0 = a.tgt_off[sg], ti1 = a.tgt_off[sg + 1];
+  f3 qcur = {0.f, 0.f, 0.f}, qprev = {0.f, 0.f, 0.f};
+  float margin = 0.f, band = 0.f;
+  // the first target's cut when it has at most 3 x 64 entries and fits the
+  // leaf list: entry and node of this lane for batches 0-2 in (pn0, pv0) ..
+  // (pn2, pv2), loaded while the position loads, the finiteness ballot and
+  // the box reductions below run; scanned before the target loop (npre > 0)
+  const uint32_t leaf_cap = a.tile == 2 ? 128u : TQ_LEAVES;  // >= 2 x 64
+  const uint32_t stack_cap = a.tile == 2 ? 32u : TQ_STACK;
+  uint32_t npre = 0;
+  int pn0 = 0, pn1 = 0, pn2 = 0;
+  NodeV pv0, pv1, pv2;
   if (!per_lane) {
-    const uint32_t mw = a.qsegs.mesh[sg];
-    const f3 q = active ? ld3(a.x, a.mt.voff[mw] + query_vertex(a, sg, mw, tq)) : f3{0.f, 0.f, 0.f};
-    const float mg = a.mesh_margin[mw];
+    qcur = ld3(a.x, g);
+    qprev = ld3(a.xprev, g);
+    margin = a.mesh_margin[m];
+    band = a.mesh_band[m];
+    if (ti0 < ti1) {
+      const uint32_t t = a.tgt[ti0];
+      const uint32_t ncut = a.cut_cnt ? a.cut_cnt[t] : 0u;
+      if (a.mesh_nfacets[t] > 0 && ncut > 0 && ncut <= CUT_CAP && ncut <= 3 * 64 && ncut <= leaf_cap) {
+        const int32_t* CL = a.cut + (size_t)t * CUT_CAP;
+        const ysim_bvh_node* T = a.tree + a.mt.noff[t];
+        npre = ncut;
+        if (lane < ncut) {
+          pn0 = CL[lane];
+          pv0 = load_node(T, pn0);
+        }
+        if (64 + lane < ncut) {
+          pn1 = CL[64 + lane];
+          pv1 = load_node(T, pn1);
+        }
+        if (128 + lane < ncut) {
+          pn2 = CL[128 + lane];
+          pv2 = load_node(T, pn2);
+        }
+      }
+    }
+    const f3 q = active ? qcur : f3{0.f, 0.f, 0.f};
+    const float mg = margin;
     const bool finite = __builtin_isfinite(q.x - mg) && __builtin_isfinite(q.y - mg) &&
                         __builtin_isfinite(q.z - mg) && __builtin_isfinite(q.x + mg) &&
                         __builtin_isfinite(q.y + mg) && __builtin_isfinite(q.z + mg);
     per_lane = __ballot(!finite) != 0;
   }
+  uint32_t jc = 0, cc = 0;  // the lane's slot row and contact count, for the cloth step
   if (per_lane) {
-    uint32_t jl = 0, cnt = 0;
     if (active) {
+      uint32_t cnt = 0;
       const uint32_t sl = seg_of(a.qsegs, tq);
       const uint32_t ml = a.qsegs.mesh[sl];
       const uint32_t pl = query_vertex(a, sl, ml, tq);
-      jl = a.qsegs.off[sl] + pl;
+      const uint32_t jl = a.qsegs.off[sl] + pl;
       const f3 qc = ld3(a.x, a.mt.voff[ml] + pl), qp = ld3(a.xprev, a.mt.voff[ml] + pl);
       const float mg = a.mesh_margin[ml];
       const f3 qlo = {qc.x - mg, qc.y - mg, qc.z - mg}, qhi = {qc.x + mg, qc.y + mg, qc.z + mg};
@@ -224,328 +267,348 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
           atomicOr(a.flags, 4u);
       a.ccount[jl] = cnt;
       if (cnt > a.cap) atomicOr(a.flags, 2u);
+      jc = jl;
+      cc = cnt;
     }
-    if (CLOTH) tile_cloth(ca, active, jl, cnt, w);  // every lane (wave reductions)
-    wave_stats(a, w, npair);
-    return;
-  }
-  const uint32_t m = a.qsegs.mesh[sg];
-  const uint32_t p = query_vertex(a, sg, m, active ? tq : last);
-  const uint32_t j = a.qsegs.off[sg] + p;
-  const uint32_t g = a.mt.voff[m] + p;
-  const f3 qcur = ld3(a.x, g);
-  const f3 qprev = ld3(a.xprev, g);
-  const float margin = a.mesh_margin[m];
-  const float band = a.mesh_band[m];
-  const f3 qmin = {qcur.x - margin, qcur.y - margin, qcur.z - margin};
-  const f3 qmax = {qcur.x + margin, qcur.y + margin, qcur.z + margin};
-  // per-lane query data for the cooperative phases; idle lanes carry an empty box
-  const float INF = __uint_as_float(0x7f800000u);
-  f3 lo = active ? qmin : f3{INF, INF, INF}, hi = active ? qmax : f3{-INF, -INF, -INF};
-  s_q[lane * 6 + 0] = qcur.x;
-  s_q[lane * 6 + 1] = qcur.y;
-  s_q[lane * 6 + 2] = qcur.z;
-  s_q[lane * 6 + 3] = qprev.x;
-  s_q[lane * 6 + 4] = qprev.y;
-  s_q[lane * 6 + 5] = qprev.z;
-  s_cnt[lane] = 0;
-  s_p[lane] = p;
-  s_j[lane] = j;
-  auto reduce = [&](int o) {
-    lo = {fminf(lo.x, __shfl_xor(lo.x, o, 64)), fminf(lo.y, __shfl_xor(lo.y, o, 64)),
-          fminf(lo.z, __shfl_xor(lo.z, o, 64))};
-    hi = {fmaxf(hi.x, __shfl_xor(hi.x, o, 64)), fmaxf(hi.y, __shfl_xor(hi.y, o, 64)),
-          fmaxf(hi.z, __shfl_xor(hi.z, o, 64))};
-  };
-  // row r holds lanes 8 r .. 8 r + 7, column c lanes c, c + 8, .., c + 56; a
-  // vertex whose box overlaps a leaf's lies in a row and a column that do too
-  auto put = [&](uint32_t b) {
-    s_rc[b * 6 + 0] = lo.x;
-    s_rc[b * 6 + 1] = lo.y;
-    s_rc[b * 6 + 2] = lo.z;
-    s_rc[b * 6 + 3] = hi.x;
-    s_rc[b * 6 + 4] = hi.y;
-    s_rc[b * 6 + 5] = hi.z;
-  };
-  const f3 lo1 = lo, hi1 = hi;
-  reduce(1);
-  reduce(2);
-  reduce(4);
-  if ((lane & 7u) == 0) put(lane >> 3);
-  reduce(8);
-  reduce(16);
-  reduce(32);
-  const f3 Ulo = lo, Uhi = hi;  // union of the wave's boxes
-  lo = lo1;
-  hi = hi1;
-  reduce(8);
-  reduce(16);
-  reduce(32);
-  if (lane < 8) put(8 + lane);
-  const uint32_t leaf_cap = a.tile == 2 ? 128u : TQ_LEAVES;  // >= 2 x 64
-  const uint32_t stack_cap = a.tile == 2 ? 32u : TQ_STACK;
-  wave_sync();
-
-  for (uint32_t ti = a.tgt_off[sg]; ti < a.tgt_off[sg + 1]; ++ti) {
-    const uint32_t t = a.tgt[ti];
-    const uint32_t nT = a.mesh_nfacets[t];
-    if (nT == 0) continue;
-    const ysim_bvh_node* T = a.tree + a.mt.noff[t];
-    const uint32_t* F = a.facets + 3 * (size_t)a.mt.foff[t];
-    const uint32_t tvb = a.mt.voff[t];
-    const bool self = (t == m);
-    const uint32_t cnt0 = s_cnt[lane], npair0 = npair;  // for a per-lane redo of this target
-    uint32_t np = 0;
-    // ---- phase 3 (after each batch of leaves, and whenever the pair buffer
-    // fills): narrow phase per pair, the triangle's corners from LDS
-    auto flush = [&]() {
-      for (uint32_t c = 0; c < np; c += 64) {
-        if (c + lane < np) {
-          const uint32_t pr = s_pair[c + lane];
-          const uint32_t v = pr & 63u, sl = pr >> 6;
-          const uint32_t fid = s_bfid[sl];
-          const float* tp = s_tri + 9 * sl;
-          const f3 t0 = {tp[0], tp[1], tp[2]}, t1 = {tp[3], tp[4], tp[5]}, t2 = {tp[6], tp[7], tp[8]};
-          const f3 vc = {s_q[v * 6 + 0], s_q[v * 6 + 1], s_q[v * 6 + 2]};
-          const f3 vp = {s_q[v * 6 + 3], s_q[v * 6 + 4], s_q[v * 6 + 5]};
-          f3 nrm;
-          float d;
-          if (narrow_test(vc, vp, t0, t1, t2, band, nrm, d)) {
-            const uint32_t slot = atomicAdd(&s_cnt[v], 1u);
-            if (slot < a.cap) {
-              const uint32_t jv = s_j[v];
-              a.cnd[(size_t)jv * a.cap + slot] = make_float4(nrm.x, nrm.y, nrm.z, d);
-              a.cid[(size_t)jv * a.cap + slot] = make_uint2(t, fid);
-            }
-          }
-        }
-      }
-      np = 0;
-      wave_sync();
+  } else {
+    const f3 qmin = {qcur.x - margin, qcur.y - margin, qcur.z - margin};
+    const f3 qmax = {qcur.x + margin, qcur.y + margin, qcur.z + margin};
+    // per-lane query data for the cooperative phases; idle lanes carry an empty box
+    const float INF = __uint_as_float(0x7f800000u);
+    f3 lo = active ? qmin : f3{INF, INF, INF}, hi = active ? qmax : f3{-INF, -INF, -INF};
+    s_q[lane * 6 + 0] = qcur.x;
+    s_q[lane * 6 + 1] = qcur.y;
+    s_q[lane * 6 + 2] = qcur.z;
+    s_q[lane * 6 + 3] = qprev.x;
+    s_q[lane * 6 + 4] = qprev.y;
+    s_q[lane * 6 + 5] = qprev.z;
+    s_cnt[lane] = 0;
+    s_p[lane] = p;
+    s_j[lane] = j;
+    auto reduce = [&](int o) {
+      lo = {fminf(lo.x, __shfl_xor(lo.x, o, 64)), fminf(lo.y, __shfl_xor(lo.y, o, 64)),
+            fminf(lo.z, __shfl_xor(lo.z, o, 64))};
+      hi = {fmaxf(hi.x, __shfl_xor(hi.x, o, 64)), fmaxf(hi.y, __shfl_xor(hi.y, o, 64)),
+            fmaxf(hi.z, __shfl_xor(hi.z, o, 64))};
     };
-    // ---- phase 2 (whenever the leaf list fills): leaves -> (vertex, facet)
-    // candidates, in batches of 64 leaves (one per lane, batch slot = lane)
-    auto leaves = [&](uint32_t nl) {
-      for (uint32_t c0 = 0; c0 < nl; c0 += 64) {
-        const bool have = c0 + lane < nl;
-        uint32_t fid = 0, f0 = 0, f1 = 0, f2 = 0;
-        uint64_t cand = 0;
-        NodeV L;
-        f3 p0, p1, p2;
-        if (have) {
-          const uint2 le = s_leaf[c0 + lane];
-          L = load_node(T, (int)le.x);
-          fid = le.y;
-          const uint32_t* f = F + 3 * (size_t)fid;
-          f0 = f[0];
-          f1 = f[1];
-          f2 = f[2];
-          p0 = ld3(a.x, tvb + f0);  // for the narrow phase; in flight during the tests
-          p1 = ld3(a.x, tvb + f1);
-          p2 = ld3(a.x, tvb + f2);
-          // rows and columns overlapping the leaf; candidates = their crossings
-          uint64_t rows = 0, cols = 0;
-#pragma unroll 2
-          for (int r = 0; r < 8; ++r) {
-            const f3 bl = {s_rc[r * 6 + 0], s_rc[r * 6 + 1], s_rc[r * 6 + 2]};
-            const f3 bh = {s_rc[r * 6 + 3], s_rc[r * 6 + 4], s_rc[r * 6 + 5]};
-            if (overlap(bl, bh, L.lo, L.hi)) rows |= 0xffull << (8 * r);
-          }
-#pragma unroll 2
-          for (int c = 0; c < 8; ++c) {
-            const f3 bl = {s_rc[48 + c * 6 + 0], s_rc[48 + c * 6 + 1], s_rc[48 + c * 6 + 2]};
-            const f3 bh = {s_rc[48 + c * 6 + 3], s_rc[48 + c * 6 + 4], s_rc[48 + c * 6 + 5]};
-            if (overlap(bl, bh, L.lo, L.hi)) cols |= 0x0101010101010101ull << c;
-          }
-          cand = rows & cols;
-        }
-        // the candidate vertices whose own box overlaps the leaf, as a mask
-        // of tile lanes
-        uint64_t vm = 0;
-        while (cand) {
-          const uint32_t v = __builtin_ctzll(cand);
-          cand &= cand - 1;
-          if (base + v >= a.nitems) continue;
-          const f3 vc = {s_q[v * 6 + 0], s_q[v * 6 + 1], s_q[v * 6 + 2]};
-          const f3 vl = {vc.x - margin, vc.y - margin, vc.z - margin};
-          const f3 vh = {vc.x + margin, vc.y + margin, vc.z + margin};
-          if (!overlap(vl, vh, L.lo, L.hi)) continue;
-          const uint32_t pv = s_p[v];
-          if (self && (pv == f0 || pv == f1 || pv == f2)) continue;
-          vm |= 1ull << v;
-        }
-        if (have) {
-          float* tp = s_tri + 9 * lane;
-          tp[0] = p0.x;
-          tp[1] = p0.y;
-          tp[2] = p0.z;
-          tp[3] = p1.x;
-          tp[4] = p1.y;
-          tp[5] = p1.z;
-          tp[6] = p2.x;
-          tp[7] = p2.y;
-          tp[8] = p2.z;
-          s_bfid[lane] = fid;
-        }
-        wave_sync();
-        // emitted in rounds of at most 4 (vertex, facet) pairs per lane
-        while (__ballot(vm != 0)) {
-          uint32_t nc = 0, cand = 0;  // up to 4 candidate lanes, 6 bits each
-#pragma unroll
-          for (int q = 0; q < 4; ++q)
-            if (vm) {
-              cand |= (uint32_t)__builtin_ctzll(vm) << (6 * nc);
-              vm &= vm - 1;
-              ++nc;
+    // row r holds lanes 8 r .. 8 r + 7, column c lanes c, c + 8, .., c + 56; a
+    // vertex whose box overlaps a leaf's lies in a row and a column that do too
+    auto put = [&](uint32_t b) {
+      s_rc[b * 6 + 0] = lo.x;
+      s_rc[b * 6 + 1] = lo.y;
+      s_rc[b * 6 + 2] = lo.z;
+      s_rc[b * 6 + 3] = hi.x;
+      s_rc[b * 6 + 4] = hi.y;
+      s_rc[b * 6 + 5] = hi.z;
+    };
+    const f3 lo1 = lo, hi1 = hi;
+    reduce(1);
+    reduce(2);
+    reduce(4);
+    if ((lane & 7u) == 0) put(lane >> 3);
+    reduce(8);
+    reduce(16);
+    reduce(32);
+    const f3 Ulo = lo, Uhi = hi;  // union of the wave's boxes
+    lo = lo1;
+    hi = hi1;
+    reduce(8);
+    reduce(16);
+    reduce(32);
+    if (lane < 8) put(8 + lane);
+    // the prefetched cut of the first target: pending pairs and leaves
+    // (sp_pre, nl_pre; no leaf flush: the cut fits the list)
+    uint32_t sp_pre = 0, nl_pre = 0;
+    bool ovf_pre = false;
+    auto pre_batch = [&](uint32_t c0, int node, const NodeV& A) {
+      const bool have = c0 + lane < npre;
+      bool pc = false, lc = false;
+      if (have) {
+        const bool h = overlap(Ulo, Uhi, A.lo, A.hi);
+        pc = h && A.a >= 0;
+        lc = h && A.a < 0;
+      }
+      const uint64_t bp = __ballot(pc), bl = __ballot(lc);
+      if (sp_pre + popc64(bp) > stack_cap) {
+        ovf_pre = true;
+        return;
+      }
+      if (pc) s_stk[sp_pre + popc64(bp & lanes_below())] = make_int2(A.a, A.b);
+      if (lc) s_leaf[nl_pre + popc64(bl & lanes_below())] = make_uint2((uint32_t)node, (uint32_t)A.b);
+      sp_pre += popc64(bp);
+      nl_pre += popc64(bl);
+    };
+    if (npre > 0) pre_batch(0, pn0, pv0);
+    if (npre > 64 && !ovf_pre) pre_batch(64, pn1, pv1);
+    if (npre > 128 && !ovf_pre) pre_batch(128, pn2, pv2);
+    wave_sync();
+
+    for (uint32_t ti = ti0; ti < ti1; ++ti) {
+      const uint32_t t = a.tgt[ti];
+      const uint32_t nT = a.mesh_nfacets[t];
+      if (nT == 0) continue;
+      const ysim_bvh_node* T = a.tree + a.mt.noff[t];
+      const uint32_t* F = a.facets + 3 * (size_t)a.mt.foff[t];
+      const uint32_t tvb = a.mt.voff[t];
+      const bool self = (t == m);
+      const uint32_t cnt0 = s_cnt[lane], npair0 = npair;  // for a per-lane redo of this target
+      uint32_t np = 0;
+      // ---- phase 3 (after each batch of leaves, and whenever the pair buffer
+      // fills): narrow phase per pair, the triangle's corners from LDS
+      auto flush = [&]() {
+        for (uint32_t c = 0; c < np; c += 64) {
+          if (c + lane < np) {
+            const uint32_t pr = s_pair[c + lane];
+            const uint32_t v = pr & 63u, sl = pr >> 6;
+            const uint32_t fid = s_bfid[sl];
+            const float* tp = s_tri + 9 * sl;
+            const f3 t0 = {tp[0], tp[1], tp[2]}, t1 = {tp[3], tp[4], tp[5]}, t2 = {tp[6], tp[7], tp[8]};
+            const f3 vc = {s_q[v * 6 + 0], s_q[v * 6 + 1], s_q[v * 6 + 2]};
+            const f3 vp = {s_q[v * 6 + 3], s_q[v * 6 + 4], s_q[v * 6 + 5]};
+            f3 nrm;
+            float d;
+            if (narrow_test(vc, vp, t0, t1, t2, band, nrm, d)) {
+              const uint32_t slot = atomicAdd(&s_cnt[v], 1u);
+              if (slot < a.cap) {
+                const uint32_t jv = s_j[v];
+                a.cnd[(size_t)jv * a.cap + slot] = make_float4(nrm.x, nrm.y, nrm.z, d);
+                a.cid[(size_t)jv * a.cap + slot] = make_uint2(t, fid);
+              }
             }
-          // exclusive scan of the candidate counts (nc <= 4) from the ballots
-          // of their three bits
-          const uint64_t c1 = __ballot(nc & 1u), c2 = __ballot(nc & 2u), c4 = __ballot(nc & 4u);
-          const uint64_t below = lanes_below();
-          const uint32_t excl = popc64(c1 & below) + 2 * popc64(c2 & below) + 4 * popc64(c4 & below);
-          const uint32_t tot = popc64(c1) + 2 * popc64(c2) + 4 * popc64(c4);
-          if (np + tot > TQ_PAIRS) flush();
-          const uint32_t o = np + excl;
-          for (uint32_t q = 0; q < nc; ++q) s_pair[o + q] = ((cand >> (6 * q)) & 63u) | (lane << 6);
-          if (a.audit) {  // one reservation per round for the wave's tot records
-            uint32_t r = 0;
-            if (lane == 0) r = atomicAdd(a.audit_cnt, tot);
-            r = __shfl(r, 0, 64) + excl;
-            for (uint32_t q = 0; q < nc; ++q)
-              if (r + q < a.audit_cap)
-                a.audit[r + q] = make_uint4(s_j[(cand >> (6 * q)) & 63u], t, fid, audit_tag(AUDIT_TILE, w));
           }
-          np += tot;
-          npair += nc;
+        }
+        np = 0;
+        wave_sync();
+      };
+      // ---- phase 2 (whenever the leaf list fills): leaves -> (vertex, facet)
+      // candidates, in batches of 64 leaves (one per lane, batch slot = lane)
+      auto leaves = [&](uint32_t nl) {
+        for (uint32_t c0 = 0; c0 < nl; c0 += 64) {
+          const bool have = c0 + lane < nl;
+          uint32_t fid = 0, f0 = 0, f1 = 0, f2 = 0;
+          uint64_t cand = 0;
+          NodeV L;
+          f3 p0, p1, p2;
+          if (have) {
+            const uint2 le = s_leaf[c0 + lane];
+            L = load_node(T, (int)le.x);
+            fid = le.y;
+            const uint32_t* f = F + 3 * (size_t)fid;
+            f0 = f[0];
+            f1 = f[1];
+            f2 = f[2];
+            p0 = ld3(a.x, tvb + f0);  // for the narrow phase; in flight during the tests
+            p1 = ld3(a.x, tvb + f1);
+            p2 = ld3(a.x, tvb + f2);
+            // rows and columns overlapping the leaf; candidates = their crossings
+            uint64_t rows = 0, cols = 0;
+#pragma unroll 2
+            for (int r = 0; r < 8; ++r) {
+              const f3 bl = {s_rc[r * 6 + 0], s_rc[r * 6 + 1], s_rc[r * 6 + 2]};
+              const f3 bh = {s_rc[r * 6 + 3], s_rc[r * 6 + 4], s_rc[r * 6 + 5]};
+              if (overlap(bl, bh, L.lo, L.hi)) rows |= 0xffull << (8 * r);
+            }
+#pragma unroll 2
+            for (int c = 0; c < 8; ++c) {
+              const f3 bl = {s_rc[48 + c * 6 + 0], s_rc[48 + c * 6 + 1], s_rc[48 + c * 6 + 2]};
+              const f3 bh = {s_rc[48 + c * 6 + 3], s_rc[48 + c * 6 + 4], s_rc[48 + c * 6 + 5]};
+              if (overlap(bl, bh, L.lo, L.hi)) cols |= 0x0101010101010101ull << c;
+            }
+            cand = rows & cols;
+          }
+          // the candidate vertices whose own box overlaps the leaf, as a mask
+          // of tile lanes
+          uint64_t vm = 0;
+          while (cand) {
+            const uint32_t v = __builtin_ctzll(cand);
+            cand &= cand - 1;
+            if (base + v >= a.nitems) continue;
+            const f3 vc = {s_q[v * 6 + 0], s_q[v * 6 + 1], s_q[v * 6 + 2]};
+            const f3 vl = {vc.x - margin, vc.y - margin, vc.z - margin};
+            const f3 vh = {vc.x + margin, vc.y + margin, vc.z + margin};
+            if (!overlap(vl, vh, L.lo, L.hi)) continue;
+            const uint32_t pv = s_p[v];
+            if (self && (pv == f0 || pv == f1 || pv == f2)) continue;
+            vm |= 1ull << v;
+          }
+          if (have) {
+            float* tp = s_tri + 9 * lane;
+            tp[0] = p0.x;
+            tp[1] = p0.y;
+            tp[2] = p0.z;
+            tp[3] = p1.x;
+            tp[4] = p1.y;
+            tp[5] = p1.z;
+            tp[6] = p2.x;
+            tp[7] = p2.y;
+            tp[8] = p2.z;
+            s_bfid[lane] = fid;
+          }
+          wave_sync();
+          // emitted in rounds of at most 4 (vertex, facet) pairs per lane
+          while (__ballot(vm != 0)) {
+            uint32_t nc = 0, cand = 0;  // up to 4 candidate lanes, 6 bits each
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              if (vm) {
+                cand |= (uint32_t)__builtin_ctzll(vm) << (6 * nc);
+                vm &= vm - 1;
+                ++nc;
+              }
+            // exclusive scan of the candidate counts (nc <= 4) from the ballots
+            // of their three bits
+            const uint64_t c1 = __ballot(nc & 1u), c2 = __ballot(nc & 2u), c4 = __ballot(nc & 4u);
+            const uint64_t below = lanes_below();
+            const uint32_t excl = popc64(c1 & below) + 2 * popc64(c2 & below) + 4 * popc64(c4 & below);
+            const uint32_t tot = popc64(c1) + 2 * popc64(c2) + 4 * popc64(c4);
+            if (np + tot > TQ_PAIRS) flush();
+            const uint32_t o = np + excl;
+            for (uint32_t q = 0; q < nc; ++q) s_pair[o + q] = ((cand >> (6 * q)) & 63u) | (lane << 6);
+            if (a.audit) {  // one reservation per round for the wave's tot records
+              uint32_t r = 0;
+              if (lane == 0) r = atomicAdd(a.audit_cnt, tot);
+              r = __shfl(r, 0, 64) + excl;
+              for (uint32_t q = 0; q < nc; ++q)
+                if (r + q < a.audit_cap)
+                  a.audit[r + q] = make_uint4(s_j[(cand >> (6 * q)) & 63u], t, fid, audit_tag(AUDIT_TILE, w));
+            }
+            np += tot;
+            npair += nc;
+            wave_sync();
+          }
+          flush();  // the next batch overwrites the triangles
+        }
+      };
+      // ---- phase 1: breadth-first walk with the union box, up to 64 pending
+      // nodes per round.  It starts from the tree's cut when the build listed
+      // one (a frontier covering every leaf once, so the leaves found are the
+      // same as from the root), in one round of independent loads instead of
+      // the top levels' dependent rounds; else from the root.
+      uint32_t sp = 0, nl = 0;
+      bool overflow = false;
+      const uint32_t ncut = a.cut_cnt ? a.cut_cnt[t] : 0u;
+      if (ti == ti0 && npre > 0) {  // scanned above
+        sp = ovf_pre ? 0u : sp_pre;
+        nl = nl_pre;
+        overflow = ovf_pre;
+      } else if (ncut > 0 && ncut <= CUT_CAP) {
+        const int32_t* CL = a.cut + (size_t)t * CUT_CAP;
+        for (uint32_t c0 = 0; c0 < ncut; c0 += 64) {
+          if (nl + 64 > leaf_cap) {
+            leaves(nl);
+            nl = 0;
+          }
+          const bool have = c0 + lane < ncut;
+          bool pc = false, lc = false;
+          int node = 0;
+          NodeV A;
+          if (have) {
+            node = CL[c0 + lane];
+            A = load_node(T, node);
+            const bool h = overlap(Ulo, Uhi, A.lo, A.hi);
+            pc = h && A.a >= 0;
+            lc = h && A.a < 0;
+          }
+          const uint64_t bp = __ballot(pc), bl = __ballot(lc);
+          if (sp + popc64(bp) > stack_cap) {
+            overflow = true;
+            break;
+          }
+          if (pc) s_stk[sp + popc64(bp & lanes_below())] = make_int2(A.a, A.b);
+          if (lc) s_leaf[nl + popc64(bl & lanes_below())] = make_uint2((uint32_t)node, (uint32_t)A.b);
+          sp += popc64(bp);
+          nl += popc64(bl);
           wave_sync();
         }
-        flush();  // the next batch overwrites the triangles
+        if (overflow) sp = 0;
+      } else {
+        const NodeV r = load_node(T, 0);
+        if (overlap(Ulo, Uhi, r.lo, r.hi)) {
+          if (r.a < 0) {
+            if (lane == 0) s_leaf[0] = make_uint2(0u, (uint32_t)r.b);
+            nl = 1;
+          } else {
+            if (lane == 0) s_stk[0] = make_int2(r.a, r.b);
+            sp = 1;
+          }
+        }
       }
-    };
-    // ---- phase 1: breadth-first walk with the union box, up to 64 pending
-    // nodes per round.  It starts from the tree's cut when the build listed
-    // one (a frontier covering every leaf once, so the leaves found are the
-    // same as from the root), in one round of independent loads instead of
-    // the top levels' dependent rounds; else from the root.
-    uint32_t sp = 0, nl = 0;
-    bool overflow = false;
-    const uint32_t ncut = a.cut_cnt ? a.cut_cnt[t] : 0u;
-    if (ncut > 0 && ncut <= CUT_CAP) {
-      const int32_t* CL = a.cut + (size_t)t * CUT_CAP;
-      for (uint32_t c0 = 0; c0 < ncut; c0 += 64) {
-        if (nl + 64 > leaf_cap) {
-          leaves(nl);
-          nl = 0;
-        }
-        const bool have = c0 + lane < ncut;
-        bool pc = false, lc = false;
-        int node = 0;
-        NodeV A;
-        if (have) {
-          node = CL[c0 + lane];
-          A = load_node(T, node);
-          const bool h = overlap(Ulo, Uhi, A.lo, A.hi);
-          pc = h && A.a >= 0;
-          lc = h && A.a < 0;
-        }
-        const uint64_t bp = __ballot(pc), bl = __ballot(lc);
-        if (sp + popc64(bp) > stack_cap) {
+      wave_sync();
+      uint32_t rounds = 0;
+      while (sp > 0) {
+        if (++rounds > 2 * nT + 2) {  // a valid tree is walked in fewer rounds
           overflow = true;
           break;
         }
-        if (pc) s_stk[sp + popc64(bp & lanes_below())] = make_int2(A.a, A.b);
-        if (lc) s_leaf[nl + popc64(bl & lanes_below())] = make_uint2((uint32_t)node, (uint32_t)A.b);
-        sp += popc64(bp);
-        nl += popc64(bl);
+        const uint32_t k = sp < 64u ? sp : 64u, b0 = sp - k;
+        if (nl + 2 * k > leaf_cap) {  // this round may add 2 leaves per node
+          leaves(nl);
+          nl = 0;
+        }
+        const bool have = lane < k;
+        int2 e = make_int2(0, 0);
+        if (have) e = s_stk[b0 + lane];
+        wave_sync();
+        sp = b0;
+        bool pA = false, pB = false, lA = false, lB = false;
+        NodeV A, B;
+        if (have) {
+          A = load_node(T, e.x);
+          B = load_node(T, e.y);
+          const bool hA = overlap(Ulo, Uhi, A.lo, A.hi), hB = overlap(Ulo, Uhi, B.lo, B.hi);
+          pA = hA && A.a >= 0;
+          pB = hB && B.a >= 0;
+          lA = hA && A.a < 0;
+          lB = hB && B.a < 0;
+        }
+        const uint64_t bp1 = __ballot(pA || pB), bp2 = __ballot(pA && pB);
+        const uint64_t bl1 = __ballot(lA || lB), bl2 = __ballot(lA && lB);
+        const uint32_t tp = popc64(bp1) + popc64(bp2), tl = popc64(bl1) + popc64(bl2);
+        if (sp + tp > stack_cap) {
+          overflow = true;
+          break;
+        }
+        uint32_t op = sp + popc64(bp1 & lanes_below()) + popc64(bp2 & lanes_below());
+        uint32_t ol = nl + popc64(bl1 & lanes_below()) + popc64(bl2 & lanes_below());
+        if (pA) s_stk[op++] = make_int2(A.a, A.b);
+        if (pB) s_stk[op++] = make_int2(B.a, B.b);
+        if (lA) s_leaf[ol++] = make_uint2((uint32_t)e.x, (uint32_t)A.b);
+        if (lB) s_leaf[ol++] = make_uint2((uint32_t)e.y, (uint32_t)B.b);
+        sp += tp;
+        nl += tl;
         wave_sync();
       }
-      if (overflow) sp = 0;
-    } else {
-      const NodeV r = load_node(T, 0);
-      if (overlap(Ulo, Uhi, r.lo, r.hi)) {
-        if (r.a < 0) {
-          if (lane == 0) s_leaf[0] = make_uint2(0u, (uint32_t)r.b);
-          nl = 1;
-        } else {
-          if (lane == 0) s_stk[0] = make_int2(r.a, r.b);
-          sp = 1;
+      if (overflow) {
+        // too wide: this target again per lane, from the slot counts it started with
+        if (a.audit && lane == 0) {  // the walk's audit records of t are void
+          const uint32_t r = atomicAdd(a.audit_cnt, 1u);
+          if (r < a.audit_cap) a.audit[r] = make_uint4(0u, t, 0u, audit_tag(AUDIT_RETRACT, w));
         }
+        wave_sync();
+        uint32_t c = cnt0;
+        npair = npair0;
+        const f3 qc = {s_q[lane * 6 + 0], s_q[lane * 6 + 1], s_q[lane * 6 + 2]};
+        const f3 qp = {s_q[lane * 6 + 3], s_q[lane * 6 + 4], s_q[lane * 6 + 5]};
+        const f3 qlo = {qc.x - margin, qc.y - margin, qc.z - margin}, qhi = {qc.x + margin, qc.y + margin, qc.z + margin};
+        if (active && !lane_target<false, TQ_SDEPTH>(a, m, s_p[lane], s_j[lane], t, qc, qp, qlo, qhi, band, fstk, 64, c,
+                                                  npair))
+          atomicOr(a.flags, 4u);
+        wave_sync();
+        s_cnt[lane] = c;
+        wave_sync();
+        continue;
       }
+      leaves(nl);
     }
     wave_sync();
-    uint32_t rounds = 0;
-    while (sp > 0) {
-      if (++rounds > 2 * nT + 2) {  // a valid tree is walked in fewer rounds
-        overflow = true;
-        break;
-      }
-      const uint32_t k = sp < 64u ? sp : 64u, b0 = sp - k;
-      if (nl + 2 * k > leaf_cap) {  // this round may add 2 leaves per node
-        leaves(nl);
-        nl = 0;
-      }
-      const bool have = lane < k;
-      int2 e = make_int2(0, 0);
-      if (have) e = s_stk[b0 + lane];
-      wave_sync();
-      sp = b0;
-      bool pA = false, pB = false, lA = false, lB = false;
-      NodeV A, B;
-      if (have) {
-        A = load_node(T, e.x);
-        B = load_node(T, e.y);
-        const bool hA = overlap(Ulo, Uhi, A.lo, A.hi), hB = overlap(Ulo, Uhi, B.lo, B.hi);
-        pA = hA && A.a >= 0;
-        pB = hB && B.a >= 0;
-        lA = hA && A.a < 0;
-        lB = hB && B.a < 0;
-      }
-      const uint64_t bp1 = __ballot(pA || pB), bp2 = __ballot(pA && pB);
-      const uint64_t bl1 = __ballot(lA || lB), bl2 = __ballot(lA && lB);
-      const uint32_t tp = popc64(bp1) + popc64(bp2), tl = popc64(bl1) + popc64(bl2);
-      if (sp + tp > stack_cap) {
-        overflow = true;
-        break;
-      }
-      uint32_t op = sp + popc64(bp1 & lanes_below()) + popc64(bp2 & lanes_below());
-      uint32_t ol = nl + popc64(bl1 & lanes_below()) + popc64(bl2 & lanes_below());
-      if (pA) s_stk[op++] = make_int2(A.a, A.b);
-      if (pB) s_stk[op++] = make_int2(B.a, B.b);
-      if (lA) s_leaf[ol++] = make_uint2((uint32_t)e.x, (uint32_t)A.b);
-      if (lB) s_leaf[ol++] = make_uint2((uint32_t)e.y, (uint32_t)B.b);
-      sp += tp;
-      nl += tl;
-      wave_sync();
+    cc = s_cnt[lane];
+    jc = j;
+    if (active) {
+      a.ccount[j] = cc;
+      if (cc > a.cap) atomicOr(a.flags, 2u);
     }
-    if (overflow) {
-      // too wide: this target again per lane, from the slot counts it started with
-      if (a.audit && lane == 0) {  // the walk's audit records of t are void
-        const uint32_t r = atomicAdd(a.audit_cnt, 1u);
-        if (r < a.audit_cap) a.audit[r] = make_uint4(0u, t, 0u, audit_tag(AUDIT_RETRACT, w));
-      }
-      wave_sync();
-      uint32_t c = cnt0;
-      npair = npair0;
-      const f3 qc = {s_q[lane * 6 + 0], s_q[lane * 6 + 1], s_q[lane * 6 + 2]};
-      const f3 qp = {s_q[lane * 6 + 3], s_q[lane * 6 + 4], s_q[lane * 6 + 5]};
-      const f3 qlo = {qc.x - margin, qc.y - margin, qc.z - margin}, qhi = {qc.x + margin, qc.y + margin, qc.z + margin};
-      if (active && !lane_target<false, TQ_SDEPTH>(a, m, s_p[lane], s_j[lane], t, qc, qp, qlo, qhi, band, fstk, 64, c,
-                                                npair))
-        atomicOr(a.flags, 4u);
-      wave_sync();
-      s_cnt[lane] = c;
-      wave_sync();
-      continue;
-    }
-    leaves(nl);
   }
-  wave_sync();
-  const uint32_t c = s_cnt[lane];
-  if (active) {
-    a.ccount[j] = c;
-    if (c > a.cap) atomicOr(a.flags, 2u);
-  }
-  if (CLOTH) tile_cloth(ca, active, j, c, w);
+  if (CLOTH) tile_cloth(ca, active, jc, cc, w);  // every lane (wave reductions)
   wave_stats(a, w, npair);
 }
 

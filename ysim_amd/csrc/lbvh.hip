@@ -905,88 +905,105 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
     return;
   }
   uint32_t clo = (uint32_t)i, chi = (uint32_t)i;  // leaf range of the current node
+  // Local climb: LDS arrivals only, one exit test per outcome and plain node
+  // stores (the branch-light loop keeps the per-level instruction count low;
+  // with 16 waves of a block per CU the loop is issue-bound).
+  bool dev = false;
+  for (int step = 0;; ++step) {
+    const int p = pnext;
+    const uint32_t up = upnext;
+    const uint32_t side = up & 1u;  // 0: child_a (left part of p's range)
+    const uint32_t psize = up >> 1;
+    const uint32_t plo = side ? chi + 1 - psize : clo, phi = side ? chi : clo + psize - 1;
+    // sibling: child_b at split + 1 for a child_a, child_a at split for a
+    // child_b; a leaf when that position ends p's range
+    const uint32_t q = side ? clo - 1 : chi + 1, qe = side ? plo : phi;
+    const int sib = (int)q + (q == qe ? nn - 1 : 0);
+    const uint32_t slot = s + (uint32_t)p - b0;  // in [0, RB) for a local p: an endpoint of its range
+    const bool bad = (p < 0) | (p >= nn - 1) | (sib < 0) | (sib >= 2 * nn - 1) | (step >= 2 * nn);
+    const bool local = !bad & (s + plo >= b0) & (s + phi < b0 + RB);
+    if (!local || slot >= (uint32_t)RB) {
+      dev = !local && !bad;  // else a corrupted topology
+      break;
+    }
+    store_node(child, lo, hi, la, lb, false);
+    float* mine = s_box[slot][side];
+    mine[0] = lo.x; mine[1] = lo.y; mine[2] = lo.z; mine[3] = hi.x; mine[4] = hi.y; mine[5] = hi.z;
+    const uint32_t old = __hip_atomic_fetch_add(&s_cnt[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (old == 0u) return;  // first arrival: the sibling's thread combines
+    const float* o = s_box[slot][side ^ 1u];
+    const f3 omin = {o[0], o[1], o[2]}, omax = {o[3], o[4], o[5]};
+    // union in (child_a, child_b) order, bvh.metal:567-573
+    lo = side ? vmin(omin, lo) : vmin(lo, omin);
+    hi = side ? vmax(omax, hi) : vmax(hi, omax);
+    la = side ? sib : child;
+    lb = side ? child : sib;
+    child = p;
+    clo = plo;
+    chi = phi;
+    const int2 l = s_link[slot];
+    pnext = l.x;
+    upnext = (uint32_t)l.y;
+    if (p == 0) {
+      store_node(0, lo, hi, la, lb, false);
+      return;
+    }
+  }
+  if (!dev) {
+    atomicOr(err, 4u);
+    return;
+  }
+  // Device-wide climb from `child`, whose parent spans blocks: the node goes
+  // out with device-coherent stores (another workgroup reads it as a
+  // sibling), then the second-arrival protocol of k_refit per level, with the
+  // next level's links loaded while the arrival resolves.
+  store_node(child, lo, hi, la, lb, true);
   for (int step = 0; step < 2 * nn; ++step) {
     const int p = pnext;
     const uint32_t up = upnext;
-    if (p < 0 || p >= nn - 1) {
-      atomicOr(err, 4u);
-      return;
-    }
-    const int side = (int)(up & 1u);  // 0: child_a (left part of p's range)
+    if (p < 0 || p >= nn - 1) break;
+    const uint32_t side = up & 1u;
     const uint32_t psize = up >> 1;
     const uint32_t plo = side ? chi + 1 - psize : clo, phi = side ? chi : clo + psize - 1;
-    const bool local = s + plo >= b0 && s + phi < b0 + RB;
-    store_node(child, lo, hi, la, lb, !local);
-    // sibling: child_b (split + 1) for a child_a, child_a (split) for a child_b
-    const uint32_t sp = side ? clo - 1 : chi;
-    const int sib = side ? (sp == plo ? nn - 1 + (int)sp : (int)sp)
-                         : (sp + 1 == phi ? nn + (int)sp : (int)sp + 1);
-    if (sib < 0 || sib >= 2 * nn - 1) {
-      atomicOr(err, 4u);
-      return;
-    }
-    if (local) {
-      const uint32_t slot = s + (uint32_t)p - b0;
-      if (slot >= (uint32_t)RB) {  // p is an endpoint of its range in a valid tree
-        atomicOr(err, 4u);
-        return;
-      }
-      float* mine = s_box[slot][side];
-      mine[0] = lo.x; mine[1] = lo.y; mine[2] = lo.z; mine[3] = hi.x; mine[4] = hi.y; mine[5] = hi.z;
-      const uint32_t old = __hip_atomic_fetch_add(&s_cnt[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (old == 0u) return;  // first arrival
-      const float* o = s_box[slot][side ^ 1];
-      const f3 omin = {o[0], o[1], o[2]}, omax = {o[3], o[4], o[5]};
-      if (side == 0) {
-        lo = vmin(lo, omin);
-        hi = vmax(hi, omax);
-      } else {
-        lo = vmin(omin, lo);
-        hi = vmax(omax, hi);
-      }
-      const int2 l = s_link[slot];
-      pnext = l.x;
-      upnext = (uint32_t)l.y;
+    const uint32_t sq = side ? clo - 1 : chi + 1, sqe = side ? plo : phi;
+    const int sib = (int)sq + (sq == sqe ? nn - 1 : 0);
+    if (sib < 0 || sib >= 2 * nn - 1) break;
+    pnext = P[p];
+    upnext = UP[p];
+    uint32_t old;
+    if (FORMAL) {
+      old = __hip_atomic_fetch_add(C + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     } else {
-      // the next level's links load while this arrival resolves
-      pnext = P[p];
-      upnext = UP[p];
-      uint32_t old;
-      if (FORMAL) {
-        old = __hip_atomic_fetch_add(C + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-      } else {
-        wait_all_mem();
-        old = __hip_atomic_fetch_add(C + p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-      if ((old & 1u) == 0u) return;
-      const float* q = reinterpret_cast<const float*>(T + sib);
-      f3 smin, smax;
-      if (FORMAL) {
-        smin = f3{q[0], q[1], q[2]};
-        smax = f3{q[4], q[5], q[6]};
-      } else {
-        const uint64_t m01 = ld_agent_u64(reinterpret_cast<const uint64_t*>(q + 0));
-        const uint64_t x01 = ld_agent_u64(reinterpret_cast<const uint64_t*>(q + 4));
-        smin = f3{lo_f(m01), hi_f(m01), ld_agent(q + 2)};
-        smax = f3{lo_f(x01), hi_f(x01), ld_agent(q + 6)};
-      }
-      if (side == 0) {
-        lo = vmin(lo, smin);
-        hi = vmax(hi, smax);
-      } else {
-        lo = vmin(smin, lo);
-        hi = vmax(smax, hi);
-      }
+      wait_all_mem();
+      old = __hip_atomic_fetch_add(C + p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if ((old & 1u) == 0u) return;
+    const float* q = reinterpret_cast<const float*>(T + sib);
+    f3 smin, smax;
+    if (FORMAL) {
+      smin = f3{q[0], q[1], q[2]};
+      smax = f3{q[4], q[5], q[6]};
+    } else {
+      const uint64_t m01 = ld_agent_u64(reinterpret_cast<const uint64_t*>(q + 0));
+      const uint64_t x01 = ld_agent_u64(reinterpret_cast<const uint64_t*>(q + 4));
+      smin = f3{lo_f(m01), hi_f(m01), ld_agent(q + 2)};
+      smax = f3{lo_f(x01), hi_f(x01), ld_agent(q + 6)};
+    }
+    if (side == 0) {
+      lo = vmin(lo, smin);
+      hi = vmax(hi, smax);
+    } else {
+      lo = vmin(smin, lo);
+      hi = vmax(smax, hi);
     }
     la = side ? sib : child;
     lb = side ? child : sib;
     child = p;
     clo = plo;
     chi = phi;
-    if (p == 0) {
-      store_node(0, lo, hi, la, lb, false);
-      return;
-    }
+    // the root is read by no other workgroup; every other spanning node is
+    store_node(p, lo, hi, la, lb, p != 0);
+    if (p == 0) return;
   }
   atomicOr(err, 4u);
 }

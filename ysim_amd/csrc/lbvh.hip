@@ -375,53 +375,113 @@ __global__ __launch_bounds__(256) void k_hierarchy(SegTable ts, MeshTables mt, c
     if (j == 0 && skip.note) __hip_atomic_store(skip.note, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     return;
   }
-  if (j >= n) return;
-  const uint32_t sg = seg_of(ts, j);
+  // lanes past the items stay for the wave's cooperative searches below
+  const bool live = j < n;
+  if (__ballot(live) == 0) return;
+  const uint32_t sg = seg_of(ts, live ? j : n - 1);
   const uint32_t m = ts.mesh[sg];
   const uint32_t s = ts.off[sg];
   const int nn = (int)(ts.off[sg + 1] - s);
-  const int i = (int)(j - s);
+  const int i = (int)((live ? j : n - 1) - s);
   const uint32_t base = mt.noff[m];
   ysim_bvh_node* T = tree + base;
   int32_t* P = parent + base;
   const bool wide = code_bits > 32;
   const uint64_t mask = (code_bits >= 64) ? ~0ull : ((1ull << code_bits) - 1);
   KarrasSeg<K> K_{keys + s, vals + s, nn, mask, wide};
-  // the sorted facet order seeds the next incremental build
-  if (perm_out) perm_out[j] = K_.idx[i];
-  // leaf slot nn-1+i: child_a = -1, child_b = facet id (the LDS refit writes
-  // the node when rl.fid is set)
-  if (rl.fid) {
-    rl.fid[base + nn - 1 + i] = K_.idx[i];
-  } else {
-    T[nn - 1 + i].child_a = -1;
-    T[nn - 1 + i].child_b = (int32_t)K_.idx[i];
+  if (live) {
+    // the sorted facet order seeds the next incremental build
+    if (perm_out) perm_out[j] = K_.idx[i];
+    // leaf slot nn-1+i: child_a = -1, child_b = facet id (the LDS refit writes
+    // the node when rl.fid is set)
+    if (rl.fid) {
+      rl.fid[base + nn - 1 + i] = K_.idx[i];
+    } else {
+      T[nn - 1 + i].child_a = -1;
+      T[nn - 1 + i].child_b = (int32_t)K_.idx[i];
+    }
+    if (i == 0) P[0] = -1;
   }
-  if (i == 0) P[0] = -1;
-  if (i >= nn - 1) return;
-  visit[base + i] = 0;
-  // determineRange, bvh.metal:240-258
-  int d = (K_.delta(i, i + 1) - K_.delta(i, i - 1) >= 0) ? 1 : -1;
-  int dmin = K_.delta(i, i - d);
-  int lmax = 2;
-  while (K_.delta(i, i + lmax * d) > dmin) lmax <<= 1;
-  int l = 0;
-  for (int t = lmax >> 1; t >= 1; t >>= 1)
-    if (K_.delta(i, i + (l + t) * d) > dmin) l += t;
-  int jj = i + l * d;
-  int lo = d < 0 ? jj : i, hi = d < 0 ? i : jj;
+  // Internal node i (i < nn - 1; the other lanes only help the wave's
+  // cooperative searches below).
+  const bool node = live && i < nn - 1;
+  if (node) visit[base + i] = 0;
+  // determineRange, bvh.metal:240-258.  delta(i, i + k d) does not increase
+  // with k (sorted (code, index) keys), so the range end is the last k whose
+  // delta exceeds dmin.  The doubling search stops at k = 64 (HSEQ); nodes of
+  // longer ranges are searched by the whole wave, 64 probes per round, which
+  // finds the same boundary in ~log64 dependent rounds instead of ~2 log2.
+  constexpr int HSEQ = 64;
+  int d = 1, dmin = -1, l = 0;
+  bool coop = false;
+  if (node) {
+    d = (K_.delta(i, i + 1) - K_.delta(i, i - 1) >= 0) ? 1 : -1;
+    dmin = K_.delta(i, i - d);
+    int lmax = 2;
+    while (lmax <= HSEQ && K_.delta(i, i + lmax * d) > dmin) lmax <<= 1;
+    coop = lmax > HSEQ;
+    if (!coop)
+      for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (K_.delta(i, i + (l + t) * d) > dmin) l += t;
+  }
   // findSplit, bvh.metal:197-224, measuring the common prefix with delta()
   // (code, then index tie-break) in every case; the reference's midpoint
   // shortcut for equal end codes contradicts that tie-break for runs of >= 3
-  // equal codes.  Identical split whenever the end codes differ.
-  const int common = K_.delta(lo, hi);
-  uint32_t sp = (uint32_t)lo, step = (uint32_t)(hi - lo);
-  do {
-    step = (step + 1) >> 1;
-    uint32_t ns = sp + step;
-    if (ns < (uint32_t)hi && K_.delta(lo, (int)ns) > common) sp = ns;
-  } while (step > 1);
-  const uint32_t split = sp;
+  // equal codes.  Identical split whenever the end codes differ.  The split
+  // is the last position x in [lo, hi) with delta(lo, x) > delta(lo, hi)
+  // (lo when none): again a monotone boundary.
+  int lo = 0, hi = 0;
+  uint32_t split = 0;
+  if (node && !coop) {
+    const int jj = i + l * d;
+    lo = d < 0 ? jj : i;
+    hi = d < 0 ? i : jj;
+    const int common = K_.delta(lo, hi);
+    uint32_t sp = (uint32_t)lo, step = (uint32_t)(hi - lo);
+    do {
+      step = (step + 1) >> 1;
+      uint32_t ns = sp + step;
+      if (ns < (uint32_t)hi && K_.delta(lo, (int)ns) > common) sp = ns;
+    } while (step > 1);
+    split = sp;
+  }
+  // Cooperative searches, one long-range node of the wave at a time (all its
+  // lanes are in one segment: coop nodes need 64 keys on one side).  Each
+  // round probes 64 evenly spaced positions of the open interval; the
+  // ballot of a monotone predicate is a prefix, whose length narrows it.
+  const uint32_t lane = __lane_id();
+  auto boundary = [&](int a, int b, auto&& pred) {  // last x in [a, b] with pred(x); pred(a) holds
+    while (b > a) {
+      const int stride = (b - a + 63) / 64;
+      const int x = a + ((int)lane + 1) * stride;
+      const uint64_t hit = __ballot(x <= b && pred(x));
+      const int c = (int)__popcll(hit);
+      b = min(b, a + (c + 1) * stride - 1);
+      a += c * stride;
+    }
+    return a;
+  };
+  uint64_t todo = __ballot(coop);
+  while (todo) {
+    const int src = (int)__builtin_ctzll(todo);
+    todo &= todo - 1;
+    const int ni = __shfl(i, src, 64), nd = __shfl(d, src, 64), ndm = __shfl(dmin, src, 64);
+    const uint32_t ns0 = __shfl(s, src, 64);  // the node's segment (lanes may lie in others)
+    const int nnn = __shfl(nn, src, 64);
+    const KarrasSeg<K> KN{keys + ns0, vals + ns0, nnn, mask, wide};
+    const int kmax = nd > 0 ? nnn - 1 - ni : ni;  // delta is -1 past the segment
+    const int nl = boundary(HSEQ, kmax, [&](int k) { return KN.delta(ni, ni + k * nd) > ndm; });
+    const int jj = ni + nl * nd;
+    const int nlo = nd < 0 ? jj : ni, nhi = nd < 0 ? ni : jj;
+    const int common = KN.delta(nlo, nhi);
+    const int nsp = boundary(nlo, nhi - 1, [&](int x) { return KN.delta(nlo, x) > common; });
+    if ((int)lane == src) {
+      lo = nlo;
+      hi = nhi;
+      split = (uint32_t)nsp;
+    }
+  }
+  if (!node) return;
   int ca = ((int)split == lo) ? (int)split + nn - 1 : (int)split;
   int cb = ((int)split + 1 == hi) ? (int)split + nn : (int)split + 1;
   if (!rl.fid) {

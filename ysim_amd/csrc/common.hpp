@@ -276,16 +276,58 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblocks) {
   return x * per + min(x, rem) + k;
 }
 
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o));
-  return v;
+// Cross-lane reductions of a wave64 without LDS: DPP moves inside rows of 16
+// lanes, then the gfx950 permlane swaps across rows.  Each step pairs every
+// lane with one partner, so every lane ends with the result:
+//   lane ^ 1, lane ^ 2 (quad_perm), lane 7 - i within 8 (row_half_mirror):
+//     the 8-lane group; lane 15 - i within 16 (row_mirror): the row;
+//   rows 0/1 and 2/3 (permlane16_swap), then the two halves (permlane32_swap).
+// lane ^ 8 (row_ror:8), then the two swaps, reduce the 8 lanes of equal lane & 7.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
 }
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
-  return v;
+constexpr int DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_HALF_MIRROR = 0x141, DPP_MIRROR = 0x140, DPP_ROR8 = 0x128;
+template <class Op>
+__device__ __forceinline__ uint32_t swap_rows_u32(uint32_t v, Op op) {
+  const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+  return op((uint32_t)p[0], (uint32_t)p[1]);
 }
+template <class Op>
+__device__ __forceinline__ uint32_t swap_halves_u32(uint32_t v, Op op) {
+  const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+  return op((uint32_t)p[0], (uint32_t)p[1]);
+}
+template <class Op>
+__device__ __forceinline__ uint32_t group8_reduce_u32(uint32_t v, Op op) {
+  v = op(v, dpp_u32<DPP_XOR1>(v));
+  v = op(v, dpp_u32<DPP_XOR2>(v));
+  return op(v, dpp_u32<DPP_HALF_MIRROR>(v));
+}
+template <class Op>
+__device__ __forceinline__ uint32_t wave_reduce_u32(uint32_t v, Op op) {
+  v = group8_reduce_u32(v, op);
+  v = op(v, dpp_u32<DPP_MIRROR>(v));
+  return swap_halves_u32(swap_rows_u32(v, op), op);
+}
+// over the lanes of equal lane & 7 (lane ^ 8, ^ 16, ^ 32)
+template <class Op>
+__device__ __forceinline__ uint32_t column8_reduce_u32(uint32_t v, Op op) {
+  v = op(v, dpp_u32<DPP_ROR8>(v));
+  return swap_halves_u32(swap_rows_u32(v, op), op);
+}
+struct MinU32 {
+  __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return min(a, b); }
+};
+struct MaxU32 {
+  __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return max(a, b); }
+};
+struct AddU32 {
+  __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a + b; }
+};
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) { return wave_reduce_u32(v, MinU32{}); }
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) { return wave_reduce_u32(v, MaxU32{}); }
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) { return wave_reduce_u32(v, AddU32{}); }
 
 // Box of a whole grid of NT-thread blocks into acc[0..5] (ordered keys fkey:
 // min x, y, z, max x, y, z), from one (lo, hi) per thread (+inf / -inf for a

@@ -281,9 +281,7 @@ __global__ __launch_bounds__(128) void k_hash_query(HashArgs h, QueryArgs a) {
   }
   if (!EMIT_PAIRS && a.wstats) {
     // per-wave running count, one word per wave (no same-address atomics)
-    uint32_t s = npair;
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+    const uint32_t s = wave_sum_u32(npair);
     if ((threadIdx.x & 63) == 0 && s) a.wstats[j >> 6] += s;
   }
 }

@@ -55,9 +55,7 @@ __device__ __forceinline__ void wave_sync() {
 
 // Adds a wave's candidate count to its own statistics word.
 __device__ __forceinline__ void wave_stats(const QueryArgs& a, uint32_t wave, uint32_t npair) {
-  uint32_t s = npair;
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+  const uint32_t s = wave_sum_u32(npair);
   if (a.wstats && __lane_id() == 0 && s) a.wstats[wave] += s;
 }
 
@@ -285,12 +283,23 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
     s_cnt[lane] = 0;
     s_p[lane] = p;
     s_j[lane] = j;
-    auto reduce = [&](int o) {
-      lo = {fminf(lo.x, __shfl_xor(lo.x, o, 64)), fminf(lo.y, __shfl_xor(lo.y, o, 64)),
-            fminf(lo.z, __shfl_xor(lo.z, o, 64))};
-      hi = {fmaxf(hi.x, __shfl_xor(hi.x, o, 64)), fmaxf(hi.y, __shfl_xor(hi.y, o, 64)),
-            fmaxf(hi.z, __shfl_xor(hi.z, o, 64))};
+    // min / max of the box bounds over lane groups, with the LDS-free
+    // reductions of common.hpp (exact in any order; the boxes are only
+    // compared, so the sign of a zero bound does not matter)
+    const auto fmin_u = [](uint32_t x, uint32_t y) { return __float_as_uint(fminf(__uint_as_float(x), __uint_as_float(y))); };
+    const auto fmax_u = [](uint32_t x, uint32_t y) { return __float_as_uint(fmaxf(__uint_as_float(x), __uint_as_float(y))); };
+    auto reduce = [&](auto&& red) {
+      lo = {__uint_as_float(red(__float_as_uint(lo.x), fmin_u)), __uint_as_float(red(__float_as_uint(lo.y), fmin_u)),
+            __uint_as_float(red(__float_as_uint(lo.z), fmin_u))};
+      hi = {__uint_as_float(red(__float_as_uint(hi.x), fmax_u)), __uint_as_float(red(__float_as_uint(hi.y), fmax_u)),
+            __uint_as_float(red(__float_as_uint(hi.z), fmax_u))};
     };
+    const auto rows8 = [](uint32_t v, auto op) { return group8_reduce_u32(v, op); };
+    const auto rows_to_wave = [](uint32_t v, auto op) {
+      v = op(v, dpp_u32<DPP_MIRROR>(v));
+      return swap_halves_u32(swap_rows_u32(v, op), op);
+    };
+    const auto cols8 = [](uint32_t v, auto op) { return column8_reduce_u32(v, op); };
     // row r holds lanes 8 r .. 8 r + 7, column c lanes c, c + 8, .., c + 56; a
     // vertex whose box overlaps a leaf's lies in a row and a column that do too
     auto put = [&](uint32_t b) {
@@ -302,19 +311,13 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
       s_rc[b * 6 + 5] = hi.z;
     };
     const f3 lo1 = lo, hi1 = hi;
-    reduce(1);
-    reduce(2);
-    reduce(4);
+    reduce(rows8);
     if ((lane & 7u) == 0) put(lane >> 3);
-    reduce(8);
-    reduce(16);
-    reduce(32);
+    reduce(rows_to_wave);
     const f3 Ulo = lo, Uhi = hi;  // union of the wave's boxes
     lo = lo1;
     hi = hi1;
-    reduce(8);
-    reduce(16);
-    reduce(32);
+    reduce(cols8);
     if (lane < 8) put(8 + lane);
     // the prefetched cut of the first target: pending pairs and leaves
     // (sp_pre, nl_pre; no leaf flush: the cut fits the list)

@@ -204,6 +204,8 @@ def batched_vs_sequential(ctx, substeps: int, log):
         "sequential_plain_substeps_per_s": round(substeps / ts, 2),
         "instance_substeps_per_s": round(nI * substeps / tb, 1),
         "speedup": round(ts / tb, 2), "identical_results": identical, "flags": flags,
+        "variant": "segmented hipGraph: one launch per stage over all instances (inter-kernel gaps measured at "
+                   "0.25% of the substep, profiles/batched_c3_graph_gaps.md, so no persistent work-queue kernel)",
     }
 
 
@@ -302,6 +304,25 @@ def broad_query_profile(n: int, reps: int = 5):
     print(json.dumps({"broad_query_pairs": out, "workload": f"{n}x{n} fold vs its own 30-bit tree"}))
 
 
+def batched_profile(substeps: int):
+    """The batched C3 scene's hipGraph replay alone (after an untimed warm-up
+    replay), for a kernel trace: the gaps between consecutive dispatches of
+    the replayed substeps are the cost a persistent kernel would remove."""
+    from ysim_amd import scenes
+    from ysim_amd.hip import HipContext, HipScene
+    ctx = HipContext(0)
+    sc = scenes.config_c3(substeps=substeps)
+    b = HipScene(ctx, sc)
+    b.step(2, graph=True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    b.step(substeps, graph=True)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    print(json.dumps({"batched_profile": {"instances": len(sc.meshes) // 2, "substeps": substeps,
+                                          "substeps_per_s": round(substeps / dt, 2), "flags": b.flags()}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -325,9 +346,14 @@ def main():
                     help="skip the C5 LBVH-vs-hash, adjoint and copy-ceiling measurements")
     ap.add_argument("--broad-query-profile", action="store_true",
                     help="only run the stage-level broad query (per-pair vs segmented emission) for a kernel trace")
+    ap.add_argument("--batched-profile", action="store_true",
+                    help="only replay the batched C3 graph (--batched-substeps substeps) for a kernel trace")
     args = ap.parse_args()
     if args.broad_query_profile:
         broad_query_profile(args.n)
+        return
+    if args.batched_profile:
+        batched_profile(args.batched_substeps)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on exactly one GPU (single-GPU scope); "

@@ -209,6 +209,54 @@ def batched_vs_sequential(ctx, substeps: int, log):
     }
 
 
+def contact_active(ctx, tuning: dict, log, start: int = 300, length: int = 20, reps: int = 10):
+    """The 512 x 512 fold after its layers meet: stepped `start` substeps (the
+    first contacts appear near substep 290; past ~320 a vertex holds more
+    contacts than its slots and the cloth soon explodes), snapshotted, then
+    `reps` episodes of `length` substeps restored from that state and replayed
+    with the headline's tuning.  Reports the rate, contacts and candidate
+    pairs per substep, and the builds the episodes deferred."""
+    from ysim_amd import scenes
+    from ysim_amd.hip import HipScene
+    sc = scenes.config_c2()
+    hs = HipScene(ctx, sc)
+    hs.set_tuning(**{k: int(v) for k, v in tuning.items() if k in (
+        "sort_digit_bits", "sort_items", "query_block", "cloth_block", "refit_block", "sort_incremental",
+        "query_waves", "query_tile", "build_mode")})
+    hs.step(start, graph=True)
+    hs.snapshot()
+    hs.step(length, graph=True)  # capture + warm-up
+    hs.restore()
+    ctx.sync()
+    hs.counters(reset=True)
+    d0 = hs.deferred_builds()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        hs.restore()
+        hs.step(length, graph=True)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    cnt = hs.counters(reset=True)
+    # contacts per substep over one more episode, substep by substep (eager)
+    hs.restore()
+    per = []
+    for _ in range(length):
+        hs.step(1, graph=False)
+        per.append(hs.counters(reset=True)["contacts"])
+    n = reps * length
+    log(f"contact-active fold: {n / dt:.1f} substeps/s over substeps {start}-{start + length}, "
+        f"contacts {per[0]}..{per[-1]}")
+    return {
+        "workload": f"{sc.name} from substep {start} (the layers in contact): {reps} episodes of {length} "
+                    "substeps restored on device, hipGraph, the headline's tuning",
+        "substeps_per_s": round(n / dt, 2), "ms_per_substep": round(1e3 * dt / n, 5),
+        "candidate_pairs_per_substep": round(cnt["pairs"] / max(1, cnt["steps"]), 1),
+        "contacts_first_substep": per[0], "contacts_last_substep": per[-1],
+        "contacts_mean_per_substep": round(sum(per) / len(per), 1),
+        "deferred_builds_timed": hs.deferred_builds() - d0, "flags": hs.flags(),
+    }
+
+
 def lbvh_vs_hash(ctx, substeps: int, log):
     """1024 x 1024 self-colliding cloth over 4 level-6 icospheres: LBVH traversal
     vs spatial hash (2^22 slots, cell 2 * thickness), same candidate set; both
@@ -530,6 +578,8 @@ def main():
     if not args.no_batched:
         out["batched"] = batched_vs_sequential(ctx, args.batched_substeps, log)
     if not args.no_extras:
+        if args.n == 512:
+            out["c2_contact_active"] = contact_active(ctx, tuning, log)
         out["c5_lbvh_vs_hash"] = lbvh_vs_hash(ctx, 20, log)
         out["adjoint"] = adjoint_timing(ctx, log)
     if not args.no_parity and args.n == 512:

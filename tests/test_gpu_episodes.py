@@ -10,6 +10,8 @@
   oracle (state, contacts, last substep's candidates).
 * 1024 x 1024 + 4 level-6 icospheres: the LBVH candidate set at substep 0
   and at substep 4 equals the oracle's.
+* The fold past the meeting of its layers (substeps 300-320, with contacts):
+  the oracle restarted from the GPU state follows it bit for bit.
 (Simulator::update substep order, include/sim/simulator.hpp:3512-3815.)
 """
 import numpy as np
@@ -69,6 +71,45 @@ def test_c2_episode_bench_tuning(ctx, c2_oracle_episode, rb):
         assert np.array_equal(nodes[f], ref["nodes"][f]), f
     got, _ = hs.audit()
     assert_same_candidates(got, ref["pairs"], f"fold substep {ref['steps'] - 1}, rb={rb}")
+    assert hs.flags() == 0
+
+
+def test_c2_contact_active_from_gpu_state(ctx, oracle_threads):
+    """The fold after its layers meet: the GPU runs substeps 0-300 under the
+    bench tuning, the oracle restarts from that state, and over substeps
+    300-320 (contacts grow from ~70 to ~1,800 per substep, no slot overflow)
+    both agree bit for bit every 5 substeps: state, contacts and, on the
+    last substep, the audited candidate set of the fused kernel."""
+    sc = scenes.config_c2()
+    hs = HipScene(ctx, sc)
+    hs.set_tuning(sort_digit_bits=8, sort_items=16, sort_incremental=1, sort_hybrid=0, query_tile=1,
+                  refit_block=1024, build_mode=0)
+    hs.step(300, graph=True)
+    os_ = OracleScene(sc)
+    os_.set_state(*hs.state())
+    counts = []
+    for k in range(4):
+        if k == 3:
+            hs.step(4, graph=True)
+            os_.step(4)
+            want = os_.pairs()
+            hs.set_audit(1 << 21)
+            hs.step(1)
+            os_.step(1)
+        else:
+            hs.step(5, graph=True)
+            os_.step(5)
+        for a, b in zip(hs.state(), os_.state()):
+            assert np.array_equal(a, b), f"substep {305 + 5 * k}"
+        off, con = hs.contacts()
+        ooff, ocon = os_.contacts()
+        assert np.array_equal(off, ooff)
+        assert_contacts_equal(con, ocon, f"contact-active fold, substep {305 + 5 * k}")
+        counts.append(len(con))
+    got, redo = hs.audit()
+    assert_same_candidates(got, want, "contact-active fold, substep 319")
+    print(f"contacts at substeps 305..320: {counts}")
+    assert counts[0] > 0 and counts[-1] > counts[0]
     assert hs.flags() == 0
 
 

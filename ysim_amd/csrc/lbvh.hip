@@ -408,10 +408,13 @@ __global__ __launch_bounds__(256) void k_hierarchy(SegTable ts, MeshTables mt, c
   if (node) visit[base + i] = 0;
   // determineRange, bvh.metal:240-258.  delta(i, i + k d) does not increase
   // with k (sorted (code, index) keys), so the range end is the last k whose
-  // delta exceeds dmin.  The doubling search stops at k = 64 (HSEQ); nodes of
-  // longer ranges are searched by the whole wave, 64 probes per round, which
-  // finds the same boundary in ~log64 dependent rounds instead of ~2 log2.
-  constexpr int HSEQ = 64;
+  // delta exceeds dmin.  The doubling search stops at k = 1024 (HSEQ); nodes
+  // of longer ranges (about one in 500) are searched by the whole wave, 64
+  // probes per round, which finds the same boundary in ~log64 dependent
+  // rounds instead of ~2 log2.  On the 512 x 512 fold: 29.2 -> 26.2 us; a
+  // threshold of 64 sends ~16 nodes of each wave through the cooperative
+  // rounds one after another (28.9 us) and fetches 3.6x the keys.
+  constexpr int HSEQ = 1024;
   int d = 1, dmin = -1, l = 0;
   bool coop = false;
   if (node) {
@@ -445,8 +448,8 @@ __global__ __launch_bounds__(256) void k_hierarchy(SegTable ts, MeshTables mt, c
     } while (step > 1);
     split = sp;
   }
-  // Cooperative searches, one long-range node of the wave at a time (all its
-  // lanes are in one segment: coop nodes need 64 keys on one side).  Each
+  // Cooperative searches, one long-range node of the wave at a time (lanes
+  // of other segments help with the node's own segment table).  Each
   // round probes 64 evenly spaced positions of the open interval; the
   // ballot of a monotone predicate is a prefix, whose length narrows it.
   const uint32_t lane = __lane_id();

@@ -172,6 +172,7 @@ struct ysim_scene {
   uint32_t nm = 0, Vtot = 0, Ttot = 0, Ntot = 0;
   std::vector<ysim_mesh_desc> meshes;
   std::vector<uint32_t> voff, foff, noff;
+  std::vector<uint32_t> h_tgt_off, h_tgt, h_grid_n;  // host copies of d_tgt_off, d_tgt, d_grid_n
   std::vector<uint32_t> dyn, stat;
   std::vector<uint32_t> dyn_voff;        // host copy of dyn_vsegs offsets
   uint32_t Vdyn = 0, Tdyn = 0;
@@ -570,6 +571,8 @@ struct ysim_scene {
     d_tgt_off = d_tgt = nullptr;
     YSIM_TRY(put(&d_tgt_off, toff));
     YSIM_TRY(put(&d_tgt, tg));
+    h_tgt_off = toff;
+    h_tgt = tg;
     return YSIM_OK;
   }
 
@@ -724,7 +727,7 @@ struct ysim_scene {
       // check failed (its kernels return at once otherwise)
       YSIM_TRY(launch_morton_inc(st, g.tsegs, mt, xc, facets, box_acc, g.code_bits, g.key64, g.perm, keys[0],
                                  vals[0], g.T, need_full, parts_box ? cloth_parts : nullptr,
-                                 parts_box ? parts_count : 0u, links.cut_cnt));
+                                 parts_box ? parts_count : 0u, links.cut_cnt, links.cut));
       if (timing) YSIM_HIP_TRY(hipEventRecord(ev[2], st));
       YSIM_TRY(launch_incremental_fix(st, g.key64, keys[0], vals[0], g.T, need_full, inc_bcnt, inc_edges));
       const bool defer = capturing && !agglo;
@@ -747,7 +750,7 @@ struct ysim_scene {
       return YSIM_OK;
     }
     YSIM_TRY(launch_morton(st, g.tsegs, mt, xc, facets, box_acc, g.code_bits, g.key64, keys[0], vals[0], g.T,
-                           g.plan, hist, counters, epoch, nullptr, links.cut_cnt));
+                           g.plan, hist, counters, epoch, nullptr, links.cut_cnt, links.cut));
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[2], st));
     if (g.plan.hybrid)
       YSIM_TRY(launch_sort_hybrid(st, g.plan, keys[0], vals[0], keys[1], vals[1], hist, status, counters, epoch,
@@ -806,6 +809,14 @@ struct ysim_scene {
     a.block = tune.query_block;
     a.waves = tune.query_waves;
     a.tile = tune.query_tile;
+    if (dyn.size() == 1 && h_tgt_off.size() == 2 && h_tgt_off[1] > h_tgt_off[0]) {
+      const uint32_t m = dyn[0], t = h_tgt[h_tgt_off[0]];
+      a.pre_t = (int32_t)t;
+      a.seg0_mesh = m;
+      a.seg0_voff = voff[m];
+      a.seg0_grid_n = h_grid_n[m];
+      a.pre_tree = tree + noff[t];
+    }
     return a;
   }
 
@@ -1764,6 +1775,7 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
         gn[m] = N;
     }
     YSIM_TRY(s.put(&s.d_grid_n, gn));
+    s.h_grid_n = gn;
   }
   YSIM_TRY(s.put(&s.d_shape, shp));
   YSIM_TRY(s.put(&s.d_margin, margin));
@@ -1777,6 +1789,7 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
   YSIM_TRY(s.alloc(&s.links.cut_cnt, num_meshes));
   YSIM_TRY(s.zero(s.links.cut_cnt, num_meshes));
   YSIM_TRY(s.alloc(&s.links.cut, (size_t)num_meshes * CUT_CAP));
+  YSIM_HIP_TRY(hipMemsetAsync(s.links.cut, 0xff, 4 * (size_t)num_meshes * CUT_CAP, s.st));  // -1: no entry
   YSIM_TRY(s.zero(s.tree, 8 * (size_t)s.Ntot));
   YSIM_TRY(s.zero(s.visit, s.Ntot));
   // segment tables

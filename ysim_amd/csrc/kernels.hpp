@@ -49,23 +49,26 @@ int launch_box(hipStream_t s, SegTable vsegs, MeshTables mt, const float* x, uin
 // of all sort passes of `plan`; zeroes the sort's block counters and advances
 // its look-back epoch (either may be null when no sort follows).
 // run_flag (optional): the kernel does nothing while *run_flag == 0.
-// cut_reset (optional, per mesh): zeroed for the group's meshes.
+// cut_reset (optional, per mesh): zeroed for the group's meshes; cut_fill
+// (optional, the cut lists at CUT_CAP per mesh): a one-segment group's list
+// is filled with -1, so readers may load entries before the count.
 int launch_morton(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x,
                   const uint32_t* facets, const uint32_t* box_acc, uint32_t code_bits,
                   bool key64, void* keys, uint32_t* vals, uint32_t nitems, const SortPlan& plan,
                   uint32_t* hist, uint32_t* counters, uint32_t* epoch, const uint32_t* run_flag = nullptr,
-                  uint32_t* cut_reset = nullptr);
+                  uint32_t* cut_reset = nullptr, int32_t* cut_fill = nullptr);
 // Incremental S1: keys in the order perm (previous sorted facets), each
 // 2048-key window sorted by (key, facet); zeroes *need_full.
 // With nparts > 0 (one segment only) the box is the reduction of nparts
 // 6-word records at box_parts (the cloth kernel's per-block boxes); every
 // block reduces them and block 0 writes the box to box_acc for the build's
 // later kernels.  Otherwise box_acc holds the box per segment.
-// cut_reset (per mesh): zeroed for the group's meshes (their build starts).
+// cut_reset (per mesh): zeroed for the group's meshes (their build starts);
+// cut_fill as for launch_morton.
 int launch_morton_inc(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x, const uint32_t* facets,
                       uint32_t* box_acc, uint32_t code_bits, bool key64, const uint32_t* perm, void* keys,
                       uint32_t* vals, uint32_t nitems, uint32_t* need_full, const uint32_t* box_parts = nullptr,
-                      uint32_t nparts = 0, uint32_t* cut_reset = nullptr);
+                      uint32_t nparts = 0, uint32_t* cut_reset = nullptr, int32_t* cut_fill = nullptr);
 // Extra links of a scene build for the LDS refit, written by the hierarchy
 // kernel (null: not written, and the refit takes the device-wide path only).
 // With fid set, the hierarchy writes no tree nodes: the LDS refit that follows
@@ -78,7 +81,8 @@ struct RefitLinks {
   // parent has more (a frontier that covers every leaf once), appended in any
   // order to cut + CUT_CAP mesh, counted in cut_cnt[mesh] (past CUT_CAP: not
   // listed, and readers walk from the root).  The build's Morton kernel zeroes
-  // the group's counts, so a build that writes no cut leaves them at zero.
+  // the group's counts (and fills a one-segment group's list with -1), so a
+  // build that writes no cut leaves them at zero.
   uint32_t* cut_cnt = nullptr;
   int32_t* cut = nullptr;
 };
@@ -158,6 +162,13 @@ struct QueryArgs {
   uint4* audit = nullptr;
   uint32_t* audit_cnt = nullptr;  // records reserved (may exceed audit_cap)
   uint32_t audit_cap = 0;
+  // One query segment (one dynamic mesh): its mesh, vertex offset and grid
+  // side, and its first target's mesh and tree, given by the host so that
+  // the tile kernel's first loads wait on no table lookups (pre_t < 0: not
+  // given; the first target is then read from tgt_off / tgt).
+  int32_t pre_t = -1;
+  uint32_t seg0_mesh = 0, seg0_voff = 0, seg0_grid_n = 0;
+  const ysim_bvh_node* pre_tree = nullptr;
 };
 // Audit record of one candidate found by a per-lane walk (a.audit set).
 __device__ __forceinline__ void audit_lane(const QueryArgs& a, uint32_t j, uint32_t t, uint32_t fid) {

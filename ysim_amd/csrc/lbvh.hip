@@ -192,13 +192,16 @@ __global__ __launch_bounds__(256) void k_morton(SegTable ts, MeshTables mt, cons
                                                 uint32_t* __restrict__ hist,
                                                 uint32_t* __restrict__ counters, uint32_t* __restrict__ epoch,
                                                 const uint32_t* __restrict__ run_flag,
-                                                uint32_t* __restrict__ cut_reset) {
+                                                uint32_t* __restrict__ cut_reset, int32_t* __restrict__ cut_fill) {
   constexpr uint32_t DIGITS = 1u << DB;
   constexpr uint32_t HW = NH * DIGITS;
   __shared__ uint32_t s_h[HW];
   if (run_flag && *run_flag == 0u) return;  // the incremental sort succeeded: nothing to do
-  if (cut_reset && blockIdx.x == 0)
+  if (cut_reset && blockIdx.x == 0) {
     for (uint32_t t = threadIdx.x; t < ts.nseg; t += 256) cut_reset[ts.mesh[t]] = 0u;
+    if (cut_fill && ts.nseg == 1)
+      for (uint32_t k = threadIdx.x; k < CUT_CAP; k += 256) cut_fill[(size_t)ts.mesh[0] * CUT_CAP + k] = -1;
+  }
   for (uint32_t i = threadIdx.x; i < passes * DIGITS; i += 256) s_h[i] = 0;
   // the next sort's block counters start at zero and its look-back words carry
   // a fresh epoch (the sort passes run after this kernel completes)
@@ -234,13 +237,14 @@ __global__ __launch_bounds__(256) void k_morton(SegTable ts, MeshTables mt, cons
 int launch_morton(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x,
                   const uint32_t* facets, const uint32_t* box_acc, uint32_t code_bits, bool key64,
                   void* keys, uint32_t* vals, uint32_t nitems, const SortPlan& plan, uint32_t* hist,
-                  uint32_t* counters, uint32_t* epoch, const uint32_t* run_flag, uint32_t* cut_reset) {
+                  uint32_t* counters, uint32_t* epoch, const uint32_t* run_flag, uint32_t* cut_reset,
+                  int32_t* cut_fill) {
   if (!nitems) return YSIM_OK;
   uint32_t blocks = ceil_div(nitems, 256 * MORTON_ITEMS);
 #define YSIM_MORTON(KT, DBV, NHV)                                                                             \
   hipLaunchKernelGGL((k_morton<KT, DBV, NHV>), dim3(blocks), dim3(256), 0, s, tsegs, mt, x, facets, box_acc, \
                      code_bits, (KT*)keys, vals, nitems, plan.passes, plan.hist_shift, hist, counters, epoch, \
-                     run_flag, cut_reset)
+                     run_flag, cut_reset, cut_fill)
   if (plan.hybrid) {
     if (key64) YSIM_MORTON(uint64_t, 11, 1); else YSIM_MORTON(uint32_t, 11, 1);
   } else if (plan.digit_bits == 11) {
@@ -269,14 +273,17 @@ __global__ __launch_bounds__(INC_THREADS) void k_morton_inc(SegTable ts, MeshTab
                                                     K* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t n,
                                                     uint32_t* __restrict__ need_full,
                                                     const uint32_t* __restrict__ box_parts, uint32_t nparts,
-                                                    uint32_t* __restrict__ cut_reset) {
+                                                    uint32_t* __restrict__ cut_reset, int32_t* __restrict__ cut_fill) {
   __shared__ K sk[INC_WINDOW];
   __shared__ uint32_t sv[INC_WINDOW];
   __shared__ uint32_t s_part[INC_THREADS];
   __shared__ uint32_t s_box[6];
   if (blockIdx.x == 0 && threadIdx.x == 0) *need_full = 0u;
-  if (cut_reset && blockIdx.x == 0)
+  if (cut_reset && blockIdx.x == 0) {
     for (uint32_t t = threadIdx.x; t < ts.nseg; t += INC_THREADS) cut_reset[ts.mesh[t]] = 0u;
+    if (cut_fill && ts.nseg == 1)
+      for (uint32_t k = threadIdx.x; k < CUT_CAP; k += INC_THREADS) cut_fill[(size_t)ts.mesh[0] * CUT_CAP + k] = -1;
+  }
   if (nparts) {  // one segment: the box from the cloth kernel's block boxes
     const uint32_t v = reduce_box_records<INC_THREADS, false>(box_parts, nparts, s_part);
     if (threadIdx.x < 6) {
@@ -320,16 +327,18 @@ __global__ __launch_bounds__(INC_THREADS) void k_morton_inc(SegTable ts, MeshTab
 int launch_morton_inc(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x, const uint32_t* facets,
                       uint32_t* box_acc, uint32_t code_bits, bool key64, const uint32_t* perm, void* keys,
                       uint32_t* vals, uint32_t nitems, uint32_t* need_full, const uint32_t* box_parts,
-                      uint32_t nparts, uint32_t* cut_reset) {
+                      uint32_t nparts, uint32_t* cut_reset, int32_t* cut_fill) {
   if (!nitems) return YSIM_OK;
   if (nparts && (!box_parts || tsegs.nseg != 1)) return YSIM_ERR_INVALID;
   const uint32_t blocks = ceil_div(nitems, INC_WINDOW);
   if (key64)
     hipLaunchKernelGGL(k_morton_inc<uint64_t>, dim3(blocks), dim3(INC_THREADS), 0, s, tsegs, mt, x, facets, box_acc,
-                       code_bits, perm, (uint64_t*)keys, vals, nitems, need_full, box_parts, nparts, cut_reset);
+                       code_bits, perm, (uint64_t*)keys, vals, nitems, need_full, box_parts, nparts, cut_reset,
+                       cut_fill);
   else
     hipLaunchKernelGGL(k_morton_inc<uint32_t>, dim3(blocks), dim3(INC_THREADS), 0, s, tsegs, mt, x, facets, box_acc,
-                       code_bits, perm, (uint32_t*)keys, vals, nitems, need_full, box_parts, nparts, cut_reset);
+                       code_bits, perm, (uint32_t*)keys, vals, nitems, need_full, box_parts, nparts, cut_reset,
+                       cut_fill);
   YSIM_CHECK_LAUNCH();
   return YSIM_OK;
 }

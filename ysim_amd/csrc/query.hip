@@ -198,11 +198,15 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
   // query box (a NaN box overlaps every box under the closed test; the union
   // box would ignore it), walks per lane over each lane's own targets
   bool per_lane = seg_of(a.qsegs, base) != sg;
-  // the cooperative path's lane data (idle lanes take the wave's last item)
-  const uint32_t m = a.qsegs.mesh[sg];
-  const uint32_t p = query_vertex(a, sg, m, active ? tq : last);
-  const uint32_t j = a.qsegs.off[sg] + p;
-  const uint32_t g = a.mt.voff[m] + p;
+  // the cooperative path's lane data (idle lanes take the wave's last item);
+  // with one query segment (pre_t >= 0) the host gave its tables: the
+  // segment starts at item 0
+  const bool one = a.pre_t >= 0;
+  const uint32_t m = one ? a.seg0_mesh : a.qsegs.mesh[sg];
+  const uint32_t it = active ? tq : last;
+  const uint32_t p = one ? (a.seg0_grid_n ? tile_order(it, a.seg0_grid_n) : it) : query_vertex(a, sg, m, it);
+  const uint32_t j = one ? p : a.qsegs.off[sg] + p;
+  const uint32_t g = (one ? a.seg0_voff : a.mt.voff[m]) + p;
   const uint32_t ti0 = a.tgt_off[sg], ti1 = a.tgt_off[sg + 1];
   f3 qcur = {0.f, 0.f, 0.f}, qprev = {0.f, 0.f, 0.f};
   float margin = 0.f, band = 0.f;
@@ -220,7 +224,20 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
     qprev = ld3(a.xprev, g);
     margin = a.mesh_margin[m];
     band = a.mesh_band[m];
-    if (ti0 < ti1) {
+    if (one) {
+      // the host lists only targets with facets, and the build filled the
+      // list of a one-segment group with -1 before appending: the entries
+      // load without the count, which decides later whether they are used
+      const int32_t* CL = a.cut + (size_t)a.pre_t * CUT_CAP;
+      pn0 = CL[lane];
+      pn1 = CL[64 + lane];
+      pn2 = CL[128 + lane];
+      if (pn0 >= 0) pv0 = load_node(a.pre_tree, pn0);
+      if (pn1 >= 0) pv1 = load_node(a.pre_tree, pn1);
+      if (pn2 >= 0) pv2 = load_node(a.pre_tree, pn2);
+      const uint32_t ncut = a.cut_cnt ? a.cut_cnt[a.pre_t] : 0u;
+      if (ncut > 0 && ncut <= CUT_CAP && ncut <= 3 * 64 && ncut <= leaf_cap) npre = ncut;
+    } else if (ti0 < ti1) {
       const uint32_t t = a.tgt[ti0];
       const uint32_t ncut = a.cut_cnt ? a.cut_cnt[t] : 0u;
       if (a.mesh_nfacets[t] > 0 && ncut > 0 && ncut <= CUT_CAP && ncut <= 3 * 64 && ncut <= leaf_cap) {

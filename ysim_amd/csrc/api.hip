@@ -288,6 +288,30 @@ struct ysim_scene {
   // dynamic trees were last built by it (their refit links are then absent)
   int2* agg_range = nullptr;
   bool tree_agglo = false;
+  // Two-kernel refit: spanning-node records of the hierarchy kernel (in
+  // links), partial boxes, and the refit block size the dynamic group's
+  // records were written for (0: none; the refit then climbs device-wide).
+  RefitSpan span;
+  uint32_t span_rb_dyn = 0;
+  // The links a group's hierarchy kernel writes: spanning nodes are recorded
+  // when the refit that follows shares the group's leaf positions (a static
+  // group's own refit at creation, or the only dynamic group).
+  bool records_span(const BuildGroup& g) const {
+    const bool dyn_only = dyn_groups.size() == 1 && &g == &dyn_groups[0];
+    const bool stat = &g >= stat_groups.data() && &g < stat_groups.data() + stat_groups.size();
+    return !formal_refit && (dyn_only || stat);
+  }
+  RefitLinks links_for(const BuildGroup& g) {
+    RefitLinks l = links;
+    if (records_span(g)) {
+      l.span_rb = tune.refit_block;
+      if (dyn_groups.size() == 1 && &g == &dyn_groups[0]) span_rb_dyn = tune.refit_block;
+    }
+    return l;
+  }
+  const RefitSpan* span_for_dyn() const {
+    return (dyn_groups.size() == 1 && span_rb_dyn == tune.refit_block && !formal_refit) ? &span : nullptr;
+  }
 
   int grow(float** p, size_t& have, size_t need) {
     if (need <= have && *p) return YSIM_OK;
@@ -740,7 +764,7 @@ struct ysim_scene {
       if (timing) YSIM_HIP_TRY(hipEventRecord(ev[3], st));
       if (defer) {
         YSIM_TRY(launch_hierarchy(st, g.tsegs, mt, keys[0], g.key64, vals[0], g.code_bits, tree, parent, visit, g.T,
-                                  box_acc, g.tsegs.nseg, hist, (uint32_t)g.plan.hist_words, g.perm, links,
+                                  box_acc, g.tsegs.nseg, hist, (uint32_t)g.plan.hist_words, g.perm, links_for(g),
                                   HierarchySkip{need_full, d_stale}));
         tree_agglo = false;
       } else {
@@ -771,10 +795,12 @@ struct ysim_scene {
                             P.swept != 0 && vc, facets, tree, parent, visit, agg_range, g.T, box_acc, g.tsegs.nseg,
                             hist, (uint32_t)g.plan.hist_words, g.perm, flags, formal_refit));
       tree_agglo = true;
+      if (dyn_groups.size() == 1 && &g == &dyn_groups[0]) span_rb_dyn = 0;
       return YSIM_OK;
     }
     YSIM_TRY(launch_hierarchy(st, g.tsegs, mt, keys[sorted], g.key64, vals[sorted], g.code_bits, tree, parent,
-                              visit, g.T, box_acc, g.tsegs.nseg, hist, (uint32_t)g.plan.hist_words, g.perm, links));
+                              visit, g.T, box_acc, g.tsegs.nseg, hist, (uint32_t)g.plan.hist_words, g.perm,
+                              links_for(g)));
     tree_agglo = false;
     return YSIM_OK;
   }
@@ -856,7 +882,8 @@ struct ysim_scene {
     // it has no refit links, so its refits climb through parent links
     if (!agglo)
       YSIM_TRY(launch_refit(st, dyn_tsegs, mt, xc, v[par], P.subh, P.swept != 0, facets, tree, parent, visit, Tdyn,
-                            flags, formal_refit, tree_agglo ? nullptr : &links, tune.refit_block));
+                            flags, formal_refit, tree_agglo ? nullptr : &links, tune.refit_block,
+                            tree_agglo ? nullptr : span_for_dyn()));
     if (timing) YSIM_HIP_TRY(hipEventRecord(ev[5], st));
     return YSIM_OK;
   }
@@ -1789,6 +1816,21 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
   YSIM_TRY(s.alloc(&s.links.cut_cnt, num_meshes));
   YSIM_TRY(s.zero(s.links.cut_cnt, num_meshes));
   YSIM_TRY(s.alloc(&s.links.cut, (size_t)num_meshes * CUT_CAP));
+  {
+    // spanning-node records and partial boxes over the leaf positions of any
+    // group (at most every facet of the scene)
+    const size_t np = std::max<uint32_t>(s.Ttot, 1u);
+    uint64_t* mask = nullptr;
+    int4* rec = nullptr;
+    float4 *part = nullptr, *blk = nullptr;
+    YSIM_TRY(s.alloc(&mask, (np + 63) / 64));
+    YSIM_TRY(s.alloc(&rec, np));
+    YSIM_TRY(s.alloc(&part, 2 * np));
+    YSIM_TRY(s.alloc(&blk, 2 * ((np + 255) / 256)));
+    s.links.span_mask = mask;
+    s.links.span_rec = rec;
+    s.span = RefitSpan{mask, rec, part, blk};
+  }
   YSIM_HIP_TRY(hipMemsetAsync(s.links.cut, 0xff, 4 * (size_t)num_meshes * CUT_CAP, s.st));  // -1: no entry
   YSIM_TRY(s.zero(s.tree, 8 * (size_t)s.Ntot));
   YSIM_TRY(s.zero(s.visit, s.Ntot));
@@ -1937,8 +1979,10 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
   for (auto* gs : {&s.stat_groups, &s.dyn_groups})
     for (auto& g : *gs) {
       YSIM_TRY(s.enqueue_build(g, s.x[0]));
+      // the group's own positions: the records its hierarchy kernel just wrote
       YSIM_TRY(launch_refit(s.st, g.tsegs, s.mt, s.x[0], s.v[0], 0.f, false, s.facets, s.tree, s.parent, s.visit,
-                            g.T, s.flags, s.formal_refit, &s.links, s.tune.refit_block));
+                            g.T, s.flags, s.formal_refit, &s.links, s.tune.refit_block,
+                            s.records_span(g) ? &s.span : nullptr));
     }
   YSIM_HIP_TRY(hipStreamSynchronize(s.st));
   uint32_t fl = 0;

@@ -85,8 +85,26 @@ struct RefitLinks {
   // build that writes no cut leaves them at zero.
   uint32_t* cut_cnt = nullptr;
   int32_t* cut = nullptr;
+  // Nodes spanning refit blocks of span_rb sorted leaves (span_rb 0: not
+  // recorded).  Position space: the leaf positions of the launch's segment
+  // table, which the refit that reads them must share.  span_mask: one bit
+  // per position (a node at its own index's position), one word per 64;
+  // span_rec[position]: (lo, hi, child_a, child_b), mesh-relative, written for
+  // spanning nodes only.
+  uint32_t span_rb = 0;
+  uint64_t* span_mask = nullptr;
+  int4* span_rec = nullptr;
 };
 constexpr uint32_t CUT_CAP = 512;
+// Partial boxes of the two-kernel refit (span_rb as above): part[position]
+// two float4 per leaf position (only positions that start or end a spanning
+// node's range are written), blk[block] the union of a block's leaves.
+struct RefitSpan {
+  const uint64_t* mask = nullptr;
+  const int4* rec = nullptr;
+  float4* part = nullptr;
+  float4* blk = nullptr;
+};
 // S3: Karras hierarchy per segment; child and parent links (and rl.up when
 // set); zeroes the refit visit counters; resets box_acc (nbox segments) and the
 // sort histogram for the next build.
@@ -115,12 +133,16 @@ int launch_agglo(hipStream_t s, SegTable tsegs, MeshTables mt, const void* keys,
                  uint32_t* err, bool formal);
 // S4+S5: leaf boxes (swept optional) + atomic bottom-up union per segment.
 // With rl (from launch_hierarchy) the levels inside blocks of `block` (256,
-// 512 or 1024) sorted leaves are combined in LDS.
+// 512 or 1024) sorted leaves are combined in LDS.  With span (its mask and
+// records written by the hierarchy kernel of this topology with span_rb ==
+// block, over the same leaf positions) the nodes spanning blocks are
+// finished by a second kernel from per-block partial boxes instead of the
+// device-wide arrival protocol (not with formal).
 bool refit_block_ok(uint32_t b);
 int launch_refit(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x, const float* v,
                  float dt, bool swept, const uint32_t* facets, ysim_bvh_node* tree,
                  const int32_t* parent, uint32_t* visit, uint32_t nitems, uint32_t* err, bool formal,
-                 const RefitLinks* rl = nullptr, uint32_t block = 256);
+                 const RefitLinks* rl = nullptr, uint32_t block = 256, const RefitSpan* span = nullptr);
 
 // ---------------------------------------------------------------- query.hip
 struct QueryArgs {

@@ -493,9 +493,16 @@ __global__ __launch_bounds__(256) void k_hierarchy(SegTable ts, MeshTables mt, c
       split = (uint32_t)nsp;
     }
   }
-  if (!node) return;
   int ca = ((int)split == lo) ? (int)split + nn - 1 : (int)split;
   int cb = ((int)split + 1 == hi) ? (int)split + nn : (int)split + 1;
+  if (rl.span_rb) {  // nodes whose leaf range crosses a refit block boundary
+    const uint32_t rb = rl.span_rb;
+    const bool sp = node && (s + (uint32_t)lo) / rb != (s + (uint32_t)hi) / rb;
+    const uint64_t bm = __ballot(sp);
+    if (lane == 0) rl.span_mask[j >> 6] = bm;  // lane 0 is live: lanes are ascending positions
+    if (sp) rl.span_rec[j] = make_int4(lo, hi, ca, cb);
+  }
+  if (!node) return;
   if (!rl.fid) {
     T[i].child_a = ca;
     T[i].child_b = cb;
@@ -887,27 +894,48 @@ __global__ __launch_bounds__(256) void k_refit(SegTable ts, MeshTables mt, const
 // internal node whose range lies inside the block is then combined through LDS
 // with no device memory access (the children's boxes and an arrival counter
 // per node; a Karras internal node is an endpoint of its own range, so its
-// index falls inside the block and addresses the LDS slot).  Only nodes
-// spanning blocks use the device-wide second-arrival protocol above, with the
-// next level's links loaded while the arrival resolves.  Unions keep
+// index falls inside the block and addresses the LDS slot).  Unions keep
 // (child_a, child_b) operand order; the boxes are identical to k_refit's.
 // Every node is written whole, box and child links: a leaf's links are
 // (-1, facet id from fids), an internal node's are its two children, known to
 // the thread that combines it (its own node and the sibling from the ranges).
-template <bool FORMAL, int RB>
+//
+// The nodes spanning blocks (their parent is an "escape" node's parent) are
+// finished in one of two ways:
+//  TOP = false: the device-wide second-arrival protocol of k_refit per level,
+//    with the next level's links loaded while the arrival resolves (chains of
+//    up to ~23 dependent device-scope round trips near block boundaries);
+//  TOP = true (spanning nodes recorded by the hierarchy kernel for RB): the
+//    block's escape nodes partition the leaves of its meshes that span blocks.
+//    A spanning node's range is [lo, hi] with lo in block a and hi in block b
+//    (a < b): its leftmost descendant escape (down the child_a side) starts at
+//    lo in block a, its rightmost (child_b side) ends at hi in block b, and
+//    the blocks in between lie inside the range.  So its box is the union of
+//    three parts: the leaves [lo, end of a] (the escapes of block a from lo
+//    on), whole blocks a+1 .. b-1, and the leaves [start of b, hi].  Each
+//    child_a escape writes the first kind at part[lo], each child_b escape
+//    the last kind at part[hi] (escapes are disjoint, so no position gets
+//    both), the block writes its box to blk, and k_refit_span combines them
+//    after this kernel: no dependent chain across workgroups.  A float min or
+//    max is exact, so unions in another order give the same bits for finite
+//    boxes without signed zeros of both signs on an axis (positions built by
+//    additions never become -0).
+template <bool FORMAL, int RB, bool TOP>
 __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, const float* __restrict__ x,
                                                   const float* __restrict__ v, float dt, int swept,
                                                   const uint32_t* __restrict__ facets, ysim_bvh_node* tree,
                                                   const int32_t* __restrict__ parent,
                                                   const uint32_t* __restrict__ up_links,
                                                   const uint32_t* __restrict__ fids, uint32_t* visit,
-                                                  uint32_t n, uint32_t* err) {
+                                                  uint32_t n, uint32_t* err, RefitSpan sp) {
   __shared__ uint32_t s_cnt[RB];
   __shared__ float s_box[RB][2][6];
   __shared__ int2 s_link[RB];  // internal node (slot) of the thread's mesh: (parent, up)
+  __shared__ uint32_t s_nesc;
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t b0 = blk * RB;
   s_cnt[threadIdx.x] = 0;
+  if (TOP && threadIdx.x == 0) s_nesc = 0;
   const uint32_t j = b0 + threadIdx.x;
   const bool valid = j < n;
   const uint32_t sg = seg_of(ts, valid ? j : n - 1);
@@ -952,7 +980,7 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
   }
   if (has_link) s_link[threadIdx.x] = link;
   __syncthreads();
-  if (!valid) return;
+  if (!TOP && !valid) return;
   int la = -1, lb = (int)fid;  // child links of the node `child`
   // A node whose box another workgroup reads during this kernel (its parent is
   // combined through the device-wide protocol) is written with device-coherent
@@ -972,112 +1000,218 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
       st_agent_u64(w + 3, pack2(b.z, __int_as_float(cb)));
     }
   };
-  if (nn <= 1) {
-    store_node(child, lo, hi, la, lb, false);
-    return;
-  }
   uint32_t clo = (uint32_t)i, chi = (uint32_t)i;  // leaf range of the current node
-  // Local climb: LDS arrivals only, one exit test per outcome and plain node
-  // stores (the branch-light loop keeps the per-level instruction count low;
-  // with 16 waves of a block per CU the loop is issue-bound).
-  bool dev = false;
-  for (int step = 0;; ++step) {
-    const int p = pnext;
-    const uint32_t up = upnext;
-    const uint32_t side = up & 1u;  // 0: child_a (left part of p's range)
-    const uint32_t psize = up >> 1;
-    const uint32_t plo = side ? chi + 1 - psize : clo, phi = side ? chi : clo + psize - 1;
-    // sibling: child_b at split + 1 for a child_a, child_a at split for a
-    // child_b; a leaf when that position ends p's range
-    const uint32_t q = side ? clo - 1 : chi + 1, qe = side ? plo : phi;
-    const int sib = (int)q + (q == qe ? nn - 1 : 0);
-    const uint32_t slot = s + (uint32_t)p - b0;  // in [0, RB) for a local p: an endpoint of its range
-    const bool bad = (p < 0) | (p >= nn - 1) | (sib < 0) | (sib >= 2 * nn - 1) | (step >= 2 * nn);
-    const bool local = !bad & (s + plo >= b0) & (s + phi < b0 + RB);
-    if (!local || slot >= (uint32_t)RB) {
-      dev = !local && !bad;  // else a corrupted topology
-      break;
-    }
+  uint32_t side = 0;                              // of the current node (0: child_a)
+  bool dev = false;                               // `child` is an escape: its parent spans blocks
+  if (valid && nn <= 1) {
     store_node(child, lo, hi, la, lb, false);
-    float* mine = s_box[slot][side];
-    mine[0] = lo.x; mine[1] = lo.y; mine[2] = lo.z; mine[3] = hi.x; mine[4] = hi.y; mine[5] = hi.z;
-    const uint32_t old = __hip_atomic_fetch_add(&s_cnt[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (old == 0u) return;  // first arrival: the sibling's thread combines
-    const float* o = s_box[slot][side ^ 1u];
-    const f3 omin = {o[0], o[1], o[2]}, omax = {o[3], o[4], o[5]};
-    // union in (child_a, child_b) order, bvh.metal:567-573
-    lo = side ? vmin(omin, lo) : vmin(lo, omin);
-    hi = side ? vmax(omax, hi) : vmax(hi, omax);
-    la = side ? sib : child;
-    lb = side ? child : sib;
-    child = p;
-    clo = plo;
-    chi = phi;
-    const int2 l = s_link[slot];
-    pnext = l.x;
-    upnext = (uint32_t)l.y;
-    if (p == 0) {
-      store_node(0, lo, hi, la, lb, false);
-      return;
+  } else if (valid) {
+    // Local climb: LDS arrivals only, one exit test per outcome and plain
+    // node stores (the branch-light loop keeps the per-level instruction
+    // count low; with 16 waves of a block per CU the loop is issue-bound).
+    for (int step = 0;; ++step) {
+      const int p = pnext;
+      const uint32_t up = upnext;
+      side = up & 1u;  // 0: child_a (left part of p's range)
+      const uint32_t psize = up >> 1;
+      const uint32_t plo = side ? chi + 1 - psize : clo, phi = side ? chi : clo + psize - 1;
+      // sibling: child_b at split + 1 for a child_a, child_a at split for a
+      // child_b; a leaf when that position ends p's range
+      const uint32_t q = side ? clo - 1 : chi + 1, qe = side ? plo : phi;
+      const int sib = (int)q + (q == qe ? nn - 1 : 0);
+      const uint32_t slot = s + (uint32_t)p - b0;  // in [0, RB) for a local p: an endpoint of its range
+      const bool bad = (p < 0) | (p >= nn - 1) | (sib < 0) | (sib >= 2 * nn - 1) | (step >= 2 * nn);
+      const bool local = !bad & (s + plo >= b0) & (s + phi < b0 + RB);
+      if (!local || slot >= (uint32_t)RB) {
+        dev = !local && !bad;  // else a corrupted topology
+        if (!dev) atomicOr(err, 4u);
+        break;
+      }
+      store_node(child, lo, hi, la, lb, false);
+      float* mine = s_box[slot][side];
+      mine[0] = lo.x; mine[1] = lo.y; mine[2] = lo.z; mine[3] = hi.x; mine[4] = hi.y; mine[5] = hi.z;
+      const uint32_t old = __hip_atomic_fetch_add(&s_cnt[slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (old == 0u) break;  // first arrival: the sibling's thread combines
+      const float* o = s_box[slot][side ^ 1u];
+      const f3 omin = {o[0], o[1], o[2]}, omax = {o[3], o[4], o[5]};
+      // union in (child_a, child_b) order, bvh.metal:567-573
+      lo = side ? vmin(omin, lo) : vmin(lo, omin);
+      hi = side ? vmax(omax, hi) : vmax(hi, omax);
+      la = side ? sib : child;
+      lb = side ? child : sib;
+      child = p;
+      clo = plo;
+      chi = phi;
+      const int2 l = s_link[slot];
+      pnext = l.x;
+      upnext = (uint32_t)l.y;
+      if (p == 0) {
+        store_node(0, lo, hi, la, lb, false);
+        break;
+      }
     }
   }
-  if (!dev) {
-    atomicOr(err, 4u);
+  if constexpr (TOP) {
+    // Escapes: stored whole (plain stores; read after the kernel), listed in
+    // LDS (s_box is free once every local node is combined), then each writes
+    // its partial union and thread 0 the block's box.
+    __syncthreads();
+    if (dev) {
+      store_node(child, lo, hi, la, lb, false);
+      const uint32_t k = atomicAdd(&s_nesc, 1u);  // < RB: escapes are disjoint leaf ranges of the block
+      float* r = &s_box[k][0][0];
+      r[0] = lo.x; r[1] = lo.y; r[2] = lo.z; r[3] = hi.x; r[4] = hi.y; r[5] = hi.z;
+      r[6] = __uint_as_float(s + clo); r[7] = __uint_as_float(s + chi);  // launch positions
+    }
+    __syncthreads();
+    const uint32_t ne = s_nesc;
+    if (dev) {
+      // child_a escape: the escapes of the block right of it (its parent
+      // starts at clo and ends past the block, so its mesh covers the rest of
+      // the block); child_b: those left of it.
+      f3 ulo = lo, uhi = hi;
+      for (uint32_t e = 0; e < ne; ++e) {
+        const float* r = &s_box[e][0][0];
+        const uint32_t elo = __float_as_uint(r[6]), ehi = __float_as_uint(r[7]);
+        if (side ? ehi < s + clo : elo > s + chi) {
+          ulo = vmin(ulo, f3{r[0], r[1], r[2]});
+          uhi = vmax(uhi, f3{r[3], r[4], r[5]});
+        }
+      }
+      float4* q = sp.part + 2 * (size_t)(s + (side ? chi : clo));
+      q[0] = make_float4(ulo.x, ulo.y, ulo.z, 0.f);
+      q[1] = make_float4(uhi.x, uhi.y, uhi.z, 0.f);
+    }
+    if (threadIdx.x == 0 && ne > 0) {
+      // the block's box: read only for blocks inside a spanning range, whose
+      // leaves all lie in that one mesh
+      const float* r = &s_box[0][0][0];
+      f3 ulo = {r[0], r[1], r[2]}, uhi = {r[3], r[4], r[5]};
+      for (uint32_t e = 1; e < ne; ++e) {
+        r = &s_box[e][0][0];
+        ulo = vmin(ulo, f3{r[0], r[1], r[2]});
+        uhi = vmax(uhi, f3{r[3], r[4], r[5]});
+      }
+      sp.blk[2 * blk] = make_float4(ulo.x, ulo.y, ulo.z, 0.f);
+      sp.blk[2 * blk + 1] = make_float4(uhi.x, uhi.y, uhi.z, 0.f);
+    }
     return;
+  } else {
+    if (!dev) return;
+    // Device-wide climb from `child`, whose parent spans blocks: the node goes
+    // out with device-coherent stores (another workgroup reads it as a
+    // sibling), then the second-arrival protocol of k_refit per level, with the
+    // next level's links loaded while the arrival resolves.
+    store_node(child, lo, hi, la, lb, true);
+    for (int step = 0; step < 2 * nn; ++step) {
+      const int p = pnext;
+      const uint32_t up = upnext;
+      if (p < 0 || p >= nn - 1) break;
+      side = up & 1u;
+      const uint32_t psize = up >> 1;
+      const uint32_t plo = side ? chi + 1 - psize : clo, phi = side ? chi : clo + psize - 1;
+      const uint32_t sq = side ? clo - 1 : chi + 1, sqe = side ? plo : phi;
+      const int sib = (int)sq + (sq == sqe ? nn - 1 : 0);
+      if (sib < 0 || sib >= 2 * nn - 1) break;
+      pnext = P[p];
+      upnext = UP[p];
+      uint32_t old;
+      if (FORMAL) {
+        old = __hip_atomic_fetch_add(C + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        wait_all_mem();
+        old = __hip_atomic_fetch_add(C + p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      if ((old & 1u) == 0u) return;
+      const float* q = reinterpret_cast<const float*>(T + sib);
+      f3 smin, smax;
+      if (FORMAL) {
+        smin = f3{q[0], q[1], q[2]};
+        smax = f3{q[4], q[5], q[6]};
+      } else {
+        const uint64_t m01 = ld_agent_u64(reinterpret_cast<const uint64_t*>(q + 0));
+        const uint64_t x01 = ld_agent_u64(reinterpret_cast<const uint64_t*>(q + 4));
+        smin = f3{lo_f(m01), hi_f(m01), ld_agent(q + 2)};
+        smax = f3{lo_f(x01), hi_f(x01), ld_agent(q + 6)};
+      }
+      if (side == 0) {
+        lo = vmin(lo, smin);
+        hi = vmax(hi, smax);
+      } else {
+        lo = vmin(smin, lo);
+        hi = vmax(smax, hi);
+      }
+      la = side ? sib : child;
+      lb = side ? child : sib;
+      child = p;
+      clo = plo;
+      chi = phi;
+      // the root is read by no other workgroup; every other spanning node is
+      store_node(p, lo, hi, la, lb, p != 0);
+      if (p == 0) return;
+    }
+    atomicOr(err, 4u);
   }
-  // Device-wide climb from `child`, whose parent spans blocks: the node goes
-  // out with device-coherent stores (another workgroup reads it as a
-  // sibling), then the second-arrival protocol of k_refit per level, with the
-  // next level's links loaded while the arrival resolves.
-  store_node(child, lo, hi, la, lb, true);
-  for (int step = 0; step < 2 * nn; ++step) {
-    const int p = pnext;
-    const uint32_t up = upnext;
-    if (p < 0 || p >= nn - 1) break;
-    const uint32_t side = up & 1u;
-    const uint32_t psize = up >> 1;
-    const uint32_t plo = side ? chi + 1 - psize : clo, phi = side ? chi : clo + psize - 1;
-    const uint32_t sq = side ? clo - 1 : chi + 1, sqe = side ? plo : phi;
-    const int sib = (int)sq + (sq == sqe ? nn - 1 : 0);
-    if (sib < 0 || sib >= 2 * nn - 1) break;
-    pnext = P[p];
-    upnext = UP[p];
-    uint32_t old;
-    if (FORMAL) {
-      old = __hip_atomic_fetch_add(C + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      wait_all_mem();
-      old = __hip_atomic_fetch_add(C + p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    if ((old & 1u) == 0u) return;
-    const float* q = reinterpret_cast<const float*>(T + sib);
-    f3 smin, smax;
-    if (FORMAL) {
-      smin = f3{q[0], q[1], q[2]};
-      smax = f3{q[4], q[5], q[6]};
-    } else {
-      const uint64_t m01 = ld_agent_u64(reinterpret_cast<const uint64_t*>(q + 0));
-      const uint64_t x01 = ld_agent_u64(reinterpret_cast<const uint64_t*>(q + 4));
-      smin = f3{lo_f(m01), hi_f(m01), ld_agent(q + 2)};
-      smax = f3{lo_f(x01), hi_f(x01), ld_agent(q + 6)};
-    }
-    if (side == 0) {
-      lo = vmin(lo, smin);
-      hi = vmax(hi, smax);
-    } else {
-      lo = vmin(smin, lo);
-      hi = vmax(smax, hi);
-    }
-    la = side ? sib : child;
-    lb = side ? child : sib;
-    child = p;
-    clo = plo;
-    chi = phi;
-    // the root is read by no other workgroup; every other spanning node is
-    store_node(p, lo, hi, la, lb, p != 0);
-    if (p == 0) return;
+}
+
+// Second kernel of the TOP refit: one wave per word of the span mask, one
+// lane per spanning node of the word.  Box = part[lo] (leaves lo .. end of
+// block a) U blk[a+1 .. b-1] U part[hi] (start of block b .. hi), as ordered
+// keys (exact min/max in any order).  A lane takes up to SPAN_LANE_BLOCKS
+// middle blocks itself; the few nodes with more (near the root) are reduced
+// by the whole wave, one after another.  The node is written whole with the
+// child links the hierarchy kernel recorded.
+constexpr uint32_t SPAN_LANE_BLOCKS = 4;
+template <int RB>
+__global__ __launch_bounds__(256) void k_refit_span(SegTable ts, MeshTables mt, ysim_bvh_node* tree, RefitSpan sp,
+                                                    uint32_t nwords) {
+  const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= nwords) return;
+  const uint64_t bits = sp.mask[w];
+  if (!bits) return;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t j = w * 64 + lane;
+  const bool mine = (bits >> lane) & 1u;
+  uint32_t k[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+  auto take = [](uint32_t* t, const float4* q) {
+    const float4 l = q[0], h = q[1];
+    t[0] = min(t[0], fkey(l.x)); t[1] = min(t[1], fkey(l.y)); t[2] = min(t[2], fkey(l.z));
+    t[3] = max(t[3], fkey(h.x)); t[4] = max(t[4], fkey(h.y)); t[5] = max(t[5], fkey(h.z));
+  };
+  int4 r = make_int4(0, 0, 0, 0);
+  uint32_t s = 0, base = 0, a = 0, b = 0;
+  if (mine) {
+    r = sp.rec[j];
+    const uint32_t sg = seg_of(ts, j);
+    s = ts.off[sg];
+    base = mt.noff[ts.mesh[sg]];
+    const uint32_t plo = s + (uint32_t)r.x, phi = s + (uint32_t)r.y;
+    a = plo / RB;
+    b = phi / RB;
+    take(k, sp.part + 2 * (size_t)plo);
+    take(k, sp.part + 2 * (size_t)phi);
+    if (b - a - 1 <= SPAN_LANE_BLOCKS)
+      for (uint32_t c = a + 1; c < b; ++c) take(k, sp.blk + 2 * (size_t)c);
   }
-  atomicOr(err, 4u);
+  uint64_t big = __ballot(mine && b - a - 1 > SPAN_LANE_BLOCKS);
+  while (big) {
+    const int src = __builtin_ctzll(big);
+    big &= big - 1;
+    const uint32_t ba = __shfl(a, src, 64), bb = __shfl(b, src, 64);
+    uint32_t t[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+    for (uint32_t c = ba + 1 + lane; c < bb; c += 64) take(t, sp.blk + 2 * (size_t)c);
+#pragma unroll
+    for (int e = 0; e < 6; ++e) t[e] = e < 3 ? wave_min_u32(t[e]) : wave_max_u32(t[e]);
+    if ((int)lane == src) {
+      k[0] = min(k[0], t[0]); k[1] = min(k[1], t[1]); k[2] = min(k[2], t[2]);
+      k[3] = max(k[3], t[3]); k[4] = max(k[4], t[4]); k[5] = max(k[5], t[5]);
+    }
+  }
+  if (mine) {
+    float4* q4 = reinterpret_cast<float4*>(tree + base + (j - s));
+    q4[0] = make_float4(funkey(k[0]), funkey(k[1]), funkey(k[2]), __int_as_float(r.z));
+    q4[1] = make_float4(funkey(k[3]), funkey(k[4]), funkey(k[5]), __int_as_float(r.w));
+  }
 }
 
 bool refit_block_ok(uint32_t b) { return b == 256 || b == 512 || b == 1024; }
@@ -1085,20 +1219,32 @@ bool refit_block_ok(uint32_t b) { return b == 256 || b == 512 || b == 1024; }
 int launch_refit(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x, const float* v, float dt,
                  bool swept, const uint32_t* facets, ysim_bvh_node* tree, const int32_t* parent,
                  uint32_t* visit, uint32_t nitems, uint32_t* err, bool formal, const RefitLinks* rl,
-                 uint32_t block) {
+                 uint32_t block, const RefitSpan* span) {
   if (!nitems) return YSIM_OK;
   if (rl && rl->up) {
     if (!refit_block_ok(block) || !rl->fid) return YSIM_ERR_INVALID;
-#define YSIM_R(F, B)                                                                                     \
-  hipLaunchKernelGGL((k_refit_lds<F, B>), dim3(ceil_div(nitems, B)), dim3(B), 0, s, tsegs, mt, x, v, dt, \
-                     (int)swept, facets, tree, parent, rl->up, rl->fid, visit, nitems, err)
+    const bool top = span && span->mask && !formal;
+    const RefitSpan sp = top ? *span : RefitSpan{};
+#define YSIM_R(F, B, TP)                                                                                     \
+  hipLaunchKernelGGL((k_refit_lds<F, B, TP>), dim3(ceil_div(nitems, B)), dim3(B), 0, s, tsegs, mt, x, v, dt, \
+                     (int)swept, facets, tree, parent, rl->up, rl->fid, visit, nitems, err, sp)
     if (formal) {
-      if (block == 256) YSIM_R(true, 256); else if (block == 512) YSIM_R(true, 512); else YSIM_R(true, 1024);
+      if (block == 256) YSIM_R(true, 256, false); else if (block == 512) YSIM_R(true, 512, false); else YSIM_R(true, 1024, false);
+    } else if (!top) {
+      if (block == 256) YSIM_R(false, 256, false); else if (block == 512) YSIM_R(false, 512, false); else YSIM_R(false, 1024, false);
     } else {
-      if (block == 256) YSIM_R(false, 256); else if (block == 512) YSIM_R(false, 512); else YSIM_R(false, 1024);
+      if (block == 256) YSIM_R(false, 256, true); else if (block == 512) YSIM_R(false, 512, true); else YSIM_R(false, 1024, true);
     }
 #undef YSIM_R
     YSIM_CHECK_LAUNCH();
+    if (top) {
+      const uint32_t nwords = ceil_div(nitems, 64);
+#define YSIM_S(B) \
+  hipLaunchKernelGGL((k_refit_span<B>), dim3(ceil_div(nwords, 4)), dim3(256), 0, s, tsegs, mt, tree, sp, nwords)
+      if (block == 256) YSIM_S(256); else if (block == 512) YSIM_S(512); else YSIM_S(1024);
+#undef YSIM_S
+      YSIM_CHECK_LAUNCH();
+    }
     return YSIM_OK;
   }
   if (formal)

@@ -700,6 +700,15 @@ struct ysim_scene {
     out.nseg = (uint32_t)ms.size();
     out.off = d_off;
     out.mesh = d_mesh;
+    if (ms.size() == 1) {  // the one-segment values travel in the kernel arguments
+      const uint32_t m = ms[0];
+      out.one_mesh = m;
+      out.one_s = 0;
+      out.one_n = off[1];
+      out.one_noff = noff[m];
+      out.one_foff = foff[m];
+      out.one_voff = voff[m];
+    }
     return YSIM_OK;
   }
 

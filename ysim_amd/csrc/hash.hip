@@ -49,21 +49,10 @@ __global__ __launch_bounds__(256) void k_hash_boxes_count(HashArgs a) {
   const uint32_t gt = blockIdx.x * 256 + threadIdx.x;
   if (gt >= a.ntri) return;
   const TriRef r = tri_ref(a, gt);
-  const uint32_t* f = a.facets + 3 * (size_t)(a.mt.foff[r.mesh] + r.fid);
   const uint32_t vb = a.mt.voff[r.mesh];
   const bool swept = a.swept && a.mesh_dynamic[r.mesh];
-  f3 lo = ld3(a.x, vb + f[0]), hi = lo;
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    f3 p = ld3(a.x, vb + f[c]);
-    lo = vmin(lo, p);
-    hi = vmax(hi, p);
-    if (swept) {
-      f3 q = p + ld3(a.v, vb + f[c]) * a.dt;
-      lo = vmin(lo, q);
-      hi = vmax(hi, q);
-    }
-  }
+  f3 lo, hi;
+  tri_box(a.x, a.v, a.dt, swept, a.facets + 3 * (size_t)(a.mt.foff[r.mesh] + r.fid), vb, lo, hi);
   float* B = a.boxes + 6 * (size_t)gt;
   B[0] = lo.x; B[1] = lo.y; B[2] = lo.z; B[3] = hi.x; B[4] = hi.y; B[5] = hi.z;
   const uint32_t grp = a.mesh_group[r.mesh];

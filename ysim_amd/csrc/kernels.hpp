@@ -15,6 +15,11 @@ struct SegTable {
   uint32_t nseg = 0;
   const uint32_t* off = nullptr;   // device, nseg + 1
   const uint32_t* mesh = nullptr;  // device, nseg
+  // One-segment tables filled by the host (one_mesh != ~0u): the segment's
+  // mesh, first item, item count and the mesh's node / facet / vertex offsets,
+  // passed with the kernel arguments so no table load precedes a thread's
+  // first data load.
+  uint32_t one_mesh = ~0u, one_s = 0, one_n = 0, one_noff = 0, one_foff = 0, one_voff = 0;
 };
 
 __device__ __forceinline__ uint32_t seg_of(const SegTable& s, uint32_t j) {
@@ -33,6 +38,52 @@ struct MeshTables {
   const uint32_t* foff = nullptr;  // facets
   const uint32_t* noff = nullptr;  // BVH node windows (2T - 1 per mesh)
 };
+
+// Item j's segment with its mesh and that mesh's offsets: from the kernel
+// arguments for a host-filled one-segment table (one cloth), else a per-lane
+// binary search, then the mesh and its offsets.
+struct SegInfo {
+  uint32_t sg, m, s, n;       // segment, mesh, first item, items
+  uint32_t noff, foff, voff;  // the mesh's node window, facets, vertices
+};
+__device__ __forceinline__ SegInfo seg_info(const SegTable& t, const MeshTables& mt, uint32_t j) {
+  if (t.nseg <= 1 && t.one_mesh != ~0u)
+    return {0u, t.one_mesh, t.one_s, t.one_n, t.one_noff, t.one_foff, t.one_voff};
+  if (t.nseg <= 1) {
+    const uint32_t m = t.mesh[0], s = t.off[0];
+    return {0u, m, s, t.off[1] - s, mt.noff[m], mt.foff[m], mt.voff[m]};
+  }
+  const uint32_t sg = seg_of(t, j);
+  const uint32_t m = t.mesh[sg], s = t.off[sg];
+  return {sg, m, s, t.off[sg + 1] - s, mt.noff[m], mt.foff[m], mt.voff[m]};
+}
+
+// Box of the triangle at f (three vertex ids from vb on); when swept, also of
+// x + v dt per vertex.  The ids load as one 12-byte record and every position
+// (and velocity) load is issued before the first use, so the gather takes two
+// dependent rounds.  Union order p0, q0, p1, q1, p2, q2 (q: swept point).
+struct __attribute__((packed, aligned(4))) pu3 {
+  uint32_t a, b, c;
+};
+// tri_box_ids: the same from the three vertex ids.
+__device__ __forceinline__ void tri_box_ids(const float* __restrict__ x, const float* __restrict__ v, float dt,
+                                            bool swept, uint32_t a, uint32_t b, uint32_t c, f3& lo, f3& hi) {
+  const f3 p0 = ld3(x, a), p1 = ld3(x, b), p2 = ld3(x, c);
+  if (swept) {
+    const f3 v0 = ld3(v, a), v1 = ld3(v, b), v2 = ld3(v, c);
+    const f3 q0 = p0 + v0 * dt, q1 = p1 + v1 * dt, q2 = p2 + v2 * dt;
+    lo = vmin(vmin(vmin(vmin(vmin(p0, q0), p1), q1), p2), q2);
+    hi = vmax(vmax(vmax(vmax(vmax(p0, q0), p1), q1), p2), q2);
+  } else {
+    lo = vmin(vmin(p0, p1), p2);
+    hi = vmax(vmax(p0, p1), p2);
+  }
+}
+__device__ __forceinline__ void tri_box(const float* __restrict__ x, const float* __restrict__ v, float dt,
+                                        bool swept, const uint32_t* f, uint32_t vb, f3& lo, f3& hi) {
+  const pu3 t = *reinterpret_cast<const pu3*>(f);
+  tri_box_ids(x, v, dt, swept, vb + t.a, vb + t.b, vb + t.c, lo, hi);
+}
 
 // ---------------------------------------------------------------- lbvh.hip
 // S0: per-segment exact min/max of vertex positions into box_acc[seg*6] (ordered keys).

@@ -303,12 +303,12 @@ __global__ __launch_bounds__(INC_THREADS) void k_morton_inc(SegTable ts, MeshTab
       continue;
     }
     const uint32_t j = start + i;
-    const uint32_t sg = seg_of(ts, j);
-    const uint32_t m = ts.mesh[sg];
+    const SegInfo si = seg_info(ts, mt, j);
+    const uint32_t sg = si.sg;
     const uint32_t fid = perm[j];
-    const uint32_t* f = facets + 3 * (size_t)(mt.foff[m] + fid);
-    const uint32_t vb = mt.voff[m];
-    const f3 x0 = ld3(x, vb + f[0]), x1 = ld3(x, vb + f[1]), x2 = ld3(x, vb + f[2]);
+    const pu3 f = *reinterpret_cast<const pu3*>(facets + 3 * (size_t)(si.foff + fid));
+    const uint32_t vb = si.voff;
+    const f3 x0 = ld3(x, vb + f.a), x1 = ld3(x, vb + f.b), x2 = ld3(x, vb + f.c);
     const uint32_t* a = box + sg * 6;
     const f3 bmin = {funkey(a[0]), funkey(a[1]), funkey(a[2])};
     const f3 bmax = {funkey(a[3]), funkey(a[4]), funkey(a[5])};
@@ -387,12 +387,12 @@ __global__ __launch_bounds__(256) void k_hierarchy(SegTable ts, MeshTables mt, c
   // lanes past the items stay for the wave's cooperative searches below
   const bool live = j < n;
   if (__ballot(live) == 0) return;
-  const uint32_t sg = seg_of(ts, live ? j : n - 1);
-  const uint32_t m = ts.mesh[sg];
-  const uint32_t s = ts.off[sg];
-  const int nn = (int)(ts.off[sg + 1] - s);
+  const SegInfo si = seg_info(ts, mt, live ? j : n - 1);
+  const uint32_t m = si.m;
+  const uint32_t s = si.s;
+  const int nn = (int)si.n;
   const int i = (int)((live ? j : n - 1) - s);
-  const uint32_t base = mt.noff[m];
+  const uint32_t base = si.noff;
   ysim_bvh_node* T = tree + base;
   int32_t* P = parent + base;
   const bool wide = code_bits > 32;
@@ -596,21 +596,9 @@ __global__ __launch_bounds__(256) void k_agglo(SegTable ts, MeshTables mt, const
   const uint32_t fid = idx[i];
   if (perm_out) perm_out[j] = fid;
 
-  const uint32_t* f = facets + 3 * (size_t)(mt.foff[m] + fid);
   const uint32_t vb = mt.voff[m];
-  f3 lo = ld3(x, vb + f[0]), hi = lo;
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    f3 p = ld3(x, vb + f[c]);
-    lo = vmin(lo, p);
-    hi = vmax(hi, p);
-    if (swept) {
-      f3 vv = ld3(v, vb + f[c]);
-      f3 q = p + vv * dt;
-      lo = vmin(lo, q);
-      hi = vmax(hi, q);
-    }
-  }
+  f3 lo, hi;
+  tri_box(x, v, dt, swept, facets + 3 * (size_t)(mt.foff[m] + fid), vb, lo, hi);
   auto store_box = [&](int node, f3 a, f3 b) {
     float* q = reinterpret_cast<float*>(T + node);
     if (FORMAL) {
@@ -808,21 +796,9 @@ __global__ __launch_bounds__(256) void k_refit(SegTable ts, MeshTables mt, const
 
   int child = nn - 1 + i;
   const uint32_t fid = (uint32_t)T[child].child_b;
-  const uint32_t* f = facets + 3 * (size_t)(mt.foff[m] + fid);
   const uint32_t vb = mt.voff[m];
-  f3 lo = ld3(x, vb + f[0]), hi = lo;
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    f3 p = ld3(x, vb + f[c]);
-    lo = vmin(lo, p);
-    hi = vmax(hi, p);
-    if (swept) {
-      f3 vv = ld3(v, vb + f[c]);
-      f3 q = p + vv * dt;
-      lo = vmin(lo, q);
-      hi = vmax(hi, q);
-    }
-  }
+  f3 lo, hi;
+  tri_box(x, v, dt, swept, facets + 3 * (size_t)(mt.foff[m] + fid), vb, lo, hi);
   auto store_box = [&](int node, f3 a, f3 b) {
     float* q = reinterpret_cast<float*>(T + node);
     if (FORMAL) {
@@ -938,12 +914,11 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
   if (TOP && threadIdx.x == 0) s_nesc = 0;
   const uint32_t j = b0 + threadIdx.x;
   const bool valid = j < n;
-  const uint32_t sg = seg_of(ts, valid ? j : n - 1);
-  const uint32_t m = ts.mesh[sg];
-  const uint32_t s = ts.off[sg];
-  const int nn = (int)(ts.off[sg + 1] - s);
+  const SegInfo si = seg_info(ts, mt, valid ? j : n - 1);
+  const uint32_t s = si.s;
+  const int nn = (int)si.n;
   const int i = (int)((valid ? j : n - 1) - s);
-  const uint32_t base = mt.noff[m];
+  const uint32_t base = si.noff;
   ysim_bvh_node* T = tree + base;
   const int32_t* P = parent + base;
   const uint32_t* UP = up_links + base;
@@ -961,22 +936,7 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
     fid = fids[base + child];
     pnext = P[child];
     upnext = UP[child];
-    const uint32_t* f = facets + 3 * (size_t)(mt.foff[m] + fid);
-    const uint32_t vb = mt.voff[m];
-    lo = ld3(x, vb + f[0]);
-    hi = lo;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      f3 p = ld3(x, vb + f[c]);
-      lo = vmin(lo, p);
-      hi = vmax(hi, p);
-      if (swept) {
-        f3 vv = ld3(v, vb + f[c]);
-        f3 q = p + vv * dt;
-        lo = vmin(lo, q);
-        hi = vmax(hi, q);
-      }
-    }
+    tri_box(x, v, dt, swept, facets + 3 * (size_t)(si.foff + fid), si.voff, lo, hi);
   }
   if (has_link) s_link[threadIdx.x] = link;
   __syncthreads();
@@ -1182,9 +1142,9 @@ __global__ __launch_bounds__(256) void k_refit_span(SegTable ts, MeshTables mt, 
   uint32_t s = 0, base = 0, a = 0, b = 0;
   if (mine) {
     r = sp.rec[j];
-    const uint32_t sg = seg_of(ts, j);
-    s = ts.off[sg];
-    base = mt.noff[ts.mesh[sg]];
+    const SegInfo si = seg_info(ts, mt, j);
+    s = si.s;
+    base = si.noff;
     const uint32_t plo = s + (uint32_t)r.x, phi = s + (uint32_t)r.y;
     a = plo / RB;
     b = phi / RB;

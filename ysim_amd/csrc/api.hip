@@ -1822,6 +1822,8 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
   YSIM_TRY(s.alloc(&s.visit, s.Ntot));
   YSIM_TRY(s.alloc(&s.links.up, s.Ntot));
   YSIM_TRY(s.alloc(&s.links.fid, s.Ntot));
+  YSIM_TRY(s.alloc(&s.links.hint, s.Ntot));
+  YSIM_TRY(s.zero(s.links.hint, 2 * (size_t)s.Ntot));
   YSIM_TRY(s.alloc(&s.links.cut_cnt, num_meshes));
   YSIM_TRY(s.zero(s.links.cut_cnt, num_meshes));
   YSIM_TRY(s.alloc(&s.links.cut, (size_t)num_meshes * CUT_CAP));

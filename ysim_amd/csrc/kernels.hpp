@@ -145,6 +145,10 @@ struct RefitLinks {
   uint32_t span_rb = 0;
   uint64_t* span_mask = nullptr;
   int4* span_rec = nullptr;
+  // [node window + internal node] the other end of the node's leaf range and
+  // its split at the last build that wrote them: checked before the searches
+  // (any contents are safe: a hint that fails its check is searched anew)
+  int2* hint = nullptr;
 };
 constexpr uint32_t CUT_CAP = 512;
 // Partial boxes of the two-kernel refit (span_rb as above): part[position]

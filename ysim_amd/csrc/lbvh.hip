@@ -284,37 +284,83 @@ __global__ __launch_bounds__(INC_THREADS) void k_morton_inc(SegTable ts, MeshTab
     if (cut_fill && ts.nseg == 1)
       for (uint32_t k = threadIdx.x; k < CUT_CAP; k += INC_THREADS) cut_fill[(size_t)ts.mesh[0] * CUT_CAP + k] = -1;
   }
-  if (nparts) {  // one segment: the box from the cloth kernel's block boxes
-    const uint32_t v = reduce_box_records<INC_THREADS, false>(box_parts, nparts, s_part);
+  // Two items per thread.  Their facet ids, then facets, then positions load
+  // first; the box records (one segment: the cloth step's per-block boxes)
+  // load with the facet ids and are reduced while the positions are in
+  // flight, so the box costs no round of its own.
+  static_assert(INC_WINDOW == 2 * INC_THREADS, "two items per thread");
+  const bool wide = code_bits > 32;
+  const uint32_t start = blockIdx.x * INC_WINDOW;
+  const uint32_t cnt = min(INC_WINDOW, n - start);
+  uint32_t fid[2], sgi[2];
+  f3 x0[2], x1[2], x2[2];
+  SegInfo si[2];
+  // items past the window's count (last window only) load the last item's
+  // data, so the loads need no branches and issue together
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const uint32_t i = min(threadIdx.x + k * INC_THREADS, cnt - 1);
+    si[k] = seg_info(ts, mt, start + i);
+    fid[k] = perm[start + i];
+  }
+  uint32_t v[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+  if (nparts)  // records of 24 bytes (8-byte aligned): three 8-byte loads each, 4 records in flight
+#pragma unroll 4
+    for (uint32_t r = threadIdx.x; r < nparts; r += INC_THREADS) {
+      const uint2* q = reinterpret_cast<const uint2*>(box_parts + 6 * (size_t)r);
+      const uint2 u0 = q[0], u1 = q[1], u2 = q[2];
+      v[0] = min(v[0], u0.x);
+      v[1] = min(v[1], u0.y);
+      v[2] = min(v[2], u1.x);
+      v[3] = max(v[3], u1.y);
+      v[4] = max(v[4], u2.x);
+      v[5] = max(v[5], u2.y);
+    }
+  pu3 f[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) f[k] = *reinterpret_cast<const pu3*>(facets + 3 * (size_t)(si[k].foff + fid[k]));
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const uint32_t vb = si[k].voff;
+    x0[k] = ld3(x, vb + f[k].a);
+    x1[k] = ld3(x, vb + f[k].b);
+    x2[k] = ld3(x, vb + f[k].c);
+    sgi[k] = si[k].sg;
+  }
+  if (nparts) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = k < 3 ? wave_min_u32(v[k]) : wave_max_u32(v[k]);
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    constexpr uint32_t NW = INC_THREADS / 64;
+    if (lane == 0)
+#pragma unroll
+      for (int k = 0; k < 6; ++k) s_part[k * NW + wv] = v[k];
+    __syncthreads();
     if (threadIdx.x < 6) {
-      s_box[threadIdx.x] = v;
-      if (blockIdx.x == 0) acc[threadIdx.x] = v;
+      const uint32_t k = threadIdx.x;
+      uint32_t o = s_part[k * NW];
+      for (uint32_t ww = 1; ww < NW; ++ww) o = k < 3 ? min(o, s_part[k * NW + ww]) : max(o, s_part[k * NW + ww]);
+      s_box[threadIdx.x] = o;
+      if (blockIdx.x == 0) acc[threadIdx.x] = o;
     }
     __syncthreads();
   }
   const uint32_t* box = nparts ? s_box : acc;
-  const bool wide = code_bits > 32;
-  const uint32_t start = blockIdx.x * INC_WINDOW;
-  const uint32_t cnt = min(INC_WINDOW, n - start);
-  for (uint32_t i = threadIdx.x; i < INC_WINDOW; i += INC_THREADS) {
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const uint32_t i = threadIdx.x + k * INC_THREADS;
     if (i >= cnt) {
       sk[i] = (K)~(K)0;
       sv[i] = ~0u;
       continue;
     }
-    const uint32_t j = start + i;
-    const SegInfo si = seg_info(ts, mt, j);
-    const uint32_t sg = si.sg;
-    const uint32_t fid = perm[j];
-    const pu3 f = *reinterpret_cast<const pu3*>(facets + 3 * (size_t)(si.foff + fid));
-    const uint32_t vb = si.voff;
-    const f3 x0 = ld3(x, vb + f.a), x1 = ld3(x, vb + f.b), x2 = ld3(x, vb + f.c);
+    const uint32_t sg = sgi[k];
     const uint32_t* a = box + sg * 6;
     const f3 bmin = {funkey(a[0]), funkey(a[1]), funkey(a[2])};
     const f3 bmax = {funkey(a[3]), funkey(a[4]), funkey(a[5])};
-    const uint64_t code = morton_of(x0, x1, x2, bmin, bmax, wide);
+    const uint64_t code = morton_of(x0[k], x1[k], x2[k], bmin, bmax, wide);
     sk[i] = (ts.nseg > 1) ? (K)(((uint64_t)sg << code_bits) | code) : (K)code;
-    sv[i] = fid;
+    sv[i] = fid[k];
   }
   __syncthreads();
   window_sort<K, INC_WINDOW, INC_THREADS>(sk, sv, cnt);

@@ -482,13 +482,14 @@ def main():
     sb = stage_bytes(V, T, 8, passes, contacts, incremental)
     # incremental: shifted windows with the check, then the fallback (Morton,
     # MSD pass, buckets) whose kernels return at once unless the check failed
-    launches = {"box": 1, "morton": 1, "sort": 4 if incremental else passes, "hierarchy": 1, "refit": 1,
+    launches = {"box": 1, "morton": 1, "sort": 4 if incremental else passes, "hierarchy": 1, "refit": 2,
                 "query": 1, "integrate": 1}
     kernels = {"box": "k_box_parts",
                "morton": "k_morton_inc<uint64_t>" if incremental else f"k_morton<uint64_t, {digit_bits}, 8>",
                "sort": "k_window_sort_shifted<uint64_t>" if incremental
                else f"k_onesweep<uint64_t, {digit_bits}, {int(tuning['sort_items'])}>",
-               "hierarchy": "k_hierarchy<uint64_t>", "refit": f"k_refit_lds<false, {int(tuning['refit_block'])}>",
+               "hierarchy": "k_hierarchy<uint64_t>",
+               "refit": f"k_refit_lds<false, {int(tuning['refit_block'])}, true> + k_refit_span<{int(tuning['refit_block'])}>",
                "query": "k_query_tile" if int(tuning.get("query_tile", 0))
                else f"k_query<false, {int(tuning['query_block'])}, {int(tuning['query_waves'])}>",
                "integrate": f"k_cloth<{int(tuning['cloth_block'])}>"}
@@ -521,6 +522,9 @@ def main():
         if s == "sort" and incremental:
             ks = ["k_window_sort_shifted<uint64_t>", "k_morton<uint64_t, 11, 1>",
                   f"k_onesweep<uint64_t, 11, {int(tuning['sort_items'])}>", "k_bucket_sort<uint64_t, 16>"]
+        elif s == "refit" and launches[s]:
+            # the block pass and the spanning-node pass
+            ks = kernels[s].split(" + ")
         else:
             ks = [kernels[s]] * launches[s] if launches[s] else []
         tbs = [pmc_traffic(k)[0] for k in ks]
@@ -531,7 +535,9 @@ def main():
     per_launch_bytes = sb[dom] / launches[dom]
     per_launch_ms = stages_ms[dom] / launches[dom]
     achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
-    traffic, traffic_src = pmc_traffic(kernels[dom])
+    parts = [pmc_traffic(k) for k in kernels[dom].split(" + ")]
+    traffic = None if any(t is None for t, _ in parts) else sum(t for t, _ in parts) // launches[dom]
+    traffic_src = parts[0][1]
     copy = None if args.no_copy else ctx.copy_bandwidth(1 << 30, 20)
     if copy:
         log(f"copy ceiling: {copy['GBps']:.0f} GB/s")

@@ -563,6 +563,13 @@ struct ysim_scene {
   int put(T** p, const std::vector<T>& h) {
     return put(p, h.data(), h.size());
   }
+  int fill(void* p, size_t words, uint32_t v) {  // a kernel: graph captures record no memset nodes
+    if (!words) return YSIM_OK;
+    hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>(4096, ceil_div(words, 256))), dim3(256), 0, st,
+                       (uint32_t*)p, (uint32_t)words, v);
+    YSIM_CHECK_LAUNCH();
+    return YSIM_OK;
+  }
   int zero(void* p, size_t words) {
     if (!words) return YSIM_OK;
     hipLaunchKernelGGL(k_zero_u32, dim3(std::min<uint32_t>(4096, ceil_div(words, 256))), dim3(256), 0, st,
@@ -804,6 +811,8 @@ struct ysim_scene {
                             P.swept != 0 && vc, facets, tree, parent, visit, agg_range, g.T, box_acc, g.tsegs.nseg,
                             hist, (uint32_t)g.plan.hist_words, g.perm, flags, formal_refit));
       tree_agglo = true;
+      // its parent links are not the Karras topology the hints describe
+      YSIM_TRY(fill(links.hint, 2 * (size_t)Ntot, 0xffffffffu));
       if (dyn_groups.size() == 1 && &g == &dyn_groups[0]) span_rb_dyn = 0;
       return YSIM_OK;
     }
@@ -1823,7 +1832,7 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
   YSIM_TRY(s.alloc(&s.links.up, s.Ntot));
   YSIM_TRY(s.alloc(&s.links.fid, s.Ntot));
   YSIM_TRY(s.alloc(&s.links.hint, s.Ntot));
-  YSIM_TRY(s.zero(s.links.hint, 2 * (size_t)s.Ntot));
+  YSIM_TRY(s.fill(s.links.hint, 2 * (size_t)s.Ntot, 0xffffffffu));  // (-1, -1): no hint
   YSIM_TRY(s.alloc(&s.links.cut_cnt, num_meshes));
   YSIM_TRY(s.zero(s.links.cut_cnt, num_meshes));
   YSIM_TRY(s.alloc(&s.links.cut, (size_t)num_meshes * CUT_CAP));

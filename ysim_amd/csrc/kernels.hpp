@@ -146,8 +146,11 @@ struct RefitLinks {
   uint64_t* span_mask = nullptr;
   int4* span_rec = nullptr;
   // [node window + internal node] the other end of the node's leaf range and
-  // its split at the last build that wrote them: checked before the searches
-  // (any contents are safe: a hint that fails its check is searched anew)
+  // its split at the last hierarchy build of these links, or (-1, -1), which
+  // never passes: checked before the searches, and a node whose hint passes
+  // has the children it had then, so it rewrites neither the hint nor its
+  // children's parent and `up` links.  Any other writer of the parent links
+  // (the agglomerative build) resets the hints to (-1, -1).
   int2* hint = nullptr;
 };
 constexpr uint32_t CUT_CAP = 512;

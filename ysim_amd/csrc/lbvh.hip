@@ -593,12 +593,16 @@ __global__ __launch_bounds__(256) void k_hierarchy(SegTable ts, MeshTables mt, c
     T[i].child_a = ca;
     T[i].child_b = cb;
   }
-  P[ca] = i;
-  P[cb] = i;
-  if (rl.up) {
-    const uint32_t size2 = (uint32_t)(hi - lo + 1) << 1;
-    rl.up[base + ca] = size2;       // left child
-    rl.up[base + cb] = size2 | 1u;  // right child
+  // a node whose range and split passed the check has the children it had
+  // at the last build, whose parent and `up` links are already these values
+  if (!hinted) {
+    P[ca] = i;
+    P[cb] = i;
+    if (rl.up) {
+      const uint32_t size2 = (uint32_t)(hi - lo + 1) << 1;
+      rl.up[base + ca] = size2;       // left child
+      rl.up[base + cb] = size2 | 1u;  // right child
+    }
   }
   if (rl.cut) {  // children of at most S leaves under a node of more
     const uint32_t S = max(1u, ((uint32_t)nn + 127u) / 128u);
@@ -1066,7 +1070,10 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
       const uint32_t q = side ? clo - 1 : chi + 1, qe = side ? plo : phi;
       const int sib = (int)q + (q == qe ? nn - 1 : 0);
       const uint32_t slot = s + (uint32_t)p - b0;  // in [0, RB) for a local p: an endpoint of its range
-      const bool bad = (p < 0) | (p >= nn - 1) | (sib < 0) | (sib >= 2 * nn - 1) | (step >= 2 * nn);
+      // (an inconsistent parent size would wrap plo or push phi past the
+      // segment: a corrupted topology, flagged, never used as a range)
+      const bool bad = (p < 0) | (p >= nn - 1) | (sib < 0) | (sib >= 2 * nn - 1) | (step >= 2 * nn) |
+                       (plo > phi) | (phi >= (uint32_t)nn);
       const bool local = !bad & (s + plo >= b0) & (s + phi < b0 + RB);
       if (!local || slot >= (uint32_t)RB) {
         dev = !local && !bad;  // else a corrupted topology

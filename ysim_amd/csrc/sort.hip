@@ -69,6 +69,10 @@ __global__ void k_zero_u32(uint32_t* __restrict__ p, uint32_t n) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
 __global__ void k_sort_epoch_one(uint32_t* epoch) { *epoch = 1u; }
 
 // One LSD pass.  Block = 4 waves; wave w owns a contiguous run of 64*ITEMS keys

@@ -55,5 +55,6 @@ int launch_sort_passes(hipStream_t s, const SortPlan& p, void* keys_a, uint32_t*
                        uint32_t* counters, const uint32_t* epoch, uint32_t* err);
 
 __global__ void k_zero_u32(uint32_t* __restrict__ p, uint32_t n);
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v);
 
 }  // namespace ysim

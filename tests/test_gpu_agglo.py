@@ -117,3 +117,30 @@ def test_builder_switches_and_batched(ctx, oracle_threads):
         for u, w in zip(a.state(), b.state()):
             assert np.array_equal(u, w)
     assert a.flags() == 0 and b.flags() == 0
+
+
+@pytest.mark.parametrize("rebuild_every", [1, 3])
+def test_karras_after_agglomerative_under_capture(ctx, oracle_threads, rebuild_every):
+    """Regression: Karras nodes whose range and split hints pass skip rewriting
+    their children's parent and `up` links.  After agglomerative builds (which
+    write other parent links) the next Karras build must rewrite them all,
+    also when the switch back captures a graph before its first (eager)
+    substep runs.  Trees and parents after each Karras segment, and states
+    after every segment, equal the oracle's (which builds with Karras)."""
+    print(f"karras after agglomerative, rebuild_every={rebuild_every}", flush=True)
+    sc = scenes.config_c2(n=96)
+    sc.params["rebuild_every"] = rebuild_every
+    hs, os_ = HipScene(ctx, sc), O.OracleScene(sc)
+    for mode in (0, 1, 0, 1, 1, 0):
+        hs.set_tuning(build_mode=mode)
+        os_.set_build_mode(mode)
+        hs.step(2 * rebuild_every + 1, graph=True)
+        os_.step(2 * rebuild_every + 1)
+        for u, w in zip(hs.state(), os_.state()):
+            assert np.array_equal(u, w)
+        if mode == 0 and rebuild_every == 1:
+            gn, gp = hs.tree(0)
+            on, op, _, _ = os_.tree(0)
+            _same_tree(gn, on)
+            assert np.array_equal(gp, op)
+    assert hs.flags() == 0

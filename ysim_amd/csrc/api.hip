@@ -293,6 +293,9 @@ struct ysim_scene {
   // records were written for (0: none; the refit then climbs device-wide).
   RefitSpan span;
   uint32_t span_rb_dyn = 0;
+  // the same view after a replay of the captured graph
+  bool graph_tree_agglo = false;
+  uint32_t graph_last_sorted = 0, graph_span_rb = 0;
   // The links a group's hierarchy kernel writes: spanning nodes are recorded
   // when the refit that follows shares the group's leaf positions (a static
   // group's own refit at creation, or the only dynamic group).
@@ -1025,6 +1028,13 @@ struct ysim_scene {
     // skips the box kernel too; the host flag is left as it was
     const bool fresh = box_fresh;
     const uint32_t parts = parts_count;
+    // the host's view of the tree in memory (which builder made it, which
+    // key buffer holds its order, the block size its spanning nodes were
+    // recorded for) changes as builds are enqueued; captured builds run only
+    // when the graph is replayed, so the view is kept as it was and the
+    // view after a replay is applied at each launch
+    const bool agglo0 = tree_agglo;
+    const uint32_t sorted0 = last_sorted, span0 = span_rb_dyn;
     box_fresh = true;
     parts_count = parts_for_config();
     capturing = true;
@@ -1033,6 +1043,12 @@ struct ysim_scene {
     capturing = false;
     box_fresh = fresh;
     parts_count = parts;
+    graph_tree_agglo = tree_agglo;
+    graph_last_sorted = last_sorted;
+    graph_span_rb = span_rb_dyn;
+    tree_agglo = agglo0;
+    last_sorted = sorted0;
+    span_rb_dyn = span0;
     hipError_t e = hipStreamEndCapture(st, &g);
     if (rc != YSIM_OK) return rc;
     if (e != hipSuccess) return fail(YSIM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -1066,6 +1082,9 @@ struct ysim_scene {
         continue;
       }
       YSIM_HIP_TRY(hipGraphLaunch(graph, st));
+      tree_agglo = graph_tree_agglo;
+      last_sorted = graph_last_sorted;
+      span_rb_dyn = graph_span_rb;
       step += graph_period;
       steps_run += graph_period;
       n -= graph_period;

@@ -204,3 +204,61 @@ def test_planted_mistake_is_caught(bits, what, kd, oracle_threads):
     pred = float((gx0 * d).sum())
     worst = max(worst, abs(fdx - pred) / (FD_REL * abs(fdx) + _fd_atol(L, 1e-5)))
     assert worst > 1.0, f"planted mistake ({what}) not detected"
+
+
+@pytest.mark.parametrize("n", [16, 24])
+def test_unfrozen_map_jumps_where_the_gradient_lives(n, oracle_threads):
+    """Why the adjoint differentiates the frozen map (each step's contacts and
+    gates held fixed), measured: on the C4 drape every spring among those
+    touching the sphere whose gradient exceeds 1e-9 changes the contact
+    signature (step, vertex, facet, projection gate) of the ordinary,
+    detecting forward under a log-stiffness perturbation of only +-1e-9, so
+    the unfrozen map jumps at theta_0 there and no finite difference resolves
+    its derivative; springs that keep the signature under +-1e-3 are where
+    both maps coincide, and there their central differences agree."""
+    sc = scenes.config_c4(n=n, steps=60, checkpoint_every=10)
+    tt, t0 = scenes.c4_theta(sc)
+    t0 = t0.astype(np.float64)
+    kd = sc.params["kd"]
+    R, Rd = O.AdjointRollout(sc, 64), O.AdjointRollout(sc, 64)
+    xT, _ = R.forward(tt, kd, 60, 10)
+    x0, _ = R.forward(t0, kd, 60, 10, mode="record")
+    c0, s0 = R.contacts()
+    assert c0 > 0
+    g, _, _, _ = R.backward(x0 - xT)
+    springs = _pick_springs(R, n, 48, seed=n)
+    relevant = [j for j in springs if abs(g[j]) > 1e-9]
+    assert len(relevant) >= 8
+    for j in relevant:
+        tp, tm = t0.copy(), t0.copy()
+        tp[j] += 1e-9
+        tm[j] -= 1e-9
+        Rd.forward(tp, kd, 60, 10, mode="detect")
+        sp = Rd.contacts()
+        Rd.forward(tm, kd, 60, 10, mode="detect")
+        sm = Rd.contacts()
+        assert sp != (c0, s0) or sm != (c0, s0), j
+    # where the signature holds (20 steps: the first contacts), the maps agree
+    steps, e = 20, FD_EPS
+    xT, _ = R.forward(tt, kd, steps, 10)
+    x0, _ = R.forward(t0, kd, steps, 10, mode="record")
+    c0, s0 = R.contacts()
+    L = _loss(x0, xT)
+    same = 0
+    for j in springs:
+        tp, tm = t0.copy(), t0.copy()
+        tp[j] += e
+        tm[j] -= e
+        xdp, _ = Rd.forward(tp, kd, steps, 10, mode="detect")
+        sp = Rd.contacts()
+        xdm, _ = Rd.forward(tm, kd, steps, 10, mode="detect")
+        sm = Rd.contacts()
+        if sp != (c0, s0) or sm != (c0, s0):
+            continue
+        same += 1
+        xfp, _ = R.forward(tp, kd, steps, 10, mode="frozen")
+        xfm, _ = R.forward(tm, kd, steps, 10, mode="frozen")
+        fd_unfrozen = (_loss(xdp, xT) - _loss(xdm, xT)) / (2 * e)
+        fd_frozen = (_loss(xfp, xT) - _loss(xfm, xT)) / (2 * e)
+        assert abs(fd_unfrozen - fd_frozen) <= 2 * _fd_atol(L, e), (j, fd_unfrozen, fd_frozen)
+    assert same >= 4, (same, len(springs))

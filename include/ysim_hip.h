@@ -146,6 +146,8 @@ const char* ysim_hip_last_error(void);
 int ysim_hip_version(void);
 int ysim_hip_device_count(int* out);
 int ysim_hip_ctx_create(int device, uint32_t flags, ysim_ctx** out);
+/* Destroy calls wait for the stream, then release everything; a failure the
+ * stream reports is returned after the release. */
 int ysim_hip_ctx_destroy(ysim_ctx* ctx);
 int ysim_hip_sync(ysim_ctx* ctx);
 int ysim_hip_alloc(ysim_ctx* ctx, size_t bytes, void** d_out);

@@ -1092,12 +1092,14 @@ struct ysim_scene {
     return YSIM_OK;
   }
 
+  // Teardown is best-effort: every handle is released whatever an earlier
+  // release returned (the destroy entry point reports the stream's state).
   ~ysim_scene() {
-    if (graph) hipGraphExecDestroy(graph);
+    if (graph) (void)hipGraphExecDestroy(graph);
     if (h_stale) (void)hipHostFree(h_stale);
     for (auto& e : ev)
-      if (e) hipEventDestroy(e);
-    for (void* p : allocs) hipFree(p);
+      if (e) (void)hipEventDestroy(e);
+    for (void* p : allocs) (void)hipFree(p);
   }
 };
 
@@ -1315,10 +1317,13 @@ int ysim_hip_ctx_create(int device, uint32_t flags, ysim_ctx** out) {
 
 int ysim_hip_ctx_destroy(ysim_ctx* ctx) {
   if (!ctx) return YSIM_OK;
-  hipStreamSynchronize(ctx->stream);
-  if (ctx->scratch) hipFree(ctx->scratch);
-  hipStreamDestroy(ctx->stream);
+  // Everything is released even when the stream reports an earlier failure;
+  // that failure is then the return value.
+  const hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  (void)hipStreamDestroy(ctx->stream);
   delete ctx;
+  if (e != hipSuccess) return fail(YSIM_ERR_HIP, std::string("ctx_destroy: ") + hipGetErrorString(e));
   return YSIM_OK;
 }
 
@@ -2033,8 +2038,9 @@ int ysim_hip_scene_create(ysim_ctx* ctx, uint32_t num_meshes, const ysim_mesh_de
 
 int ysim_hip_scene_destroy(ysim_scene* scene) {
   if (!scene) return YSIM_OK;
-  hipStreamSynchronize(scene->st);
+  const hipError_t e = hipStreamSynchronize(scene->st);
   delete scene;
+  if (e != hipSuccess) return fail(YSIM_ERR_HIP, std::string("scene_destroy: ") + hipGetErrorString(e));
   return YSIM_OK;
 }
 

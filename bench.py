@@ -576,6 +576,7 @@ def main():
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "traffic_source": traffic_src,
+            "traffic_kind": "upper bound (FETCH_SIZE counts L2 fills, some served by the MALL)",
             "bytes_per_launch": int(per_launch_bytes), "ms_per_launch": round(per_launch_ms, 5),
             "copy_ceiling_GBps": None if copy is None else round(copy["GBps"], 1),
         },

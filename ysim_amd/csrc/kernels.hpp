@@ -300,40 +300,74 @@ struct SpringIdx {
   }
 };
 
-// Visit the (up to) 12 grid springs of local vertex i in the gather order of
-// compute_cloth_grid_forces_fast (src/metal/physics.metal:195-218):
-// fn(local neighbour, rest length, family k, spring index).
+// Visit the (up to) 12 grid springs of local vertex (row, col) in the gather
+// order of compute_cloth_grid_forces_fast (src/metal/physics.metal:195-218):
+// fn(local neighbour, row offset, column offset, rest length, family k,
+// spring index).
+template <class Fn>
+__device__ __forceinline__ void for_grid_springs_rc(const ysim_fastgrid_params& P, uint32_t row, uint32_t col,
+                                                    Fn&& fn) {
+  const uint32_t N = P.particle_num_1d;
+  const SpringIdx S(N);
+  const uint32_t i = row * N + col;
+  if (col > 0) fn(i - 1, 0, -1, P.stretch_rest_x, P.kstretch, row * (N - 1) + col - 1);
+  if (col < N - 1) fn(i + 1, 0, 1, P.stretch_rest_x, P.kstretch, row * (N - 1) + col);
+  if (row > 0) fn(i - N, -1, 0, P.stretch_rest_y, P.kstretch, S.SY + (row - 1) * N + col);
+  if (row < N - 1) fn(i + N, 1, 0, P.stretch_rest_y, P.kstretch, S.SY + row * N + col);
+  if (col > 0 && row > 0) fn(i - N - 1, -1, -1, P.shear_rest_a, P.kshear, S.SA + (row - 1) * (N - 1) + col - 1);
+  if (col < N - 1 && row > 0) fn(i - N + 1, -1, 1, P.shear_rest_b, P.kshear, S.SB + (row - 1) * (N - 1) + col);
+  if (col > 0 && row < N - 1) fn(i + N - 1, 1, -1, P.shear_rest_b, P.kshear, S.SB + row * (N - 1) + col - 1);
+  if (col < N - 1 && row < N - 1) fn(i + N + 1, 1, 1, P.shear_rest_a, P.kshear, S.SA + row * (N - 1) + col);
+  if (col > 1) fn(i - 2, 0, -2, P.bend_rest_x, P.kbend, S.BX + row * (N - 2) + col - 2);
+  if (col < N - 2) fn(i + 2, 0, 2, P.bend_rest_x, P.kbend, S.BX + row * (N - 2) + col);
+  if (row > 1) fn(i - 2 * N, -2, 0, P.bend_rest_y, P.kbend, S.BY + (row - 2) * N + col);
+  if (row < N - 2) fn(i + 2 * N, 2, 0, P.bend_rest_y, P.kbend, S.BY + row * N + col);
+}
+// The same for local vertex i: fn(local neighbour, rest length, family k,
+// spring index).
 template <class Fn>
 __device__ __forceinline__ void for_grid_springs(const ysim_fastgrid_params& P, uint32_t i, Fn&& fn) {
   const uint32_t N = P.particle_num_1d;
-  const SpringIdx S(N);
-  const uint32_t row = i / N, col = i % N;
-  if (col > 0) fn(row * N + col - 1, P.stretch_rest_x, P.kstretch, row * (N - 1) + col - 1);
-  if (col < N - 1) fn(row * N + col + 1, P.stretch_rest_x, P.kstretch, row * (N - 1) + col);
-  if (row > 0) fn((row - 1) * N + col, P.stretch_rest_y, P.kstretch, S.SY + (row - 1) * N + col);
-  if (row < N - 1) fn((row + 1) * N + col, P.stretch_rest_y, P.kstretch, S.SY + row * N + col);
-  if (col > 0 && row > 0) fn((row - 1) * N + col - 1, P.shear_rest_a, P.kshear, S.SA + (row - 1) * (N - 1) + col - 1);
-  if (col < N - 1 && row > 0) fn((row - 1) * N + col + 1, P.shear_rest_b, P.kshear, S.SB + (row - 1) * (N - 1) + col);
-  if (col > 0 && row < N - 1) fn((row + 1) * N + col - 1, P.shear_rest_b, P.kshear, S.SB + row * (N - 1) + col - 1);
-  if (col < N - 1 && row < N - 1) fn((row + 1) * N + col + 1, P.shear_rest_a, P.kshear, S.SA + row * (N - 1) + col);
-  if (col > 1) fn(row * N + col - 2, P.bend_rest_x, P.kbend, S.BX + row * (N - 2) + col - 2);
-  if (col < N - 2) fn(row * N + col + 2, P.bend_rest_x, P.kbend, S.BX + row * (N - 2) + col);
-  if (row > 1) fn((row - 2) * N + col, P.bend_rest_y, P.kbend, S.BY + (row - 2) * N + col);
-  if (row < N - 2) fn((row + 2) * N + col, P.bend_rest_y, P.kbend, S.BY + row * N + col);
+  for_grid_springs_rc(P, i / N, i % N,
+                      [&](uint32_t q, int, int, float rest, float k, uint32_t sidx) { fn(q, rest, k, sidx); });
 }
 
-// Spring + damping + air force on local vertex i (global offset vb): ext + kair v
-// + the 12 calc_spring terms in gather order.  KS (optional) holds one
-// stiffness per spring and replaces the family constants.
+// Neighbour states for grid_force_rc: vertex vb + q of X and V.
+struct NbrGlobal {
+  const float* X;
+  const float* V;
+  uint32_t vb;
+  __device__ __forceinline__ void operator()(uint32_t q, int, int, f3& x, f3& v) const {
+    x = ld3(X, vb + q);
+    v = ld3(V, vb + q);
+  }
+};
+
+// Spring + damping + air force on the vertex at (row, col): ext + kair v + the
+// 12 calc_spring terms in gather order, neighbour states from nbr.  KSPRING:
+// KS holds one stiffness per spring and replaces the family constants (a
+// template switch, so the family constants stay in registers).
+template <bool KSPRING, class Nbr>
+__device__ __forceinline__ f3 grid_force_rc(const ysim_fastgrid_params& P, uint32_t row, uint32_t col, const Nbr& nbr,
+                                            f3 pos0, f3 vel0, f3 ext, float kair, float kd, const float* KS) {
+  f3 force = ext + vel0 * kair;
+  for_grid_springs_rc(P, row, col, [&](uint32_t q, int dr, int dc, float rest, float k, uint32_t sidx) {
+    f3 xq, vq;
+    nbr(q, dr, dc, xq, vq);
+    force = force + calc_spring(pos0, vel0, xq, vq, rest, KSPRING ? KS[sidx] : k, kd);
+  });
+  return force;
+}
+
+// grid_force_rc on local vertex i (global offset vb), neighbours from X and V;
+// KS (optional) holds one stiffness per spring.
 __device__ __forceinline__ f3 grid_force(const ysim_fastgrid_params& P, uint32_t i, uint32_t vb, const float* X,
                                          const float* V, f3 pos0, f3 vel0, f3 ext, float kair, float kd,
                                          const float* KS) {
-  f3 force = ext + vel0 * kair;
-  for_grid_springs(P, i, [&](uint32_t q, float rest, float k, uint32_t sidx) {
-    const float kk = KS ? KS[sidx] : k;
-    force = force + calc_spring(pos0, vel0, ld3(X, vb + q), ld3(V, vb + q), rest, kk, kd);
-  });
-  return force;
+  const uint32_t N = P.particle_num_1d, row = i / N, col = i % N;
+  const NbrGlobal nbr{X, V, vb};
+  return KS ? grid_force_rc<true>(P, row, col, nbr, pos0, vel0, ext, kair, kd, KS)
+            : grid_force_rc<false>(P, row, col, nbr, pos0, vel0, ext, kair, kd, KS);
 }
 
 // compute_tri_spring_forces (src/metal/physics.metal:319-378) on global vertex
@@ -394,22 +428,67 @@ bool cloth_block_ok(uint32_t b);
 uint32_t cloth_box_parts_words(uint32_t nitems);  // enough for every block size
 uint32_t cloth_box_parts(uint32_t nitems, uint32_t block);
 
+// Where cloth_vertex reads the states of a vertex and its spring neighbours:
+// x and v in device memory (NoTile), or a GridTile staged in LDS by the
+// caller: the 12 x 12 vertices (rows r0-2 .. r0+9, columns c0-2 .. c0+9) of
+// an 8 x 8 tile of an N x N FastGrid cloth with its two-vertex halo, as
+// float4 (xyz, unused w); entry (r - r0 + 2) * 12 + (c - c0 + 2) holds
+// vertex (r, c), h is the caller's own entry and (row, col) its vertex.
+// Entries outside the cloth are never read: the springs to them do not exist.
+struct NoTile {
+  static constexpr bool staged = false;
+};
+struct GridTile {
+  static constexpr bool staged = true;
+  static constexpr int W = 12;  // halo row length
+  const float4* X;
+  const float4* V;
+  uint32_t row, col, h;
+};
+// Neighbour states for grid_force_rc from a GridTile, relative to the
+// caller's entry (X, V point at it).
+struct NbrTile {
+  const float4* X;
+  const float4* V;
+  __device__ __forceinline__ void operator()(uint32_t, int dr, int dc, f3& x, f3& v) const {
+    const int o = dr * GridTile::W + dc;
+    const float4 a = X[o], b = V[o];
+    x = {a.x, a.y, a.z};
+    v = {b.x, b.y, b.z};
+  }
+};
+
 // One vertex j < a.nitems: forces, integrate, contacts, guard; stores v_out
 // and x_out and returns the new position.  count >= 0: the vertex's contact
-// count, known to the caller (else read from a.ccount).
-__device__ __forceinline__ f3 cloth_vertex(const ClothArgs& a, uint32_t j, int32_t count = -1) {
+// count, known to the caller (else read from a.ccount).  A GridTile (the
+// vertex is (tile.row, tile.col) of a FastGrid cloth) supplies the states of
+// the vertex and its neighbours; the arithmetic is the same either way.
+template <class Tile = NoTile>
+__device__ __forceinline__ f3 cloth_vertex(const ClothArgs& a, uint32_t j, int32_t count = -1,
+                                           const Tile& tile = Tile{}) {
   const uint32_t sg = seg_of(a.dsegs, j);
   const uint32_t m = a.dsegs.mesh[sg];
   const uint32_t i = j - a.dsegs.off[sg];
   const uint32_t vb = a.mt.voff[m];
   const uint32_t g = vb + i;
   const ysim_fastgrid_params P = a.cloth[m];
-  const f3 pos0 = ld3(a.x, g), vel0 = ld3(a.v, g);
   const float* KS = (a.kspring && sg == 0) ? a.kspring : nullptr;
-  const f3 force = (a.behavior && a.behavior[m] == YSIM_BEHAVIOR_TRIANGULAR_CLOTH)
-                       ? tri_force(P, g, a.tri_off, a.tri_mid, a.tri_ent, a.x, a.v, pos0, vel0, ld3(a.ext, g), a.kair,
-                                   a.kd)
-                       : grid_force(P, i, vb, a.x, a.v, pos0, vel0, ld3(a.ext, g), a.kair, a.kd, KS);
+  f3 pos0, vel0, force;
+  if constexpr (Tile::staged) {
+    const float4 xs = tile.X[tile.h], vs = tile.V[tile.h];
+    pos0 = {xs.x, xs.y, xs.z};
+    vel0 = {vs.x, vs.y, vs.z};
+    const NbrTile nbr{tile.X + tile.h, tile.V + tile.h};
+    const f3 ext = ld3(a.ext, g);
+    force = KS ? grid_force_rc<true>(P, tile.row, tile.col, nbr, pos0, vel0, ext, a.kair, a.kd, KS)
+               : grid_force_rc<false>(P, tile.row, tile.col, nbr, pos0, vel0, ext, a.kair, a.kd, KS);
+  } else {
+    pos0 = ld3(a.x, g);
+    vel0 = ld3(a.v, g);
+    force = (a.behavior && a.behavior[m] == YSIM_BEHAVIOR_TRIANGULAR_CLOTH)
+                ? tri_force(P, g, a.tri_off, a.tri_mid, a.tri_ent, a.x, a.v, pos0, vel0, ld3(a.ext, g), a.kair, a.kd)
+                : grid_force(P, i, vb, a.x, a.v, pos0, vel0, ld3(a.ext, g), a.kair, a.kd, KS);
+  }
   if (a.f_out) st3(a.f_out, g, force);
 
   // integrate_cloth_grid

@@ -40,6 +40,7 @@ static_assert(TQ_LEAVES >= 2 * 64, "a walk round adds up to 2 leaves per lane");
 // each) and their facet ids; the whole buffer is also the fallback per-lane stack
 constexpr uint32_t TQ_BUF_WORDS = 2 * TQ_STACK + TQ_PAIRS + 64 * 9 + 64;
 static_assert(TQ_BUF_WORDS % 128 == 0, "whole per-lane stack entries");
+static_assert(TQ_BUF_WORDS * 4 >= 2 * 144 * 16, "the cloth step stages a 12 x 12 halo of x and v in the buffer");
 constexpr int TQ_SDEPTH = TQ_BUF_WORDS / 128;  // fallback per-lane stack depth of the tile kernel
 
 __device__ __forceinline__ uint64_t lanes_below() { return (1ull << __lane_id()) - 1ull; }
@@ -142,16 +143,48 @@ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __global__ __launch_bounds__(QB) 
 // before the wave barrier (the wave's loads of its own slot lines, never read
 // before in this kernel, then come from L2); the count comes in a register.
 __device__ __forceinline__ void tile_cloth(const ClothArgs& ca, bool active, uint32_t j, uint32_t count,
-                                           uint32_t tile) {
+                                           uint32_t tile, uint32_t grid_n, uint32_t vb, float4* s_halo) {
   wait_all_mem();
   wave_sync();
   const float INF = __uint_as_float(0x7f800000u);
   f3 lo = {INF, INF, INF}, hi = {-INF, -INF, -INF};
-  if (active) lo = hi = cloth_vertex(ca, j, (int32_t)count);
+  const uint32_t lane = __lane_id();
+  const uint32_t nt = grid_n >> 3;
+  if (grid_n && tile < nt * nt) {
+    // a full 8 x 8 tile of an N x N grid cloth (lane 8 r + c: vertex (r0 + r,
+    // c0 + c), every lane active): its 12 x 12 halo into LDS, one batch of
+    // loads, then the springs read their neighbours from there
+    const uint32_t r0 = (tile / nt) * 8, c0 = (tile % nt) * 8;
+    float4* HX = s_halo;
+    float4* HV = s_halo + 144;
+    f3 hx[3], hv[3];
+    bool ok[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const uint32_t e = lane + 64u * k;
+      const int r = (int)(r0 + e / 12u) - 2, c = (int)(c0 + e % 12u) - 2;
+      ok[k] = e < 144u && r >= 0 && c >= 0 && r < (int)grid_n && c < (int)grid_n;
+      if (ok[k]) {
+        const uint32_t q = vb + (uint32_t)r * grid_n + (uint32_t)c;
+        hx[k] = ld3(ca.x, q);
+        hv[k] = ld3(ca.v, q);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if (ok[k]) {
+        HX[lane + 64u * k] = make_float4(hx[k].x, hx[k].y, hx[k].z, 0.f);
+        HV[lane + 64u * k] = make_float4(hv[k].x, hv[k].y, hv[k].z, 0.f);
+      }
+    wave_sync();
+    const uint32_t r = lane >> 3, c = lane & 7u;
+    lo = hi = cloth_vertex(ca, j, (int32_t)count, GridTile{HX, HV, r0 + r, c0 + c, (r + 2) * 12 + c + 2});
+  } else if (active) {
+    lo = hi = cloth_vertex(ca, j, (int32_t)count);
+  }
   if (!ca.box_parts) return;
   const uint32_t r0 = wave_min_u32(fkey(lo.x)), r1 = wave_min_u32(fkey(lo.y)), r2 = wave_min_u32(fkey(lo.z));
   const uint32_t r3 = wave_max_u32(fkey(hi.x)), r4 = wave_max_u32(fkey(hi.y)), r5 = wave_max_u32(fkey(hi.z));
-  const uint32_t lane = __lane_id();
   if (lane < 6)
     ca.box_parts[6 * (size_t)tile + lane] =
         lane == 0 ? r0 : lane == 1 ? r1 : lane == 2 ? r2 : lane == 3 ? r3 : lane == 4 ? r4 : r5;
@@ -176,7 +209,7 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
   // candidates (lane | batch slot << 6), the corners and facet ids of the
   // current batch of leaves; the whole buffer is the per-lane stack
   // (TQ_SDEPTH entries) of the fallback walks
-  __shared__ int2 s_buf[TQ_BUF_WORDS / 2];
+  __shared__ __align__(16) int2 s_buf[TQ_BUF_WORDS / 2];
   __shared__ uint2 s_leaf[TQ_LEAVES];  // (leaf node, facet) overlapping the union box
   __shared__ float s_q[64 * 6];        // qcur, qprev per lane
   __shared__ float s_rc[16 * 6];       // boxes of the tile's 8 rows, then of its 8 columns
@@ -628,7 +661,8 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
       if (cc > a.cap) atomicOr(a.flags, 2u);
     }
   }
-  if (CLOTH) tile_cloth(ca, active, jc, cc, w);  // every lane (wave reductions)
+  if (CLOTH)  // every lane (wave reductions); s_buf is free again
+    tile_cloth(ca, active, jc, cc, w, one ? a.seg0_grid_n : 0u, a.seg0_voff, reinterpret_cast<float4*>(s_buf));
   wave_stats(a, w, npair);
 }
 

@@ -433,7 +433,8 @@ uint32_t cloth_box_parts(uint32_t nitems, uint32_t block);
 // caller: the 12 x 12 vertices (rows r0-2 .. r0+9, columns c0-2 .. c0+9) of
 // an 8 x 8 tile of an N x N FastGrid cloth with its two-vertex halo, as
 // float4 (xyz, unused w); entry (r - r0 + 2) * 12 + (c - c0 + 2) holds
-// vertex (r, c), h is the caller's own entry and (row, col) its vertex.
+// vertex (r, c), h is the caller's own entry and (row, col) its vertex, ext
+// its external force.
 // Entries outside the cloth are never read: the springs to them do not exist.
 struct NoTile {
   static constexpr bool staged = false;
@@ -444,6 +445,7 @@ struct GridTile {
   const float4* X;
   const float4* V;
   uint32_t row, col, h;
+  f3 ext;  // the caller's external force, loaded with the halo
 };
 // Neighbour states for grid_force_rc from a GridTile, relative to the
 // caller's entry (X, V point at it).
@@ -458,37 +460,45 @@ struct NbrTile {
   }
 };
 
-// One vertex j < a.nitems: forces, integrate, contacts, guard; stores v_out
-// and x_out and returns the new position.  count >= 0: the vertex's contact
-// count, known to the caller (else read from a.ccount).  A GridTile (the
+// The state and total force of local vertex i of mesh m (global vertex vb +
+// i, dynamic segment sg): the first half of cloth_vertex.  A GridTile (the
 // vertex is (tile.row, tile.col) of a FastGrid cloth) supplies the states of
 // the vertex and its neighbours; the arithmetic is the same either way.
+struct ClothForce {
+  f3 pos0, vel0, force;
+};
 template <class Tile = NoTile>
-__device__ __forceinline__ f3 cloth_vertex(const ClothArgs& a, uint32_t j, int32_t count = -1,
-                                           const Tile& tile = Tile{}) {
-  const uint32_t sg = seg_of(a.dsegs, j);
-  const uint32_t m = a.dsegs.mesh[sg];
-  const uint32_t i = j - a.dsegs.off[sg];
-  const uint32_t vb = a.mt.voff[m];
+__device__ __forceinline__ ClothForce cloth_force_at(const ClothArgs& a, uint32_t sg, uint32_t m, uint32_t i,
+                                                     uint32_t vb, const Tile& tile = Tile{}) {
   const uint32_t g = vb + i;
   const ysim_fastgrid_params P = a.cloth[m];
   const float* KS = (a.kspring && sg == 0) ? a.kspring : nullptr;
-  f3 pos0, vel0, force;
+  ClothForce cf;
   if constexpr (Tile::staged) {
     const float4 xs = tile.X[tile.h], vs = tile.V[tile.h];
-    pos0 = {xs.x, xs.y, xs.z};
-    vel0 = {vs.x, vs.y, vs.z};
+    cf.pos0 = {xs.x, xs.y, xs.z};
+    cf.vel0 = {vs.x, vs.y, vs.z};
     const NbrTile nbr{tile.X + tile.h, tile.V + tile.h};
-    const f3 ext = ld3(a.ext, g);
-    force = KS ? grid_force_rc<true>(P, tile.row, tile.col, nbr, pos0, vel0, ext, a.kair, a.kd, KS)
-               : grid_force_rc<false>(P, tile.row, tile.col, nbr, pos0, vel0, ext, a.kair, a.kd, KS);
+    cf.force = KS ? grid_force_rc<true>(P, tile.row, tile.col, nbr, cf.pos0, cf.vel0, tile.ext, a.kair, a.kd, KS)
+                  : grid_force_rc<false>(P, tile.row, tile.col, nbr, cf.pos0, cf.vel0, tile.ext, a.kair, a.kd, KS);
   } else {
-    pos0 = ld3(a.x, g);
-    vel0 = ld3(a.v, g);
-    force = (a.behavior && a.behavior[m] == YSIM_BEHAVIOR_TRIANGULAR_CLOTH)
-                ? tri_force(P, g, a.tri_off, a.tri_mid, a.tri_ent, a.x, a.v, pos0, vel0, ld3(a.ext, g), a.kair, a.kd)
-                : grid_force(P, i, vb, a.x, a.v, pos0, vel0, ld3(a.ext, g), a.kair, a.kd, KS);
+    cf.pos0 = ld3(a.x, g);
+    cf.vel0 = ld3(a.v, g);
+    cf.force = (a.behavior && a.behavior[m] == YSIM_BEHAVIOR_TRIANGULAR_CLOTH)
+                   ? tri_force(P, g, a.tri_off, a.tri_mid, a.tri_ent, a.x, a.v, cf.pos0, cf.vel0, ld3(a.ext, g),
+                               a.kair, a.kd)
+                   : grid_force(P, i, vb, a.x, a.v, cf.pos0, cf.vel0, ld3(a.ext, g), a.kair, a.kd, KS);
   }
+  return cf;
+}
+
+// The rest of vertex j's substep (global vertex g of mesh m) from its state
+// and force: integrate, contacts, guard; stores v_out and x_out (and f_out)
+// and returns the new position.  count >= 0: the vertex's contact count,
+// known to the caller (else read from a.ccount).
+__device__ __forceinline__ f3 cloth_apply_at(const ClothArgs& a, uint32_t j, uint32_t m, uint32_t g, int32_t count,
+                                             const ClothForce& cf) {
+  const f3 pos0 = cf.pos0, vel0 = cf.vel0, force = cf.force;
   if (a.f_out) st3(a.f_out, g, force);
 
   // integrate_cloth_grid
@@ -500,7 +510,7 @@ __device__ __forceinline__ f3 cloth_vertex(const ClothArgs& a, uint32_t j, int32
   vel.y = vel.y + ((subh * force.y) / mass) * mask;
   vel.z = vel.z + ((subh * force.z) / mass) * mask;
 
-  const float thickness = P.thickness;
+  const float thickness = a.cloth[m].thickness;
   auto apply = [&](float4 nd) {
     f3 n = {nd.x, nd.y, nd.z};
     float nlen2 = dot(n, n);
@@ -559,6 +569,14 @@ __device__ __forceinline__ f3 cloth_vertex(const ClothArgs& a, uint32_t j, int32
   st3(a.v_out, g, vel);
   st3(a.x_out, g, pos);
   return pos;
+}
+
+// One vertex j < a.nitems: forces, integrate, contacts, guard; stores v_out
+// and x_out and returns the new position (count as for cloth_apply_at).
+__device__ __forceinline__ f3 cloth_vertex(const ClothArgs& a, uint32_t j, int32_t count = -1) {
+  const uint32_t sg = seg_of(a.dsegs, j);
+  const uint32_t m = a.dsegs.mesh[sg], vb = a.mt.voff[m], i = j - a.dsegs.off[sg];
+  return cloth_apply_at(a, j, m, vb + i, count, cloth_force_at(a, sg, m, i, vb));
 }
 
 int launch_cloth(hipStream_t s, const ClothArgs& a);

@@ -137,52 +137,77 @@ __attribute__((amdgpu_waves_per_eu(WPE, WPE))) __global__ __launch_bounds__(QB) 
   if (!EMIT_PAIRS) wave_stats(a, (xcd_block(blockIdx.x, gridDim.x) * QB + threadIdx.x) >> 6, npair);
 }
 
+// The 12 x 12 halo of x and v around the full 8 x 8 tile (r0, c0) of an N x N
+// grid cloth at global vertex offset vb (entry e = lane + 64 k, vertex (r0 +
+// e / 12 - 2, c0 + e % 12 - 2), loaded when inside the cloth), in registers
+// until halo_store writes it to LDS; every load is issued before any is used.
+struct TileHalo {
+  f3 x[3], v[3];
+  bool ok[3];
+};
+__device__ __forceinline__ TileHalo halo_load(const ClothArgs& ca, uint32_t r0, uint32_t c0, uint32_t N,
+                                              uint32_t vb) {
+  const uint32_t lane = __lane_id();
+  TileHalo h;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const uint32_t e = lane + 64u * k;
+    const int r = (int)(r0 + e / 12u) - 2, c = (int)(c0 + e % 12u) - 2;
+    h.ok[k] = e < 144u && r >= 0 && c >= 0 && r < (int)N && c < (int)N;
+    if (h.ok[k]) {
+      const uint32_t q = vb + (uint32_t)r * N + (uint32_t)c;
+      h.x[k] = ld3(ca.x, q);
+      h.v[k] = ld3(ca.v, q);
+    }
+  }
+  return h;
+}
+// Entries as float4 at HX[e] and HV[e] (GridTile layout); visible to the
+// whole wave after the wave_sync.
+__device__ __forceinline__ void halo_store(const TileHalo& h, float4* HX, float4* HV) {
+  const uint32_t lane = __lane_id();
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    if (h.ok[k]) {
+      HX[lane + 64u * k] = make_float4(h.x[k].x, h.x[k].y, h.x[k].z, 0.f);
+      HV[lane + 64u * k] = make_float4(h.v[k].x, h.v[k].y, h.v[k].z, 0.f);
+    }
+  wave_sync();
+}
+
 // After a tile's query (CLOTH variant): the cloth step of its vertices, and
 // the box of their new positions as 6 ordered keys at box_parts + 6 tile.
+// grid_n > 0: the lanes are vertices of one N x N grid cloth, mesh m at
+// global vertex offset vb, in tile order (j local); a full 8 x 8 tile stages
+// its 12 x 12 halo of x and v in LDS (s_halo, 288 float4) and the springs
+// read their neighbours there.
 // Contact slots were written by any lane of the wave: their stores complete
 // before the wave barrier (the wave's loads of its own slot lines, never read
 // before in this kernel, then come from L2); the count comes in a register.
 __device__ __forceinline__ void tile_cloth(const ClothArgs& ca, bool active, uint32_t j, uint32_t count,
-                                           uint32_t tile, uint32_t grid_n, uint32_t vb, float4* s_halo) {
+                                           uint32_t tile, uint32_t grid_n, uint32_t m, uint32_t vb,
+                                           float4* s_halo) {
   wait_all_mem();
   wave_sync();
   const float INF = __uint_as_float(0x7f800000u);
   f3 lo = {INF, INF, INF}, hi = {-INF, -INF, -INF};
-  const uint32_t lane = __lane_id();
   const uint32_t nt = grid_n >> 3;
   if (grid_n && tile < nt * nt) {
-    // a full 8 x 8 tile of an N x N grid cloth (lane 8 r + c: vertex (r0 + r,
-    // c0 + c), every lane active): its 12 x 12 halo into LDS, one batch of
-    // loads, then the springs read their neighbours from there
+    // a full tile (every lane active, lane 8 r + c on vertex (r0 + r, c0 + c))
     const uint32_t r0 = (tile / nt) * 8, c0 = (tile % nt) * 8;
+    const TileHalo halo = halo_load(ca, r0, c0, grid_n, vb);
+    const f3 ext = ld3(ca.ext, vb + j);
     float4* HX = s_halo;
     float4* HV = s_halo + 144;
-    f3 hx[3], hv[3];
-    bool ok[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const uint32_t e = lane + 64u * k;
-      const int r = (int)(r0 + e / 12u) - 2, c = (int)(c0 + e % 12u) - 2;
-      ok[k] = e < 144u && r >= 0 && c >= 0 && r < (int)grid_n && c < (int)grid_n;
-      if (ok[k]) {
-        const uint32_t q = vb + (uint32_t)r * grid_n + (uint32_t)c;
-        hx[k] = ld3(ca.x, q);
-        hv[k] = ld3(ca.v, q);
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-      if (ok[k]) {
-        HX[lane + 64u * k] = make_float4(hx[k].x, hx[k].y, hx[k].z, 0.f);
-        HV[lane + 64u * k] = make_float4(hv[k].x, hv[k].y, hv[k].z, 0.f);
-      }
-    wave_sync();
-    const uint32_t r = lane >> 3, c = lane & 7u;
-    lo = hi = cloth_vertex(ca, j, (int32_t)count, GridTile{HX, HV, r0 + r, c0 + c, (r + 2) * 12 + c + 2});
+    halo_store(halo, HX, HV);
+    const uint32_t r = __lane_id() >> 3, c = __lane_id() & 7u;
+    const ClothForce cf = cloth_force_at(ca, 0u, m, j, vb, GridTile{HX, HV, r0 + r, c0 + c, (r + 2) * 12 + c + 2, ext});
+    lo = hi = cloth_apply_at(ca, j, m, vb + j, (int32_t)count, cf);
   } else if (active) {
     lo = hi = cloth_vertex(ca, j, (int32_t)count);
   }
   if (!ca.box_parts) return;
+  const uint32_t lane = __lane_id();
   const uint32_t r0 = wave_min_u32(fkey(lo.x)), r1 = wave_min_u32(fkey(lo.y)), r2 = wave_min_u32(fkey(lo.z));
   const uint32_t r3 = wave_max_u32(fkey(hi.x)), r4 = wave_max_u32(fkey(hi.y)), r5 = wave_max_u32(fkey(hi.z));
   if (lane < 6)
@@ -265,9 +290,11 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
       pn0 = CL[lane];
       pn1 = CL[64 + lane];
       pn2 = CL[128 + lane];
-      if (pn0 >= 0) pv0 = load_node(a.pre_tree, pn0);
-      if (pn1 >= 0) pv1 = load_node(a.pre_tree, pn1);
-      if (pn2 >= 0) pv2 = load_node(a.pre_tree, pn2);
+      // no branch between the three node loads (an unused entry loads node 0,
+      // which every listed target has), so they are in flight together
+      pv0 = load_node(a.pre_tree, pn0 >= 0 ? pn0 : 0);
+      pv1 = load_node(a.pre_tree, pn1 >= 0 ? pn1 : 0);
+      pv2 = load_node(a.pre_tree, pn2 >= 0 ? pn2 : 0);
       const uint32_t ncut = a.cut_cnt ? a.cut_cnt[a.pre_t] : 0u;
       if (ncut > 0 && ncut <= CUT_CAP && ncut <= 3 * 64 && ncut <= leaf_cap) npre = ncut;
     } else if (ti0 < ti1) {
@@ -662,7 +689,8 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
     }
   }
   if (CLOTH)  // every lane (wave reductions); s_buf is free again
-    tile_cloth(ca, active, jc, cc, w, one ? a.seg0_grid_n : 0u, a.seg0_voff, reinterpret_cast<float4*>(s_buf));
+    tile_cloth(ca, active, jc, cc, w, one ? a.seg0_grid_n : 0u, a.seg0_mesh, a.seg0_voff,
+               reinterpret_cast<float4*>(s_buf));
   wave_stats(a, w, npair);
 }
 

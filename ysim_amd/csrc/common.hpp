@@ -148,6 +148,29 @@ YSIM_HD bool point_in_tri(f3 p, f3 v0, f3 v1) {
   return a >= 0.0f && b >= 0.0f && c >= 0.0f;
 }
 
+// Square root of a squared length whose argument is 0, at least 2^-96 or not
+// finite, or whose root is only compared with a bound above 2^-48 when the
+// argument is smaller (its callers: narrow_test and the contact projection
+// after a guard of 1e-12, calc_spring, which compares the root with 1e-7).
+// On the device: the hardware root,
+// then the neighbour below or above it whose square brackets the argument
+// (two fma residuals) -- the compiler's correctly rounded square root without
+// its scaling of arguments below 2^-96 and its return of +-0 / +inf as given,
+// which leave every such argument's result unchanged.
+YSIM_HD float sqrt_len(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+  const float su = __uint_as_float(__float_as_uint(s) + 1u);
+  float r = s;
+  if (fmaf(-sd, s, x) <= 0.0f) r = sd;
+  if (fmaf(-su, s, x) > 0.0f) r = su;
+  return r;
+#else
+  return sqrtf(x);
+#endif
+}
+
 // narrow_pt_tri geometry, src/metal/bruteforce.metal:114-158: swept segment
 // [qprev, qcur] against the triangle plane; n oriented toward qprev's side;
 // fires when crossed or d_cur < band; contact point projected into the plane
@@ -158,7 +181,7 @@ YSIM_HD bool narrow_test(f3 qcur, f3 qprev, f3 t0, f3 t1, f3 t2, float band,
   f3 n = cross(v0, v1);
   float nlen2 = dot(n, n);
   if (nlen2 < 1e-12f) return false;
-  float inv = 1.0f / sqrtf(nlen2);
+  float inv = 1.0f / sqrt_len(nlen2);
   n = n * inv;
   float dprev = dot(n, qprev - t0);
   float dcur = dot(n, qcur - t0);
@@ -198,7 +221,7 @@ YSIM_HD bool overlap(f3 qmin, f3 qmax, f3 nmin, f3 nmax) {
 YSIM_HD f3 calc_spring(f3 p0, f3 v0, f3 p1, f3 v1, float rest, float ks, float kd) {
   f3 dx = p1 - p0;
   f3 dv = v1 - v0;
-  float len = sqrtf(dot(dx, dx));
+  float len = sqrt_len(dot(dx, dx));
   if (len < 1e-7f) return f3{0.f, 0.f, 0.f};
   f3 ndx = {dx.x / len, dx.y / len, dx.z / len};
   float s = ks * (len - rest) + kd * dot(dv, ndx);

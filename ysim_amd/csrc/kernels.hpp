@@ -515,7 +515,7 @@ __device__ __forceinline__ f3 cloth_apply_at(const ClothArgs& a, uint32_t j, uin
     f3 n = {nd.x, nd.y, nd.z};
     float nlen2 = dot(n, n);
     if (nlen2 < 1e-12f) return;
-    float inv = 1.0f / sqrtf(nlen2);
+    float inv = 1.0f / sqrt_len(nlen2);
     n = n * inv;
     float vn = dot(vel, n);
     float dist = nd.w;

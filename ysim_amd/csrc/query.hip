@@ -44,6 +44,15 @@ static_assert(TQ_BUF_WORDS * 4 >= 2 * 144 * 16, "the cloth step stages a 12 x 12
 constexpr int TQ_SDEPTH = TQ_BUF_WORDS / 128;  // fallback per-lane stack depth of the tile kernel
 
 __device__ __forceinline__ uint64_t lanes_below() { return (1ull << __lane_id()) - 1ull; }
+// The crossings of rows rb and columns cb of an 8 x 8 tile as a lane mask
+// (lane 8 r + c): bit r of rb spread to row r's byte (times 0x204081 moves
+// bit i to bit 8 i, times 0xff fills the byte), every byte masked by cb.
+__device__ __forceinline__ uint64_t cross_mask(uint32_t rb, uint32_t cb) {
+  const uint32_t c4 = cb * 0x01010101u;
+  const uint32_t lo = ((((rb & 0xfu) * 0x00204081u) & 0x01010101u) * 0xffu) & c4;
+  const uint32_t hi = ((((rb >> 4) * 0x00204081u) & 0x01010101u) * 0xffu) & c4;
+  return ((uint64_t)hi << 32) | lo;
+}
 __device__ __forceinline__ uint32_t popc64(uint64_t v) { return (uint32_t)__popcll(v); }
 
 // LDS handoff between the lanes of one wave (the tile kernel runs one wave per
@@ -480,21 +489,22 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
             p0 = ld3(a.x, tvb + f0);  // for the narrow phase; in flight during the tests
             p1 = ld3(a.x, tvb + f1);
             p2 = ld3(a.x, tvb + f2);
-            // rows and columns overlapping the leaf; candidates = their crossings
-            uint64_t rows = 0, cols = 0;
+            // rows and columns overlapping the leaf (bit r, bit c);
+            // candidates = their crossings
+            uint32_t rb = 0, cb = 0;
 #pragma unroll 2
             for (int r = 0; r < 8; ++r) {
               const f3 bl = {s_rc[r * 6 + 0], s_rc[r * 6 + 1], s_rc[r * 6 + 2]};
               const f3 bh = {s_rc[r * 6 + 3], s_rc[r * 6 + 4], s_rc[r * 6 + 5]};
-              if (overlap(bl, bh, L.lo, L.hi)) rows |= 0xffull << (8 * r);
+              if (overlap(bl, bh, L.lo, L.hi)) rb |= 1u << r;
             }
 #pragma unroll 2
             for (int c = 0; c < 8; ++c) {
               const f3 bl = {s_rc[48 + c * 6 + 0], s_rc[48 + c * 6 + 1], s_rc[48 + c * 6 + 2]};
               const f3 bh = {s_rc[48 + c * 6 + 3], s_rc[48 + c * 6 + 4], s_rc[48 + c * 6 + 5]};
-              if (overlap(bl, bh, L.lo, L.hi)) cols |= 0x0101010101010101ull << c;
+              if (overlap(bl, bh, L.lo, L.hi)) cb |= 1u << c;
             }
-            cand = rows & cols;
+            cand = cross_mask(rb, cb);
           }
           // the candidate vertices whose own box overlaps the leaf, as a mask
           // of tile lanes

@@ -37,8 +37,10 @@ struct NodeV {
   int a, b;
 };
 
+// i >= 0 (a node of T's window); indexed unsigned, so the address needs no
+// sign extension
 YSIM_HD NodeV load_node(const ysim_bvh_node* T, int i) {
-  const ysim_bvh_node& n = T[i];
+  const ysim_bvh_node& n = T[(uint32_t)i];
   return NodeV{{n.min[0], n.min[1], n.min[2]}, {n.max[0], n.max[1], n.max[2]}, n.child_a, n.child_b};
 }
 

@@ -44,13 +44,15 @@ static_assert(TQ_BUF_WORDS * 4 >= 2 * 144 * 16, "the cloth step stages a 12 x 12
 constexpr int TQ_SDEPTH = TQ_BUF_WORDS / 128;  // fallback per-lane stack depth of the tile kernel
 
 __device__ __forceinline__ uint64_t lanes_below() { return (1ull << __lane_id()) - 1ull; }
-// The crossings of rows rb and columns cb of an 8 x 8 tile as a lane mask
-// (lane 8 r + c): bit r of rb spread to row r's byte (times 0x204081 moves
-// bit i to bit 8 i, times 0xff fills the byte), every byte masked by cb.
+// The crossings of rows rb and columns cb (8 bits each) of an 8 x 8 tile as
+// a lane mask (lane 8 r + c): bit r of rb spread to row r's byte (times
+// 0x204081, a 24-bit product, moves bit i of a nibble to bit 8 i; x 255 as
+// (y << 8) - y fills the byte), every byte masked by cb.
 __device__ __forceinline__ uint64_t cross_mask(uint32_t rb, uint32_t cb) {
-  const uint32_t c4 = cb * 0x01010101u;
-  const uint32_t lo = ((((rb & 0xfu) * 0x00204081u) & 0x01010101u) * 0xffu) & c4;
-  const uint32_t hi = ((((rb >> 4) * 0x00204081u) & 0x01010101u) * 0xffu) & c4;
+  const uint32_t c2 = cb | (cb << 8), c4 = c2 | (c2 << 16);
+  const uint32_t ylo = __umul24(rb & 0xfu, 0x00204081u) & 0x01010101u;
+  const uint32_t yhi = __umul24((rb >> 4) & 0xfu, 0x00204081u) & 0x01010101u;
+  const uint32_t lo = ((ylo << 8) - ylo) & c4, hi = ((yhi << 8) - yhi) & c4;
   return ((uint64_t)hi << 32) | lo;
 }
 __device__ __forceinline__ uint32_t popc64(uint64_t v) { return (uint32_t)__popcll(v); }

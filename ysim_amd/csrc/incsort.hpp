@@ -3,9 +3,11 @@
 //
 // Between substeps the Morton order of a moving cloth changes very little (a
 // few positions, for a small fraction of triangles).  The incremental build
-// computes the keys in the previous substep's sorted order, sorts aligned and
-// then half-shifted windows in LDS, and checks every adjacent pair; only if
-// that check fails does the full sort run.  Because a stable sort by key is
+// computes the keys in the previous substep's sorted order, sorts aligned
+// windows of INC_MORTON_WINDOW keys and then windows of INC_WINDOW keys
+// shifted by INC_SHIFT in LDS (every boundary of the first lies at least
+// INC_SHIFT keys inside one of the second), and checks every adjacent pair;
+// only if that check fails does the full sort run.  Because a stable sort by key is
 // exactly the order by (key, facet id) — facet ids are distinct, or keys carry
 // the mesh segment — the result equals the full sort's whenever the check
 // passes.
@@ -14,7 +16,12 @@
 
 namespace ysim {
 
-constexpr uint32_t INC_WINDOW = 2048;  // keys per window (power of two)
+constexpr uint32_t INC_WINDOW = 2048;         // keys per shifted window (power of two)
+constexpr uint32_t INC_MORTON_WINDOW = 1024;  // keys per window of the Morton kernel
+constexpr uint32_t INC_SHIFT = 512;           // shifted window w starts at w INC_WINDOW - INC_SHIFT
+static_assert(INC_WINDOW % INC_MORTON_WINDOW == 0 && INC_SHIFT % INC_MORTON_WINDOW != 0 &&
+                  INC_SHIFT <= INC_MORTON_WINDOW / 2 && INC_SHIFT < INC_WINDOW,
+              "every Morton window boundary lies INC_SHIFT or more keys inside a shifted window");
 
 template <typename K>
 __device__ __forceinline__ bool kv_greater(K a, uint32_t fa, K b, uint32_t fb) {

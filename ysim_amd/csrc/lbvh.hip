@@ -260,11 +260,13 @@ int launch_morton(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x,
 // S1 of the incremental build: Morton keys in the previous substep's sorted
 // order (perm[j] = facet at sorted position j; the positions of a mesh stay in
 // its segment because keys carry the segment above the code bits), each block's
-// 2048 positions then sorted in LDS by (key, facet).  Clears need_full for the
-// check that follows.
-// 1024 threads per 2048-key window: the per-key chain of dependent loads
-// (order -> facet -> positions) needs many waves in flight.
-constexpr uint32_t INC_THREADS = 1024;
+// INC_MORTON_WINDOW positions then sorted in LDS by (key, facet).  Clears
+// need_full for the check that follows.
+// 512 threads per 1024-key window, two blocks per CU: one block's code
+// arithmetic and sort overlap the other's chain of dependent loads (order ->
+// facet -> positions); one 1024-thread block per CU ran those phases in
+// lockstep.
+constexpr uint32_t INC_THREADS = 512;
 
 template <typename K>
 __global__ __launch_bounds__(INC_THREADS) void k_morton_inc(SegTable ts, MeshTables mt, const float* __restrict__ x,
@@ -274,8 +276,8 @@ __global__ __launch_bounds__(INC_THREADS) void k_morton_inc(SegTable ts, MeshTab
                                                     uint32_t* __restrict__ need_full,
                                                     const uint32_t* __restrict__ box_parts, uint32_t nparts,
                                                     uint32_t* __restrict__ cut_reset, int32_t* __restrict__ cut_fill) {
-  __shared__ K sk[INC_WINDOW];
-  __shared__ uint32_t sv[INC_WINDOW];
+  __shared__ K sk[INC_MORTON_WINDOW];
+  __shared__ uint32_t sv[INC_MORTON_WINDOW];
   __shared__ uint32_t s_part[INC_THREADS];
   __shared__ uint32_t s_box[6];
   if (blockIdx.x == 0 && threadIdx.x == 0) *need_full = 0u;
@@ -288,10 +290,10 @@ __global__ __launch_bounds__(INC_THREADS) void k_morton_inc(SegTable ts, MeshTab
   // first; the box records (one segment: the cloth step's per-block boxes)
   // load with the facet ids and are reduced while the positions are in
   // flight, so the box costs no round of its own.
-  static_assert(INC_WINDOW == 2 * INC_THREADS, "two items per thread");
+  static_assert(INC_MORTON_WINDOW == 2 * INC_THREADS, "two items per thread");
   const bool wide = code_bits > 32;
-  const uint32_t start = blockIdx.x * INC_WINDOW;
-  const uint32_t cnt = min(INC_WINDOW, n - start);
+  const uint32_t start = blockIdx.x * INC_MORTON_WINDOW;
+  const uint32_t cnt = min(INC_MORTON_WINDOW, n - start);
   uint32_t fid[2], sgi[2];
   f3 x0[2], x1[2], x2[2];
   SegInfo si[2];
@@ -363,7 +365,7 @@ __global__ __launch_bounds__(INC_THREADS) void k_morton_inc(SegTable ts, MeshTab
     sv[i] = fid[k];
   }
   __syncthreads();
-  window_sort<K, INC_WINDOW, INC_THREADS>(sk, sv, cnt);
+  window_sort<K, INC_MORTON_WINDOW, INC_THREADS>(sk, sv, cnt);
   for (uint32_t i = threadIdx.x; i < cnt; i += INC_THREADS) {
     keys[start + i] = sk[i];
     vals[start + i] = sv[i];
@@ -376,7 +378,7 @@ int launch_morton_inc(hipStream_t s, SegTable tsegs, MeshTables mt, const float*
                       uint32_t nparts, uint32_t* cut_reset, int32_t* cut_fill) {
   if (!nitems) return YSIM_OK;
   if (nparts && (!box_parts || tsegs.nseg != 1)) return YSIM_ERR_INVALID;
-  const uint32_t blocks = ceil_div(nitems, INC_WINDOW);
+  const uint32_t blocks = ceil_div(nitems, INC_MORTON_WINDOW);
   if (key64)
     hipLaunchKernelGGL(k_morton_inc<uint64_t>, dim3(blocks), dim3(INC_THREADS), 0, s, tsegs, mt, x, facets, box_acc,
                        code_bits, perm, (uint64_t*)keys, vals, nitems, need_full, box_parts, nparts, cut_reset,

@@ -562,8 +562,9 @@ int launch_sort_hybrid(hipStream_t s, const SortPlan& p, void* keys_a, uint32_t*
 
 
 // ------------------------------------------------------------------ incremental
-// Second window pass of the incremental sort: windows straddling the aligned
-// window boundaries (block b covers [b*W - W/2, b*W + W/2)), sorted in place,
+// Second window pass of the incremental sort: windows straddling the Morton
+// kernel's window boundaries (block b covers [b W - INC_SHIFT, (b + 1) W -
+// INC_SHIFT), W = INC_WINDOW), sorted in place,
 // then the exact check.  Every adjacent pair inside a window is ordered once
 // the window is sorted, so only the pairs across the shifted windows'
 // boundaries remain: each block publishes its window's first and last (key,
@@ -580,7 +581,7 @@ __global__ __launch_bounds__(WS_THREADS) void k_window_sort_shifted(K* __restric
   __shared__ K sk[INC_WINDOW];
   __shared__ uint32_t sv[INC_WINDOW];
   const uint32_t w = blockIdx.x;
-  const int64_t s0 = (int64_t)w * INC_WINDOW - INC_WINDOW / 2;
+  const int64_t s0 = (int64_t)w * INC_WINDOW - INC_SHIFT;
   const uint32_t lo = (uint32_t)(s0 < 0 ? 0 : s0);
   const uint32_t hi = (uint32_t)min<int64_t>(s0 + INC_WINDOW, (int64_t)n);
   if (lo >= hi) return;
@@ -596,7 +597,7 @@ __global__ __launch_bounds__(WS_THREADS) void k_window_sort_shifted(K* __restric
       vals[lo + i] = sv[i];
     }
   if (threadIdx.x != 0) return;
-  const uint32_t nw = (n + INC_WINDOW / 2 + INC_WINDOW - 1) / INC_WINDOW;  // non-empty windows
+  const uint32_t nw = (n + INC_SHIFT + INC_WINDOW - 1) / INC_WINDOW;  // non-empty windows
   uint64_t* e = edges + 3 * (size_t)w;
   st_agent_u64(e + 0, (uint64_t)sk[0]);
   st_agent_u64(e + 1, (uint64_t)sk[cnt - 1]);

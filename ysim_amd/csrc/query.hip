@@ -43,7 +43,10 @@ static_assert(TQ_BUF_WORDS % 128 == 0, "whole per-lane stack entries");
 static_assert(TQ_BUF_WORDS * 4 >= 2 * 144 * 16, "the cloth step stages a 12 x 12 halo of x and v in the buffer");
 constexpr int TQ_SDEPTH = TQ_BUF_WORDS / 128;  // fallback per-lane stack depth of the tile kernel
 
-__device__ __forceinline__ uint64_t lanes_below() { return (1ull << __lane_id()) - 1ull; }
+// Set bits of the wave-uniform mask m at lanes below this one.
+__device__ __forceinline__ uint32_t count_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
 // The crossings of rows rb and columns cb (8 bits each) of an 8 x 8 tile as
 // a lane mask (lane 8 r + c): bit r of rb spread to row r's byte (times
 // 0x204081, a 24-bit product, moves bit i of a nibble to bit 8 i; x 255 as
@@ -412,20 +415,16 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
     uint32_t sp_pre = 0, nl_pre = 0;
     bool ovf_pre = false;
     auto pre_batch = [&](uint32_t c0, int node, const NodeV& A) {
-      const bool have = c0 + lane < npre;
-      bool pc = false, lc = false;
-      if (have) {
-        const bool h = overlap(Ulo, Uhi, A.lo, A.hi);
-        pc = h && A.a >= 0;
-        lc = h && A.a < 0;
-      }
+      // branch-free: the flags stay lane masks for the ballots
+      const bool h = (c0 + lane < npre) & overlap(Ulo, Uhi, A.lo, A.hi);
+      const bool pc = h & (A.a >= 0), lc = h & (A.a < 0);
       const uint64_t bp = __ballot(pc), bl = __ballot(lc);
       if (sp_pre + popc64(bp) > stack_cap) {
         ovf_pre = true;
         return;
       }
-      if (pc) s_stk[sp_pre + popc64(bp & lanes_below())] = make_int2(A.a, A.b);
-      if (lc) s_leaf[nl_pre + popc64(bl & lanes_below())] = make_uint2((uint32_t)node, (uint32_t)A.b);
+      if (pc) s_stk[sp_pre + count_below(bp)] = make_int2(A.a, A.b);
+      if (lc) s_leaf[nl_pre + count_below(bl)] = make_uint2((uint32_t)node, (uint32_t)A.b);
       sp_pre += popc64(bp);
       nl_pre += popc64(bl);
     };
@@ -550,8 +549,7 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
             // exclusive scan of the candidate counts (nc <= 4) from the ballots
             // of their three bits
             const uint64_t c1 = __ballot(nc & 1u), c2 = __ballot(nc & 2u), c4 = __ballot(nc & 4u);
-            const uint64_t below = lanes_below();
-            const uint32_t excl = popc64(c1 & below) + 2 * popc64(c2 & below) + 4 * popc64(c4 & below);
+            const uint32_t excl = count_below(c1) + 2 * count_below(c2) + 4 * count_below(c4);
             const uint32_t tot = popc64(c1) + 2 * popc64(c2) + 4 * popc64(c4);
             if (np + tot > TQ_PAIRS) flush();
             const uint32_t o = np + excl;
@@ -590,24 +588,20 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
             leaves(nl);
             nl = 0;
           }
+          // branch-free: lanes past the cut load its first entry of this
+          // batch (a valid node) and drop the result
           const bool have = c0 + lane < ncut;
-          bool pc = false, lc = false;
-          int node = 0;
-          NodeV A;
-          if (have) {
-            node = CL[c0 + lane];
-            A = load_node(T, node);
-            const bool h = overlap(Ulo, Uhi, A.lo, A.hi);
-            pc = h && A.a >= 0;
-            lc = h && A.a < 0;
-          }
+          const int node = CL[c0 + (have ? lane : 0u)];
+          const NodeV A = load_node(T, node);
+          const bool h = have & overlap(Ulo, Uhi, A.lo, A.hi);
+          const bool pc = h & (A.a >= 0), lc = h & (A.a < 0);
           const uint64_t bp = __ballot(pc), bl = __ballot(lc);
           if (sp + popc64(bp) > stack_cap) {
             overflow = true;
             break;
           }
-          if (pc) s_stk[sp + popc64(bp & lanes_below())] = make_int2(A.a, A.b);
-          if (lc) s_leaf[nl + popc64(bl & lanes_below())] = make_uint2((uint32_t)node, (uint32_t)A.b);
+          if (pc) s_stk[sp + count_below(bp)] = make_int2(A.a, A.b);
+          if (lc) s_leaf[nl + count_below(bl)] = make_uint2((uint32_t)node, (uint32_t)A.b);
           sp += popc64(bp);
           nl += popc64(bl);
           wave_sync();
@@ -637,31 +631,25 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
           leaves(nl);
           nl = 0;
         }
+        // branch-free, so the flags stay lane masks for the ballots: lanes
+        // past the k pending pairs load pair b0 (a valid one) and drop it
         const bool have = lane < k;
-        int2 e = make_int2(0, 0);
-        if (have) e = s_stk[b0 + lane];
+        const int2 e = s_stk[b0 + (have ? lane : 0u)];
         wave_sync();
         sp = b0;
-        bool pA = false, pB = false, lA = false, lB = false;
-        NodeV A, B;
-        if (have) {
-          A = load_node(T, e.x);
-          B = load_node(T, e.y);
-          const bool hA = overlap(Ulo, Uhi, A.lo, A.hi), hB = overlap(Ulo, Uhi, B.lo, B.hi);
-          pA = hA && A.a >= 0;
-          pB = hB && B.a >= 0;
-          lA = hA && A.a < 0;
-          lB = hB && B.a < 0;
-        }
-        const uint64_t bp1 = __ballot(pA || pB), bp2 = __ballot(pA && pB);
-        const uint64_t bl1 = __ballot(lA || lB), bl2 = __ballot(lA && lB);
+        const NodeV A = load_node(T, e.x), B = load_node(T, e.y);
+        const bool hA = have & overlap(Ulo, Uhi, A.lo, A.hi), hB = have & overlap(Ulo, Uhi, B.lo, B.hi);
+        const bool pA = hA & (A.a >= 0), pB = hB & (B.a >= 0);
+        const bool lA = hA & (A.a < 0), lB = hB & (B.a < 0);
+        const uint64_t bp1 = __ballot(pA | pB), bp2 = __ballot(pA & pB);
+        const uint64_t bl1 = __ballot(lA | lB), bl2 = __ballot(lA & lB);
         const uint32_t tp = popc64(bp1) + popc64(bp2), tl = popc64(bl1) + popc64(bl2);
         if (sp + tp > stack_cap) {
           overflow = true;
           break;
         }
-        uint32_t op = sp + popc64(bp1 & lanes_below()) + popc64(bp2 & lanes_below());
-        uint32_t ol = nl + popc64(bl1 & lanes_below()) + popc64(bl2 & lanes_below());
+        uint32_t op = sp + count_below(bp1) + count_below(bp2);
+        uint32_t ol = nl + count_below(bl1) + count_below(bl2);
         if (pA) s_stk[op++] = make_int2(A.a, A.b);
         if (pB) s_stk[op++] = make_int2(B.a, B.b);
         if (lA) s_leaf[ol++] = make_uint2((uint32_t)e.x, (uint32_t)A.b);

@@ -126,7 +126,11 @@ int launch_morton_inc(hipStream_t s, SegTable tsegs, MeshTables mt, const float*
 // writes each node whole (box and child links) in one pass, so no node is
 // written in part twice per rebuild.
 struct RefitLinks {
-  uint32_t* up = nullptr;   // [node window + node] parent's leaf count << 1 | (node is child_b)
+  // [node window + node] parent's leaf count << 2 | (the parent's index is
+  // the high end of its leaf range) << 1 | (node is child_b): with the node's
+  // own range this gives the parent's range and index (a Karras node is an
+  // endpoint of its range)
+  uint32_t* up = nullptr;
   uint32_t* fid = nullptr;  // [node window + leaf slot] the leaf's facet id
   // Tree cut per mesh: the nodes of at most S = ceil(leaves / 128) leaves whose
   // parent has more (a frontier that covers every leaf once), appended in any

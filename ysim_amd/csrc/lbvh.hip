@@ -600,10 +600,10 @@ __global__ __launch_bounds__(256) void k_hierarchy(SegTable ts, MeshTables mt, c
   if (!hinted) {
     P[ca] = i;
     P[cb] = i;
-    if (rl.up) {
-      const uint32_t size2 = (uint32_t)(hi - lo + 1) << 1;
-      rl.up[base + ca] = size2;       // left child
-      rl.up[base + cb] = size2 | 1u;  // right child
+    if (rl.up) {  // this node is the high end of its range when d < 0
+      const uint32_t size4 = ((uint32_t)(hi - lo + 1) << 2) | (d < 0 ? 2u : 0u);
+      rl.up[base + ca] = size4;       // left child
+      rl.up[base + cb] = size4 | 1u;  // right child
     }
   }
   if (rl.cut) {  // children of at most S leaves under a node of more
@@ -955,10 +955,11 @@ __global__ __launch_bounds__(256) void k_refit(SegTable ts, MeshTables mt, const
 // Same refit with the lower levels resolved inside the workgroup.  A block
 // owns RB (256, 512 or 1024; chosen by the autotuner) consecutive sorted leaves.
 // Each thread tracks the leaf range of its current node; the parent's range
-// follows from the parent's leaf count and the side, both in `up` (written by
-// the hierarchy kernel), and so does the sibling: a Karras node's children are
-// the nodes at its split and split + 1.  The block first copies the parent
-// links of its leaves and of the internal nodes indexed inside it to LDS.  An
+// follows from the parent's leaf count and the side, and the parent's index
+// from which end of that range it is, all in `up` (written by the hierarchy
+// kernel), and so does the sibling: a Karras node's children are the nodes at
+// its split and split + 1.  So the parent array is not read.  The block first
+// copies the `up` words of the internal nodes indexed inside it to LDS.  An
 // internal node whose range lies inside the block is then combined through LDS
 // with no device memory access (the children's boxes and an arrival counter
 // per node; a Karras internal node is an endpoint of its own range, so its
@@ -992,13 +993,12 @@ template <bool FORMAL, int RB, bool TOP>
 __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, const float* __restrict__ x,
                                                   const float* __restrict__ v, float dt, int swept,
                                                   const uint32_t* __restrict__ facets, ysim_bvh_node* tree,
-                                                  const int32_t* __restrict__ parent,
                                                   const uint32_t* __restrict__ up_links,
                                                   const uint32_t* __restrict__ fids, uint32_t* visit,
                                                   uint32_t n, uint32_t* err, RefitSpan sp) {
   __shared__ uint32_t s_cnt[RB];
   __shared__ float s_box[RB][2][6];
-  __shared__ int2 s_link[RB];  // internal node (slot) of the thread's mesh: (parent, up)
+  __shared__ uint32_t s_up[RB];  // `up` of the internal node at the slot (of the thread's mesh)
   __shared__ uint32_t s_nesc;
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t b0 = blk * RB;
@@ -1012,25 +1012,22 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
   const int i = (int)((valid ? j : n - 1) - s);
   const uint32_t base = si.noff;
   ysim_bvh_node* T = tree + base;
-  const int32_t* P = parent + base;
   const uint32_t* UP = up_links + base;
   uint32_t* C = visit + base;
   // the block's internal links load while the leaf's box is gathered; they
   // reach LDS after it
   const bool has_link = valid && i < nn - 1;
-  int2 link = make_int2(0, 0);
-  if (has_link) link = make_int2(P[i], (int)UP[i]);
+  uint32_t link = 0;
+  if (has_link) link = UP[i];
   int child = nn - 1 + i;
   uint32_t fid = 0, upnext = 0;
-  int pnext = 0;
   f3 lo = {0.f, 0.f, 0.f}, hi = lo;
   if (valid) {
     fid = fids[base + child];
-    pnext = P[child];
     upnext = UP[child];
     tri_box(x, v, dt, swept, facets + 3 * (size_t)(si.foff + fid), si.voff, lo, hi);
   }
-  if (has_link) s_link[threadIdx.x] = link;
+  if (has_link) s_up[threadIdx.x] = link;
   __syncthreads();
   if (!TOP && !valid) return;
   int la = -1, lb = (int)fid;  // child links of the node `child`
@@ -1062,11 +1059,11 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
     // node stores (the branch-light loop keeps the per-level instruction
     // count low; with 16 waves of a block per CU the loop is issue-bound).
     for (int step = 0;; ++step) {
-      const int p = pnext;
       const uint32_t up = upnext;
       side = up & 1u;  // 0: child_a (left part of p's range)
-      const uint32_t psize = up >> 1;
+      const uint32_t psize = up >> 2;
       const uint32_t plo = side ? chi + 1 - psize : clo, phi = side ? chi : clo + psize - 1;
+      const int p = (int)((up & 2u) ? phi : plo);
       // sibling: child_b at split + 1 for a child_a, child_a at split for a
       // child_b; a leaf when that position ends p's range
       const uint32_t q = side ? clo - 1 : chi + 1, qe = side ? plo : phi;
@@ -1097,9 +1094,7 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
       child = p;
       clo = plo;
       chi = phi;
-      const int2 l = s_link[slot];
-      pnext = l.x;
-      upnext = (uint32_t)l.y;
+      upnext = s_up[slot];
       if (p == 0) {
         store_node(0, lo, hi, la, lb, false);
         break;
@@ -1159,16 +1154,15 @@ __global__ __launch_bounds__(RB) void k_refit_lds(SegTable ts, MeshTables mt, co
     // next level's links loaded while the arrival resolves.
     store_node(child, lo, hi, la, lb, true);
     for (int step = 0; step < 2 * nn; ++step) {
-      const int p = pnext;
       const uint32_t up = upnext;
-      if (p < 0 || p >= nn - 1) break;
       side = up & 1u;
-      const uint32_t psize = up >> 1;
+      const uint32_t psize = up >> 2;
       const uint32_t plo = side ? chi + 1 - psize : clo, phi = side ? chi : clo + psize - 1;
+      const int p = (int)((up & 2u) ? phi : plo);
+      if (plo > phi || p < 0 || p >= nn - 1) break;
       const uint32_t sq = side ? clo - 1 : chi + 1, sqe = side ? plo : phi;
       const int sib = (int)sq + (sq == sqe ? nn - 1 : 0);
       if (sib < 0 || sib >= 2 * nn - 1) break;
-      pnext = P[p];
       upnext = UP[p];
       uint32_t old;
       if (FORMAL) {
@@ -1282,7 +1276,7 @@ int launch_refit(hipStream_t s, SegTable tsegs, MeshTables mt, const float* x, c
     const RefitSpan sp = top ? *span : RefitSpan{};
 #define YSIM_R(F, B, TP)                                                                                     \
   hipLaunchKernelGGL((k_refit_lds<F, B, TP>), dim3(ceil_div(nitems, B)), dim3(B), 0, s, tsegs, mt, x, v, dt, \
-                     (int)swept, facets, tree, parent, rl->up, rl->fid, visit, nitems, err, sp)
+                     (int)swept, facets, tree, rl->up, rl->fid, visit, nitems, err, sp)
     if (formal) {
       if (block == 256) YSIM_R(true, 256, false); else if (block == 512) YSIM_R(true, 512, false); else YSIM_R(true, 1024, false);
     } else if (!top) {

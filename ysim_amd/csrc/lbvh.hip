@@ -435,13 +435,13 @@ __global__ __launch_bounds__(256) void k_hierarchy(SegTable ts, MeshTables mt, c
   // reset accumulators for the next build (their readers ran in earlier kernels)
   if (j < nbox * 6) box_acc[j] = ((j % 6) < 3) ? 0xffffffffu : 0u;
   if (j < hist_words) hist[j] = 0;
-  if (skip.flag && *skip.flag) {
-    if (j == 0 && skip.note) __hip_atomic_store(skip.note, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    return;
-  }
   // lanes past the items stay for the wave's cooperative searches below
   const bool live = j < n;
-  if (__ballot(live) == 0) return;
+  if (__ballot(live) == 0) {
+    if (j == 0 && skip.flag && *skip.flag && skip.note)
+      __hip_atomic_store(skip.note, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    return;
+  }
   const SegInfo si = seg_info(ts, mt, live ? j : n - 1);
   const uint32_t m = si.m;
   const uint32_t s = si.s;
@@ -453,22 +453,33 @@ __global__ __launch_bounds__(256) void k_hierarchy(SegTable ts, MeshTables mt, c
   const bool wide = code_bits > 32;
   const uint64_t mask = (code_bits >= 64) ? ~0ull : ((1ull << code_bits) - 1);
   KarrasSeg<K> K_{keys + s, vals + s, nn, mask, wide};
-  if (live) {
-    // the sorted facet order seeds the next incremental build
-    if (perm_out) perm_out[j] = K_.idx[i];
-    // leaf slot nn-1+i: child_a = -1, child_b = facet id (the LDS refit writes
-    // the node when rl.fid is set)
-    if (rl.fid) {
-      rl.fid[base + nn - 1 + i] = K_.idx[i];
-    } else {
-      T[nn - 1 + i].child_a = -1;
-      T[nn - 1 + i].child_b = (int32_t)K_.idx[i];
-    }
-    if (i == 0) P[0] = -1;
-  }
   // Internal node i (i < nn - 1; the other lanes only help the wave's
   // cooperative searches below).
   const bool node = live && i < nn - 1;
+  // The item's facet, the node's hint and its neighbours' keys load before
+  // the deferral flag is read (every address is valid either way), so a
+  // replayed build pays one round trip for all of them; nothing is stored
+  // before the flag is known.
+  const uint32_t idx_i = K_.idx[i];
+  const int2 h = (rl.hint && node) ? rl.hint[base + i] : make_int2(i, i);
+  const uint64_t ci = K_.code(i), cl = K_.code_at(i - 1), cr = K_.code_at(i + 1);
+  if (skip.flag && *skip.flag) {
+    if (j == 0 && skip.note) __hip_atomic_store(skip.note, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    return;
+  }
+  if (live) {
+    // the sorted facet order seeds the next incremental build
+    if (perm_out) perm_out[j] = idx_i;
+    // leaf slot nn-1+i: child_a = -1, child_b = facet id (the LDS refit writes
+    // the node when rl.fid is set)
+    if (rl.fid) {
+      rl.fid[base + nn - 1 + i] = idx_i;
+    } else {
+      T[nn - 1 + i].child_a = -1;
+      T[nn - 1 + i].child_b = (int32_t)idx_i;
+    }
+    if (i == 0) P[0] = -1;
+  }
   if (node) visit[base + i] = 0;
   // determineRange, bvh.metal:240-258.  delta(i, i + k d) does not increase
   // with k (sorted (code, index) keys), so the range end is the last k whose
@@ -496,9 +507,7 @@ __global__ __launch_bounds__(256) void k_hierarchy(SegTable ts, MeshTables mt, c
   int lo = 0, hi = 0;
   uint32_t split = 0;
   if (node) {
-    const int2 h = rl.hint ? rl.hint[base + i] : make_int2(i, i);
     const int hd = h.x > i ? 1 : -1;
-    const uint64_t ci = K_.code(i), cl = K_.code_at(i - 1), cr = K_.code_at(i + 1);
     const uint64_t cj = K_.code_at(h.x), cjn = K_.code_at(h.x + hd), cs = K_.code_at(h.y), cs1 = K_.code_at(h.y + 1);
     const int dr = K_.delta_c(i, i + 1, ci, cr), dl = K_.delta_c(i, i - 1, ci, cl);
     d = (dr - dl >= 0) ? 1 : -1;

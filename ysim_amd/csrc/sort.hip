@@ -603,28 +603,25 @@ __global__ __launch_bounds__(WS_THREADS) void k_window_sort_shifted(K* __restric
   st_agent_u64(e + 1, (uint64_t)sk[cnt - 1]);
   st_agent_u64(e + 2, (uint64_t)sv[0] | ((uint64_t)sv[cnt - 1] << 32));
   wait_all_mem();
-  for (int k = 0; k < 2; ++k) {
-    const int64_t b = (int64_t)w - 1 + k;  // boundary between windows b and b + 1
-    if (b < 0 || b + 1 >= (int64_t)nw) continue;
-    const uint32_t old = __hip_atomic_fetch_add(bcnt + b, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if ((old & 1u) == 0u) continue;  // first arrival: the neighbour checks
-    K last_k, first_k;
-    uint32_t last_v, first_v;
-    if (b == (int64_t)w) {
-      last_k = sk[cnt - 1];
-      last_v = sv[cnt - 1];
-      const uint64_t* o = edges + 3 * (size_t)(b + 1);
-      first_k = (K)ld_agent_u64(o + 0);
-      first_v = (uint32_t)ld_agent_u64(o + 2);
-    } else {
-      const uint64_t* o = edges + 3 * (size_t)b;
-      last_k = (K)ld_agent_u64(o + 1);
-      last_v = (uint32_t)(ld_agent_u64(o + 2) >> 32);
-      first_k = sk[0];
-      first_v = sv[0];
-    }
-    if (!kv_greater(first_k, first_v, last_k, last_v)) atomicOr(need_full, 1u);
+  // both arrivals go out together, then the second arrivals' neighbour loads
+  // (three round trips after the window instead of up to five)
+  const bool okl = w >= 1 && w < nw, okr = w + 1 < nw;  // boundaries w - 1 and w
+  const uint32_t oldl = okl ? __hip_atomic_fetch_add(bcnt + (w - 1), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+  const uint32_t oldr = okr ? __hip_atomic_fetch_add(bcnt + w, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+  const bool chkl = okl && (oldl & 1u), chkr = okr && (oldr & 1u);  // second arrivals check
+  uint64_t lk = 0, lv = 0, rk = 0, rv = 0;
+  if (chkl) {  // window w - 1's last (key, facet)
+    const uint64_t* o = edges + 3 * (size_t)(w - 1);
+    lk = ld_agent_u64(o + 1);
+    lv = ld_agent_u64(o + 2);
   }
+  if (chkr) {  // window w + 1's first (key, facet)
+    const uint64_t* o = edges + 3 * (size_t)(w + 1);
+    rk = ld_agent_u64(o + 0);
+    rv = ld_agent_u64(o + 2);
+  }
+  if (chkl && !kv_greater(sk[0], sv[0], (K)lk, (uint32_t)(lv >> 32))) atomicOr(need_full, 1u);
+  if (chkr && !kv_greater((K)rk, (uint32_t)rv, sk[cnt - 1], sv[cnt - 1])) atomicOr(need_full, 1u);
 }
 
 uint32_t incremental_windows(uint32_t n) { return n / INC_WINDOW + 2; }

@@ -33,6 +33,7 @@ constexpr int SDEPTH = 8;          // LDS stack entries per thread (deeper stack
 constexpr uint32_t TQ_STACK = 128; // tile walk: pending child pairs
 constexpr uint32_t TQ_LEAVES = 192;
 constexpr uint32_t TQ_PAIRS = 256;  // one emission round adds up to 4 pairs per lane
+constexpr uint32_t TQ_HEAVY = 16;   // a leaf with more candidate vertices is tested by the whole wave
 static_assert(TQ_PAIRS >= 64 * 4, "an emission round must fit the pair buffer after a flush");
 static_assert(TQ_LEAVES >= 2 * 64, "a walk round adds up to 2 leaves per lane");
 // LDS words of the tile kernel's shared buffer: pending child pairs (2 each),
@@ -508,8 +509,39 @@ __attribute__((amdgpu_waves_per_eu(CLOTH ? 4 : 5, CLOTH ? 4 : 5))) __global__ __
             cand = cross_mask(rb, cb);
           }
           // the candidate vertices whose own box overlaps the leaf, as a mask
-          // of tile lanes
+          // of tile lanes.  A leaf with more than TQ_HEAVY candidates (a
+          // stretched or fast triangle spanning the tile) is tested by the
+          // whole wave below, one vertex per lane, instead of one candidate
+          // after another by its own lane
+          const bool heavy = popc64(cand) > TQ_HEAVY;
+          uint64_t hv = __ballot(heavy);
+          const uint64_t hcand = cand;
+          if (heavy) cand = 0;
           uint64_t vm = 0;
+          if (hv) {
+            // this lane's vertex, as the loop below reads a candidate's
+            const f3 vc = {s_q[lane * 6 + 0], s_q[lane * 6 + 1], s_q[lane * 6 + 2]};
+            const f3 vl = {vc.x - margin, vc.y - margin, vc.z - margin};
+            const f3 vh = {vc.x + margin, vc.y + margin, vc.z + margin};
+            const uint32_t pv = s_p[lane];
+            const bool act = base + lane < a.nitems;
+            do {  // wave-uniform: leaf l's box, corners and mask read from its lane
+              const uint32_t l = (uint32_t)__builtin_ctzll(hv);
+              hv &= hv - 1;
+              const auto rl = [l](float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); };
+              const f3 llo = {rl(L.lo.x), rl(L.lo.y), rl(L.lo.z)}, lhi = {rl(L.hi.x), rl(L.hi.y), rl(L.hi.z)};
+              const uint32_t g0 = (uint32_t)__builtin_amdgcn_readlane((int)f0, l);
+              const uint32_t g1 = (uint32_t)__builtin_amdgcn_readlane((int)f1, l);
+              const uint32_t g2 = (uint32_t)__builtin_amdgcn_readlane((int)f2, l);
+              const uint32_t clo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)hcand, l);
+              const uint32_t chi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(hcand >> 32), l);
+              const uint64_t cl = ((uint64_t)chi << 32) | clo;
+              const bool ok = ((cl >> lane) & 1u) != 0 && act && overlap(vl, vh, llo, lhi) &&
+                              !(self && (pv == g0 || pv == g1 || pv == g2));
+              const uint64_t m1 = __ballot(ok);
+              if (lane == l) vm = m1;
+            } while (hv);
+          }
           while (cand) {
             const uint32_t v = __builtin_ctzll(cand);
             cand &= cand - 1;

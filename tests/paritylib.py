@@ -70,3 +70,38 @@ def cut_cover(nodes, cut) -> np.ndarray:
                 stack.append(int(ca[k]))
                 stack.append(int(cb[k]))
     return cover
+
+
+def heavy_tile_leaves(x, v, facets, n: int, margin: float, dt: float, heavy: int = 16) -> int:
+    """(tile, leaf) pairs of an n x n grid cloth's self query that the tile
+    walk tests with the whole wave: the leaf's box (the facet's corners at x
+    and x + v dt) overlaps more than `heavy` crossings of the 8 x 8 tile's row
+    and column boxes (each the union of its 8 vertices' [x - m, x + m]).
+    Counted on the CPU from a state, so a GPU test can show that the case it
+    audits takes that path."""
+    x = np.asarray(x, np.float32).reshape(n, n, 3)
+    s = (np.asarray(x, np.float32).reshape(-1, 3) + np.asarray(v, np.float32).reshape(-1, 3) * np.float32(dt))
+    f = np.asarray(facets).reshape(-1, 3)
+    X = x.reshape(-1, 3)
+    llo = np.minimum(X[f].min(1), s[f].min(1))
+    lhi = np.maximum(X[f].max(1), s[f].max(1))
+    # a leaf crossing more than 16 of a tile's 64 row x column crossings
+    # spans at least 5 rows or 5 columns, so it is wider than 1.5 grid steps
+    ext = (lhi - llo).max(1)
+    step = float(np.median(np.linalg.norm(x[:, 1:] - x[:, :-1], axis=-1)))
+    big = np.nonzero(ext > 1.5 * step)[0]
+    if big.size == 0:
+        return 0
+    t = x.reshape(n // 8, 8, n // 8, 8, 3)
+    m = np.float32(margin)
+    # tile rows: (tile row, row, tile column, 3) -> (tiles, 8, 3)
+    rlo = (t - m).min(axis=3).transpose(0, 2, 1, 3).reshape(-1, 8, 3)
+    rhi = (t + m).max(axis=3).transpose(0, 2, 1, 3).reshape(-1, 8, 3)
+    clo = (t - m).min(axis=1).reshape(-1, 8, 3)  # tile columns
+    chi = (t + m).max(axis=1).reshape(-1, 8, 3)
+    count = 0
+    for i in big:
+        ro = np.all((rlo <= lhi[i]) & (rhi >= llo[i]), axis=-1).sum(1)
+        co = np.all((clo <= lhi[i]) & (chi >= llo[i]), axis=-1).sum(1)
+        count += int(np.count_nonzero(ro * co > heavy))
+    return count

@@ -143,3 +143,28 @@ def test_cut_cover_on_oracle_tree():
     assert np.all(cut_cover(nodes, [0]) == 1)
     assert np.any(cut_cover(nodes, [0, a]) == 2)
     assert np.any(cut_cover(nodes, [a]) == 0)
+
+
+def test_heavy_tile_leaves_known_case():
+    """A flat 16 x 16 grid (unit steps, margin 0.5) with its own cell
+    triangles and one planted triangle over rows and columns 0-10: the
+    planted leaf crosses 64, 24, 24 and 9 row x column crossings of the four
+    tiles, so exactly three (tile, leaf) pairs pass the 16 threshold; the
+    cell triangles cross at most 3 x 3."""
+    from paritylib import heavy_tile_leaves
+    n = 16
+    r, c = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    x = np.stack([c, r, np.zeros_like(c)], -1).astype(np.float32).reshape(-1, 3)
+    cells = []
+    for i in range(n - 1):
+        for j in range(n - 1):
+            a, b, d, e = i * n + j, i * n + j + 1, (i + 1) * n + j, (i + 1) * n + j + 1
+            cells += [[a, b, e], [a, e, d]]
+    cells = np.array(cells, np.uint32)
+    v = np.zeros_like(x)
+    assert heavy_tile_leaves(x, v, cells, n, 0.5, 0.0) == 0
+    planted = np.vstack([cells, [[0, 10, 10 * n + 10]]]).astype(np.uint32)
+    assert heavy_tile_leaves(x, v, planted, n, 0.5, 0.0) == 3
+    # a sweep along +x by 2 steps reaches column 12: 8 x 5 in the right tiles
+    v[[0, 10, 10 * n + 10], 0] = 2.0
+    assert heavy_tile_leaves(x, v, planted, n, 0.5, 1.0) == 3

@@ -18,7 +18,7 @@ import numpy as np
 import pytest
 
 from oracle.oracle import OracleScene
-from paritylib import assert_same_candidates
+from paritylib import heavy_tile_leaves, assert_same_candidates
 from ysim_amd import scenes
 from ysim_amd.hip import HipScene
 
@@ -93,6 +93,12 @@ def test_c2_contact_active_from_gpu_state(ctx, oracle_threads):
             hs.step(4, graph=True)
             os_.step(4)
             want = os_.pairs()
+            # the audited substep has leaves the tile walk tests with the
+            # whole wave (more than 16 row x column crossings of a tile)
+            xo, vo, _ = os_.state()
+            heavy = heavy_tile_leaves(xo, vo, sc.meshes[0].facets, 512, sc.params["margin"], sc.params["subh"])
+            print(f"heavy (tile, leaf) pairs at substep 319: {heavy}", flush=True)
+            assert heavy > 0
             hs.set_audit(1 << 21)
             hs.step(1)
             os_.step(1)
